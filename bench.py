@@ -1,0 +1,157 @@
+"""Device-load benchmark: instance loads/s of one load pass
+(updateState + updateSecondaryState + loadDAEVectors + loadDAEMatrices) over
+all instances of one model type, on one MI355X.
+
+Workload: the device-evaluation microbenchmark -- 2^20 instances of the
+model, private nodes, terminal voltages i.i.d. U[-0.5, 1.5] V, W/L lognormal
+(sigma 0.2) around 90/45 nm (AREA around 1 for diode/BJT), transient
+newtonIter = 1 with voltage limiting.  Inputs are resident in HBM before
+timing starts.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(model, n_sample, seconds):
+    """The oracle's serial load pass on a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle
+    from xyce_amd.workloads import CARDS, SEED, instance_params
+    rng = np.random.default_rng(np.random.SeedSequence(SEED))
+    m = oracle.Master(model)
+    cards = CARDS[model]
+    for level, card in cards:
+        m.add_model(card, level=level)
+    mi = rng.integers(0, len(cards), size=n_sample).astype(np.int32)
+    m.add_instances(n_sample, mi, instance_params(model, n_sample, rng))
+    m.setup()
+    n_ext, n_int, n_sta, n_sto = m.counts()
+    nn = int((n_ext + n_int).sum())
+    m.register_lids(np.arange(nn, dtype=np.int32), np.arange(int(n_sta.sum()), dtype=np.int32),
+                    np.arange(int(n_sto.sum()), dtype=np.int32))
+    r, c = m.jac_pairs()
+    key = np.unique(r.astype(np.int64) * nn + c)
+    rr = key // nn
+    rp = np.zeros(nn + 1, np.int64)
+    np.add.at(rp, rr + 1, 1)
+    m.register_csr(np.cumsum(rp).astype(np.int32), (key % nn).astype(np.int32))
+    m.finalize()
+    h = {
+        "nextSol": rng.uniform(-0.5, 1.5, nn), "currSol": np.zeros(nn),
+        "nextSto": rng.uniform(-0.5, 1.5, max(int(n_sto.sum()), 1)),
+        "currSto": rng.uniform(-0.5, 1.5, max(int(n_sto.sum()), 1)),
+        "nextSta": np.zeros(max(int(n_sta.sum()), 1)), "currSta": np.zeros(max(int(n_sta.sum()), 1)),
+        "lastSta": np.zeros(max(int(n_sta.sum()), 1)), "nextStaDeriv": np.zeros(max(int(n_sta.sum()), 1)),
+        "f": np.zeros(nn), "q": np.zeros(nn), "b": np.zeros(nn), "dFdxdVp": np.zeros(nn),
+        "dQdxdVp": np.zeros(nn), "dFdx": np.zeros(len(key)), "dQdx": np.zeros(len(key)),
+    }
+    v = oracle.make_vectors(**h)
+    ss = oracle.SolverState()
+    ss.transientFlag = 1
+    ss.newtonIter = 1
+    ss.gainScale = 1.0
+    ss.nltermScale = 1.0
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        m.updateState(ss, v)
+        m.updateSecondaryState(ss, v)
+        m.loadDAEVectors(ss, v)
+        m.loadDAEMatrices(ss, v)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return n_sample * reps / el, f"{reps} load passes over {n_sample} instances in {el:.1f} s"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--model", default="diode", choices=["diode", "mosfet1", "bjt", "bsim4"])
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fused", action="store_true", help="time the one-call fused load")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from xyce_amd.workloads import Workload
+
+    print(f"[bench] building {args.model} workload n={args.n}", file=sys.stderr, flush=True)
+    wl = Workload(args.model, args.n)
+    step = wl.load_fused if args.fused else wl.load_pass
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("[bench] timing", file=sys.stderr, flush=True)
+    wl.master.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_up, ms_vec, ms_mat = wl.master.last_times()
+    wl.master.enable_timing(False)
+    ms_step = (t1 - t0) / args.steps * 1e3
+    value = args.n / (ms_step * 1e-3)
+    bu, bv, bm = wl.master.traffic_bytes()
+    if args.fused:
+        dom_bytes, dom_ms, dom_name = bu + bv + bm, ms_up, "fused load"
+    else:
+        dom_bytes, dom_ms, dom_name = bu, ms_up, "update_state"
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        n_sample = args.cpu_sample or {"diode": 1 << 17, "mosfet1": 1 << 15, "bjt": 1 << 15,
+                                       "bsim4": 1 << 13}[args.model]
+        print(f"[bench] cpu baseline on {n_sample} instances", file=sys.stderr, flush=True)
+        cv, desc = cpu_baseline(args.model, n_sample, args.cpu_seconds)
+        cpu = {"value": cv, "unit": "instance loads/s", "cores": 1, "kind": "port",
+               "sample": desc}
+    out = {
+        "metric": f"{args.model.upper()} instance loads/s (eval+stamp)",
+        "value": value,
+        "unit": "instance loads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"device-eval microbenchmark, {args.n} {args.model} instances, "
+                               "private nodes, V~U[-0.5,1.5], one load pass per step",
+                   "instances": args.n, "model": args.model,
+                   "path": "fused load_all" if args.fused else
+                           "updateState+loadDAEVectors+loadDAEMatrices"},
+        "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": dom_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * xyce_amd.h -- C ABI of the MI355X device-load library.
+ *
+ * The library evaluates nonlinear semiconductor device instances (junction
+ * diode, Level-1 MOSFET, Gummel-Poon BJT, BSIM4 4.6.1) on one AMD Instinct
+ * GPU and stamps their DAE contributions: the F and Q residual vectors, the
+ * voltage-limiter vectors dFdx*dVp and dQdx*dVp, lead currents, and the dF/dx
+ * and dQ/dx Jacobians held as CSR value arrays.
+ *
+ * Every entry point takes plain pointers and sizes.  Pointers to circuit
+ * vectors and CSR value arrays are DEVICE pointers unless stated otherwise.
+ * All GPU work is ordered on the context's stream; calls are not re-entrant
+ * for one context.  Errors are returned as XA_ERR_* codes with a message in
+ * xa_last_error(); nothing throws across the ABI.
+ *
+ * Reference interfaces replaced (Xyce 6.2 source tree):
+ *   Device::updateState / updateSecondaryState / loadDAEVectors /
+ *   loadDAEMatrices           src/DeviceModelPKG/Core/include/N_DEV_Device.h
+ *                             (default loops N_DEV_DeviceMaster.h:643-725)
+ *   per-model Master overrides, e.g. Diode::Master N_DEV_Diode.C:2051-2171,
+ *   MOSFET1::Master N_DEV_MOSFET1.C:4069-4688, BJT::Master N_DEV_BJT.C:3891-4366,
+ *   MOSFET_B4::Master N_DEV_MOSFET_B4.C:16180-17268
+ *   Instance::registerLIDs / registerStateLIDs / registerStoreLIDs /
+ *   jacobianStamp / registerJacLIDs   (e.g. N_DEV_Diode.C:479-687)
+ *   Model / Instance constructors + processParams + updateTemperature
+ *                             (parameter intake, e.g. N_DEV_Diode.C:1816-1940)
+ */
+#ifndef XYCE_AMD_H
+#define XYCE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XA_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define XA_OK              0
+#define XA_ERR_ARG        -1   /* bad argument / unknown parameter name      */
+#define XA_ERR_STATE      -2   /* call made in the wrong setup phase         */
+#define XA_ERR_HIP        -3   /* HIP runtime error                          */
+#define XA_ERR_UNSUPPORTED -4  /* model option outside the implemented set   */
+#define XA_ERR_NOMEM      -5
+#define XA_ERR_DEVICE     -6   /* a device instance reported failure         */
+
+/* ---- model types (device letter, level) -------------------------------- */
+#define XA_DIODE    1   /* "d", levels 1,2       N_DEV_Diode.C:2179-2186     */
+#define XA_MOSFET1  2   /* "m", level 1          N_DEV_MOSFET1.C:4984-4996   */
+#define XA_BJT      3   /* "q", level 1          N_DEV_BJT.C:4368-4380       */
+#define XA_BSIM4    4   /* "m", levels 14,54     N_DEV_MOSFET_B4.C:17270-17280 */
+
+/* ---- output kinds of a load ------------------------------------------- */
+#define XA_KIND_F        0   /* daeFVector rows                              */
+#define XA_KIND_Q        1   /* daeQVector rows                              */
+#define XA_KIND_FLIM     2   /* dFdxdVpVector rows (voltage limiter)         */
+#define XA_KIND_QLIM     3   /* dQdxdVpVector rows                           */
+#define XA_KIND_DFDX     4   /* dFdx CSR value slots                         */
+#define XA_KIND_DQDX     5   /* dQdx CSR value slots                         */
+#define XA_NKINDS        6
+
+typedef struct xa_context_s* xa_ctx;
+typedef struct xa_group_s*   xa_group;
+
+/* Solver-state flags read by the device loads: the subset of
+ * N_DEV_SolverState.h:93-190 that the four models consult (filled in the
+ * reference by DeviceMgr::setupSolverInfo_, N_DEV_DeviceMgr.C:3218-3379). */
+typedef struct xa_solver_state {
+  int32_t newtonIter;
+  int32_t dcopFlag;
+  int32_t initJctFlag;
+  int32_t initFixFlag;
+  int32_t inputOPFlag;
+  int32_t locaEnabledFlag;
+  int32_t initTranFlag;
+  int32_t beginIntegrationFlag;
+  int32_t tranopFlag;
+  int32_t acopFlag;
+  int32_t transientFlag;
+  int32_t artParameterFlag;     /* MOSFET gain/nonlinearity homotopy        */
+  int32_t bjtArtParameterFlag;  /* BJT homotopy                             */
+  int32_t _pad;
+  double  pdt;                  /* alpha/dt                                 */
+  double  currTimeStep;
+  double  lastTimeStep;
+  double  currTime;
+  double  gainScale;            /* gainScale[0] of the reference            */
+  double  nltermScale;
+} xa_solver_state;
+
+/* Device options read by the loads (N_DEV_DeviceOptions.C:66-140). */
+typedef struct xa_device_options {
+  double  gmin;                 /* default 1e-12                            */
+  double  tnom;                 /* kelvin, default 300.15                   */
+  double  temp;                 /* kelvin, default 300.15                   */
+  double  defl, defw, defad, defas;  /* MOS defaults 1e-4,1e-4,0,0          */
+  double  excessPhaseScalar1, excessPhaseScalar2;  /* 1.0, 1.0              */
+  int32_t voltageLimiterFlag;   /* default 1                                */
+  int32_t newMeyerFlag;         /* default 0                                */
+  int32_t newExcessPhase;       /* default 0                                */
+  int32_t lambertWFlag;         /* default 0                                */
+} xa_device_options;
+
+/* Circuit vectors for one load call (device pointers; NULL where unused).
+ * Field names follow N_DEV_ExternData.h. */
+typedef struct xa_vectors {
+  const double* nextSol;
+  const double* currSol;
+  const double* flagSol;        /* optional: inputOP flag vector            */
+  double*       nextSta;
+  double*       currSta;        /* written on the first initTran iteration  */
+  const double* lastSta;
+  const double* nextStaDeriv;   /* updateSecondaryState                     */
+  double*       nextSto;
+  double*       currSto;
+  double*       f;
+  double*       q;
+  double*       b;
+  double*       dFdxdVp;
+  double*       dQdxdVp;
+  double*       storeLeadF;
+  double*       storeLeadQ;
+  double*       dFdx;           /* CSR values (nnz)                         */
+  double*       dQdx;           /* CSR values (nnz)                         */
+} xa_vectors;
+
+/* ---- context ------------------------------------------------------------ */
+int         xa_ctx_create(int device_ordinal, xa_ctx* out);
+int         xa_ctx_destroy(xa_ctx ctx);
+int         xa_ctx_sync(xa_ctx ctx);
+void*       xa_ctx_stream(xa_ctx ctx);          /* the hipStream_t used    */
+/* Use a caller-owned stream (e.g. the framework's current stream). */
+int         xa_ctx_set_stream(xa_ctx ctx, void* stream);
+const char* xa_last_error(void);
+int         xa_abi_version(void);
+void        xa_default_options(xa_device_options* opt);
+void        xa_default_solver_state(xa_solver_state* ss);
+
+/* ---- group setup (one group = all instances of one model type) ---------- */
+/* Replaces Config<Traits>::registerDevice + Traits::factory. */
+int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
+                    xa_group* out);
+int xa_group_destroy(xa_group g);
+
+/* .MODEL card: name=value pairs; a pair given here is "given" in the
+ * reference sense.  Unknown names -> XA_ERR_ARG.  Returns the model index. */
+int xa_group_add_model(xa_group g, int level, int n_params,
+                       const char* const* names, const double* values,
+                       int* model_index);
+
+/* Instances: n instances of model `model_index[i]`.  Instance parameters are
+ * given as n_params columns; values[p][i] is parameter p of instance i. */
+int xa_group_add_instances(xa_group g, int n, const int* model_index,
+                           int n_params, const char* const* names,
+                           const double* const* values);
+
+/* Runs Model::processParams, Instance::processParams / updateTemperature for
+ * every instance on the host and fixes the per-instance variable counts. */
+int xa_group_setup(xa_group g);
+
+/* Per-instance variable counts after setup (host arrays of length n):
+ * external nodes, internal nodes, state vars, store vars (incl. lead
+ * current slot when requested). */
+int xa_group_counts(xa_group g, int* n_ext, int* n_int, int* n_sta, int* n_sto);
+int xa_group_num_instances(xa_group g);
+
+/* Lead currents requested per instance (host array of n flags, may be NULL
+ * for none).  Must precede xa_group_setup. */
+int xa_group_request_lead_current(xa_group g, const int32_t* flags);
+
+/* Instance::registerLIDs/registerStateLIDs/registerStoreLIDs: flattened host
+ * arrays, instance-major, lengths sum(n_ext+n_int), sum(n_sta), sum(n_sto).
+ * Ground is -1. */
+int xa_group_register_lids(xa_group g, const int32_t* node_lids,
+                           const int32_t* sta_lids, const int32_t* sto_lids);
+
+/* jacobianStamp(): the (row LID, col LID) pairs the group stamps into dFdx
+ * and dQdx, after node collapse, ground rows/cols dropped, duplicates kept.
+ * Query with rows==NULL to get the count. */
+int xa_group_jac_pairs(xa_group g, int64_t* count, int32_t* rows, int32_t* cols);
+
+/* registerJacLIDs: the CSR structure of dFdx/dQdx (host arrays); the library
+ * resolves every stamp to a value slot (Indexor::matrixGlobalToLocal). */
+int xa_group_register_csr(xa_group g, int n_rows, const int32_t* row_ptr,
+                          const int32_t* col_idx);
+
+/* Builds the fixed-order scatter plans and uploads everything. */
+int xa_group_finalize(xa_group g);
+
+/* ---- loads (Master virtuals) ------------------------------------------- */
+/* Master::updateState(solVec, staVec, stoVec) */
+int xa_update_state(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
+/* Master::updateSecondaryState(staDerivVec, stoVec) */
+int xa_update_secondary_state(xa_group g, const xa_solver_state* ss,
+                              const xa_vectors* v);
+/* Master::loadDAEVectors(solVec, fVec, qVec, bVec, storeLeadF, storeLeadQ) */
+int xa_load_vectors(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
+/* Master::loadDAEMatrices(dFdx, dQdx) */
+int xa_load_matrices(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
+/* updateState + updateSecondaryState + loadDAEVectors + loadDAEMatrices in
+ * one pass (no reference counterpart as one call; same results as the four
+ * calls in sequence). */
+int xa_load_all(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
+
+/* Device-side status of the last update: number of instances whose
+ * evaluation returned false (0 = all succeeded). Synchronizes. */
+int xa_group_failures(xa_group g, int64_t* n_failed);
+
+/* Per-instance intermediate read-back for tests (host array of n doubles).
+ * `name` is a model-specific intermediate ("Id", "Gd", "Vd", ...). */
+int xa_group_read_intermediate(xa_group g, const char* name, double* out);
+
+/* Algorithmic HBM bytes of one load pass of this group (inputs read once,
+ * outputs written once), for roofline reporting. */
+int xa_group_traffic_bytes(xa_group g, double* bytes_update_state,
+                           double* bytes_load_vectors, double* bytes_load_matrices);
+
+/* Kernel timing on the context stream with HIP events.  Enabling (re)starts
+ * recording; last_times returns the mean milliseconds per call of each phase
+ * (update_state, load_vectors, load_matrices; load_all counts as update)
+ * over the calls recorded since, 0 when none.  Synchronizes. */
+int xa_group_enable_timing(xa_group g, int on);
+int xa_group_last_times(xa_group g, float* ms_update, float* ms_vectors,
+                        float* ms_matrices);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XYCE_AMD_H */

@@ -1,0 +1,106 @@
+/* or_internal.h -- oracle internals (test infrastructure only). */
+#ifndef XA_OR_INTERNAL_H
+#define XA_OR_INTERNAL_H
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+/* N_DEV_Const.h:50-114 */
+#define OR_Q        (1.6021918e-19)
+#define OR_CtoK     (273.15)
+#define OR_BOLTZ    (1.3806226e-23)
+#define OR_REFTEMP  (300.15)
+#define OR_KoverQ   (OR_BOLTZ / OR_Q)
+#define OR_Eg300    (1.1150877)
+#define OR_Eg0      (1.16)
+#define OR_alphaEg  (7.02e-4)
+#define OR_betaEg   (1108.0)
+#define OR_Ni0      (1.45e10)
+#define OR_EPSOX    (3.453133e-11)
+#define OR_EPSSI    (1.03594e-10)
+#define OR_MAX_EXP  (5.834617425e14)
+#define OR_MIN_EXP  (1.713908431e-15)
+#define OR_EXP_THRESHOLD (34.0)
+#define OR_MAX_EXP_ARG (100.0)
+
+typedef struct {
+  const char* name;
+  int slot;
+  double def;
+  int is_temp;
+} or_param;
+
+typedef struct {
+  int level;
+  double* v;
+  char* given;
+} or_card;
+
+struct or_model_ops;
+
+struct or_group_s {
+  int type;
+  xa_device_options opt;
+  const struct or_model_ops* ops;
+  int nmodels;
+  or_card* models;
+  int64_t n;
+  int* inst_model;
+  double** ival;      /* [slot][inst] */
+  char* igiven;       /* [slot] */
+  int32_t* lead;
+  int* n_ext; int* n_int; int* n_sta; int* n_sto;
+  void* inst;         /* model-specific instance array */
+  void* mdata;        /* model-specific processed model array */
+  int phase;
+  int n_rows;
+  const int32_t* row_ptr;  /* borrowed during register_csr only */
+  const int32_t* col_idx;
+};
+
+typedef struct or_model_ops {
+  const char* name;
+  const or_param* mparams; int n_mparams; int n_mslots;
+  const or_param* iparams; int n_iparams; int n_islots;
+  int (*setup)(struct or_group_s*);
+  void (*register_lids)(struct or_group_s*, int64_t i, const int32_t* nodes,
+                        const int32_t* sta, const int32_t* sto);
+  /* appends (row, col) LID pairs of instance i into rows/cols (if non-null),
+     returns count */
+  int64_t (*jac_pairs)(struct or_group_s*, int64_t i, int32_t* rows, int32_t* cols);
+  int (*register_jac)(struct or_group_s*, int64_t i);
+  int (*update_state)(struct or_group_s*, const xa_solver_state*, const xa_vectors*);
+  int (*update_secondary)(struct or_group_s*, const xa_solver_state*, const xa_vectors*);
+  int (*load_vectors)(struct or_group_s*, const xa_solver_state*, const xa_vectors*);
+  int (*load_matrices)(struct or_group_s*, const xa_solver_state*, const xa_vectors*);
+  int (*read)(struct or_group_s*, const char* name, double* out);
+  void (*destroy)(struct or_group_s*);
+} or_model_ops;
+
+extern const or_model_ops or_diode_ops;
+extern const or_model_ops or_mosfet1_ops;
+extern const or_model_ops or_bjt_ops;
+extern const or_model_ops or_bsim4_ops;
+
+int or_fail(int code, const char* msg);
+/* Offset of column `col` in CSR row `row`, as Indexor::matrixGlobalToLocal
+ * (N_TOP_Indexor.C:171-200) resolves a (row, col) pair to a row-local slot;
+ * returns the absolute value index, or -1 if either is ground. */
+int64_t or_slot(const struct or_group_s* g, int32_t row, int32_t col);
+extern int or_threads;
+
+/* DeviceSupport restatements (or_support.c) */
+double or_limvds(double vnew, double vold);
+double or_pnjlim(double vnew, double vold, double vt, double vcrit, int* icheck);
+double or_fetlim(double vnew, double vold, double vto);
+void   or_lambertw(double x, double* w, int* ierr, double* xi);
+void   or_qmeyer(double vgs, double vgd, double vgb, double von, double vdsat,
+                 double* capgs, double* capgd, double* capgb, double phi, double cox);
+void   or_qmeyerderivs(double vgs, double vgd, double vgb, double von, double vdsat,
+                       double* d, double phi, double cox, int Dtype);
+static inline double or_min(double a, double b) { return a < b ? a : b; }
+static inline double or_max(double a, double b) { return a > b ? a : b; }
+
+#endif

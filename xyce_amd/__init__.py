@@ -1,0 +1,228 @@
+"""xyce_amd -- MI355X device-load library: Python host mirror of Xyce's
+device Master interface over the C ABI in include/xyce_amd.h.
+
+A :class:`Master` holds every instance of one model type on the GPU and
+offers the reference's four per-type calls (N_DEV_Device.h; default loops
+N_DEV_DeviceMaster.h:643-725)::
+
+    updateState(...)            # Master::updateState(solVec, staVec, stoVec)
+    updateSecondaryState(...)   # Master::updateSecondaryState(staDerivVec, stoVec)
+    loadDAEVectors(...)         # Master::loadDAEVectors(solVec, fVec, qVec, bVec, leadF, leadQ)
+    loadDAEMatrices(...)        # Master::loadDAEMatrices(dFdx, dQdx)
+
+Vectors and CSR value arrays are torch CUDA (HIP) float64 tensors; torch is
+used only for device memory and streams.  The evaluation and stamping run in
+the hand-written HIP kernels of libxyce_amd.so; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (DeviceOptions, SolverState, Vectors, VECTOR_FIELDS, MODEL_TYPES,
+                   XaError, check, lib)
+
+__all__ = ["Context", "Master", "DeviceOptions", "SolverState", "Vectors",
+           "default_options", "default_solver_state", "MODEL_TYPES", "XaError", "lib"]
+
+
+def default_options(**kw):
+    o = DeviceOptions()
+    lib().xa_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def default_solver_state(**kw):
+    s = SolverState()
+    lib().xa_default_solver_state(C.byref(s))
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        if not t.is_cuda or t.dtype.itemsize != 8 or not t.is_contiguous():
+            raise ValueError("vectors must be contiguous 8-byte device tensors")
+        return t.data_ptr()
+    raise TypeError("expected a torch device tensor")
+
+
+def make_vectors(**tensors):
+    v = Vectors()
+    for k, t in tensors.items():
+        if k not in VECTOR_FIELDS:
+            raise KeyError(k)
+        setattr(v, k, _ptr(t))
+    return v
+
+
+class Context:
+    """One GPU and one stream (xa_ctx).  By default the library is bound to
+    torch's current stream so torch allocations and library kernels order."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        self.h = C.c_void_p()
+        check(lib().xa_ctx_create(device, C.byref(self.h)), "xa_ctx_create")
+        self.device = device
+        if use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(device)
+            check(lib().xa_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream)), "set_stream")
+
+    def sync(self):
+        check(lib().xa_ctx_sync(self.h), "xa_ctx_sync")
+
+    def close(self):
+        if self.h:
+            lib().xa_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _cstrs(names):
+    arr = (C.c_char_p * max(len(names), 1))()
+    for i, n in enumerate(names):
+        arr[i] = n.encode()
+    return arr
+
+
+class Master:
+    """All instances of one model type on one GPU (an xa_group)."""
+
+    def __init__(self, ctx, model_type, options=None):
+        self.ctx = ctx
+        self.type = MODEL_TYPES[model_type] if isinstance(model_type, str) else model_type
+        self.h = C.c_void_p()
+        opt = options if options is not None else default_options()
+        check(lib().xa_group_create(ctx.h, self.type, C.byref(opt), C.byref(self.h)),
+              "xa_group_create")
+        self._keep = []
+
+    # ---- setup, mirroring Model/Instance construction and topology ----
+    def add_model(self, params, level=1):
+        names = list(params.keys())
+        vals = (C.c_double * max(len(names), 1))(*[float(params[k]) for k in names])
+        idx = C.c_int()
+        check(lib().xa_group_add_model(self.h, level, len(names), _cstrs(names), vals,
+                                       C.byref(idx)), "add_model")
+        return idx.value
+
+    def add_instances(self, n, model_index=None, params=None):
+        params = params or {}
+        names = list(params.keys())
+        cols = [np.ascontiguousarray(np.broadcast_to(np.asarray(params[k], np.float64), (n,)))
+                for k in names]
+        self._keep.extend(cols)
+        ptrs = (C.POINTER(C.c_double) * max(len(cols), 1))(
+            *[c.ctypes.data_as(C.POINTER(C.c_double)) for c in cols])
+        mi = None
+        if model_index is not None:
+            mi_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(model_index, np.int32), (n,)))
+            self._keep.append(mi_arr)
+            mi = mi_arr.ctypes.data_as(C.POINTER(C.c_int))
+        check(lib().xa_group_add_instances(self.h, n, mi, len(names), _cstrs(names), ptrs),
+              "add_instances")
+
+    def request_lead_current(self, flags):
+        f = np.ascontiguousarray(flags, np.int32)
+        check(lib().xa_group_request_lead_current(self.h, f.ctypes.data_as(C.POINTER(C.c_int32))),
+              "lead")
+
+    def setup(self):
+        check(lib().xa_group_setup(self.h), "setup")
+
+    @property
+    def n(self):
+        return lib().xa_group_num_instances(self.h)
+
+    def counts(self):
+        n = self.n
+        out = [np.zeros(n, np.int32) for _ in range(4)]
+        check(lib().xa_group_counts(self.h, *[o.ctypes.data_as(C.POINTER(C.c_int)) for o in out]),
+              "counts")
+        return out
+
+    def register_lids(self, node_lids, sta_lids, sto_lids):
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (node_lids, sta_lids, sto_lids)]
+        check(lib().xa_group_register_lids(
+            self.h, *[a.ctypes.data_as(C.POINTER(C.c_int32)) for a in arrs]), "register_lids")
+
+    def jac_pairs(self):
+        cnt = C.c_int64()
+        check(lib().xa_group_jac_pairs(self.h, C.byref(cnt), None, None), "jac_pairs")
+        r = np.zeros(cnt.value, np.int32)
+        c = np.zeros(cnt.value, np.int32)
+        check(lib().xa_group_jac_pairs(self.h, C.byref(cnt), r.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       c.ctypes.data_as(C.POINTER(C.c_int32))), "jac_pairs")
+        return r, c
+
+    def register_csr(self, row_ptr, col_idx):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        check(lib().xa_group_register_csr(self.h, len(rp) - 1, rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          ci.ctypes.data_as(C.POINTER(C.c_int32))), "register_csr")
+
+    def finalize(self):
+        check(lib().xa_group_finalize(self.h), "finalize")
+
+    # ---- the four per-type loads ----
+    def updateState(self, ss, vectors):
+        check(lib().xa_update_state(self.h, C.byref(ss), C.byref(vectors)), "updateState")
+
+    def updateSecondaryState(self, ss, vectors):
+        check(lib().xa_update_secondary_state(self.h, C.byref(ss), C.byref(vectors)),
+              "updateSecondaryState")
+
+    def loadDAEVectors(self, ss, vectors):
+        check(lib().xa_load_vectors(self.h, C.byref(ss), C.byref(vectors)), "loadDAEVectors")
+
+    def loadDAEMatrices(self, ss, vectors):
+        check(lib().xa_load_matrices(self.h, C.byref(ss), C.byref(vectors)), "loadDAEMatrices")
+
+    def loadAll(self, ss, vectors):
+        check(lib().xa_load_all(self.h, C.byref(ss), C.byref(vectors)), "loadAll")
+
+    def failures(self):
+        c = C.c_int64()
+        check(lib().xa_group_failures(self.h, C.byref(c)), "failures")
+        return c.value
+
+    def read(self, name):
+        out = np.zeros(self.n, np.float64)
+        check(lib().xa_group_read_intermediate(self.h, name.encode(),
+                                               out.ctypes.data_as(C.POINTER(C.c_double))), "read")
+        return out
+
+    def traffic_bytes(self):
+        b = [C.c_double() for _ in range(3)]
+        check(lib().xa_group_traffic_bytes(self.h, *[C.byref(x) for x in b]), "traffic")
+        return [x.value for x in b]
+
+    def enable_timing(self, on=True):
+        check(lib().xa_group_enable_timing(self.h, 1 if on else 0), "timing")
+
+    def last_times(self):
+        t = [C.c_float() for _ in range(3)]
+        check(lib().xa_group_last_times(self.h, *[C.byref(x) for x in t]), "times")
+        return [x.value for x in t]
+
+    def close(self):
+        if self.h:
+            lib().xa_group_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

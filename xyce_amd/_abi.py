@@ -1,0 +1,115 @@
+"""ctypes declarations of the C ABI in include/xyce_amd.h.
+
+The shared library ``libxyce_amd.so`` is built in-tree (``xyce_amd/``) by
+``__graft_entry__.build()``; importing this module loads it and fails loudly
+when it is missing -- there is no CPU fallback for the device loads.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxyce_amd.so")
+
+XA_OK = 0
+XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4 = 1, 2, 3, 4
+MODEL_TYPES = {"diode": XA_DIODE, "mosfet1": XA_MOSFET1, "bjt": XA_BJT, "bsim4": XA_BSIM4}
+
+
+class SolverState(C.Structure):
+    """xa_solver_state: N_DEV_SolverState.h:93-190 subset."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "newtonIter", "dcopFlag", "initJctFlag", "initFixFlag", "inputOPFlag",
+        "locaEnabledFlag", "initTranFlag", "beginIntegrationFlag", "tranopFlag",
+        "acopFlag", "transientFlag", "artParameterFlag", "bjtArtParameterFlag", "_pad")] + \
+        [(n, C.c_double) for n in (
+            "pdt", "currTimeStep", "lastTimeStep", "currTime", "gainScale", "nltermScale")]
+
+
+class DeviceOptions(C.Structure):
+    """xa_device_options: N_DEV_DeviceOptions.C:66-140 subset."""
+    _fields_ = [(n, C.c_double) for n in (
+        "gmin", "tnom", "temp", "defl", "defw", "defad", "defas",
+        "excessPhaseScalar1", "excessPhaseScalar2")] + \
+        [(n, C.c_int32) for n in (
+            "voltageLimiterFlag", "newMeyerFlag", "newExcessPhase", "lambertWFlag")]
+
+
+VECTOR_FIELDS = ("nextSol", "currSol", "flagSol", "nextSta", "currSta", "lastSta",
+                 "nextStaDeriv", "nextSto", "currSto", "f", "q", "b", "dFdxdVp",
+                 "dQdxdVp", "storeLeadF", "storeLeadQ", "dFdx", "dQdx")
+
+
+class Vectors(C.Structure):
+    """xa_vectors: device pointers, names after N_DEV_ExternData.h."""
+    _fields_ = [(n, C.c_void_p) for n in VECTOR_FIELDS]
+
+
+def _proto(lib):
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    P = C.POINTER
+    sig = {
+        "xa_ctx_create": (C.c_int, [C.c_int, P(vp)]),
+        "xa_ctx_destroy": (C.c_int, [vp]),
+        "xa_ctx_sync": (C.c_int, [vp]),
+        "xa_ctx_set_stream": (C.c_int, [vp, vp]),
+        "xa_ctx_stream": (vp, [vp]),
+        "xa_last_error": (C.c_char_p, []),
+        "xa_abi_version": (C.c_int, []),
+        "xa_default_options": (None, [P(DeviceOptions)]),
+        "xa_default_solver_state": (None, [P(SolverState)]),
+        "xa_group_create": (C.c_int, [vp, C.c_int, P(DeviceOptions), P(vp)]),
+        "xa_group_destroy": (C.c_int, [vp]),
+        "xa_group_add_model": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_char_p),
+                                          P(C.c_double), P(C.c_int)]),
+        "xa_group_add_instances": (C.c_int, [vp, C.c_int, P(C.c_int), C.c_int,
+                                              P(C.c_char_p), P(P(C.c_double))]),
+        "xa_group_request_lead_current": (C.c_int, [vp, P(i32)]),
+        "xa_group_setup": (C.c_int, [vp]),
+        "xa_group_counts": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "xa_group_num_instances": (C.c_int, [vp]),
+        "xa_group_register_lids": (C.c_int, [vp, P(i32), P(i32), P(i32)]),
+        "xa_group_jac_pairs": (C.c_int, [vp, P(i64), P(i32), P(i32)]),
+        "xa_group_register_csr": (C.c_int, [vp, C.c_int, P(i32), P(i32)]),
+        "xa_group_finalize": (C.c_int, [vp]),
+        "xa_update_state": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_update_secondary_state": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_load_vectors": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_load_matrices": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_load_all": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_group_failures": (C.c_int, [vp, P(i64)]),
+        "xa_group_read_intermediate": (C.c_int, [vp, C.c_char_p, P(C.c_double)]),
+        "xa_group_traffic_bytes": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "xa_group_enable_timing": (C.c_int, [vp, C.c_int]),
+        "xa_group_last_times": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    """Load libxyce_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build(); "
+                "the device loads have no CPU fallback")
+        _lib = _proto(C.CDLL(LIB_PATH))
+    return _lib
+
+
+class XaError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != XA_OK:
+        msg = lib().xa_last_error().decode()
+        raise XaError(f"{what}: rc={rc}: {msg}")
+    return rc

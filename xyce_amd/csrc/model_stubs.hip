@@ -1,0 +1,8 @@
+// Placeholders for model back ends not yet built: group creation fails
+// with XA_ERR_UNSUPPORTED.
+#include "xa_internal.h"
+namespace xa {
+const ModelImpl* model_mosfet1() { return nullptr; }
+const ModelImpl* model_bjt() { return nullptr; }
+const ModelImpl* model_bsim4() { return nullptr; }
+}  // namespace xa

@@ -1,0 +1,159 @@
+// xa_internal.h -- host-side structures shared by the runtime and the
+// per-model back ends (not part of the public ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "xyce_amd.h"
+
+namespace xa {
+
+// Internal output kinds: the public six plus the two lead-current store
+// vectors, which are assigned (not accumulated) like
+// storeLeadF[li] = Id in Diode::Master::loadDAEVectors (N_DEV_Diode.C:2111-2118).
+enum Kind { K_F = 0, K_Q, K_FLIM, K_QLIM, K_DFDX, K_DQDX, K_LEADF, K_LEADQ, NK };
+
+inline bool kind_is_matrix(int k) { return k == K_DFDX || k == K_DQDX; }
+inline bool kind_is_assign(int k) { return k == K_LEADF || k == K_LEADQ; }
+
+// One stamp entry of an instance: local row node and local column node
+// (vector kinds use `row` only; lead kinds use `row` as local store index).
+struct Stamp {
+  int16_t row, col;
+};
+
+// Parameter table row: name as written on a .MODEL / instance line, slot in
+// the card vector, default value, and whether the value is a temperature
+// given in Celsius (converted to kelvin on intake, DeviceEntity::setParam,
+// N_DEV_DeviceEntity.C:285-300).
+struct ParamDef {
+  const char* name;
+  int index;
+  double def;
+  int is_temp;
+};
+
+struct Card {
+  int level = 1;
+  std::vector<double> v;
+  std::vector<char> given;
+  bool has(int idx) const { return given[idx] != 0; }
+};
+
+// Fixed-order segmented scatter plan of one kind.  Target t receives
+// vals[src[seg[t]]], vals[src[seg[t]+1]], ... added to out[target[t]] in that
+// order, which is instance order then stamp order -- the order in which the
+// reference's per-instance loops execute their `+=` (DeviceMaster default
+// loops, N_DEV_DeviceMaster.h:643-725).
+struct Plan {
+  int E = 0;                       // value slots per instance
+  int64_t n_targets = 0, n_src = 0;
+  int32_t* d_target = nullptr;
+  int64_t* d_seg = nullptr;
+  int64_t* d_src = nullptr;
+  double* d_vals = nullptr;        // n * E, instance-major
+  bool private_targets = true;     // every target written by one instance only
+  int32_t* d_direct = nullptr;     // n * E targets (-1 none) for the fused path
+};
+
+struct Group;
+
+// Per-model back end.
+struct ModelImpl {
+  int type;
+  const char* name;
+  const ParamDef* mparams; int n_mparams;   // model-card parameters (+aliases)
+  int n_mslots;                             // card vector length
+  const ParamDef* iparams; int n_iparams;   // instance parameters (+aliases)
+  int n_islots;
+  int max_nodes;            // length of the full local node list
+  int max_sto, max_sta;     // local store / state slots used by the kernels
+  int max_entries[NK];      // value slots per instance per kind
+  int (*setup)(Group&);     // processParams / updateTemperature on the host
+  // map registered LIDs (ext then int) to the full local node list
+  void (*resolve_nodes)(const Group&, int64_t inst, const int32_t* reg, int32_t* full);
+  // stamp list of one instance for one kind (length <= max_entries[kind])
+  void (*stamps)(const Group&, int64_t inst, int kind, std::vector<Stamp>& out);
+  int (*launch_update)(Group&, const xa_solver_state&, const xa_vectors&, bool fused);
+  int (*launch_secondary)(Group&, const xa_solver_state&, const xa_vectors&);
+  bool (*skip_kind)(const Group&, const xa_solver_state&, int kind);  // may be null
+  const char* const* state_names; int n_state;  // persistent per-instance doubles
+  double (*traffic)(const Group&, int phase);   // algorithmic bytes per instance
+};
+
+const ModelImpl* model_diode();
+const ModelImpl* model_mosfet1();
+const ModelImpl* model_bjt();
+const ModelImpl* model_bsim4();
+
+struct Ctx;
+
+enum Phase { PH_CREATED = 0, PH_SETUP, PH_LIDS, PH_CSR, PH_FINAL };
+
+struct Group {
+  Ctx* ctx = nullptr;
+  const ModelImpl* impl = nullptr;
+  xa_device_options opt{};
+  int phase = PH_CREATED;
+
+  std::vector<Card> models;
+  int64_t n = 0;
+  std::vector<int32_t> inst_model;
+  // instance parameters: card-like per instance, stored per slot column
+  std::vector<std::vector<double>> ival;   // [slot][inst]
+  std::vector<char> igiven;                // [slot] given for the whole group
+  std::vector<int32_t> lead;               // [inst] lead current requested
+
+  // filled by setup
+  std::vector<int32_t> n_ext, n_int, n_sta, n_sto;
+  int n_dfield = 0, n_ifield = 0;
+  std::vector<double> dfield;    // [field][inst]
+  std::vector<int32_t> ifield;   // [field][inst]
+  std::vector<double> mcard;     // device model cards, model-major
+  int mcard_stride = 0;
+  int n_state = 0;
+  std::vector<double> state_init;  // [state][inst]
+
+  // filled by register_lids / csr
+  std::vector<int32_t> nodes;    // [local node][inst]
+  std::vector<int32_t> sto_lid;  // [local sto][inst]
+  std::vector<int32_t> sta_lid;  // [local sta][inst]
+  std::vector<int64_t> reg_off;  // prefix offsets of registered node lids
+  std::vector<int32_t> tgt[NK];  // [inst*E + e] target, -1 none
+
+  // device copies
+  double* d_dfield = nullptr;
+  int32_t* d_ifield = nullptr;
+  double* d_mcard = nullptr;
+  int32_t* d_nodes = nullptr;
+  int32_t* d_sto = nullptr;
+  int32_t* d_sta = nullptr;
+  double* d_state = nullptr;
+  int32_t* d_fail = nullptr;
+  Plan plan[NK];
+
+  // per-phase HIP event pairs recorded while timing is on
+  bool timing = false;
+  std::vector<hipEvent_t> ev[3];
+  size_t n_ev[3] = {0, 0, 0};
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+// error reporting
+int fail(int code, const std::string& msg);
+int hip_check(hipError_t e, const char* what);
+#define XA_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) return ::xa::hip_check(_e, #call); } while (0)
+
+// block size used by the per-instance kernels
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace xa

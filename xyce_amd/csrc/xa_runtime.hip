@@ -1,0 +1,710 @@
+// xa_runtime.hip -- C ABI implementation: contexts, instance groups, the
+// setup sequence (parameters -> LIDs -> CSR slots -> scatter plans) and the
+// fixed-order segmented scatter that assembles F, Q, limiter vectors and the
+// dF/dx, dQ/dx CSR values without atomics.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "xa_internal.h"
+#include "xa_kernel.h"
+
+namespace xa {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  return fail(XA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static const ModelImpl* impl_for(int type) {
+  switch (type) {
+    case XA_DIODE: return model_diode();
+    case XA_MOSFET1: return model_mosfet1();
+    case XA_BJT: return model_bjt();
+    case XA_BSIM4: return model_bsim4();
+    default: return nullptr;
+  }
+}
+
+static std::string upper(const char* s) {
+  std::string r(s);
+  for (auto& c : r) c = (char)toupper((unsigned char)c);
+  return r;
+}
+
+static const ParamDef* find_param(const ParamDef* tab, int n, const std::string& name) {
+  for (int k = 0; k < n; ++k)
+    if (name == tab[k].name) return &tab[k];
+  return nullptr;
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src) {
+  if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+  if (src.empty()) return XA_OK;
+  XA_HIP(hipMalloc((void**)dst, src.size() * sizeof(T)));
+  XA_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return XA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-order segmented scatter.  One thread per output target; the target's
+// contributions are added in plan order starting from the value already in
+// the output, which reproduces the sequential `*ptr += value` of the
+// reference bit for bit.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_scatter_add(int64_t nt, const int32_t* __restrict__ target,
+              const int64_t* __restrict__ seg, const int64_t* __restrict__ src,
+              const double* __restrict__ vals, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t o = target[t];
+  const int64_t s1 = seg[t + 1];
+  double acc = out[o];
+  for (int64_t j = seg[t]; j < s1; ++j) acc += vals[src[j]];
+  out[o] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+k_scatter_assign(int64_t nt, const int32_t* __restrict__ target,
+                 const int64_t* __restrict__ seg, const int64_t* __restrict__ src,
+                 const double* __restrict__ vals, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  out[target[t]] = vals[src[seg[t + 1] - 1]];
+}
+
+static double* kind_out(const xa_vectors& v, int k) {
+  switch (k) {
+    case K_F: return v.f;
+    case K_Q: return v.q;
+    case K_FLIM: return v.dFdxdVp;
+    case K_QLIM: return v.dQdxdVp;
+    case K_DFDX: return v.dFdx;
+    case K_DQDX: return v.dQdx;
+    case K_LEADF: return v.storeLeadF;
+    case K_LEADQ: return v.storeLeadQ;
+  }
+  return nullptr;
+}
+
+static int run_scatter(Group& g, const xa_solver_state& ss, const xa_vectors& v,
+                       int k) {
+  Plan& p = g.plan[k];
+  double* out = kind_out(v, k);
+  if (!out || p.n_targets == 0) return XA_OK;
+  if (g.impl->skip_kind && g.impl->skip_kind(g, ss, k)) return XA_OK;
+  if (kind_is_assign(k))
+    hipLaunchKernelGGL(k_scatter_assign, dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
+                       g.ctx->stream, p.n_targets, p.d_target, p.d_seg, p.d_src,
+                       p.d_vals, out);
+  else
+    hipLaunchKernelGGL(k_scatter_add, dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
+                       g.ctx->stream, p.n_targets, p.d_target, p.d_seg, p.d_src,
+                       p.d_vals, out);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
+void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
+                bool fused, KArgs& a) {
+  memset(&a, 0, sizeof(a));
+  a.n = g.n;
+  a.dfield = g.d_dfield;
+  a.ifield = g.d_ifield;
+  a.mcard = g.d_mcard;
+  a.mstride = g.mcard_stride;
+  a.nodes = g.d_nodes;
+  a.sto = g.d_sto;
+  a.sta = g.d_sta;
+  a.state = g.d_state;
+  a.fail = g.d_fail;
+  a.ss = ss;
+  a.opt = g.opt;
+  a.v = v;
+  a.kmask = 0;
+  for (int k = 0; k < NK; ++k) {
+    a.vals[k] = g.plan[k].d_vals;
+    a.E[k] = g.plan[k].E;
+    a.direct[k] = g.plan[k].d_direct;
+    double* o = kind_out(v, k);
+    a.out[k] = o;
+    bool on = g.plan[k].E > 0;
+    // fused mode writes straight to outputs: needs an output and a target map
+    // (a kind with no live target at all has no map and is skipped)
+    if (fused)
+      on = on && o != nullptr && g.plan[k].d_direct != nullptr &&
+           !(g.impl->skip_kind && g.impl->skip_kind(g, ss, k));
+    if (on) a.kmask |= 1u << k;
+  }
+}
+
+}  // namespace xa
+
+using namespace xa;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* xa_last_error(void) { return g_err.c_str(); }
+int xa_abi_version(void) { return XA_ABI_VERSION; }
+
+void xa_default_options(xa_device_options* o) {
+  memset(o, 0, sizeof(*o));
+  o->gmin = 1.0e-12;
+  o->tnom = kRefTemp;
+  o->temp = kRefTemp;
+  o->defl = 1.0e-4;
+  o->defw = 1.0e-4;
+  o->defad = 0.0;
+  o->defas = 0.0;
+  o->excessPhaseScalar1 = 1.0;
+  o->excessPhaseScalar2 = 1.0;
+  o->voltageLimiterFlag = 1;
+  o->newMeyerFlag = 0;
+  o->newExcessPhase = 0;
+  o->lambertWFlag = 0;
+}
+
+void xa_default_solver_state(xa_solver_state* s) {
+  memset(s, 0, sizeof(*s));
+  s->gainScale = 1.0;
+  s->nltermScale = 1.0;
+}
+
+int xa_ctx_create(int device_ordinal, xa_ctx* out) {
+  if (!out) return fail(XA_ERR_ARG, "xa_ctx_create: null out");
+  int ndev = 0;
+  XA_HIP(hipGetDeviceCount(&ndev));
+  if (device_ordinal < 0 || device_ordinal >= ndev)
+    return fail(XA_ERR_ARG, "xa_ctx_create: no such device");
+  XA_HIP(hipSetDevice(device_ordinal));
+  Ctx* c = new Ctx();
+  c->device = device_ordinal;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return hip_check(e, "hipStreamCreate"); }
+  c->own_stream = true;
+  *out = reinterpret_cast<xa_ctx>(c);
+  return XA_OK;
+}
+
+int xa_ctx_set_stream(xa_ctx ctx, void* stream) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(XA_ERR_ARG, "null ctx");
+  if (c->own_stream && c->stream) XA_HIP(hipStreamDestroy(c->stream));
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  c->own_stream = false;
+  return XA_OK;
+}
+
+int xa_ctx_destroy(xa_ctx ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return XA_OK;
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return XA_OK;
+}
+
+int xa_ctx_sync(xa_ctx ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(XA_ERR_ARG, "null ctx");
+  XA_HIP(hipStreamSynchronize(c->stream));
+  return XA_OK;
+}
+
+void* xa_ctx_stream(xa_ctx ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  return c ? (void*)c->stream : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// group setup
+// ---------------------------------------------------------------------------
+int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
+                    xa_group* out) {
+  const ModelImpl* impl = impl_for(model_type);
+  if (!impl) return fail(XA_ERR_ARG, "xa_group_create: unknown model type");
+  if (!out) return fail(XA_ERR_ARG, "xa_group_create: null out");
+  Group* g = new Group();
+  g->ctx = reinterpret_cast<Ctx*>(ctx);
+  g->impl = impl;
+  if (opt) g->opt = *opt; else xa_default_options(&g->opt);
+  g->ival.assign(impl->n_islots, {});
+  g->igiven.assign(impl->n_islots, 0);
+  *out = reinterpret_cast<xa_group>(g);
+  return XA_OK;
+}
+
+static void free_plan(Plan& p) {
+  (void)hipFree(p.d_target); (void)hipFree(p.d_seg); (void)hipFree(p.d_src);
+  (void)hipFree(p.d_vals); (void)hipFree(p.d_direct);
+  p = Plan();
+}
+
+int xa_group_destroy(xa_group gh) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return XA_OK;
+  (void)hipFree(g->d_dfield); (void)hipFree(g->d_ifield); (void)hipFree(g->d_mcard);
+  (void)hipFree(g->d_nodes); (void)hipFree(g->d_sto); (void)hipFree(g->d_sta);
+  (void)hipFree(g->d_state); (void)hipFree(g->d_fail);
+  for (int k = 0; k < NK; ++k) free_plan(g->plan[k]);
+  for (auto& v : g->ev) for (auto& e : v) (void)hipEventDestroy(e);
+  delete g;
+  return XA_OK;
+}
+
+int xa_group_add_model(xa_group gh, int level, int n_params, const char* const* names,
+                       const double* values, int* model_index) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_CREATED) return fail(XA_ERR_STATE, "add_model after setup");
+  const ModelImpl* m = g->impl;
+  Card c;
+  c.level = level;
+  c.v.assign(m->n_mslots, 0.0);
+  c.given.assign(m->n_mslots, 0);
+  // defaults: first table row of each slot carries the default
+  std::vector<char> seen(m->n_mslots, 0);
+  for (int k = 0; k < m->n_mparams; ++k) {
+    const ParamDef& d = m->mparams[k];
+    if (!seen[d.index]) { c.v[d.index] = d.def; seen[d.index] = 1; }
+  }
+  for (int p = 0; p < n_params; ++p) {
+    std::string nm = upper(names[p]);
+    const ParamDef* d = find_param(m->mparams, m->n_mparams, nm);
+    if (!d) return fail(XA_ERR_ARG, std::string(m->name) + ": unknown model parameter " + nm);
+    c.v[d->index] = values[p] + (d->is_temp ? kCtoK : 0.0);
+    c.given[d->index] = 1;
+  }
+  g->models.push_back(std::move(c));
+  if (model_index) *model_index = (int)g->models.size() - 1;
+  return XA_OK;
+}
+
+int xa_group_add_instances(xa_group gh, int n, const int* model_index, int n_params,
+                           const char* const* names, const double* const* values) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_CREATED) return fail(XA_ERR_STATE, "add_instances after setup");
+  if (n < 0) return fail(XA_ERR_ARG, "negative instance count");
+  const ModelImpl* m = g->impl;
+  const int64_t n0 = g->n;
+  // validate names first
+  std::vector<const ParamDef*> defs(n_params);
+  for (int p = 0; p < n_params; ++p) {
+    defs[p] = find_param(m->iparams, m->n_iparams, upper(names[p]));
+    if (!defs[p]) return fail(XA_ERR_ARG, std::string(m->name) + ": unknown instance parameter " + names[p]);
+  }
+  for (int i = 0; i < n; ++i) {
+    const int mi = model_index ? model_index[i] : 0;
+    if (mi < 0 || mi >= (int)g->models.size())
+      return fail(XA_ERR_ARG, "instance refers to a model index not added");
+  }
+  if (n0 > 0) {
+    // a later batch must give the same parameter set (given flags are per group)
+    for (int s = 0; s < m->n_islots; ++s) {
+      bool in_batch = false;
+      for (int p = 0; p < n_params; ++p) in_batch |= defs[p]->index == s;
+      if (in_batch != (bool)g->igiven[s])
+        return fail(XA_ERR_ARG, "instance batches must give the same parameters");
+    }
+  }
+  g->n = n0 + n;
+  for (int i = 0; i < n; ++i) g->inst_model.push_back(model_index ? model_index[i] : 0);
+  std::vector<char> seen(m->n_islots, 0);
+  for (int k = 0; k < m->n_iparams; ++k) {
+    const ParamDef& d = m->iparams[k];
+    if (seen[d.index]) continue;
+    seen[d.index] = 1;
+    g->ival[d.index].resize(g->n, d.def);
+  }
+  for (int p = 0; p < n_params; ++p) {
+    const ParamDef* d = defs[p];
+    g->igiven[d->index] = 1;
+    for (int i = 0; i < n; ++i)
+      g->ival[d->index][n0 + i] = values[p][i] + (d->is_temp ? kCtoK : 0.0);
+  }
+  g->lead.resize(g->n, 0);
+  return XA_OK;
+}
+
+int xa_group_request_lead_current(xa_group gh, const int32_t* flags) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_CREATED) return fail(XA_ERR_STATE, "lead current request after setup");
+  g->lead.assign(g->n, 0);
+  if (flags) for (int64_t i = 0; i < g->n; ++i) g->lead[i] = flags[i] != 0;
+  return XA_OK;
+}
+
+int xa_group_setup(xa_group gh) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_CREATED) return fail(XA_ERR_STATE, "setup called twice");
+  if (g->models.empty()) return fail(XA_ERR_STATE, "setup without a model card");
+  g->n_ext.assign(g->n, 0); g->n_int.assign(g->n, 0);
+  g->n_sta.assign(g->n, 0); g->n_sto.assign(g->n, 0);
+  int rc = g->impl->setup(*g);
+  if (rc != XA_OK) return rc;
+  g->phase = PH_SETUP;
+  return XA_OK;
+}
+
+int xa_group_num_instances(xa_group gh) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  return g ? (int)g->n : -1;
+}
+
+int xa_group_counts(xa_group gh, int* n_ext, int* n_int, int* n_sta, int* n_sto) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase < PH_SETUP) return fail(XA_ERR_STATE, "counts before setup");
+  for (int64_t i = 0; i < g->n; ++i) {
+    if (n_ext) n_ext[i] = g->n_ext[i];
+    if (n_int) n_int[i] = g->n_int[i];
+    if (n_sta) n_sta[i] = g->n_sta[i];
+    if (n_sto) n_sto[i] = g->n_sto[i];
+  }
+  return XA_OK;
+}
+
+int xa_group_register_lids(xa_group gh, const int32_t* node_lids, const int32_t* sta_lids,
+                           const int32_t* sto_lids) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_SETUP) return fail(XA_ERR_STATE, "register_lids out of order");
+  const ModelImpl* m = g->impl;
+  const int64_t n = g->n;
+  g->nodes.assign((size_t)m->max_nodes * n, -1);
+  g->sto_lid.assign((size_t)std::max(m->max_sto, 1) * n, -1);
+  g->sta_lid.assign((size_t)std::max(m->max_sta, 1) * n, -1);
+  std::vector<int32_t> full(m->max_nodes);
+  int64_t off = 0, osta = 0, osto = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int nreg = g->n_ext[i] + g->n_int[i];
+    m->resolve_nodes(*g, i, node_lids + off, full.data());
+    for (int j = 0; j < m->max_nodes; ++j) g->nodes[j * n + i] = full[j];
+    off += nreg;
+    for (int j = 0; j < g->n_sta[i]; ++j) {
+      if (j < m->max_sta) g->sta_lid[j * n + i] = sta_lids ? sta_lids[osta + j] : -1;
+    }
+    osta += g->n_sta[i];
+    for (int j = 0; j < g->n_sto[i]; ++j) {
+      if (j < m->max_sto) g->sto_lid[j * n + i] = sto_lids ? sto_lids[osto + j] : -1;
+    }
+    osto += g->n_sto[i];
+  }
+  g->phase = PH_LIDS;
+  return XA_OK;
+}
+
+int xa_group_jac_pairs(xa_group gh, int64_t* count, int32_t* rows, int32_t* cols) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase < PH_LIDS) return fail(XA_ERR_STATE, "jac_pairs before register_lids");
+  const int64_t n = g->n;
+  int64_t c = 0;
+  std::vector<Stamp> st;
+  for (int k : {K_DFDX, K_DQDX}) {
+    for (int64_t i = 0; i < n; ++i) {
+      g->impl->stamps(*g, i, k, st);
+      for (const Stamp& s : st) {
+        const int32_t r = g->nodes[s.row * n + i], q = g->nodes[s.col * n + i];
+        if (r < 0 || q < 0) continue;
+        if (rows) { rows[c] = r; cols[c] = q; }
+        ++c;
+      }
+    }
+  }
+  if (count) *count = c;
+  return XA_OK;
+}
+
+int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
+                          const int32_t* col_idx) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_LIDS) return fail(XA_ERR_STATE, "register_csr out of order");
+  const ModelImpl* m = g->impl;
+  const int64_t n = g->n;
+  std::vector<Stamp> st;
+  for (int k = 0; k < NK; ++k) {
+    const int E = m->max_entries[k];
+    g->tgt[k].assign((size_t)E * n, -1);
+    for (int64_t i = 0; i < n; ++i) {
+      g->impl->stamps(*g, i, k, st);
+      if ((int)st.size() > E) return fail(XA_ERR_STATE, "stamp list exceeds max_entries");
+      for (size_t e = 0; e < st.size(); ++e) {
+        int32_t t = -1;
+        if (kind_is_assign(k)) {
+          t = (st[e].row < m->max_sto) ? g->sto_lid[st[e].row * n + i] : -1;
+        } else if (!kind_is_matrix(k)) {
+          t = g->nodes[st[e].row * n + i];
+        } else {
+          const int32_t r = g->nodes[st[e].row * n + i], q = g->nodes[st[e].col * n + i];
+          if (r >= 0 && q >= 0) {
+            if (r >= n_rows) return fail(XA_ERR_ARG, "stamp row outside the CSR structure");
+            const int32_t* b = col_idx + row_ptr[r];
+            const int32_t* en = col_idx + row_ptr[r + 1];
+            const int32_t* f = std::lower_bound(b, en, q);
+            if (f == en || *f != q) {
+              // unsorted row: fall back to a linear search
+              f = std::find(b, en, q);
+              if (f == en) return fail(XA_ERR_ARG, "stamp (row,col) missing from CSR structure");
+            }
+            t = (int32_t)(f - col_idx);
+          }
+        }
+        g->tgt[k][i * E + e] = t;
+      }
+    }
+  }
+  g->phase = PH_CSR;
+  return XA_OK;
+}
+
+// Counting sort of (target, instance, entry) triples by target; stable, so
+// contributions of one target stay in instance order then stamp order.
+static int build_plan(Group& g, int k) {
+  Plan& p = g.plan[k];
+  const int E = g.impl->max_entries[k];
+  p.E = E;
+  const std::vector<int32_t>& tg = g.tgt[k];
+  if (E == 0 || g.n == 0) return XA_OK;
+  int32_t tmax = -1;
+  for (int32_t t : tg) tmax = std::max(tmax, t);
+  if (tmax < 0) {
+    XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
+    return XA_OK;
+  }
+  std::vector<int64_t> cnt((size_t)tmax + 2, 0);
+  std::vector<int32_t> owner((size_t)tmax + 1, -1);
+  p.private_targets = true;
+  for (int64_t idx = 0; idx < (int64_t)tg.size(); ++idx) {
+    const int32_t t = tg[idx];
+    if (t < 0) continue;
+    cnt[t + 1]++;
+    const int32_t inst = (int32_t)(idx / E);
+    if (owner[t] < 0) owner[t] = inst;
+    else if (owner[t] != inst) p.private_targets = false;
+  }
+  std::vector<int32_t> target;
+  std::vector<int64_t> seg;
+  target.reserve(tg.size());
+  seg.reserve(tg.size() + 1);
+  seg.push_back(0);
+  std::vector<int64_t> start((size_t)tmax + 1, -1);
+  int64_t acc = 0;
+  for (int32_t t = 0; t <= tmax; ++t) {
+    if (cnt[t + 1] == 0) continue;
+    start[t] = acc;
+    acc += cnt[t + 1];
+    target.push_back(t);
+    seg.push_back(acc);
+  }
+  std::vector<int64_t> src(acc);
+  std::vector<int64_t> fillp(start);
+  for (int64_t idx = 0; idx < (int64_t)tg.size(); ++idx) {
+    const int32_t t = tg[idx];
+    if (t < 0) continue;
+    src[fillp[t]++] = idx;
+  }
+  p.n_targets = (int64_t)target.size();
+  p.n_src = acc;
+  int rc;
+  if ((rc = upload(&p.d_target, target)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_seg, seg)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_src, src)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_direct, tg)) != XA_OK) return rc;
+  XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
+  XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
+  return XA_OK;
+}
+
+int xa_group_finalize(xa_group gh) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_CSR) return fail(XA_ERR_STATE, "finalize out of order");
+  XA_HIP(hipSetDevice(g->ctx->device));
+  int rc;
+  if ((rc = upload(&g->d_dfield, g->dfield)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_ifield, g->ifield)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_mcard, g->mcard)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_nodes, g->nodes)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_sto, g->sto_lid)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_sta, g->sta_lid)) != XA_OK) return rc;
+  if ((rc = upload(&g->d_state, g->state_init)) != XA_OK) return rc;
+  XA_HIP(hipMalloc((void**)&g->d_fail, sizeof(int32_t)));
+  XA_HIP(hipMemset(g->d_fail, 0, sizeof(int32_t)));
+  for (int k = 0; k < NK; ++k)
+    if ((rc = build_plan(*g, k)) != XA_OK) return rc;
+  g->phase = PH_FINAL;
+  return XA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// loads
+// ---------------------------------------------------------------------------
+static int ready(Group* g) {
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase != PH_FINAL) return fail(XA_ERR_STATE, "load before finalize");
+  return XA_OK;
+}
+
+static void tic(Group* g, int which) {
+  if (!g->timing) return;
+  std::vector<hipEvent_t>& v = g->ev[which];
+  const size_t k = 2 * g->n_ev[which];
+  while (v.size() < k + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { g->timing = false; return; }
+    v.push_back(e);
+  }
+  (void)hipEventRecord(v[k], g->ctx->stream);
+}
+static void toc(Group* g, int which) {
+  if (!g->timing) return;
+  (void)hipEventRecord(g->ev[which][2 * g->n_ev[which] + 1], g->ctx->stream);
+  g->n_ev[which]++;
+}
+
+int xa_update_state(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0) return XA_OK;
+  tic(g, 0);
+  rc = g->impl->launch_update(*g, *ss, *v, false);
+  toc(g, 0);
+  return rc;
+}
+
+int xa_update_secondary_state(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0 || !g->impl->launch_secondary) return XA_OK;
+  return g->impl->launch_secondary(*g, *ss, *v);
+}
+
+int xa_load_vectors(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  tic(g, 1);
+  for (int k : {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ})
+    if ((rc = run_scatter(*g, *ss, *v, k)) != XA_OK) return rc;
+  toc(g, 1);
+  return XA_OK;
+}
+
+int xa_load_matrices(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  tic(g, 2);
+  for (int k : {K_DFDX, K_DQDX})
+    if ((rc = run_scatter(*g, *ss, *v, k)) != XA_OK) return rc;
+  toc(g, 2);
+  return XA_OK;
+}
+
+int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0) return XA_OK;
+  bool all_private = true;
+  for (int k = 0; k < NK; ++k) all_private = all_private && g->plan[k].private_targets;
+  // the fused kernel needs the secondary state folded in; models with a
+  // secondary pass take the split route
+  if (all_private && !g->impl->launch_secondary) {
+    tic(g, 0);
+    rc = g->impl->launch_update(*g, *ss, *v, true);
+    toc(g, 0);
+    return rc;
+  }
+  if ((rc = xa_update_state(gh, ss, v)) != XA_OK) return rc;
+  if ((rc = xa_update_secondary_state(gh, ss, v)) != XA_OK) return rc;
+  if ((rc = xa_load_vectors(gh, ss, v)) != XA_OK) return rc;
+  return xa_load_matrices(gh, ss, v);
+}
+
+int xa_group_failures(xa_group gh, int64_t* n_failed) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  int32_t h = 0;
+  XA_HIP(hipMemcpyAsync(&h, g->d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+  XA_HIP(hipStreamSynchronize(g->ctx->stream));
+  if (n_failed) *n_failed = h;
+  return XA_OK;
+}
+
+int xa_group_read_intermediate(xa_group gh, const char* name, double* out) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  for (int s = 0; s < g->impl->n_state; ++s) {
+    if (strcmp(g->impl->state_names[s], name) == 0) {
+      XA_HIP(hipStreamSynchronize(g->ctx->stream));
+      XA_HIP(hipMemcpy(out, g->d_state + (size_t)s * g->n, sizeof(double) * g->n,
+                       hipMemcpyDeviceToHost));
+      return XA_OK;
+    }
+  }
+  return fail(XA_ERR_ARG, std::string("unknown intermediate ") + name);
+}
+
+int xa_group_traffic_bytes(xa_group gh, double* bu, double* bv, double* bm) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (!g->impl->traffic) return fail(XA_ERR_UNSUPPORTED, "no traffic model");
+  if (bu) *bu = g->impl->traffic(*g, 0) * (double)g->n;
+  if (bv) *bv = g->impl->traffic(*g, 1) * (double)g->n;
+  if (bm) *bm = g->impl->traffic(*g, 2) * (double)g->n;
+  return XA_OK;
+}
+
+int xa_group_enable_timing(xa_group gh, int on) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  g->timing = on != 0;
+  for (int w = 0; w < 3; ++w) g->n_ev[w] = 0;
+  return XA_OK;
+}
+
+int xa_group_last_times(xa_group gh, float* mu, float* mv, float* mm) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  float* outs[3] = {mu, mv, mm};
+  for (int w = 0; w < 3; ++w) {
+    double tot = 0.0;
+    for (size_t k = 0; k < g->n_ev[w]; ++k) {
+      float ms = 0.f;
+      XA_HIP(hipEventSynchronize(g->ev[w][2 * k + 1]));
+      XA_HIP(hipEventElapsedTime(&ms, g->ev[w][2 * k], g->ev[w][2 * k + 1]));
+      tot += ms;
+    }
+    if (outs[w]) *outs[w] = g->n_ev[w] ? (float)(tot / g->n_ev[w]) : 0.f;
+  }
+  return XA_OK;
+}
+
+}  // extern "C"
