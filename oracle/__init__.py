@@ -76,6 +76,7 @@ def lib():
             ("or_last_error", C.c_char_p, []),
             ("or_set_threads", None, [C.c_int]),
             ("or_default_options", None, [P(DeviceOptions)]),
+            ("or_set_magnitude_mode", None, [C.c_int]),
         ]:
             f = getattr(L, name)
             f.restype = res

@@ -9,6 +9,8 @@
 
 static __thread char or_err[512];
 int or_threads = 1;
+int or_mag = 0;
+void or_set_magnitude_mode(int on) { or_mag = on != 0; }
 
 int or_fail(int code, const char* msg) {
   snprintf(or_err, sizeof(or_err), "%s", msg);

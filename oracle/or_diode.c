@@ -447,37 +447,37 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
     dinst* di = &in[i];
     double Ir = di->Gspr * (di->Vp - di->Vpp);
     if (v->f) {
-      if (di->li_Pos >= 0) v->f[di->li_Pos] -= -Ir;
-      if (di->li_Neg >= 0) v->f[di->li_Neg] -= di->Id;
-      if (di->li_Pri >= 0) v->f[di->li_Pri] -= (-di->Id + Ir);
+      OR_SUB(v->f, di->li_Pos, -Ir);
+      OR_SUB(v->f, di->li_Neg, di->Id);
+      OR_SUB(v->f, di->li_Pri, (-di->Id + Ir));
     }
     if (v->q) {
-      if (di->li_Neg >= 0) v->q[di->li_Neg] -= di->Qd;
-      if (di->li_Pri >= 0) v->q[di->li_Pri] -= -di->Qd;
+      OR_SUB(v->q, di->li_Neg, di->Qd);
+      OR_SUB(v->q, di->li_Pri, -di->Qd);
     }
     if (g->opt.voltageLimiterFlag) {
       double Vd_diff = di->Vd - di->Vd_orig;
       double Cd_Jdxp = -(di->Cd) * Vd_diff;
       double Gd_Jdxp = -(di->Gd) * Vd_diff;
       if (v->dFdxdVp) {
-        if (di->li_Neg >= 0) v->dFdxdVp[di->li_Neg] += Gd_Jdxp;
-        if (di->li_Pri >= 0) v->dFdxdVp[di->li_Pri] -= Gd_Jdxp;
+        OR_ADD(v->dFdxdVp, di->li_Neg, Gd_Jdxp);
+        OR_SUB(v->dFdxdVp, di->li_Pri, Gd_Jdxp);
       }
       if (v->dQdxdVp) {
-        if (di->li_Neg >= 0) v->dQdxdVp[di->li_Neg] += Cd_Jdxp;
-        if (di->li_Pri >= 0) v->dQdxdVp[di->li_Pri] -= Cd_Jdxp;
+        OR_ADD(v->dQdxdVp, di->li_Neg, Cd_Jdxp);
+        OR_SUB(v->dQdxdVp, di->li_Pri, Cd_Jdxp);
       }
     }
     if (di->loadLeadCurrent && di->li_store_dev_i >= 0) {
-      if (v->storeLeadF) v->storeLeadF[di->li_store_dev_i] = di->Id;
-      if (di->model->CJO != 0.0 && v->storeLeadQ) v->storeLeadQ[di->li_store_dev_i] = di->Qd;
+      if (v->storeLeadF) OR_SET(v->storeLeadF, di->li_store_dev_i, di->Id);
+      if (di->model->CJO != 0.0 && v->storeLeadQ) OR_SET(v->storeLeadQ, di->li_store_dev_i, di->Qd);
     }
   }
   return XA_OK;
 }
 
-#define ADD(M, s, val) do { if ((s) >= 0) (M)[s] += (val); } while (0)
-#define SUB(M, s, val) do { if ((s) >= 0) (M)[s] -= (val); } while (0)
+#define ADD OR_ADD
+#define SUB OR_SUB
 
 /* Master::loadDAEMatrices, N_DEV_Diode.C:2132-2171 */
 static int load_matrices(struct or_group_s* g, const xa_solver_state* ss, const xa_vectors* v) {

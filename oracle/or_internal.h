@@ -100,6 +100,13 @@ void   or_qmeyer(double vgs, double vgd, double vgb, double von, double vdsat,
                  double* capgs, double* capgd, double* capgb, double phi, double cox);
 void   or_qmeyerderivs(double vgs, double vgd, double vgb, double von, double vdsat,
                        double* d, double phi, double cox, int Dtype);
+/* Magnitude mode: the stamping macros accumulate |contribution| instead of
+ * the signed contribution, giving each assembled entry its error scale
+ * sum(|terms|) for the parity tests. */
+extern int or_mag;
+#define OR_ADD(vec, i, val) do { if ((i) >= 0) (vec)[i] += or_mag ? fabs(val) : (val); } while (0)
+#define OR_SUB(vec, i, val) do { if ((i) >= 0) (vec)[i] -= or_mag ? -fabs(val) : (val); } while (0)
+#define OR_SET(vec, i, val) do { if ((i) >= 0) (vec)[i] = or_mag ? fabs(val) : (val); } while (0)
 static inline double or_min(double a, double b) { return a < b ? a : b; }
 static inline double or_max(double a, double b) { return a > b ? a : b; }
 

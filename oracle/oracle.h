@@ -49,6 +49,8 @@ const char* or_last_error(void);
 /* OpenMP threads for the load loops (1 = the reference's serial order). */
 void or_set_threads(int n);
 void or_default_options(xa_device_options* o);
+/* 1: loads accumulate |contribution| (error scale of each entry). */
+void or_set_magnitude_mode(int on);
 
 #ifdef __cplusplus
 }

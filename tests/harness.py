@@ -167,6 +167,18 @@ class Pair:
         self.orc.loadDAEVectors(so, vo)
         self.orc.loadDAEMatrices(so, vo)
         res_o = {k: self.h[k].copy() for k in OUT_VECS + STATE_VECS + ("dFdx", "dQdx")}
+        # error scale of each assembled entry: sum of |contributions|
+        import oracle
+        mag = {k: np.zeros_like(self.h[k]) for k in OUT_VECS + ("dFdx", "dQdx")}
+        vm = oracle.make_vectors(**{k: (mag[k] if k in mag else v)
+                                    for k, v in self.h.items() if len(v)})
+        oracle.lib().or_set_magnitude_mode(1)
+        try:
+            self.orc.loadDAEVectors(so, vm)
+            self.orc.loadDAEMatrices(so, vm)
+        finally:
+            oracle.lib().or_set_magnitude_mode(0)
+        self.mag = mag
         if self.gpu is None:
             return res_o, None
         for k in OUT_VECS + ("dFdx", "dQdx"):
@@ -203,10 +215,15 @@ def max_rel_err(a, b, floor=1e-300):
     return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
 
 
-def assert_close(a, b, rtol, atol=0.0, what=""):
+def assert_close(a, b, rtol, atol=0.0, what="", scale=None):
+    """|a-b| <= rtol * max(|a|, |b|, scale) + atol, entrywise.  `scale` is
+    the sum of |contributions| of an assembled entry (harness Pair.mag)."""
     a = np.asarray(a)
     b = np.asarray(b)
-    bad = np.abs(a - b) > rtol * np.maximum(np.abs(a), np.abs(b)) + atol
+    ref = np.maximum(np.abs(a), np.abs(b))
+    if scale is not None:
+        ref = np.maximum(ref, np.asarray(scale))
+    bad = np.abs(a - b) > rtol * ref + atol
     if np.any(bad):
         idx = np.nonzero(bad)[0][:5]
         raise AssertionError(
