@@ -1,0 +1,166 @@
+/* or_bsim4.h -- oracle restatement of the Xyce BSIM4 (level 54) device:
+ * data layout (test infrastructure only).
+ * Reference: src/DeviceModelPKG/OpenModels/include/N_DEV_MOSFET_B4.h
+ *   SizeDependParam :88-321, Instance :322-1149, Model :1151-2138.
+ * Field names follow the reference so that each restated statement reads
+ * like the line it restates. */
+#ifndef OR_BSIM4_H
+#define OR_BSIM4_H
+#include "or_internal.h"
+
+#define B4_EPS0      (8.854214871e-12)      /* N_DEV_Const.h CONSTperm0 */
+#define B4_EPSSI     (1.03594e-10)
+#define B4_KboQ      (8.617087e-5)          /* N_DEV_MOSFET_B4.C:106 */
+#define B4_vt0       (OR_BOLTZ * (27.0 + OR_CtoK) / OR_Q)
+#define B4_MM        3
+#define B4_DELTA     (1.0E-9)
+#define B4_DELTA_1   (0.02)
+#define B4_DELTA_3   (0.02)
+#define B4_DELTA_4   (0.02)
+#define B4_Charge_q  (1.60219e-19)
+
+typedef struct { char dummy; } b4_unused;
+
+/* ---- model card: every table parameter as a named field ------------- */
+typedef struct {
+#define B4M(NAME, mem, def) double mem;
+#include "../include/xa_bsim4_params.def"
+#undef B4M
+} b4params;
+
+typedef struct {
+#define B4M(NAME, mem, def) char mem;
+#include "../include/xa_bsim4_params.def"
+#undef B4M
+} b4given;
+
+typedef struct b4size_s b4size;
+
+typedef struct {
+  b4params p;           /* M(x) */
+  b4given g;            /* MG(x) */
+  int dtype;            /* +1 NMOS, -1 PMOS */
+  /* derived, temperature dependent (written by every instance's
+     updateTemperature; the last one wins, as in the reference) */
+  double coxe, coxp, vcrit, factor1, vtm0, vtm, Eg0;
+  double SjctTempSatCurDensity, SjctSidewallTempSatCurDensity, SjctGateSidewallTempSatCurDensity;
+  double DjctTempSatCurDensity, DjctSidewallTempSatCurDensity, DjctGateSidewallTempSatCurDensity;
+  double SunitAreaTempJctCap, DunitAreaTempJctCap;
+  double SunitLengthSidewallTempJctCap, DunitLengthSidewallTempJctCap;
+  double SunitLengthGateSidewallTempJctCap, DunitLengthGateSidewallTempJctCap;
+  double PhiBS, PhiBD, PhiBSWS, PhiBSWD, PhiBSWGS, PhiBSWGD;
+  double njtsstemp, njtsswstemp, njtsswgstemp, njtsdtemp, njtsswdtemp, njtsswgdtemp;
+  /* size-dependent parameter cache keyed by (L, W, NF) (N_DEV_MOSFET_B4.C:6018-6049) */
+  b4size** sizes;
+  int nsizes, capsizes;
+} b4model;
+
+/* ---- size-dependent parameters (SizeDependParam) ---------------------- */
+struct b4size_s {
+  double Length, Width, NFinger, referenceTemperature;
+#define B4BIN(sm, base, l, w, p) double sm;
+#include "../include/xa_bsim4_binning.def"
+#undef B4BIN
+  double dl, dlc, dw, dwc, dwj, leff, weff, leffCV, weffCV, weffCJ;
+  double abulkCVfactor, vsattemp, rds0, rdswmin, rd0, rdwmin, rs0, rswmin;
+  double u0temp, tfactor, lc, cgdo, cgso, cgbo, phi, sqrtPhi, phis3, Xdep0, sqrtXdep0;
+  double litl, vbi, vfbsd, cdep0, ToxRatio, ToxRatioEdge, Aechvb, Bechvb;
+  double AechvbEdgeS, AechvbEdgeD, BechvbEdge, mstar, mstarcv, voffcbn, voffcbncv;
+  double ldeb, k1ox, theta0vb0, thetaRout, vfbzbfactor, ku0, kvth0, ku0temp;
+  double inv_od_ref, rho_ref;
+};
+
+/* ---- instance ----------------------------------------------------------- */
+typedef struct {
+  b4model* model;
+  b4size* paramPtr;
+  /* instance parameters (B4I) and given flags */
+#define B4I(NAME, mem, def) double mem;
+#include "../include/xa_bsim4_params.def"
+#undef B4I
+#define B4I(NAME, mem, def) char mem##_given;
+#include "../include/xa_bsim4_params.def"
+#undef B4I
+  int irbodyMod, irgateMod, igeoMod, irgeoMod, itrnqsMod, iacnqsMod, imin;
+  int drainMOSFET_B4Exists, sourceMOSFET_B4Exists;
+  int loadLeadCurrent;
+  /* temperature / geometry dependent */
+  double u0temp, vsattemp, vth0, vfb, eta0, k2, k2ox, vtfbphi1, vtfbphi2, vbsc, vfbzb;
+  double cgso, cgdo, grbdb, grbpb, grbps, grbsb, grbpd, grgeltd;
+  double DMCGeff, DMCIeff, DMDGeff, Pseff, Pdeff, Aseff, Adeff;
+  double sourceConductance, drainConductance;
+  double XExpBVS, XExpBVD, vjsmFwd, vjsmRev, vjdmFwd, vjdmRev;
+  double IVjsmFwd, IVjsmRev, IVjdmFwd, IVjdmRev, SslpFwd, SslpRev, DslpFwd, DslpRev;
+  double SjctTempRevSatCur, DjctTempRevSatCur, SswTempRevSatCur, DswTempRevSatCur;
+  double SswgTempRevSatCur, DswgTempRevSatCur;
+  /* evaluation state kept between calls and read by the loads */
+  double Vd, Vs, Vb, Vsp, Vdp, Vgp, Vbp, Vge, Vgm, Vdb, Vsb, Qtotal;
+  double Vddp, Vssp, Vdbb, Vdbbp, Vsbb, Vsbbp, Vbpb, Vgegp, Vgegm, Vgmgp;
+  double vds, vgs, vbs, vges, vgms, vdbs, vsbs, vses, vdes, qdef;
+  double vbd, vdbd, vgd, vged, vgmd, vgmb, vgb, vbs_jct, vbd_jct;
+  double vbd_orig, vbs_orig, vgs_orig, vds_orig, vgd_orig, vges_orig, vgms_orig, vdes_orig;
+  double vses_orig, vdbs_orig, vsbs_orig, vdbd_orig, vged_orig, vgmd_orig, vbs_jct_orig;
+  double vbd_jct_orig, vgmb_orig, vgb_orig, Vgegp_orig, Vgegm_orig, Vgmgp_orig;
+  int origFlag, limitedFlag, newtonIterOld, mode, ChargeComputationNeeded;
+  double von, thetavth, nstar, vgs_eff, vgd_eff, dvgs_eff_dvg, dvgd_eff_dvg;
+  double Vgsteff, Abulk, ueff, EsatL, vdsat, Vdseff, AbovVgst2Vtm, grdsw;
+  double gbs, cbs, gbd, cbd, csub, gbbs, gbgs, gbds, cdrain, gds, gm, gmbs, IdovVds;
+  double gcrg, gcrgd, gcrgg, gcrgs, gcrgb, gstot, gstotd, gstotg, gstots, gstotb;
+  double gdtot, gdtotd, gdtotg, gdtots, gdtotb;
+  double Igidl, ggidld, ggidlg, ggidlb, Igisl, ggisls, ggislg, ggislb, ggidls, ggisld;
+  double Igcs, gIgcsg, gIgcsd, gIgcsb, gIgcss, Igcd, gIgcdg, gIgcdd, gIgcdb, gIgcds;
+  double Igs, gIgsg, gIgss, Igd, gIgdg, gIgdd, Igb, gIgbg, gIgbd, gIgbs, gIgbb, cd;
+  double qinv, qgate, qdrn, qsrc, qbulk, qgmid, qgdo, qgso, qgb, qgmb, qchqs, qcheq;
+  double cggb, cgsb, cgdb, cdgb, cdsb, cddb, cbgb, cbsb, cbdb, csgb, cssb, csdb;
+  double cgbb, csbb, cdbb, cbbb, cqdb, cqsb, cqgb, cqbb, gtau, CoxWL, taunet;
+  double qbs, qbd, capbs, capbd, Igate, IgateMid, Igate_Jdxp, IgateMid_Jdxp;
+  double Idrain, Isource, Idbb, Idbbp, Isbb, Isbbp, Ibpb;
+  double gcggb, gcgdb, gcgsb, gcgbb, gcdgb, gcsgb, gcbgb, gcddb, gcdsb, gcsdb, gcssb;
+  double gcdbb, gcsbb, gcbdb, gcbsb, gcdbdb, gcsbsb, gcbbb, gcgmgmb, gcgmdb, gcgmsb;
+  double gcgmbb, gcdgmb, gcsgmb, gcbgmb, gcqgb, gcqdb, gcqsb, gcqbb, gqdef;
+  double ggtg, ggtd, ggts, ggtb, gtg, gtd, gts, gtb;
+  double dxpart, sxpart, ddxpart_dVd, ddxpart_dVg, ddxpart_dVb, ddxpart_dVs;
+  double dsxpart_dVd, dsxpart_dVg, dsxpart_dVb, dsxpart_dVs;
+  double CAPcggb, CAPcgdb, CAPcgsb, CAPcgbb, CAPcdgb, CAPcsgb, CAPcbgb, CAPcddb, CAPcdsb;
+  double CAPcsdb, CAPcssb, CAPcdbb, CAPcsbb, CAPcbdb, CAPcbsb, CAPcdbdb, CAPcsbsb, CAPcbbb;
+  double CAPcgmgmb, CAPcgmdb, CAPcgmsb, CAPcgmbb, CAPcdgmb, CAPcsgmb, CAPcbgmb;
+  double CAPcqgb, CAPcqdb, CAPcqsb, CAPcqbb;
+  double qg, qd, qb, ScalingFactor;
+  /* setupFVectorVars / auxChargeCalculations products */
+  double Gm, Gmbs, FwdSum, RevSum, ceqdrn, ceqbd, ceqbs, ceqjs, ceqjd, ceqgcrg;
+  double ceqgstot, ceqgdtot, gbbdp, gbbsp, gbdpg, gbdpdp, gbdpb, gbdpsp;
+  double gbspg, gbspdp, gbspb, gbspsp;
+  double gIstotg, gIstotd, gIstots, gIstotb, Istoteq, gIdtotg, gIdtotd, gIdtots, gIdtotb;
+  double Idtoteq, gIbtotg, gIbtotd, gIbtots, gIbtotb, Ibtoteq, gIgtotg, gIgtotd, gIgtots;
+  double gIgtotb, Igtoteq;
+  double ceqdrn_Jdxp, ceqbd_Jdxp, ceqbs_Jdxp, Istoteq_Jdxp, Idtoteq_Jdxp, Ibtoteq_Jdxp;
+  double Igtoteq_Jdxp, ceqgcrg_Jdxp, ceqgstot_Jdxp, ceqgdtot_Jdxp, ceqjs_Jdxp, ceqjd_Jdxp;
+  double Qeqqg_Jdxp, Qeqqd_Jdxp, Qeqqb_Jdxp, Qeqqgmid_Jdxp, Qeqqjs_Jdxp, Qeqqjd_Jdxp;
+  double gjbd, gjbs, gdpr, gspr, geltd;
+  /* LIDs and CSR slots */
+  int li_Drain, li_GateExt, li_Source, li_Body, li_DrainPrime, li_GatePrime, li_GateMid;
+  int li_SourcePrime, li_BodyPrime, li_DrainBody, li_SourceBody;
+  int li_store_vbd, li_store_vbs, li_store_vgs, li_store_vds, li_store_vges, li_store_vgms;
+  int li_store_vdes, li_store_vses, li_store_vdbs, li_store_vsbs, li_store_vdbd;
+  int li_store_vged, li_store_vgmd, li_store_von;
+  int li_state_qb, li_state_qg, li_state_qd;
+  int64_t s_GPgp, s_GPdp, s_GPsp, s_GPbp, s_DPdp, s_DPd, s_DPgp, s_DPsp, s_DPbp;
+  int64_t s_Ddp, s_Dd, s_SPdp, s_SPgp, s_SPsp, s_SPs, s_SPbp, s_Ssp, s_Ss;
+  int64_t s_BPdp, s_BPgp, s_BPsp, s_BPbp;
+} b4inst;
+
+#define M(x)  (m->p.x)
+#define MG(x) (m->g.x)
+
+/* setup / temperature (or_bsim4_setup.c) */
+int  b4_model_init(b4model* m, const or_card* c, const xa_device_options* opt);
+int  b4_instance_init(struct or_group_s* g, b4inst* I, int64_t i);
+void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* opt);
+/* evaluation (or_bsim4_eval.c) */
+int  b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device_options* opt,
+                            const xa_vectors* v);
+/* load helpers (or_bsim4_load.c) */
+void b4_setup_capacitors_old(b4inst* I, const xa_solver_state* ss);
+void b4_setup_capacitors_new(b4inst* I);
+
+#endif

@@ -2,4 +2,3 @@
 #include "or_internal.h"
 const or_model_ops or_mosfet1_ops = {"mosfet1"};
 const or_model_ops or_bjt_ops = {"bjt"};
-const or_model_ops or_bsim4_ops = {"bsim4"};
