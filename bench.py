@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--model", default="diode", choices=["diode", "mosfet1", "bjt", "bsim4"])
+    ap.add_argument("--model", default="bsim4", choices=["diode", "mosfet1", "bjt", "bsim4"])
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -141,7 +141,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"device-eval microbenchmark, {args.n} {args.model} instances, "
                                "private nodes, V~U[-0.5,1.5], one load pass per step",
-                   "instances": args.n, "model": args.model,
+                   "instances": args.n, "device": args.model,
                    "path": "fused load_all" if args.fused else
                            "updateState+loadDAEVectors+loadDAEMatrices"},
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},

@@ -33,3 +33,67 @@ def diode_spec(n, seed=20261203, cards=None, temp_spread=False):
 
 def terminal_voltages(rng, n, lo=-0.5, hi=1.5):
     return rng.uniform(lo, hi, n)
+
+
+# BSIM4 cards spanning the evaluation's branches.  Card 0/1 are the default
+# NMOS/PMOS (the device-eval microbenchmark cards); the others switch on
+# internal drain/source nodes (sheet resistance), junction models dioMod 0
+# and 2 with breakdown and trap-assisted tunnelling, GIDL/GISL, gate
+# polysilicon depletion, mobility models 1 and 2, charge partition 50/50 and
+# 0/100, cvchargeMod 1, tempMod 1 and 2 with TNOM != TEMP (whose
+# temperature coefficients are relative: AT, UA1.. given as such), velocity
+# saturation and the velocity-overshoot term (separate card, below), binning terms, stress (LOD) and well proximity.
+BSIM4_CARDS = [
+    (54, {"TYPE": 1.0}),
+    (54, {"TYPE": -1.0}),
+    (54, {"TYPE": 1.0, "RSH": 20.0, "DIOMOD": 2.0, "JSS": 1e-4, "JSWS": 1e-10, "JSWGS": 1e-10,
+          "CJS": 1e-3, "CJSWS": 1e-10, "CJSWGS": 1e-10, "BVS": 3.0, "XJBVS": 1.0,
+          "AGIDL": 1e-9, "BGIDL": 1e9, "CGIDL": 0.5, "EGIDL": 0.8, "NGATE": 1e20,
+          "CGSL": 1e-11, "CKAPPAS": 0.6, "TOXE": 1.8e-9, "TOXP": 1.5e-9, "TEMPMOD": 2.0,
+          "AT": 1e-3, "UA1": 1e-3, "UB1": -1e-3, "UC1": 1e-3, "PRT": 1e-3}),
+    (54, {"TYPE": -1.0, "DIOMOD": 0.0, "MOBMOD": 1.0, "XPART": 0.5, "CVCHARGEMOD": 1.0,
+          "TEMPMOD": 1.0, "AT": 1e-3, "UA1": 1e-3, "UB1": -1e-3, "UC1": 1e-3, "PRT": 1e-3,
+          "TNOM": 40.0, "JTSS": 1e-7, "NJTS": 15.0, "XTSS": 0.02, "VTSS": 10.0,
+          "ACDE": 0.8, "MOIN": 12.0, "NOFF": 1.2, "VOFFCV": -0.05}),
+    (54, {"TYPE": 1.0, "MOBMOD": 2.0, "XPART": 1.0,
+          "LVTH0": 1e-9, "WVTH0": 2e-9, "PVTH0": 1e-17, "KU0": 1e-8, "KVTH0": 1e-8,
+          "SAREF": 1e-6, "SBREF": 1e-6, "WEB": 1e-3, "WEC": 1e-4, "KVTH0WE": 1e-3,
+          "PDITS": 0.1, "PDITSD": 0.1, "PDITSL": 1e6, "FPROUT": 0.2, "DVTP0": 1e-8, "DVTP1": 0.05}),
+]
+# Source-end velocity limit (VTL given).  The reference's Fsevl derivatives
+# leave out the bias dependence of Coxeff, so its Jacobian is approximate
+# there (N_DEV_MOSFET_B4.C:9905-9945): parity holds, a finite-difference
+# check does not.
+BSIM4_VTL_CARD = (54, {"TYPE": 1.0, "VTL": 2e5, "LAMBDA": 0.2, "MOBMOD": 2.0})
+
+
+def bsim4_spec(n, seed=20261203, cards=None, geometry=True, temp_spread=False, nf_mix=False,
+               sizes=None):
+    """W/L lognormal (sigma 0.2) around 90 nm / 45 nm.  `geometry` adds the
+    source/drain geometry, stress and multiplier instance parameters; `sizes`
+    draws (W, L) from that many distinct pairs so size-dependent records are
+    shared between instances."""
+    rng = splitmix64(seed)
+    cards = cards if cards is not None else BSIM4_CARDS
+    mi = rng.integers(0, len(cards), size=n).astype(np.int32)
+    if sizes:
+        w_set = lognormal(rng, 90e-9, 0.2, sizes)
+        l_set = lognormal(rng, 45e-9, 0.2, sizes)
+        pick = rng.integers(0, sizes, size=n)
+        params = {"W": w_set[pick], "L": l_set[pick]}
+    else:
+        params = {"W": lognormal(rng, 90e-9, 0.2, n), "L": lognormal(rng, 45e-9, 0.2, n)}
+    if geometry:
+        w = params["W"]
+        params.update({
+            "AS": w * 100e-9, "AD": w * 120e-9, "PS": 2 * (w + 100e-9), "PD": 2 * (w + 120e-9),
+            "NRS": rng.uniform(0.5, 2.0, n), "NRD": rng.uniform(0.5, 2.0, n),
+            "SA": lognormal(rng, 0.5e-6, 0.3, n), "SB": lognormal(rng, 0.5e-6, 0.3, n),
+            "SCA": rng.uniform(0.0, 5.0, n), "M": rng.integers(1, 3, size=n).astype(float),
+            "DELVTO": rng.uniform(-0.02, 0.02, n),
+        })
+    if nf_mix:
+        params["NF"] = rng.integers(1, 3, size=n).astype(float)
+    if temp_spread:
+        params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return {"models": cards, "n": n, "model_index": mi, "inst_params": params}

@@ -1,0 +1,2866 @@
+// model_bsim4.hip -- BSIM4 (level 54) MOSFET: parameter tables, packing of
+// the setup results into per-instance SoA columns and a per-model device
+// card, and the per-instance evaluation + stamp kernel.
+//
+// Reference: src/DeviceModelPKG/OpenModels/src/N_DEV_MOSFET_B4.C
+//   updateIntermediateVars :7744-11919, setupCapacitors_oldDAE :12616-13031,
+//   setupCapacitors_newDAE :12357-12606, auxChargeCalculations :12216-12347,
+//   setupFVectorVars :13525-13838, registerLIDs :4996-5128,
+//   registerStateLIDs :5200-5257, registerStoreLIDs :5267-5290,
+//   Master::updateState :16180-16293, Master::loadDAEVectors :16303-16567,
+//   Master::loadDAEMatrices :16577-17268.
+// Option subset: capMod = 2, mtrlMod = 0, igcMod = igbMod = 0, rdsMod = 0,
+// rgateMod = rbodyMod = trnqsMod = acnqsMod = 0 (setup rejects the rest);
+// every other branch of updateIntermediateVars is evaluated.
+#include <cmath>
+#include <cstring>
+#include "bsim4_host.h"
+#include "xa_kernel.h"
+
+namespace xa {
+namespace bsim4 {
+
+// ---- parameter tables (names as the reference registers them) ------------
+static ParamDef g_mparams[] = {
+#define B4M(NAME, mem, def) {#NAME, MS_##mem, (double)(def), 0},
+#include "xa_bsim4_params.def"
+#undef B4M
+  {"TYPE", MS_TYPE, 1.0, 0},
+};
+static ParamDef g_iparams[] = {
+#define B4I(NAME, mem, def) {#NAME, IS_##mem, (double)(def), 0},
+#include "xa_bsim4_params.def"
+#undef B4I
+};
+
+// ---- device-side layout --------------------------------------------------
+// Model values the evaluation reads: card parameters after defaulting and
+// the temperature-dependent values the model object holds.
+struct DevCard {
+  double DbulkJctBotGradingCoeff;
+  double DbulkJctGateSideGradingCoeff;
+  double DbulkJctSideGradingCoeff;
+  double DjctEmissionCoeff;
+  double DjctGateSidewallTempSatCurDensity;
+  double DjctSidewallTempSatCurDensity;
+  double DjctTempSatCurDensity;
+  double DunitAreaTempJctCap;
+  double DunitLengthGateSidewallTempJctCap;
+  double DunitLengthSidewallTempJctCap;
+  double PhiBD;
+  double PhiBS;
+  double PhiBSWD;
+  double PhiBSWGD;
+  double PhiBSWGS;
+  double PhiBSWS;
+  double SbulkJctBotGradingCoeff;
+  double SbulkJctGateSideGradingCoeff;
+  double SbulkJctSideGradingCoeff;
+  double SjctEmissionCoeff;
+  double SjctGateSidewallTempSatCurDensity;
+  double SjctSidewallTempSatCurDensity;
+  double SjctTempSatCurDensity;
+  double SunitAreaTempJctCap;
+  double SunitLengthGateSidewallTempJctCap;
+  double SunitLengthSidewallTempJctCap;
+  double ados;
+  double bdos;
+  double bvd;
+  double bvs;
+  double coxe;
+  double coxp;
+  double cvchargeMod;
+  double dioMod;
+  double dtype;
+  double epsrox;
+  double factor1;
+  double lambda;
+  double mobMod;
+  double njtsdtemp;
+  double njtsstemp;
+  double njtsswdtemp;
+  double njtsswgdtemp;
+  double njtsswgstemp;
+  double njtsswstemp;
+  double pditsl;
+  double rdsMod;
+  double tempMod;
+  double tnoiMod;
+  double tnom;
+  double toxe;
+  double toxp;
+  double vcrit;
+  double vtl;
+  double vtm;
+  double vtm0;
+  double vtsd;
+  double vtss;
+  double vtsswd;
+  double vtsswgd;
+  double vtsswgs;
+  double vtssws;
+  double xjbvd;
+  double xjbvs;
+  double xpart;
+  double lambdaGiven;
+  double vtlGiven;
+};
+
+// Per-instance double columns: size-dependent (binned) parameters, then
+// instance values fixed at setup.
+#define B4_PFIELDS \
+  B4P(Xdep0) \
+  B4P(a0) \
+  B4P(a1) \
+  B4P(a2) \
+  B4P(abulkCVfactor) \
+  B4P(acde) \
+  B4P(agidl) \
+  B4P(agisl) \
+  B4P(ags) \
+  B4P(alpha0) \
+  B4P(alpha1) \
+  B4P(b0) \
+  B4P(b1) \
+  B4P(beta0) \
+  B4P(bgidl) \
+  B4P(bgisl) \
+  B4P(cdep0) \
+  B4P(cdsc) \
+  B4P(cdscb) \
+  B4P(cdscd) \
+  B4P(cgbo) \
+  B4P(cgdl) \
+  B4P(cgdo) \
+  B4P(cgidl) \
+  B4P(cgisl) \
+  B4P(cgsl) \
+  B4P(cgso) \
+  B4P(cit) \
+  B4P(ckappad) \
+  B4P(ckappas) \
+  B4P(delta) \
+  B4P(dvt0) \
+  B4P(dvt0w) \
+  B4P(dvt1) \
+  B4P(dvt1w) \
+  B4P(dvt2) \
+  B4P(dvt2w) \
+  B4P(dvtp0) \
+  B4P(dvtp1) \
+  B4P(dwb) \
+  B4P(dwg) \
+  B4P(egidl) \
+  B4P(egisl) \
+  B4P(etab) \
+  B4P(eu) \
+  B4P(fprout) \
+  B4P(k1) \
+  B4P(k1ox) \
+  B4P(k3) \
+  B4P(k3b) \
+  B4P(keta) \
+  B4P(kt1) \
+  B4P(kt1l) \
+  B4P(kt2) \
+  B4P(lambda) \
+  B4P(ldeb) \
+  B4P(leff) \
+  B4P(leffCV) \
+  B4P(litl) \
+  B4P(lpe0) \
+  B4P(lpeb) \
+  B4P(moin) \
+  B4P(mstar) \
+  B4P(mstarcv) \
+  B4P(nfactor) \
+  B4P(ngate) \
+  B4P(noff) \
+  B4P(pclm) \
+  B4P(pdiblb) \
+  B4P(pdits) \
+  B4P(pditsd) \
+  B4P(phi) \
+  B4P(prwb) \
+  B4P(prwg) \
+  B4P(pscbe1) \
+  B4P(pscbe2) \
+  B4P(pvag) \
+  B4P(rds0) \
+  B4P(rdswmin) \
+  B4P(sqrtPhi) \
+  B4P(tfactor) \
+  B4P(theta0vb0) \
+  B4P(thetaRout) \
+  B4P(ua) \
+  B4P(ub) \
+  B4P(uc) \
+  B4P(ud) \
+  B4P(vbi) \
+  B4P(voffcbn) \
+  B4P(voffcbncv) \
+  B4P(voffcv) \
+  B4P(vtl) \
+  B4P(w0) \
+  B4P(weff) \
+  B4P(weffCJ) \
+  B4P(weffCV) \
+  B4P(xj) \
+
+#define B4_VFIELDS \
+  B4V(Adeff) \
+  B4V(Aseff) \
+  B4V(DjctTempRevSatCur) \
+  B4V(DslpFwd) \
+  B4V(DslpRev) \
+  B4V(DswTempRevSatCur) \
+  B4V(DswgTempRevSatCur) \
+  B4V(IVjdmFwd) \
+  B4V(IVjdmRev) \
+  B4V(IVjsmFwd) \
+  B4V(IVjsmRev) \
+  B4V(OFF) \
+  B4V(Pdeff) \
+  B4V(Pseff) \
+  B4V(SjctTempRevSatCur) \
+  B4V(SslpFwd) \
+  B4V(SslpRev) \
+  B4V(SswTempRevSatCur) \
+  B4V(SswgTempRevSatCur) \
+  B4V(XExpBVD) \
+  B4V(XExpBVS) \
+  B4V(drainConductance) \
+  B4V(eta0) \
+  B4V(k2ox) \
+  B4V(nf) \
+  B4V(numberParallel) \
+  B4V(sourceConductance) \
+  B4V(temp) \
+  B4V(u0temp) \
+  B4V(vbsc) \
+  B4V(vfb) \
+  B4V(vfbzb) \
+  B4V(vjdmFwd) \
+  B4V(vjdmRev) \
+  B4V(vjsmFwd) \
+  B4V(vjsmRev) \
+  B4V(vsattemp) \
+  B4V(vtfbphi1) \
+  B4V(vtfbphi2) \
+  B4V(vth0) \
+
+
+enum DField {
+#define B4P(x) P_##x,
+#define B4V(x) V_##x,
+  B4_PFIELDS
+  B4_VFIELDS
+#undef B4P
+#undef B4V
+  ND
+};
+enum IField { F_MODEL, F_DEXIST, F_SEXIST, NIF };
+enum State { S_VGB, NS };
+static const char* const kStateNames[NS] = {"vgb"};
+// Local nodes.  GatePrime, GateMid and GateExt are one node, as are
+// BodyPrime, DrainBody, SourceBody and Body (rgateMod = rbodyMod = 0);
+// DP/SP are the drain/source nodes themselves when no internal node exists.
+enum Node { N_D, N_G, N_S, N_B, N_DP, N_SP, NNODE };
+enum Sto {
+  T_vbd, T_vbs, T_vgs, T_vds, T_vges, T_vgms, T_vdes, T_vses, T_vdbs, T_vsbs,
+  T_vdbd, T_vged, T_vgmd, T_von, NSTO
+};
+enum Sta { A_qb, A_qg, A_qd, NSTA };
+
+// ---- host setup ------------------------------------------------------------
+static DevCard make_card(const Model& m) {
+  DevCard c{};
+    c.DbulkJctBotGradingCoeff = m.p.DbulkJctBotGradingCoeff;
+    c.DbulkJctGateSideGradingCoeff = m.p.DbulkJctGateSideGradingCoeff;
+    c.DbulkJctSideGradingCoeff = m.p.DbulkJctSideGradingCoeff;
+    c.DjctEmissionCoeff = m.p.DjctEmissionCoeff;
+    c.DjctGateSidewallTempSatCurDensity = m.DjctGateSidewallTempSatCurDensity;
+    c.DjctSidewallTempSatCurDensity = m.DjctSidewallTempSatCurDensity;
+    c.DjctTempSatCurDensity = m.DjctTempSatCurDensity;
+    c.DunitAreaTempJctCap = m.DunitAreaTempJctCap;
+    c.DunitLengthGateSidewallTempJctCap = m.DunitLengthGateSidewallTempJctCap;
+    c.DunitLengthSidewallTempJctCap = m.DunitLengthSidewallTempJctCap;
+    c.PhiBD = m.PhiBD;
+    c.PhiBS = m.PhiBS;
+    c.PhiBSWD = m.PhiBSWD;
+    c.PhiBSWGD = m.PhiBSWGD;
+    c.PhiBSWGS = m.PhiBSWGS;
+    c.PhiBSWS = m.PhiBSWS;
+    c.SbulkJctBotGradingCoeff = m.p.SbulkJctBotGradingCoeff;
+    c.SbulkJctGateSideGradingCoeff = m.p.SbulkJctGateSideGradingCoeff;
+    c.SbulkJctSideGradingCoeff = m.p.SbulkJctSideGradingCoeff;
+    c.SjctEmissionCoeff = m.p.SjctEmissionCoeff;
+    c.SjctGateSidewallTempSatCurDensity = m.SjctGateSidewallTempSatCurDensity;
+    c.SjctSidewallTempSatCurDensity = m.SjctSidewallTempSatCurDensity;
+    c.SjctTempSatCurDensity = m.SjctTempSatCurDensity;
+    c.SunitAreaTempJctCap = m.SunitAreaTempJctCap;
+    c.SunitLengthGateSidewallTempJctCap = m.SunitLengthGateSidewallTempJctCap;
+    c.SunitLengthSidewallTempJctCap = m.SunitLengthSidewallTempJctCap;
+    c.ados = m.p.ados;
+    c.bdos = m.p.bdos;
+    c.bvd = m.p.bvd;
+    c.bvs = m.p.bvs;
+    c.coxe = m.coxe;
+    c.coxp = m.coxp;
+    c.cvchargeMod = m.p.cvchargeMod;
+    c.dioMod = m.p.dioMod;
+    c.dtype = m.dtype;
+    c.epsrox = m.p.epsrox;
+    c.factor1 = m.factor1;
+    c.lambda = m.p.lambda;
+    c.mobMod = m.p.mobMod;
+    c.njtsdtemp = m.njtsdtemp;
+    c.njtsstemp = m.njtsstemp;
+    c.njtsswdtemp = m.njtsswdtemp;
+    c.njtsswgdtemp = m.njtsswgdtemp;
+    c.njtsswgstemp = m.njtsswgstemp;
+    c.njtsswstemp = m.njtsswstemp;
+    c.pditsl = m.p.pditsl;
+    c.rdsMod = m.p.rdsMod;
+    c.tempMod = m.p.tempMod;
+    c.tnoiMod = m.p.tnoiMod;
+    c.tnom = m.p.tnom;
+    c.toxe = m.p.toxe;
+    c.toxp = m.p.toxp;
+    c.vcrit = m.vcrit;
+    c.vtl = m.p.vtl;
+    c.vtm = m.vtm;
+    c.vtm0 = m.vtm0;
+    c.vtsd = m.p.vtsd;
+    c.vtss = m.p.vtss;
+    c.vtsswd = m.p.vtsswd;
+    c.vtsswgd = m.p.vtsswgd;
+    c.vtsswgs = m.p.vtsswgs;
+    c.vtssws = m.p.vtssws;
+    c.xjbvd = m.p.xjbvd;
+    c.xjbvs = m.p.xjbvs;
+    c.xpart = m.p.xpart;
+    c.lambdaGiven = m.g.lambda ? 1.0 : 0.0;
+    c.vtlGiven = m.g.vtl ? 1.0 : 0.0;
+  return c;
+}
+
+static int setup(Group& g) {
+  const int64_t n = g.n;
+  std::vector<Model> models(g.models.size());
+  for (size_t k = 0; k < g.models.size(); ++k) {
+    if (g.models[k].level != 54 && g.models[k].level != 14)
+      return fail(XA_ERR_UNSUPPORTED, "bsim4: level must be 14 or 54");
+    const int rc = model_init(models[k], g.models[k], g.opt);
+    if (rc != XA_OK) return rc;
+  }
+  g.n_dfield = ND;
+  g.n_ifield = NIF;
+  g.dfield.assign((size_t)ND * n, 0.0);
+  g.ifield.assign((size_t)NIF * n, 0);
+  g.n_state = NS;
+  g.state_init.assign((size_t)NS * n, 0.0);
+  auto D = [&](int f, int64_t i) -> double& { return g.dfield[f * n + i]; };
+  auto I = [&](int f, int64_t i) -> int32_t& { return g.ifield[f * n + i]; };
+  std::unique_ptr<Inst> rec(new Inst());
+  for (int64_t i = 0; i < n; ++i) {
+    Inst& in = *rec;
+    in = Inst();
+    in.model = &models[g.inst_model[i]];
+    const double l = g.ival[IS_l][i], w = g.ival[IS_w][i], nf = g.ival[IS_nf][i];
+    if (std::isnan(l) || std::isnan(w) || std::isnan(nf))
+      return fail(XA_ERR_ARG, "bsim4: L, W and NF must be numbers");
+    const int rc = instance_init(g, in, i);
+    if (rc != XA_OK) return rc;
+    const Size& p = *in.paramPtr;
+#define B4P(x) D(P_##x, i) = p.x;
+#define B4V(x) D(V_##x, i) = in.x;
+    B4_PFIELDS
+    B4_VFIELDS
+#undef B4P
+#undef B4V
+    I(F_MODEL, i) = g.inst_model[i];
+    I(F_DEXIST, i) = in.drainMOSFET_B4Exists;
+    I(F_SEXIST, i) = in.sourceMOSFET_B4Exists;
+  }
+  // cards after every instance: the model's temperature values are those
+  // the last instance's updateTemperature wrote
+  g.mcard_stride = sizeof(DevCard) / sizeof(double);
+  g.mcard.assign(g.models.size() * g.mcard_stride, 0.0);
+  for (size_t k = 0; k < models.size(); ++k) {
+    const DevCard c = make_card(models[k]);
+    std::memcpy(&g.mcard[k * g.mcard_stride], &c, sizeof(c));
+  }
+  return XA_OK;
+}
+
+static void resolve_nodes(const Group& g, int64_t i, const int32_t* reg, int32_t* full) {
+  const int64_t n = g.n;
+  full[N_D] = reg[0];
+  full[N_G] = reg[1];
+  full[N_S] = reg[2];
+  full[N_B] = reg[3];
+  int loc = 4;
+  full[N_DP] = g.ifield[F_DEXIST * n + i] ? reg[loc++] : reg[0];
+  full[N_SP] = g.ifield[F_SEXIST * n + i] ? reg[loc++] : reg[2];
+}
+
+// Stamp tables in the order of the reference's loads.  A row of -1 marks an
+// entry the instance does not load (no internal drain/source node).
+constexpr int16_t N_NONE = -1;
+static const Stamp kStF[] = {{N_DP, 0}, {N_G, 0}, {N_B, 0}, {N_SP, 0}, {N_D, 0}, {N_DP, 0}, {N_S, 0}, {N_SP, 0}};
+static const Stamp kStQ[] = {{N_DP, 0}, {N_G, 0}, {N_B, 0}, {N_SP, 0}};
+static const Stamp kStLim[] = {{N_DP, 0}, {N_G, 0}, {N_B, 0}, {N_SP, 0}};
+static const Stamp kStDF[] = {{N_G, N_G}, {N_G, N_DP}, {N_G, N_SP}, {N_G, N_B}, {N_DP, N_DP}, {N_DP, N_D}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_D, N_DP}, {N_D, N_D}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_S}, {N_SP, N_B}, {N_S, N_SP}, {N_S, N_S}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}, {N_DP, N_DP}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}};
+static const Stamp kStDQ[] = {{N_G, N_G}, {N_G, N_DP}, {N_G, N_SP}, {N_G, N_B}, {N_DP, N_DP}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}};
+
+static void stamps(const Group& g, int64_t i, int kind, std::vector<Stamp>& out) {
+  out.clear();
+  auto add = [&](const Stamp* s, int m) { out.assign(s, s + m); };
+  switch (kind) {
+    case K_F: {
+      add(kStF, 8);
+      const int64_t n = g.n;
+      if (!g.ifield[F_DEXIST * n + i]) out[4].row = out[5].row = N_NONE;
+      if (!g.ifield[F_SEXIST * n + i]) out[6].row = out[7].row = N_NONE;
+      break;
+    }
+    case K_Q: add(kStQ, 4); break;
+    case K_FLIM: case K_QLIM: if (g.opt.voltageLimiterFlag) add(kStLim, 4); break;
+    case K_DFDX: add(kStDF, 38); break;
+    case K_DQDX: add(kStDQ, 16); break;
+    default: break;   // the BSIM4 Master loads store no lead currents
+  }
+}
+
+// dQdx is not loaded on the first iteration of a transient step
+// (Master::loadDAEMatrices, N_DEV_MOSFET_B4.C:17230-17236).
+static bool skip_kind(const Group&, const xa_solver_state& ss, int kind) {
+  return kind == K_DQDX && !ss.dcopFlag && ss.initTranFlag && ss.newtonIter == 0;
+}
+
+// ---- device evaluation ---------------------------------------------------
+// DEXP of the reference: exp with linear extrapolation above the threshold.
+XA_HD void dexp3(double A, double& B, double& dB) {
+  if (A > kExpThreshold) { B = kMaxExp * (1.0 + A - kExpThreshold); dB = kMaxExp; }
+  else if (A < -kExpThreshold) { B = kMinExp; dB = 0; }
+  else { B = exp(A); dB = B; }
+}
+
+// polyDepletion (N_DEV_MOSFET_B4.C:14252-14282)
+XA_HD void polyDepletion(double phi, double ngate, double epsgate, double coxe, double Vgs_arg,
+                         double& Vgs_eff, double& dVgs_eff_dVg) {
+  if ((ngate > 1.0e18) && (ngate < 1.0e25) && (Vgs_arg > phi) && (epsgate != 0)) {
+    const double T1 = 1.0e6 * kQ * epsgate * ngate / (coxe * coxe);
+    const double T8 = Vgs_arg - phi;
+    const double T4 = sqrt(1.0 + 2.0 * T8 / T1);
+    const double T2 = 2.0 * T8 / (T4 + 1.0);
+    const double T3 = 0.5 * T2 * T2 / T1;
+    const double T7 = 1.12 - T3 - 0.05;
+    const double T6 = sqrt(T7 * T7 + 0.224);
+    const double T5 = 1.12 - 0.5 * (T7 + T6);
+    Vgs_eff = Vgs_arg - T5;
+    dVgs_eff_dVg = 1.0 - (0.5 - 0.5 / T4) * (1.0 + T7 / T6);
+  } else {
+    Vgs_eff = Vgs_arg;
+    dVgs_eff_dVg = 1.0;
+  }
+}
+
+// store-vector value, 0 for an unregistered slot
+XA_HD double stv(const double* s, int32_t lid) { return lid >= 0 ? s[lid] : 0.0; }
+
+#define PP(x) dfld(a, P_##x, i)
+#define IV(x) dfld(a, V_##x, i)
+
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_update(KArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const DevCard& C = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
+  const xa_solver_state& ss = a.ss;
+  const xa_device_options& opt = a.opt;
+  const double gmin = opt.gmin;
+  const double nf = IV(nf);
+  const int irbodyMod = 0;
+  const int32_t li_Drain = node(a, N_D, i), li_GateExt = node(a, N_G, i);
+  const int32_t li_Source = node(a, N_S, i), li_Body = node(a, N_B, i);
+  const int32_t li_DrainPrime = node(a, N_DP, i), li_SourcePrime = node(a, N_SP, i);
+  const int32_t li_GatePrime = li_GateExt, li_GateMid = li_GateExt;
+  const int32_t li_BodyPrime = li_Body, li_DrainBody = li_Body, li_SourceBody = li_Body;
+  const int32_t li_store_vbd = stolid(a, T_vbd, i), li_store_vbs = stolid(a, T_vbs, i);
+  const int32_t li_store_vgs = stolid(a, T_vgs, i), li_store_vds = stolid(a, T_vds, i);
+  const int32_t li_store_vges = stolid(a, T_vges, i), li_store_vgms = stolid(a, T_vgms, i);
+  const int32_t li_store_vdes = stolid(a, T_vdes, i), li_store_vses = stolid(a, T_vses, i);
+  const int32_t li_store_vdbs = stolid(a, T_vdbs, i), li_store_vsbs = stolid(a, T_vsbs, i);
+  const int32_t li_store_vdbd = stolid(a, T_vdbd, i), li_store_vged = stolid(a, T_vged, i);
+  const int32_t li_store_vgmd = stolid(a, T_vgmd, i), li_store_von = stolid(a, T_von, i);
+
+  // evaluation state of the instance
+  double Vd, Vs, Vb, Vsp, Vdp, Vgp, Vbp, Vge, Vgm, Vdb, Vsb, Qtotal;
+  double Vddp, Vssp, Vdbb, Vdbbp, Vsbb, Vsbbp, Vbpb, Vgegp, Vgegm, Vgmgp;
+  double vds, vgs, vbs, vges, vgms, vdbs, vsbs, vses, vdes, qdef;
+  double vbd, vdbd, vgd, vged, vgmd, vgmb, vgb, vbs_jct, vbd_jct;
+  double vbd_orig, vbs_orig, vgs_orig, vds_orig, vgd_orig, vges_orig, vgms_orig, vdes_orig;
+  double vses_orig, vdbs_orig, vsbs_orig, vdbd_orig, vged_orig, vgmd_orig, vbs_jct_orig;
+  double vbd_jct_orig, vgmb_orig, vgb_orig, Vgegp_orig, Vgegm_orig, Vgmgp_orig;
+  int origFlag, limitedFlag, newtonIterOld, mode, ChargeComputationNeeded;
+  double von, thetavth, nstar, vgs_eff, vgd_eff, dvgs_eff_dvg, dvgd_eff_dvg;
+  double Vgsteff, Abulk, ueff, EsatL, vdsat, Vdseff, AbovVgst2Vtm, grdsw;
+  double gbs, cbs, gbd, cbd, csub, gbbs, gbgs, gbds, cdrain, gds, gm, gmbs, IdovVds;
+  double gcrg, gcrgd, gcrgg, gcrgs, gcrgb, gstot, gstotd, gstotg, gstots, gstotb;
+  double gdtot, gdtotd, gdtotg, gdtots, gdtotb;
+  double Igidl, ggidld, ggidlg, ggidlb, Igisl, ggisls, ggislg, ggislb, ggidls, ggisld;
+  double Igcs, gIgcsg, gIgcsd, gIgcsb, gIgcss, Igcd, gIgcdg, gIgcdd, gIgcdb, gIgcds;
+  double Igs, gIgsg, gIgss, Igd, gIgdg, gIgdd, Igb, gIgbg, gIgbd, gIgbs, gIgbb, cd;
+  double qinv, qgate, qdrn, qsrc, qbulk, qgmid, qgdo, qgso, qgb, qgmb, qchqs, qcheq;
+  double cggb, cgsb, cgdb, cdgb, cdsb, cddb, cbgb, cbsb, cbdb, csgb, cssb, csdb;
+  double cgbb, csbb, cdbb, cbbb, cqdb, cqsb, cqgb, cqbb, gtau, CoxWL, taunet;
+  double qbs, qbd, capbs, capbd, Igate, IgateMid, Igate_Jdxp, IgateMid_Jdxp;
+  double Idrain, Isource, Idbb, Idbbp, Isbb, Isbbp, Ibpb;
+  double gcggb, gcgdb, gcgsb, gcgbb, gcdgb, gcsgb, gcbgb, gcddb, gcdsb, gcsdb, gcssb;
+  double gcdbb, gcsbb, gcbdb, gcbsb, gcdbdb, gcsbsb, gcbbb, gcgmgmb, gcgmdb, gcgmsb;
+  double gcgmbb, gcdgmb, gcsgmb, gcbgmb, gcqgb, gcqdb, gcqsb, gcqbb, gqdef;
+  double ggtg, ggtd, ggts, ggtb, gtg, gtd, gts, gtb;
+  double dxpart, sxpart, ddxpart_dVd, ddxpart_dVg, ddxpart_dVb, ddxpart_dVs;
+  double dsxpart_dVd, dsxpart_dVg, dsxpart_dVb, dsxpart_dVs;
+  double CAPcggb, CAPcgdb, CAPcgsb, CAPcgbb, CAPcdgb, CAPcsgb, CAPcbgb, CAPcddb, CAPcdsb;
+  double CAPcsdb, CAPcssb, CAPcdbb, CAPcsbb, CAPcbdb, CAPcbsb, CAPcdbdb, CAPcsbsb, CAPcbbb;
+  double CAPcgmgmb, CAPcgmdb, CAPcgmsb, CAPcgmbb, CAPcdgmb, CAPcsgmb, CAPcbgmb;
+  double CAPcqgb, CAPcqdb, CAPcqsb, CAPcqbb;
+  double qg, qd, qb;
+  double Gm, Gmbs, FwdSum, RevSum, ceqdrn, ceqbd, ceqbs, ceqjs, ceqjd, ceqgcrg;
+  double ceqgstot, ceqgdtot, gbbdp, gbbsp, gbdpg, gbdpdp, gbdpb, gbdpsp;
+  double gbspg, gbspdp, gbspb, gbspsp;
+  double gIstotg, gIstotd, gIstots, gIstotb, Istoteq, gIdtotg, gIdtotd, gIdtots, gIdtotb;
+  double Idtoteq, gIbtotg, gIbtotd, gIbtots, gIbtotb, Ibtoteq, gIgtotg, gIgtotd, gIgtots;
+  double gIgtotb, Igtoteq;
+  double ceqdrn_Jdxp, ceqbd_Jdxp, ceqbs_Jdxp, Istoteq_Jdxp, Idtoteq_Jdxp, Ibtoteq_Jdxp;
+  double Igtoteq_Jdxp, ceqgcrg_Jdxp, ceqgstot_Jdxp, ceqgdtot_Jdxp, ceqjs_Jdxp, ceqjd_Jdxp;
+  double Qeqqg_Jdxp, Qeqqd_Jdxp, Qeqqb_Jdxp, Qeqqgmid_Jdxp, Qeqqjs_Jdxp, Qeqqjd_Jdxp;
+  double gjbd, gjbs, gdpr, gspr, geltd;
+  double cgdo, cgso;
+  // vgb keeps the previous evaluation's value until it is recomputed
+  vgb = st(a, S_VGB, i);
+
+  // ---- Instance::updateIntermediateVars ----
+
+  double vgd_old, SourceSatCurrent, DrainSatCurrent, evbd, evbs, arg, sarg;
+  double Vfbeff, dVfbeff_dVg, dVfbeff_dVb, V3, V4, MJD, MJSWD, MJSWGD, MJS, MJSWS, MJSWGS;
+  double Ggidld, Ggidlg, Ggidlb, Ggislg, Ggislb, Ggisls, VgstNVt, ExpVgst;
+  double czbd, czbdsw, czbdswg, czbs, czbssw, czbsswg;
+  double Vgs_eff, dVgs_eff_dVg, Vth_NarrowW, Phis, dPhis_dVb, sqrtPhis, dsqrtPhis_dVb;
+  double Vth, dVth_dVb, dVth_dVd, Vgst, Nvtms, Nvtmd, Vtm, Vtm0, n, dn_dVb, dn_dVd, voffcv;
+  double noff, dnoff_dVd, dnoff_dVb, CoxWLcen, QovCox, LINK, V0, DeltaPhi, dDeltaPhi_dVg;
+  double VgDP, dVgDP_dVg, Cox, Tox, Tcen, dTcen_dVg, dTcen_dVd, dTcen_dVb, Ccen;
+  double Coxeff, dCoxeff_dVd, dCoxeff_dVg, dCoxeff_dVb, Denomi, dDenomi_dVg, dDenomi_dVd, dDenomi_dVb;
+  double dueff_dVg, dueff_dVd, dueff_dVb, Esat, Vdsat, dEsatL_dVg, dEsatL_dVd, dEsatL_dVb;
+  double dVdsat_dVg, dVdsat_dVb, dVdsat_dVd, Vasat, dVasat_dVg, dVasat_dVb, dVasat_dVd;
+  double Va, dVa_dVd, dVa_dVg, dVa_dVb, Vbseff, dVbseff_dVb, VbseffCV, dVbseffCV_dVb;
+  double T0, dT0_dVg, dT0_dVd, dT0_dVb, T1, dT1_dVg, dT1_dVd, dT1_dVb;
+  double T2, dT2_dVg, dT2_dVd, dT2_dVb, T3, dT3_dVg, dT3_dVd, dT3_dVb;
+  double T4, dT4_dVd, dT4_dVb, T5, dT5_dVg, dT5_dVd, dT5_dVb;
+  double T6, dT6_dVg, dT6_dVd, dT6_dVb, T7, T8, dT8_dVg, dT8_dVd, dT8_dVb;
+  double T9, dT9_dVg, dT9_dVd, dT9_dVb, T10, dT10_dVg, dT10_dVb, dT10_dVd;
+  double T11, T12, T13, T14, tmp, dAbulk_dVb, Abulk0, dAbulk0_dVb;
+  double Cclm, dCclm_dVg, dCclm_dVd, dCclm_dVb, FP, dFP_dVg, PvagTerm, dPvagTerm_dVg;
+  double dPvagTerm_dVd, dPvagTerm_dVb, VADITS, dVADITS_dVg, dVADITS_dVd, Lpe_Vb;
+  double dDITS_Sft_dVb, dDITS_Sft_dVd, VACLM, dVACLM_dVg, dVACLM_dVd, dVACLM_dVb;
+  double VADIBL, dVADIBL_dVg, dVADIBL_dVd, dVADIBL_dVb, Xdep, dXdep_dVb;
+  double lt1, dlt1_dVb, ltw, dltw_dVb, Delt_vth, dDelt_vth_dVb, Theta0, dTheta0_dVb;
+  double TempRatio, tmp1, tmp2, tmp3, tmp4, DIBL_Sft, dDIBL_Sft_dVd, Lambda, dLambda_dVg, a1;
+  double dVgsteff_dVg, dVgsteff_dVd, dVgsteff_dVb, dVdseff_dVg, dVdseff_dVd, dVdseff_dVb;
+  double VdseffCV, dVdseffCV_dVg, dVdseffCV_dVd, dVdseffCV_dVb, diffVds, dAbulk_dVg;
+  double beta, dbeta_dVg, dbeta_dVd, dbeta_dVb, gche, dgche_dVg, dgche_dVd, dgche_dVb;
+  double fgche1, dfgche1_dVg, dfgche1_dVd, dfgche1_dVb, fgche2, dfgche2_dVg, dfgche2_dVd, dfgche2_dVb;
+  double Idl, dIdl_dVg, dIdl_dVd, dIdl_dVb, Idsa, dIdsa_dVg, dIdsa_dVd, dIdsa_dVb, Ids, Gmb;
+  double devbs_dvb, devbd_dvb, Isub, Gbd, Gbg, Gbb, Gds, VASCBE, dVASCBE_dVg, dVASCBE_dVd, dVASCBE_dVb;
+  double CoxeffWovL, Rds, dRds_dVg, dRds_dVb, WVCox, WVCoxRds, Vgst2Vtm, VdsatCV;
+  double Leff, Weff, dWeff_dVg, dWeff_dVb, AbulkCV, dAbulkCV_dVb;
+  double Cgg1, Cgb1, Cgd1, Cbg1, Cbb1, Cbd1, Qac0, Qsub0, dQac0_dVg, dQac0_dVb;
+  double dQsub0_dVg, dQsub0_dVd, dQsub0_dVb, Csg, Csd, Csb, Cbg, Cbd, Cbb, Cgg, Cgd, Cgb;
+  double Nvtmrss, Nvtmrssws, Nvtmrsswgs, Nvtmrsd, Nvtmrsswd, Nvtmrsswgd;
+  double vs, Fsevl, dvs_dVg, dvs_dVd, dvs_dVb, dFsevl_dVg, dFsevl_dVd, dFsevl_dVb;
+  double vgdx, vgsx, epssub, toxe, epsrox, von_local, Vds, Vgs, Vbs, Vdb_, Gm_, Vdseff_;
+  double vbd_old, vbs_old, vgs_old, vds_old, vges_old, vgms_old, vdes_old, vses_old;
+  double vdbs_old, vsbs_old, vdbd_old, vged_old, vgmd_old;
+  int Check = 0;
+
+  dT1_dVb = dT2_dVb = dT3_dVb = dT4_dVb = dT5_dVb = dT6_dVb = 0.0;
+  dT0_dVb = 0.0;
+  devbs_dvb = devbd_dvb = 0.0;
+  ChargeComputationNeeded = (ss.tranopFlag || ss.acopFlag || ss.transientFlag) ? 1 : 0;
+  limitedFlag = 0;
+
+  /* node voltages (:7919-7974) */
+  Vd = solv(a.v.nextSol, li_Drain);
+  Vs = solv(a.v.nextSol, li_Source);
+  Vb = solv(a.v.nextSol, li_Body);
+  Vsp = solv(a.v.nextSol, li_SourcePrime);
+  Vdp = solv(a.v.nextSol, li_DrainPrime);
+  Vgp = solv(a.v.nextSol, li_GatePrime);
+  Vbp = solv(a.v.nextSol, li_BodyPrime);
+  Vge = solv(a.v.nextSol, li_GateExt);
+  Vgm = (li_GateMid >= 0) ? solv(a.v.nextSol, li_GateMid) : 0.0;
+  Vdb = solv(a.v.nextSol, li_DrainBody);
+  Vsb = solv(a.v.nextSol, li_SourceBody);
+  Qtotal = 0.0;
+  Vddp = Vd - Vdp;
+  Vssp = Vs - Vsp;
+  Vdbb = Vdb - Vb;
+  Vdbbp = Vdb - Vbp;
+  Vsbb = Vsb - Vb;
+  Vsbbp = Vsb - Vbp;
+  Vbpb = Vbp - Vb;
+  vds = C.dtype * (Vdp - Vsp);
+  vgs = C.dtype * (Vgp - Vsp);
+  vbs = C.dtype * (Vbp - Vsp);
+  vges = C.dtype * (Vge - Vsp);
+  vgms = C.dtype * (Vgm - Vsp);
+  vdbs = C.dtype * (Vdb - Vsp);
+  vsbs = C.dtype * (Vsb - Vsp);
+  vses = C.dtype * (Vs - Vsp);
+  vdes = C.dtype * (Vd - Vsp);
+  qdef = C.dtype * (Qtotal);
+  vbd = vbs - vds;
+  vdbd = vdbs - vds;
+  vgd = vgs - vds;
+  vged = vges - vds;
+  vgmd = vgms - vds;
+  vgmb = vgms - vbs;
+  vdbd = vdbs - vds;
+  vbs_jct = (!irbodyMod) ? vbs : vsbs;
+  vbd_jct = (!irbodyMod) ? vbd : vdbd;
+  Vgegp = Vge - Vgp;
+  Vgegm = Vge - Vgm;
+  Vgmgp = Vgm - Vgp;
+  origFlag = 1;
+  vbd_orig = vbd;
+  vbs_orig = vbs;
+  vgs_orig = vgs;
+  vds_orig = vds;
+  vgd_orig = vgd;
+  vges_orig = vges;
+  vgms_orig = vgms;
+  vdes_orig = vdes;
+  vses_orig = vses;
+  vdbs_orig = vdbs;
+  vsbs_orig = vsbs;
+  vdbd_orig = vdbd;
+  vged_orig = vged;
+  vgmd_orig = vgmd;
+  vbs_jct_orig = vbs_jct;
+  vbd_jct_orig = vbd_jct;
+  vgmb_orig = vgmb;
+  vgb_orig = vgb;        /* the previous call's vgb: vgb is updated below */
+  Vgegp_orig = Vgegp;
+  Vgegm_orig = Vgegm;
+  Vgmgp_orig = Vgmgp;
+
+  /* initial conditions (:8052-8097) */
+  if (ss.initJctFlag && !IV(OFF) && opt.voltageLimiterFlag) {
+    int setv = 1;
+    if (ss.inputOPFlag) {
+      const double* fl = a.v.flagSol;
+      const int lids[11] = {li_Drain, li_GateExt, li_Source, li_Body, li_DrainPrime,
+                            li_GatePrime, li_GateMid, li_SourcePrime, li_BodyPrime,
+                            li_DrainBody, li_SourceBody};
+      int k, any0 = (fl == nullptr);
+      for (k = 0; k < 11 && !any0; ++k) any0 = (lids[k] < 0) || fl[lids[k]] == 0;
+      setv = any0;
+    }
+    if (setv) {
+      vds = 0.1;
+      vdes = 0.11;
+      vses = -0.01;
+      vgs = vges = vgms = C.dtype * IV(vth0) + 0.1;
+      origFlag = 0;
+    }
+    vbs = vdbs = vsbs = 0.0;
+    vbd = vbs - vds;
+    vdbd = vdbs - vds;
+    vgd = vgs - vds;
+    vged = vges - vds;
+    vgmd = vgms - vds;
+  } else if ((ss.initFixFlag || ss.initJctFlag) && IV(OFF)) {
+    vds = vgs = vbs = vges = vgms = 0.0;
+    vds = vsbs = vdes = vses = qdef = 0.0;
+  }
+
+  /* previous iteration's junction voltages (:8100-8158) */
+  if (ss.newtonIter == 0) {
+    newtonIterOld = 0;
+    if (!ss.dcopFlag || (ss.locaEnabledFlag && ss.dcopFlag)) {
+      const double* s = a.v.currSto;
+      vbd_old = stv(s, li_store_vbd);
+      vbs_old = stv(s, li_store_vbs);
+      vgs_old = stv(s, li_store_vgs);
+      vds_old = stv(s, li_store_vds);
+      vges_old = stv(s, li_store_vges);
+      vgms_old = stv(s, li_store_vgms);
+      vdes_old = stv(s, li_store_vdes);
+      vses_old = stv(s, li_store_vses);
+      vdbs_old = stv(s, li_store_vdbs);
+      vsbs_old = stv(s, li_store_vsbs);
+      vdbd_old = stv(s, li_store_vdbd);
+      vged_old = stv(s, li_store_vged);
+      vgmd_old = stv(s, li_store_vgmd);
+      von_local = stv(s, li_store_von);
+    } else {
+      vbd_old = vbd;
+      vbs_old = vbs;
+      vgs_old = vgs;
+      vds_old = vds;
+      vges_old = vges;
+      vgms_old = vgms;
+      vdes_old = vdes;
+      vses_old = vses;
+      vdbs_old = vdbs;
+      vsbs_old = vsbs;
+      vdbd_old = vdbd;
+      vged_old = vged;
+      vgmd_old = vgmd;
+      von_local = 0.0;
+    }
+  } else {
+    const double* s = a.v.nextSto;
+    vbd_old = stv(s, li_store_vbd);
+    vbs_old = stv(s, li_store_vbs);
+    vgs_old = stv(s, li_store_vgs);
+    vds_old = stv(s, li_store_vds);
+    vges_old = stv(s, li_store_vges);
+    vgms_old = stv(s, li_store_vgms);
+    vdes_old = stv(s, li_store_vdes);
+    vses_old = stv(s, li_store_vses);
+    vdbs_old = stv(s, li_store_vdbs);
+    vsbs_old = stv(s, li_store_vsbs);
+    vdbd_old = stv(s, li_store_vdbd);
+    vged_old = stv(s, li_store_vged);
+    vgmd_old = stv(s, li_store_vgmd);
+    von_local = stv(s, li_store_von);
+  }
+  vgd_old = vgs_old - vds_old;
+  (void)vged_old; (void)vgmd_old; (void)vges_old; (void)vgms_old; (void)vdbs_old;
+  (void)vsbs_old; (void)vdbd_old; (void)vbd_old;
+
+  /* voltage limiting (:8165-8359), rgateMod = rbodyMod = rdsMod = 0 */
+  if (opt.voltageLimiterFlag && !(ss.initFixFlag && IV(OFF))) {
+    if (ss.newtonIter >= 0 && !(ss.initJctFlag)) {
+      if (vds_old >= 0.0) {
+        vgs = fetlim(vgs, vgs_old, von_local);
+        vds = vgs - vgd;
+        vds = limvds(vds, vds_old);
+        vgd = vgs - vds;
+      } else {
+        vgd = fetlim(vgd, vgd_old, von_local);
+        vds = vgs - vgd;
+        vds = -limvds(-vds, -vds_old);
+        vgs = vgd + vds;
+      }
+      if (vds >= 0.0) {
+        vbs = pnjlim(vbs, vbs_old, kVt0, C.vcrit, &Check);
+        vbd = vbs - vds;
+      } else {
+        vbd = pnjlim(vbd, vbd_old, kVt0, C.vcrit, &Check);
+        vbs = vbd + vds;
+      }
+    }
+    if (Check == 1) limitedFlag = 1;
+    {
+      const double machprec = 2.220446049250313e-16;
+      if (fabs(vbd_orig - vbd) > machprec || fabs(vbs_orig - vbs) > machprec ||
+          fabs(vgs_orig - vgs) > machprec || fabs(vds_orig - vds) > machprec ||
+          fabs(vgd_orig - vgd) > machprec || fabs(vges_orig - vges) > machprec ||
+          fabs(vgms_orig - vgms) > machprec || fabs(vdes_orig - vdes) > machprec ||
+          fabs(vses_orig - vses) > machprec || fabs(vdbs_orig - vdbs) > machprec ||
+          fabs(vsbs_orig - vsbs) > machprec || fabs(vdbd_orig - vdbd) > machprec ||
+          fabs(vbs_jct_orig - vbs_jct) > machprec || fabs(vbd_jct_orig - vbd_jct) > machprec ||
+          fabs(vgmb_orig - vgmb) > machprec || fabs(vgb_orig - vgb) > machprec ||
+          fabs(vged_orig - vged) > machprec || fabs(vgmd_orig - vgmd) > machprec)
+        origFlag = 0;
+    }
+  }
+  if (ss.newtonIter != 0 && ss.newtonIter != newtonIterOld) newtonIterOld = ss.newtonIter;
+  vbd = vbs - vds;
+  vgd = vgs - vds;
+  vgb = vgs - vbs;
+  vged = vges - vds;
+  vgmd = vgms - vds;
+  vgmb = vgms - vbs;
+  vdbd = vdbs - vds;
+  vbs_jct = (!irbodyMod) ? vbs : vsbs;
+  vbd_jct = (!irbodyMod) ? vbd : vdbd;
+
+  /* source/drain junction diodes (:8425-8769) */
+  Nvtms = C.vtm * C.SjctEmissionCoeff;
+  if ((IV(Aseff) <= 0.0) && (IV(Pseff) <= 0.0)) SourceSatCurrent = 1.0e-14;
+  else SourceSatCurrent = IV(Aseff) * C.SjctTempSatCurDensity + IV(Pseff) * C.SjctSidewallTempSatCurDensity
+                          + PP(weffCJ) * nf * C.SjctGateSidewallTempSatCurDensity;
+  if (SourceSatCurrent <= 0.0) {
+    gbs = gmin;
+    cbs = gbs * vbs_jct;
+  } else {
+    switch ((int)C.dioMod) {
+      case 0:
+        evbs = exp(vbs_jct / Nvtms);
+        T1 = C.xjbvs * exp(-(C.bvs + vbs_jct) / Nvtms);
+        gbs = SourceSatCurrent * (evbs + T1) / Nvtms + gmin;
+        cbs = SourceSatCurrent * (evbs + IV(XExpBVS) - T1 - 1.0) + gmin * vbs_jct;
+        break;
+      case 1:
+        T2 = vbs_jct / Nvtms;
+        if (T2 < -kExpThreshold) {
+          gbs = gmin;
+          cbs = SourceSatCurrent * (kMinExp - 1.0) + gmin * vbs_jct;
+        } else if (vbs_jct <= IV(vjsmFwd)) {
+          evbs = exp(T2);
+          gbs = SourceSatCurrent * evbs / Nvtms + gmin;
+          cbs = SourceSatCurrent * (evbs - 1.0) + gmin * vbs_jct;
+        } else {
+          T0 = IV(IVjsmFwd) / Nvtms;
+          gbs = T0 + gmin;
+          cbs = IV(IVjsmFwd) - SourceSatCurrent + T0 * (vbs_jct - IV(vjsmFwd)) + gmin * vbs_jct;
+        }
+        break;
+      case 2:
+        if (vbs_jct < IV(vjsmRev)) {
+          T0 = vbs_jct / Nvtms;
+          if (T0 < -kExpThreshold) {
+            evbs = kMinExp;
+            devbs_dvb = 0.0;
+          } else {
+            evbs = exp(T0);
+            devbs_dvb = evbs / Nvtms;
+          }
+          T1 = evbs - 1.0;
+          T2 = IV(IVjsmRev) + IV(SslpRev) * (vbs_jct - IV(vjsmRev));
+          gbs = devbs_dvb * T2 + T1 * IV(SslpRev) + gmin;
+          cbs = T1 * T2 + gmin * vbs_jct;
+        } else if (vbs_jct <= IV(vjsmFwd)) {
+          T0 = vbs_jct / Nvtms;
+          if (T0 < -kExpThreshold) {
+            evbs = kMinExp;
+            devbs_dvb = 0.0;
+          } else {
+            evbs = exp(T0);
+            devbs_dvb = evbs / Nvtms;
+          }
+          T1 = (C.bvs + vbs_jct) / Nvtms;
+          if (T1 > kExpThreshold) {
+            T2 = kMinExp;
+            T3 = 0.0;
+          } else {
+            T2 = exp(-T1);
+            T3 = -T2 / Nvtms;
+          }
+          gbs = SourceSatCurrent * (devbs_dvb - C.xjbvs * T3) + gmin;
+          cbs = SourceSatCurrent * (evbs + IV(XExpBVS) - 1.0 - C.xjbvs * T2) + gmin * vbs_jct;
+        } else {
+          gbs = IV(SslpFwd) + gmin;
+          cbs = IV(IVjsmFwd) + IV(SslpFwd) * (vbs_jct - IV(vjsmFwd)) + gmin * vbs_jct;
+        }
+        break;
+      default: break;
+    }
+  }
+  Nvtmd = C.vtm * C.DjctEmissionCoeff;
+  if ((IV(Adeff) <= 0.0) && (IV(Pdeff) <= 0.0)) DrainSatCurrent = 1.0e-14;
+  else DrainSatCurrent = IV(Adeff) * C.DjctTempSatCurDensity + IV(Pdeff) * C.DjctSidewallTempSatCurDensity
+                         + PP(weffCJ) * nf * C.DjctGateSidewallTempSatCurDensity;
+  if (DrainSatCurrent <= 0.0) {
+    gbd = gmin;
+    cbd = gbd * vbd_jct;
+  } else {
+    switch ((int)C.dioMod) {
+      case 0:
+        evbd = exp(vbd_jct / Nvtmd);
+        T1 = C.xjbvd * exp(-(C.bvd + vbd_jct) / Nvtmd);
+        gbd = DrainSatCurrent * (evbd + T1) / Nvtmd + gmin;
+        cbd = DrainSatCurrent * (evbd + IV(XExpBVD) - T1 - 1.0) + gmin * vbd_jct;
+        break;
+      case 1:
+        T2 = vbd_jct / Nvtmd;
+        if (T2 < -kExpThreshold) {
+          gbd = gmin;
+          cbd = DrainSatCurrent * (kMinExp - 1.0) + gmin * vbd_jct;
+        } else if (vbd_jct <= IV(vjdmFwd)) {
+          evbd = exp(T2);
+          gbd = DrainSatCurrent * evbd / Nvtmd + gmin;
+          cbd = DrainSatCurrent * (evbd - 1.0) + gmin * vbd_jct;
+        } else {
+          T0 = IV(IVjdmFwd) / Nvtmd;
+          gbd = T0 + gmin;
+          cbd = IV(IVjdmFwd) - DrainSatCurrent + T0 * (vbd_jct - IV(vjdmFwd)) + gmin * vbd_jct;
+        }
+        break;
+      case 2:
+        if (vbd_jct < IV(vjdmRev)) {
+          T0 = vbd_jct / Nvtmd;
+          if (T0 < -kExpThreshold) {
+            evbd = kMinExp;
+            devbd_dvb = 0.0;
+          } else {
+            evbd = exp(T0);
+            devbd_dvb = evbd / Nvtmd;
+          }
+          T1 = evbd - 1.0;
+          T2 = IV(IVjdmRev) + IV(DslpRev) * (vbd_jct - IV(vjdmRev));
+          gbd = devbd_dvb * T2 + T1 * IV(DslpRev) + gmin;
+          cbd = T1 * T2 + gmin * vbd_jct;
+        } else if (vbd_jct <= IV(vjdmFwd)) {
+          T0 = vbd_jct / Nvtmd;
+          if (T0 < -kExpThreshold) {
+            evbd = kMinExp;
+            devbd_dvb = 0.0;
+          } else {
+            evbd = exp(T0);
+            devbd_dvb = evbd / Nvtmd;
+          }
+          T1 = (C.bvd + vbd_jct) / Nvtmd;
+          if (T1 > kExpThreshold) {
+            T2 = kMinExp;
+            T3 = 0.0;
+          } else {
+            T2 = exp(-T1);
+            T3 = -T2 / Nvtmd;
+          }
+          gbd = DrainSatCurrent * (devbd_dvb - C.xjbvd * T3) + gmin;
+          cbd = DrainSatCurrent * (evbd + IV(XExpBVD) - 1.0 - C.xjbvd * T2) + gmin * vbd_jct;
+        } else {
+          gbd = IV(DslpFwd) + gmin;
+          cbd = IV(IVjdmFwd) + IV(DslpFwd) * (vbd_jct - IV(vjdmFwd)) + gmin * vbd_jct;
+        }
+        break;
+      default: break;
+    }
+  }
+
+  /* trap-assisted tunneling and recombination (:8655-8769) */
+  Nvtmrssws = C.vtm0 * C.njtsswstemp;
+  Nvtmrsswgs = C.vtm0 * C.njtsswgstemp;
+  Nvtmrss = C.vtm0 * C.njtsstemp;
+  Nvtmrsswd = C.vtm0 * C.njtsswdtemp;
+  Nvtmrsswgd = C.vtm0 * C.njtsswgdtemp;
+  Nvtmrsd = C.vtm0 * C.njtsdtemp;
+  if ((C.vtss - vbs_jct) < (C.vtss * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbs_jct / Nvtmrss * T9;
+    dexp3(T0, T1, T10);
+    dT1_dVb = T10 / Nvtmrss * T9;
+  } else {
+    T9 = 1.0 / (C.vtss - vbs_jct);
+    T0 = -vbs_jct / Nvtmrss * C.vtss * T9;
+    dT0_dVb = C.vtss / Nvtmrss * (T9 + vbs_jct * T9 * T9);
+    dexp3(T0, T1, T10);
+    dT1_dVb = T10 * dT0_dVb;
+  }
+  if ((C.vtsd - vbd_jct) < (C.vtsd * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbd_jct / Nvtmrsd * T9;
+    dexp3(T0, T2, T10);
+    dT2_dVb = T10 / Nvtmrsd * T9;
+  } else {
+    T9 = 1.0 / (C.vtsd - vbd_jct);
+    T0 = -vbd_jct / Nvtmrsd * C.vtsd * T9;
+    dT0_dVb = C.vtsd / Nvtmrsd * (T9 + vbd_jct * T9 * T9);
+    dexp3(T0, T2, T10);
+    dT2_dVb = T10 * dT0_dVb;
+  }
+  if ((C.vtssws - vbs_jct) < (C.vtssws * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbs_jct / Nvtmrssws * T9;
+    dexp3(T0, T3, T10);
+    dT3_dVb = T10 / Nvtmrssws * T9;
+  } else {
+    T9 = 1.0 / (C.vtssws - vbs_jct);
+    T0 = -vbs_jct / Nvtmrssws * C.vtssws * T9;
+    dT0_dVb = C.vtssws / Nvtmrssws * (T9 + vbs_jct * T9 * T9);
+    dexp3(T0, T3, T10);
+    dT3_dVb = T10 * dT0_dVb;
+  }
+  if ((C.vtsswd - vbd_jct) < (C.vtsswd * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbd_jct / Nvtmrsswd * T9;
+    dexp3(T0, T4, T10);
+    dT4_dVb = T10 / Nvtmrsswd * T9;
+  } else {
+    T9 = 1.0 / (C.vtsswd - vbd_jct);
+    T0 = -vbd_jct / Nvtmrsswd * C.vtsswd * T9;
+    dT0_dVb = C.vtsswd / Nvtmrsswd * (T9 + vbd_jct * T9 * T9);
+    dexp3(T0, T4, T10);
+    dT4_dVb = T10 * dT0_dVb;
+  }
+  if ((C.vtsswgs - vbs_jct) < (C.vtsswgs * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbs_jct / Nvtmrsswgs * T9;
+    dexp3(T0, T5, T10);
+    dT5_dVb = T10 / Nvtmrsswgs * T9;
+  } else {
+    T9 = 1.0 / (C.vtsswgs - vbs_jct);
+    T0 = -vbs_jct / Nvtmrsswgs * C.vtsswgs * T9;
+    dT0_dVb = C.vtsswgs / Nvtmrsswgs * (T9 + vbs_jct * T9 * T9);
+    dexp3(T0, T5, T10);
+    dT5_dVb = T10 * dT0_dVb;
+  }
+  if ((C.vtsswgd - vbd_jct) < (C.vtsswgd * 1e-3)) {
+    T9 = 1.0e3;
+    T0 = -vbd_jct / Nvtmrsswgd * T9;
+    dexp3(T0, T6, T10);
+    dT6_dVb = T10 / Nvtmrsswgd * T9;
+  } else {
+    T9 = 1.0 / (C.vtsswgd - vbd_jct);
+    T0 = -vbd_jct / Nvtmrsswgd * C.vtsswgd * T9;
+    dT0_dVb = C.vtsswgd / Nvtmrsswgd * (T9 + vbd_jct * T9 * T9);
+    dexp3(T0, T6, T10);
+    dT6_dVb = T10 * dT0_dVb;
+  }
+  gbs += IV(SjctTempRevSatCur) * dT1_dVb + IV(SswTempRevSatCur) * dT3_dVb
+            + IV(SswgTempRevSatCur) * dT5_dVb;
+  cbs -= IV(SjctTempRevSatCur) * (T1 - 1.0) + IV(SswTempRevSatCur) * (T3 - 1.0)
+            + IV(SswgTempRevSatCur) * (T5 - 1.0);
+  gbd += IV(DjctTempRevSatCur) * dT2_dVb + IV(DswTempRevSatCur) * dT4_dVb
+            + IV(DswgTempRevSatCur) * dT6_dVb;
+  cbd -= IV(DjctTempRevSatCur) * (T2 - 1.0) + IV(DswTempRevSatCur) * (T4 - 1.0)
+            + IV(DswgTempRevSatCur) * (T6 - 1.0);
+
+  /* mode (:8773-8788) */
+  if (vds >= 0.0) {
+    mode = 1;
+    Vds = vds;
+    Vgs = vgs;
+    Vbs = vbs;
+    Vdb_ = vds - vbs;
+  } else {
+    mode = -1;
+    Vds = -vds;
+    Vgs = vgd;
+    Vbs = vbd;
+    Vdb_ = -vbs;
+  }
+  (void)Vdb_;
+  epsrox = C.epsrox;
+  toxe = C.toxe;
+  epssub = kEpsSi;
+  /* MOSFET homotopy (:8818-8838), one gain-scale block, no staggering */
+  if (ss.artParameterFlag) {
+    const double alpha = ss.gainScale;
+    const double vgstConst = 4.5, vdsScaleMin = 0.3;
+    Vds = Vds * (ss.nltermScale * (1.0 - vdsScaleMin) + vdsScaleMin);
+    Vgs = (alpha) * Vgs + (1.0 - alpha) * vgstConst;
+  }
+
+  /* Vbseff (:8849-8868) */
+  T0 = Vbs - IV(vbsc) - 0.001;
+  T1 = sqrt(T0 * T0 - 0.004 * IV(vbsc));
+  if (T0 >= 0.0) {
+    Vbseff = IV(vbsc) + 0.5 * (T0 + T1);
+    dVbseff_dVb = 0.5 * (1.0 + T0 / T1);
+  } else {
+    T2 = -0.002 / (T1 - T0);
+    Vbseff = IV(vbsc) * (1.0 + T2);
+    dVbseff_dVb = T2 * IV(vbsc) / T1;
+  }
+  T9 = 0.95 * PP(phi);
+  T0 = T9 - Vbseff - 0.001;
+  T1 = sqrt(T0 * T0 + 0.004 * T9);
+  Vbseff = T9 - 0.5 * (T0 + T1);
+  dVbseff_dVb *= 0.5 * (1.0 + T0 / T1);
+  Phis = PP(phi) - Vbseff;
+  dPhis_dVb = -1.0;
+  sqrtPhis = sqrt(Phis);
+  dsqrtPhis_dVb = -0.5 / sqrtPhis;
+  Xdep = PP(Xdep0) * sqrtPhis / PP(sqrtPhi);
+  dXdep_dVb = (PP(Xdep0) / PP(sqrtPhi)) * dsqrtPhis_dVb;
+  Leff = PP(leff);
+  Vtm = C.vtm;
+  Vtm0 = C.vtm0;
+
+  /* threshold voltage (:8883-9052) */
+  T3 = sqrt(Xdep);
+  V0 = PP(vbi) - PP(phi);
+  T0 = PP(dvt2) * Vbseff;
+  if (T0 >= -0.5) {
+    T1 = 1.0 + T0;
+    T2 = PP(dvt2);
+  } else {
+    T4 = 1.0 / (3.0 + 8.0 * T0);
+    T1 = (1.0 + 3.0 * T0) * T4;
+    T2 = PP(dvt2) * T4 * T4;
+  }
+  lt1 = C.factor1 * T3 * T1;
+  dlt1_dVb = C.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+  T0 = PP(dvt2w) * Vbseff;
+  if (T0 >= -0.5) {
+    T1 = 1.0 + T0;
+    T2 = PP(dvt2w);
+  } else {
+    T4 = 1.0 / (3.0 + 8.0 * T0);
+    T1 = (1.0 + 3.0 * T0) * T4;
+    T2 = PP(dvt2w) * T4 * T4;
+  }
+  ltw = C.factor1 * T3 * T1;
+  dltw_dVb = C.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+  T0 = PP(dvt1) * Leff / lt1;
+  if (T0 < kExpThreshold) {
+    T1 = exp(T0);
+    T2 = T1 - 1.0;
+    T3 = T2 * T2;
+    T4 = T3 + 2.0 * T1 * kMinExp;
+    Theta0 = T1 / T4;
+    dT1_dVb = -T0 * T1 * dlt1_dVb / lt1;
+    dTheta0_dVb = dT1_dVb * (T4 - 2.0 * T1 * (T2 + kMinExp)) / T4 / T4;
+  } else {
+    Theta0 = 1.0 / (kMaxExp - 2.0);
+    dTheta0_dVb = 0.0;
+  }
+  thetavth = PP(dvt0) * Theta0;
+  Delt_vth = thetavth * V0;
+  dDelt_vth_dVb = PP(dvt0) * dTheta0_dVb * V0;
+  T0 = PP(dvt1w) * PP(weff) * Leff / ltw;
+  if (T0 < kExpThreshold) {
+    T1 = exp(T0);
+    T2 = T1 - 1.0;
+    T3 = T2 * T2;
+    T4 = T3 + 2.0 * T1 * kMinExp;
+    T5 = T1 / T4;
+    dT1_dVb = -T0 * T1 * dltw_dVb / ltw;
+    dT5_dVb = dT1_dVb * (T4 - 2.0 * T1 * (T2 + kMinExp)) / T4 / T4;
+  } else {
+    T5 = 1.0 / (kMaxExp - 2.0);
+    dT5_dVb = 0.0;
+  }
+  T0 = PP(dvt0w) * T5;
+  T2 = T0 * V0;
+  dT2_dVb = PP(dvt0w) * dT5_dVb * V0;
+  TempRatio = IV(temp) / C.tnom - 1.0;
+  T0 = sqrt(1.0 + PP(lpe0) / Leff);
+  T1 = PP(k1ox) * (T0 - 1.0) * PP(sqrtPhi) + (PP(kt1) + PP(kt1l) / Leff + PP(kt2) * Vbseff) * TempRatio;
+  Vth_NarrowW = toxe * PP(phi) / (PP(weff) + PP(w0));
+  T3 = IV(eta0) + PP(etab) * Vbseff;
+  if (T3 < 1.0e-4) {
+    T9 = 1.0 / (3.0 - 2.0e4 * T3);
+    T3 = (2.0e-4 - T3) * T9;
+    T4 = T9 * T9;
+  } else {
+    T4 = 1.0;
+  }
+  dDIBL_Sft_dVd = T3 * PP(theta0vb0);
+  DIBL_Sft = dDIBL_Sft_dVd * Vds;
+  Lpe_Vb = sqrt(1.0 + PP(lpeb) / Leff);
+  Vth = C.dtype * IV(vth0) + (PP(k1ox) * sqrtPhis - PP(k1) * PP(sqrtPhi)) * Lpe_Vb
+        - IV(k2ox) * Vbseff - Delt_vth - T2 + (PP(k3) + PP(k3b) * Vbseff) * Vth_NarrowW + T1 - DIBL_Sft;
+  dVth_dVb = Lpe_Vb * PP(k1ox) * dsqrtPhis_dVb - IV(k2ox) - dDelt_vth_dVb - dT2_dVb
+             + PP(k3b) * Vth_NarrowW - PP(etab) * Vds * PP(theta0vb0) * T4 + PP(kt2) * TempRatio;
+  dVth_dVd = -dDIBL_Sft_dVd;
+  tmp1 = epssub / Xdep;
+  nstar = C.vtm / kQ * (C.coxe + tmp1 + PP(cit));
+  tmp2 = PP(nfactor) * tmp1;
+  tmp3 = PP(cdsc) + PP(cdscb) * Vbseff + PP(cdscd) * Vds;
+  tmp4 = (tmp2 + tmp3 * Theta0 + PP(cit)) / C.coxe;
+  if (tmp4 >= -0.5) {
+    n = 1.0 + tmp4;
+    dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + PP(cdscb) * Theta0) / C.coxe;
+    dn_dVd = PP(cdscd) * Theta0 / C.coxe;
+  } else {
+    T0 = 1.0 / (3.0 + 8.0 * tmp4);
+    n = (1.0 + 3.0 * tmp4) * T0;
+    T0 *= T0;
+    dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + PP(cdscb) * Theta0) / C.coxe * T0;
+    dn_dVd = PP(cdscd) * Theta0 / C.coxe * T0;
+  }
+  if (PP(dvtp0) > 0.0) {
+    T0 = -PP(dvtp1) * Vds;
+    if (T0 < -kExpThreshold) {
+      T2 = kMinExp;
+      dT2_dVd = 0.0;
+    } else {
+      T2 = exp(T0);
+      dT2_dVd = -PP(dvtp1) * T2;
+    }
+    T3 = Leff + PP(dvtp0) * (1.0 + T2);
+    dT3_dVd = PP(dvtp0) * dT2_dVd;
+    if (C.tempMod < 2) {
+      T4 = Vtm * log(Leff / T3);
+      dT4_dVd = -Vtm * dT3_dVd / T3;
+    } else {
+      T4 = C.vtm0 * log(Leff / T3);
+      dT4_dVd = -C.vtm0 * dT3_dVd / T3;
+    }
+    dDITS_Sft_dVd = dn_dVd * T4 + n * dT4_dVd;
+    dDITS_Sft_dVb = T4 * dn_dVb;
+    Vth -= n * T4;
+    dVth_dVd -= dDITS_Sft_dVd;
+    dVth_dVb -= dDITS_Sft_dVb;
+  }
+  von = Vth;
+
+  /* poly gate depletion (:9055-9080) */
+  T0 = IV(vfb) + PP(phi);
+  T1 = kEpsSi;
+  polyDepletion(T0, PP(ngate), T1, C.coxe, vgs, vgs_eff, dvgs_eff_dvg);
+  polyDepletion(T0, PP(ngate), T1, C.coxe, vgd, vgd_eff, dvgd_eff_dvg);
+  if (mode > 0) {
+    Vgs_eff = vgs_eff;
+    dVgs_eff_dVg = dvgs_eff_dvg;
+  } else {
+    Vgs_eff = vgd_eff;
+    dVgs_eff_dVg = dvgd_eff_dvg;
+  }
+  Vgst = Vgs_eff - Vth;
+
+  /* effective Vgst (:9085-9150) */
+  T0 = n * Vtm;
+  T1 = PP(mstar) * Vgst;
+  T2 = T1 / T0;
+  if (T2 > kExpThreshold) {
+    T10 = T1;
+    dT10_dVg = PP(mstar) * dVgs_eff_dVg;
+    dT10_dVd = -dVth_dVd * PP(mstar);
+    dT10_dVb = -dVth_dVb * PP(mstar);
+  } else if (T2 < -kExpThreshold) {
+    T10 = Vtm * log(1.0 + kMinExp);
+    dT10_dVg = 0.0;
+    dT10_dVd = T10 * dn_dVd;
+    dT10_dVb = T10 * dn_dVb;
+    T10 *= n;
+  } else {
+    ExpVgst = exp(T2);
+    T3 = Vtm * log(1.0 + ExpVgst);
+    T10 = n * T3;
+    dT10_dVg = PP(mstar) * ExpVgst / (1.0 + ExpVgst);
+    dT10_dVb = T3 * dn_dVb - dT10_dVg * (dVth_dVb + Vgst * dn_dVb / n);
+    dT10_dVd = T3 * dn_dVd - dT10_dVg * (dVth_dVd + Vgst * dn_dVd / n);
+    dT10_dVg *= dVgs_eff_dVg;
+  }
+  T1 = PP(voffcbn) - (1.0 - PP(mstar)) * Vgst;
+  T2 = T1 / T0;
+  if (T2 < -kExpThreshold) {
+    T3 = C.coxe * kMinExp / PP(cdep0);
+    T9 = PP(mstar) + T3 * n;
+    dT9_dVg = 0.0;
+    dT9_dVd = dn_dVd * T3;
+    dT9_dVb = dn_dVb * T3;
+  } else if (T2 > kExpThreshold) {
+    T3 = C.coxe * kMaxExp / PP(cdep0);
+    T9 = PP(mstar) + T3 * n;
+    dT9_dVg = 0.0;
+    dT9_dVd = dn_dVd * T3;
+    dT9_dVb = dn_dVb * T3;
+  } else {
+    ExpVgst = exp(T2);
+    T3 = C.coxe / PP(cdep0);
+    T4 = T3 * ExpVgst;
+    T5 = T1 * T4 / T0;
+    T9 = PP(mstar) + n * T4;
+    dT9_dVg = T3 * (PP(mstar) - 1.0) * ExpVgst / Vtm;
+    dT9_dVb = T4 * dn_dVb - dT9_dVg * dVth_dVb - T5 * dn_dVb;
+    dT9_dVd = T4 * dn_dVd - dT9_dVg * dVth_dVd - T5 * dn_dVd;
+    dT9_dVg *= dVgs_eff_dVg;
+  }
+  Vgsteff = T10 / T9;
+  T11 = T9 * T9;
+  dVgsteff_dVg = (T9 * dT10_dVg - T10 * dT9_dVg) / T11;
+  dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
+  dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
+
+  /* effective channel geometry and series resistance (:9153-9198) */
+  T9 = sqrtPhis - PP(sqrtPhi);
+  Weff = PP(weff) - 2.0 * (PP(dwg) * Vgsteff + PP(dwb) * T9);
+  dWeff_dVg = -2.0 * PP(dwg);
+  dWeff_dVb = -2.0 * PP(dwb) * dsqrtPhis_dVb;
+  if (Weff < 2.0e-8) {
+    T0 = 1.0 / (6.0e-8 - 2.0 * Weff);
+    Weff = 2.0e-8 * (4.0e-8 - Weff) * T0;
+    T0 *= T0 * 4.0e-16;
+    dWeff_dVg *= T0;
+    dWeff_dVb *= T0;
+  }
+  if (C.rdsMod == 1) {
+    Rds = dRds_dVg = dRds_dVb = 0.0;
+  } else {
+    T0 = 1.0 + PP(prwg) * Vgsteff;
+    dT0_dVg = -PP(prwg) / T0 / T0;
+    T1 = PP(prwb) * T9;
+    dT1_dVb = PP(prwb) * dsqrtPhis_dVb;
+    T2 = 1.0 / T0 + T1;
+    T3 = T2 + sqrt(T2 * T2 + 0.01);
+    dT3_dVg = 1.0 + T2 / (T3 - T2);
+    dT3_dVb = dT3_dVg * dT1_dVb;
+    dT3_dVg *= dT0_dVg;
+    T4 = PP(rds0) * 0.5;
+    Rds = PP(rdswmin) + T3 * T4;
+    dRds_dVg = T4 * dT3_dVg;
+    dRds_dVb = T4 * dT3_dVb;
+    if (Rds > 0.0) grdsw = 1.0 / Rds;
+    else grdsw = 0.0;
+  }
+
+  /* bulk charge effect (:9201-9258) */
+  T9 = 0.5 * PP(k1ox) * Lpe_Vb / sqrtPhis;
+  T1 = T9 + IV(k2ox) - PP(k3b) * Vth_NarrowW;
+  dT1_dVb = -T9 / sqrtPhis * dsqrtPhis_dVb;
+  T9 = sqrt(PP(xj) * Xdep);
+  tmp1 = Leff + 2.0 * T9;
+  T5 = Leff / tmp1;
+  tmp2 = PP(a0) * T5;
+  tmp3 = PP(weff) + PP(b1);
+  tmp4 = PP(b0) / tmp3;
+  T2 = tmp2 + tmp4;
+  dT2_dVb = -T9 / tmp1 / Xdep * dXdep_dVb;
+  T6 = T5 * T5;
+  T7 = T5 * T6;
+  Abulk0 = 1.0 + T1 * T2;
+  dAbulk0_dVb = T1 * tmp2 * dT2_dVb + T2 * dT1_dVb;
+  T8 = PP(ags) * PP(a0) * T7;
+  dAbulk_dVg = -T1 * T8;
+  Abulk = Abulk0 + dAbulk_dVg * Vgsteff;
+  dAbulk_dVb = dAbulk0_dVb - T8 * Vgsteff * (dT1_dVb + 3.0 * T1 * dT2_dVb);
+  if (Abulk0 < 0.1) {
+    T9 = 1.0 / (3.0 - 20.0 * Abulk0);
+    Abulk0 = (0.2 - Abulk0) * T9;
+    dAbulk0_dVb *= T9 * T9;
+  }
+  if (Abulk < 0.1) {
+    T9 = 1.0 / (3.0 - 20.0 * Abulk);
+    Abulk = (0.2 - Abulk) * T9;
+    T10 = T9 * T9;
+    dAbulk_dVb *= T10;
+    dAbulk_dVg *= T10;
+  }
+  T2 = PP(keta) * Vbseff;
+  if (T2 >= -0.9) {
+    T0 = 1.0 / (1.0 + T2);
+    dT0_dVb = -PP(keta) * T0 * T0;
+  } else {
+    T1 = 1.0 / (0.8 + T2);
+    T0 = (17.0 + 20.0 * T2) * T1;
+    dT0_dVb = -PP(keta) * T1 * T1;
+  }
+  dAbulk_dVg *= T0;
+  dAbulk_dVb = dAbulk_dVb * T0 + Abulk * dT0_dVb;
+  dAbulk0_dVb = dAbulk0_dVb * T0 + Abulk0 * dT0_dVb;
+  Abulk *= T0;
+  Abulk0 *= T0;
+
+  /* mobility (:9261-9341) */
+  T14 = 0.0;
+  if (C.mobMod == 0) {
+    T0 = Vgsteff + Vth + Vth - T14;
+    T2 = PP(ua) + PP(uc) * Vbseff;
+    T3 = T0 / toxe;
+    T12 = sqrt(Vth * Vth + 0.0001);
+    T9 = 1.0 / (Vgsteff + 2 * T12);
+    T10 = T9 * toxe;
+    T8 = PP(ud) * T10 * T10 * Vth;
+    T6 = T8 * Vth;
+    T5 = T3 * (T2 + PP(ub) * T3) + T6;
+    T7 = -2.0 * T6 * T9;
+    T11 = T7 * Vth / T12;
+    dDenomi_dVg = (T2 + 2.0 * PP(ub) * T3) / toxe;
+    T13 = 2.0 * (dDenomi_dVg + T11 + T8);
+    dDenomi_dVd = T13 * dVth_dVd;
+    dDenomi_dVb = T13 * dVth_dVb + PP(uc) * T3;
+    dDenomi_dVg += T7;
+  } else if (C.mobMod == 1) {
+    T0 = Vgsteff + Vth + Vth - T14;
+    T2 = 1.0 + PP(uc) * Vbseff;
+    T3 = T0 / toxe;
+    T4 = T3 * (PP(ua) + PP(ub) * T3);
+    T12 = sqrt(Vth * Vth + 0.0001);
+    T9 = 1.0 / (Vgsteff + 2 * T12);
+    T10 = T9 * toxe;
+    T8 = PP(ud) * T10 * T10 * Vth;
+    T6 = T8 * Vth;
+    T5 = T4 * T2 + T6;
+    T7 = -2.0 * T6 * T9;
+    T11 = T7 * Vth / T12;
+    dDenomi_dVg = (PP(ua) + 2.0 * PP(ub) * T3) * T2 / toxe;
+    T13 = 2.0 * (dDenomi_dVg + T11 + T8);
+    dDenomi_dVd = T13 * dVth_dVd;
+    dDenomi_dVb = T13 * dVth_dVb + PP(uc) * T4;
+    dDenomi_dVg += T7;
+  } else {
+    T0 = (Vgsteff + IV(vtfbphi1)) / toxe;
+    T1 = exp(PP(eu) * log(T0));
+    dT1_dVg = T1 * PP(eu) / T0 / toxe;
+    T2 = PP(ua) + PP(uc) * Vbseff;
+    T3 = T0 / toxe;
+    T12 = sqrt(Vth * Vth + 0.0001);
+    T9 = 1.0 / (Vgsteff + 2 * T12);
+    T10 = T9 * toxe;
+    T8 = PP(ud) * T10 * T10 * Vth;
+    T6 = T8 * Vth;
+    T5 = T1 * T2 + T6;
+    T7 = -2.0 * T6 * T9;
+    T11 = T7 * Vth / T12;
+    dDenomi_dVg = T2 * dT1_dVg + T7;
+    T13 = 2.0 * (T11 + T8);
+    dDenomi_dVd = T13 * dVth_dVd;
+    dDenomi_dVb = T13 * dVth_dVb + T1 * PP(uc);
+  }
+  if (T5 >= -0.8) {
+    Denomi = 1.0 + T5;
+  } else {
+    T9 = 1.0 / (7.0 + 10.0 * T5);
+    Denomi = (0.6 + T5) * T9;
+    T9 *= T9;
+    dDenomi_dVg *= T9;
+    dDenomi_dVd *= T9;
+    dDenomi_dVb *= T9;
+  }
+  ueff = IV(u0temp) / Denomi;
+  T9 = -ueff / Denomi;
+  dueff_dVg = T9 * dDenomi_dVg;
+  dueff_dVd = T9 * dDenomi_dVd;
+  dueff_dVb = T9 * dDenomi_dVb;
+
+  /* saturation field and Vdsat (:9344-9450) */
+  WVCox = Weff * IV(vsattemp) * C.coxe;
+  WVCoxRds = WVCox * Rds;
+  Esat = 2.0 * IV(vsattemp) / ueff;
+  EsatL = Esat * Leff;
+  T0 = -EsatL / ueff;
+  dEsatL_dVg = T0 * dueff_dVg;
+  dEsatL_dVd = T0 * dueff_dVd;
+  dEsatL_dVb = T0 * dueff_dVb;
+  a1 = PP(a1);
+  if (a1 == 0.0) {
+    Lambda = PP(a2);
+    dLambda_dVg = 0.0;
+  } else if (a1 > 0.0) {
+    T0 = 1.0 - PP(a2);
+    T1 = T0 - PP(a1) * Vgsteff - 0.0001;
+    T2 = sqrt(T1 * T1 + 0.0004 * T0);
+    Lambda = PP(a2) + T0 - 0.5 * (T1 + T2);
+    dLambda_dVg = 0.5 * PP(a1) * (1.0 + T1 / T2);
+  } else {
+    T1 = PP(a2) + PP(a1) * Vgsteff - 0.0001;
+    T2 = sqrt(T1 * T1 + 0.0004 * PP(a2));
+    Lambda = 0.5 * (T1 + T2);
+    dLambda_dVg = 0.5 * PP(a1) * (1.0 + T1 / T2);
+  }
+  Vgst2Vtm = Vgsteff + 2.0 * Vtm;
+  if (Rds > 0) {
+    tmp2 = dRds_dVg / Rds + dWeff_dVg / Weff;
+    tmp3 = dRds_dVb / Rds + dWeff_dVb / Weff;
+  } else {
+    tmp2 = dWeff_dVg / Weff;
+    tmp3 = dWeff_dVb / Weff;
+  }
+  if ((Rds == 0.0) && (Lambda == 1.0)) {
+    T0 = 1.0 / (Abulk * EsatL + Vgst2Vtm);
+    tmp1 = 0.0;
+    T1 = T0 * T0;
+    T2 = Vgst2Vtm * T0;
+    T3 = EsatL * Vgst2Vtm;
+    Vdsat = T3 * T0;
+    dT0_dVg = -(Abulk * dEsatL_dVg + EsatL * dAbulk_dVg + 1.0) * T1;
+    dT0_dVd = -(Abulk * dEsatL_dVd) * T1;
+    dT0_dVb = -(Abulk * dEsatL_dVb + dAbulk_dVb * EsatL) * T1;
+    dVdsat_dVg = T3 * dT0_dVg + T2 * dEsatL_dVg + EsatL * T0;
+    dVdsat_dVd = T3 * dT0_dVd + T2 * dEsatL_dVd;
+    dVdsat_dVb = T3 * dT0_dVb + T2 * dEsatL_dVb;
+  } else {
+    tmp1 = dLambda_dVg / (Lambda * Lambda);
+    T9 = Abulk * WVCoxRds;
+    T8 = Abulk * T9;
+    T7 = Vgst2Vtm * T9;
+    T6 = Vgst2Vtm * WVCoxRds;
+    T0 = 2.0 * Abulk * (T9 - 1.0 + 1.0 / Lambda);
+    dT0_dVg = 2.0 * (T8 * tmp2 - Abulk * tmp1 + (2.0 * T9 + 1.0 / Lambda - 1.0) * dAbulk_dVg);
+    dT0_dVb = 2.0 * (T8 * (2.0 / Abulk * dAbulk_dVb + tmp3) + (1.0 / Lambda - 1.0) * dAbulk_dVb);
+    dT0_dVd = 0.0;
+    T1 = Vgst2Vtm * (2.0 / Lambda - 1.0) + Abulk * EsatL + 3.0 * T7;
+    dT1_dVg = (2.0 / Lambda - 1.0) - 2.0 * Vgst2Vtm * tmp1 + Abulk * dEsatL_dVg
+              + EsatL * dAbulk_dVg + 3.0 * (T9 + T7 * tmp2 + T6 * dAbulk_dVg);
+    dT1_dVb = Abulk * dEsatL_dVb + EsatL * dAbulk_dVb + 3.0 * (T6 * dAbulk_dVb + T7 * tmp3);
+    dT1_dVd = Abulk * dEsatL_dVd;
+    T2 = Vgst2Vtm * (EsatL + 2.0 * T6);
+    dT2_dVg = EsatL + Vgst2Vtm * dEsatL_dVg + T6 * (4.0 + 2.0 * Vgst2Vtm * tmp2);
+    dT2_dVb = Vgst2Vtm * (dEsatL_dVb + 2.0 * T6 * tmp3);
+    dT2_dVd = Vgst2Vtm * dEsatL_dVd;
+    T3 = sqrt(T1 * T1 - 2.0 * T0 * T2);
+    Vdsat = (T1 - T3) / T0;
+    dT3_dVg = (T1 * dT1_dVg - 2.0 * (T0 * dT2_dVg + T2 * dT0_dVg)) / T3;
+    dT3_dVd = (T1 * dT1_dVd - 2.0 * (T0 * dT2_dVd + T2 * dT0_dVd)) / T3;
+    dT3_dVb = (T1 * dT1_dVb - 2.0 * (T0 * dT2_dVb + T2 * dT0_dVb)) / T3;
+    dVdsat_dVg = (dT1_dVg - (T1 * dT1_dVg - dT0_dVg * T2 - T0 * dT2_dVg) / T3 - Vdsat * dT0_dVg) / T0;
+    dVdsat_dVb = (dT1_dVb - (T1 * dT1_dVb - dT0_dVb * T2 - T0 * dT2_dVb) / T3 - Vdsat * dT0_dVb) / T0;
+    dVdsat_dVd = (dT1_dVd - (T1 * dT1_dVd - T0 * dT2_dVd) / T3) / T0;
+  }
+  vdsat = Vdsat;
+
+  /* effective Vds (:9453-9497) */
+  T1 = Vdsat - Vds - PP(delta);
+  dT1_dVg = dVdsat_dVg;
+  dT1_dVd = dVdsat_dVd - 1.0;
+  dT1_dVb = dVdsat_dVb;
+  T2 = sqrt(T1 * T1 + 4.0 * PP(delta) * Vdsat);
+  T0 = T1 / T2;
+  T9 = 2.0 * PP(delta);
+  T3 = T9 / T2;
+  dT2_dVg = T0 * dT1_dVg + T3 * dVdsat_dVg;
+  dT2_dVd = T0 * dT1_dVd + T3 * dVdsat_dVd;
+  dT2_dVb = T0 * dT1_dVb + T3 * dVdsat_dVb;
+  if (T1 >= 0.0) {
+    Vdseff_ = Vdsat - 0.5 * (T1 + T2);
+    dVdseff_dVg = dVdsat_dVg - 0.5 * (dT1_dVg + dT2_dVg);
+    dVdseff_dVd = dVdsat_dVd - 0.5 * (dT1_dVd + dT2_dVd);
+    dVdseff_dVb = dVdsat_dVb - 0.5 * (dT1_dVb + dT2_dVb);
+  } else {
+    T4 = T9 / (T2 - T1);
+    T5 = 1.0 - T4;
+    T6 = Vdsat * T4 / (T2 - T1);
+    Vdseff_ = Vdsat * T5;
+    dVdseff_dVg = dVdsat_dVg * T5 + T6 * (dT2_dVg - dT1_dVg);
+    dVdseff_dVd = dVdsat_dVd * T5 + T6 * (dT2_dVd - dT1_dVd);
+    dVdseff_dVb = dVdsat_dVb * T5 + T6 * (dT2_dVb - dT1_dVb);
+  }
+  if (Vds == 0.0) {
+    Vdseff_ = 0.0;
+    dVdseff_dVg = 0.0;
+    dVdseff_dVb = 0.0;
+  }
+  if (Vdseff_ > Vds) Vdseff_ = Vds;
+  diffVds = Vds - Vdseff_;
+  Vdseff = Vdseff_;
+
+  /* velocity overshoot (:9500-9540) */
+  if ((C.lambdaGiven) && (C.lambda > 0.0)) {
+    T1 = Leff * ueff;
+    T2 = PP(lambda) / T1;
+    T3 = -T2 / T1 * Leff;
+    dT2_dVd = T3 * dueff_dVd;
+    dT2_dVg = T3 * dueff_dVg;
+    dT2_dVb = T3 * dueff_dVb;
+    T5 = 1.0 / (Esat * PP(litl));
+    T4 = -T5 / EsatL;
+    dT5_dVg = dEsatL_dVg * T4;
+    dT5_dVd = dEsatL_dVd * T4;
+    dT5_dVb = dEsatL_dVb * T4;
+    T6 = 1.0 + diffVds * T5;
+    dT6_dVg = dT5_dVg * diffVds - dVdseff_dVg * T5;
+    dT6_dVd = dT5_dVd * diffVds + (1.0 - dVdseff_dVd) * T5;
+    dT6_dVb = dT5_dVb * diffVds - dVdseff_dVb * T5;
+    T7 = 2.0 / (T6 * T6 + 1.0);
+    T8 = 1.0 - T7;
+    T9 = T6 * T7 * T7;
+    dT8_dVg = T9 * dT6_dVg;
+    dT8_dVd = T9 * dT6_dVd;
+    dT8_dVb = T9 * dT6_dVb;
+    T10 = 1.0 + T2 * T8;
+    dT10_dVg = dT2_dVg * T8 + T2 * dT8_dVg;
+    dT10_dVd = dT2_dVd * T8 + T2 * dT8_dVd;
+    dT10_dVb = dT2_dVb * T8 + T2 * dT8_dVb;
+    if (T10 == 1.0) dT10_dVg = dT10_dVd = dT10_dVb = 0.0;
+    dEsatL_dVg *= T10;
+    dEsatL_dVg += EsatL * dT10_dVg;
+    dEsatL_dVd *= T10;
+    dEsatL_dVd += EsatL * dT10_dVd;
+    dEsatL_dVb *= T10;
+    dEsatL_dVb += EsatL * dT10_dVb;
+    EsatL *= T10;
+  }
+
+  /* Vasat (:9543-9565) */
+  tmp4 = 1.0 - 0.5 * Abulk * Vdsat / Vgst2Vtm;
+  T9 = WVCoxRds * Vgsteff;
+  T8 = T9 / Vgst2Vtm;
+  T0 = EsatL + Vdsat + 2.0 * T9 * tmp4;
+  T7 = 2.0 * WVCoxRds * tmp4;
+  dT0_dVg = dEsatL_dVg + dVdsat_dVg + T7 * (1.0 + tmp2 * Vgsteff)
+            - T8 * (Abulk * dVdsat_dVg - Abulk * Vdsat / Vgst2Vtm + Vdsat * dAbulk_dVg);
+  dT0_dVb = dEsatL_dVb + dVdsat_dVb + T7 * tmp3 * Vgsteff
+            - T8 * (dAbulk_dVb * Vdsat + Abulk * dVdsat_dVb);
+  dT0_dVd = dEsatL_dVd + dVdsat_dVd - T8 * Abulk * dVdsat_dVd;
+  T9 = WVCoxRds * Abulk;
+  T1 = 2.0 / Lambda - 1.0 + T9;
+  dT1_dVg = -2.0 * tmp1 + WVCoxRds * (Abulk * tmp2 + dAbulk_dVg);
+  dT1_dVb = dAbulk_dVb * WVCoxRds + T9 * tmp3;
+  Vasat = T0 / T1;
+  dVasat_dVg = (dT0_dVg - Vasat * dT1_dVg) / T1;
+  dVasat_dVb = (dT0_dVb - Vasat * dT1_dVb) / T1;
+  dVasat_dVd = dT0_dVd / T1;
+
+  /* effective oxide capacitance, beta, Idl (:9568-9622) */
+  tmp1 = IV(vtfbphi2);
+  tmp2 = 2.0e8 * C.toxp;
+  dT0_dVg = 1.0 / tmp2;
+  T0 = (Vgsteff + tmp1) * dT0_dVg;
+  tmp3 = exp(C.bdos * 0.7 * log(T0));
+  T1 = 1.0 + tmp3;
+  T2 = C.bdos * 0.7 * tmp3 / T0;
+  Tcen = C.ados * 1.9e-9 / T1;
+  dTcen_dVg = -Tcen * T2 * dT0_dVg / T1;
+  Coxeff = epssub * C.coxp / (epssub + C.coxp * Tcen);
+  dCoxeff_dVg = -Coxeff * Coxeff * dTcen_dVg / epssub;
+  CoxeffWovL = Coxeff * Weff / Leff;
+  beta = ueff * CoxeffWovL;
+  T3 = ueff / Leff;
+  dbeta_dVg = CoxeffWovL * dueff_dVg + T3 * (Weff * dCoxeff_dVg + Coxeff * dWeff_dVg);
+  dbeta_dVd = CoxeffWovL * dueff_dVd;
+  dbeta_dVb = CoxeffWovL * dueff_dVb + T3 * Coxeff * dWeff_dVb;
+  AbovVgst2Vtm = Abulk / Vgst2Vtm;
+  T0 = 1.0 - 0.5 * Vdseff_ * AbovVgst2Vtm;
+  dT0_dVg = -0.5 * (Abulk * dVdseff_dVg - Abulk * Vdseff_ / Vgst2Vtm + Vdseff_ * dAbulk_dVg) / Vgst2Vtm;
+  dT0_dVd = -0.5 * Abulk * dVdseff_dVd / Vgst2Vtm;
+  dT0_dVb = -0.5 * (Abulk * dVdseff_dVb + dAbulk_dVb * Vdseff_) / Vgst2Vtm;
+  fgche1 = Vgsteff * T0;
+  dfgche1_dVg = Vgsteff * dT0_dVg + T0;
+  dfgche1_dVd = Vgsteff * dT0_dVd;
+  dfgche1_dVb = Vgsteff * dT0_dVb;
+  T9 = Vdseff_ / EsatL;
+  fgche2 = 1.0 + T9;
+  dfgche2_dVg = (dVdseff_dVg - T9 * dEsatL_dVg) / EsatL;
+  dfgche2_dVd = (dVdseff_dVd - T9 * dEsatL_dVd) / EsatL;
+  dfgche2_dVb = (dVdseff_dVb - T9 * dEsatL_dVb) / EsatL;
+  gche = beta * fgche1 / fgche2;
+  dgche_dVg = (beta * dfgche1_dVg + fgche1 * dbeta_dVg - gche * dfgche2_dVg) / fgche2;
+  dgche_dVd = (beta * dfgche1_dVd + fgche1 * dbeta_dVd - gche * dfgche2_dVd) / fgche2;
+  dgche_dVb = (beta * dfgche1_dVb + fgche1 * dbeta_dVb - gche * dfgche2_dVb) / fgche2;
+  T0 = 1.0 + gche * Rds;
+  Idl = gche / T0;
+  T1 = (1.0 - Idl * Rds) / T0;
+  T2 = Idl * Idl;
+  dIdl_dVg = T1 * dgche_dVg - T2 * dRds_dVg;
+  dIdl_dVd = T1 * dgche_dVd;
+  dIdl_dVb = T1 * dgche_dVb - T2 * dRds_dVb;
+
+  /* output resistance: CLM, DIBL, DITS, SCBE (:9625-9832) */
+  if (PP(fprout) <= 0.0) {
+    FP = 1.0;
+    dFP_dVg = 0.0;
+  } else {
+    T9 = PP(fprout) * sqrt(Leff) / Vgst2Vtm;
+    FP = 1.0 / (1.0 + T9);
+    dFP_dVg = FP * FP * T9 / Vgst2Vtm;
+  }
+  T8 = PP(pvag) / EsatL;
+  T9 = T8 * Vgsteff;
+  if (T9 > -0.9) {
+    PvagTerm = 1.0 + T9;
+    dPvagTerm_dVg = T8 * (1.0 - Vgsteff * dEsatL_dVg / EsatL);
+    dPvagTerm_dVb = -T9 * dEsatL_dVb / EsatL;
+    dPvagTerm_dVd = -T9 * dEsatL_dVd / EsatL;
+  } else {
+    T4 = 1.0 / (17.0 + 20.0 * T9);
+    PvagTerm = (0.8 + T9) * T4;
+    T4 *= T4;
+    dPvagTerm_dVg = T8 * (1.0 - Vgsteff * dEsatL_dVg / EsatL) * T4;
+    T9 *= T4 / EsatL;
+    dPvagTerm_dVb = -T9 * dEsatL_dVb;
+    dPvagTerm_dVd = -T9 * dEsatL_dVd;
+  }
+  if ((PP(pclm) > kMinExp) && (diffVds > 1.0e-10)) {
+    T0 = 1.0 + Rds * Idl;
+    dT0_dVg = dRds_dVg * Idl + Rds * dIdl_dVg;
+    dT0_dVd = Rds * dIdl_dVd;
+    dT0_dVb = dRds_dVb * Idl + Rds * dIdl_dVb;
+    T2 = Vdsat / Esat;
+    T1 = Leff + T2;
+    dT1_dVg = (dVdsat_dVg - T2 * dEsatL_dVg / Leff) / Esat;
+    dT1_dVd = (dVdsat_dVd - T2 * dEsatL_dVd / Leff) / Esat;
+    dT1_dVb = (dVdsat_dVb - T2 * dEsatL_dVb / Leff) / Esat;
+    Cclm = FP * PvagTerm * T0 * T1 / (PP(pclm) * PP(litl));
+    dCclm_dVg = Cclm * (dFP_dVg / FP + dPvagTerm_dVg / PvagTerm + dT0_dVg / T0 + dT1_dVg / T1);
+    dCclm_dVb = Cclm * (dPvagTerm_dVb / PvagTerm + dT0_dVb / T0 + dT1_dVb / T1);
+    dCclm_dVd = Cclm * (dPvagTerm_dVd / PvagTerm + dT0_dVd / T0 + dT1_dVd / T1);
+    VACLM = Cclm * diffVds;
+    dVACLM_dVg = dCclm_dVg * diffVds - dVdseff_dVg * Cclm;
+    dVACLM_dVb = dCclm_dVb * diffVds - dVdseff_dVb * Cclm;
+    dVACLM_dVd = dCclm_dVd * diffVds + (1.0 - dVdseff_dVd) * Cclm;
+  } else {
+    VACLM = Cclm = kMaxExp;
+    dVACLM_dVd = dVACLM_dVg = dVACLM_dVb = 0.0;
+    dCclm_dVd = dCclm_dVg = dCclm_dVb = 0.0;
+  }
+  if (PP(thetaRout) > kMinExp) {
+    T8 = Abulk * Vdsat;
+    T0 = Vgst2Vtm * T8;
+    dT0_dVg = Vgst2Vtm * Abulk * dVdsat_dVg + T8 + Vgst2Vtm * Vdsat * dAbulk_dVg;
+    dT0_dVb = Vgst2Vtm * (dAbulk_dVb * Vdsat + Abulk * dVdsat_dVb);
+    dT0_dVd = Vgst2Vtm * Abulk * dVdsat_dVd;
+    T1 = Vgst2Vtm + T8;
+    dT1_dVg = 1.0 + Abulk * dVdsat_dVg + Vdsat * dAbulk_dVg;
+    dT1_dVb = Abulk * dVdsat_dVb + dAbulk_dVb * Vdsat;
+    dT1_dVd = Abulk * dVdsat_dVd;
+    T9 = T1 * T1;
+    T2 = PP(thetaRout);
+    VADIBL = (Vgst2Vtm - T0 / T1) / T2;
+    dVADIBL_dVg = (1.0 - dT0_dVg / T1 + T0 * dT1_dVg / T9) / T2;
+    dVADIBL_dVb = (-dT0_dVb / T1 + T0 * dT1_dVb / T9) / T2;
+    dVADIBL_dVd = (-dT0_dVd / T1 + T0 * dT1_dVd / T9) / T2;
+    T7 = PP(pdiblb) * Vbseff;
+    if (T7 >= -0.9) {
+      T3 = 1.0 / (1.0 + T7);
+      VADIBL *= T3;
+      dVADIBL_dVg *= T3;
+      dVADIBL_dVb = (dVADIBL_dVb - VADIBL * PP(pdiblb)) * T3;
+      dVADIBL_dVd *= T3;
+    } else {
+      T4 = 1.0 / (0.8 + T7);
+      T3 = (17.0 + 20.0 * T7) * T4;
+      dVADIBL_dVg *= T3;
+      dVADIBL_dVb = dVADIBL_dVb * T3 - VADIBL * PP(pdiblb) * T4 * T4;
+      dVADIBL_dVd *= T3;
+      VADIBL *= T3;
+    }
+    dVADIBL_dVg = dVADIBL_dVg * PvagTerm + VADIBL * dPvagTerm_dVg;
+    dVADIBL_dVb = dVADIBL_dVb * PvagTerm + VADIBL * dPvagTerm_dVb;
+    dVADIBL_dVd = dVADIBL_dVd * PvagTerm + VADIBL * dPvagTerm_dVd;
+    VADIBL *= PvagTerm;
+  } else {
+    VADIBL = kMaxExp;
+    dVADIBL_dVd = dVADIBL_dVg = dVADIBL_dVb = 0.0;
+  }
+  Va = Vasat + VACLM;
+  dVa_dVg = dVasat_dVg + dVACLM_dVg;
+  dVa_dVb = dVasat_dVb + dVACLM_dVb;
+  dVa_dVd = dVasat_dVd + dVACLM_dVd;
+  T0 = PP(pditsd) * Vds;
+  if (T0 > kExpThreshold) {
+    T1 = kMaxExp;
+    dT1_dVd = 0;
+  } else {
+    T1 = exp(T0);
+    dT1_dVd = T1 * PP(pditsd);
+  }
+  if (PP(pdits) > kMinExp) {
+    T2 = 1.0 + C.pditsl * Leff;
+    VADITS = (1.0 + T2 * T1) / PP(pdits);
+    dVADITS_dVg = VADITS * dFP_dVg;
+    dVADITS_dVd = FP * T2 * dT1_dVd / PP(pdits);
+    VADITS *= FP;
+  } else {
+    VADITS = kMaxExp;
+    dVADITS_dVg = dVADITS_dVd = 0;
+  }
+  if (PP(pscbe2) > 0.0) {
+    if (diffVds > PP(pscbe1) * PP(litl) / kExpThreshold) {
+      T0 = PP(pscbe1) * PP(litl) / diffVds;
+      VASCBE = Leff * exp(T0) / PP(pscbe2);
+      T1 = T0 * VASCBE / diffVds;
+      dVASCBE_dVg = T1 * dVdseff_dVg;
+      dVASCBE_dVd = -T1 * (1.0 - dVdseff_dVd);
+      dVASCBE_dVb = T1 * dVdseff_dVb;
+    } else {
+      VASCBE = kMaxExp * Leff / PP(pscbe2);
+      dVASCBE_dVg = dVASCBE_dVd = dVASCBE_dVb = 0.0;
+    }
+  } else {
+    VASCBE = kMaxExp;
+    dVASCBE_dVg = dVASCBE_dVd = dVASCBE_dVb = 0.0;
+  }
+  T9 = diffVds / VADIBL;
+  T0 = 1.0 + T9;
+  Idsa = Idl * T0;
+  dIdsa_dVg = T0 * dIdl_dVg - Idl * (dVdseff_dVg + T9 * dVADIBL_dVg) / VADIBL;
+  dIdsa_dVd = T0 * dIdl_dVd + Idl * (1.0 - dVdseff_dVd - T9 * dVADIBL_dVd) / VADIBL;
+  dIdsa_dVb = T0 * dIdl_dVb - Idl * (dVdseff_dVb + T9 * dVADIBL_dVb) / VADIBL;
+  T9 = diffVds / VADITS;
+  T0 = 1.0 + T9;
+  dIdsa_dVg = T0 * dIdsa_dVg - Idsa * (dVdseff_dVg + T9 * dVADITS_dVg) / VADITS;
+  dIdsa_dVd = T0 * dIdsa_dVd + Idsa * (1.0 - dVdseff_dVd - T9 * dVADITS_dVd) / VADITS;
+  dIdsa_dVb = T0 * dIdsa_dVb - Idsa * dVdseff_dVb / VADITS;
+  Idsa *= T0;
+  T0 = log(Va / Vasat);
+  dT0_dVg = dVa_dVg / Va - dVasat_dVg / Vasat;
+  dT0_dVb = dVa_dVb / Va - dVasat_dVb / Vasat;
+  dT0_dVd = dVa_dVd / Va - dVasat_dVd / Vasat;
+  T1 = T0 / Cclm;
+  T9 = 1.0 + T1;
+  dT9_dVg = (dT0_dVg - T1 * dCclm_dVg) / Cclm;
+  dT9_dVb = (dT0_dVb - T1 * dCclm_dVb) / Cclm;
+  dT9_dVd = (dT0_dVd - T1 * dCclm_dVd) / Cclm;
+  dIdsa_dVg = dIdsa_dVg * T9 + Idsa * dT9_dVg;
+  dIdsa_dVb = dIdsa_dVb * T9 + Idsa * dT9_dVb;
+  dIdsa_dVd = dIdsa_dVd * T9 + Idsa * dT9_dVd;
+  Idsa *= T9;
+
+  /* substrate current (:9835-9877) */
+  tmp = PP(alpha0) + PP(alpha1) * Leff;
+  if ((tmp <= 0.0) || (PP(beta0) <= 0.0)) {
+    Isub = Gbd = Gbb = Gbg = 0.0;
+  } else {
+    T2 = tmp / Leff;
+    if (diffVds > PP(beta0) / kExpThreshold) {
+      T0 = -PP(beta0) / diffVds;
+      T1 = T2 * diffVds * exp(T0);
+      T3 = T1 / diffVds * (T0 - 1.0);
+      dT1_dVg = T3 * dVdseff_dVg;
+      dT1_dVd = T3 * (dVdseff_dVd - 1.0);
+      dT1_dVb = T3 * dVdseff_dVb;
+    } else {
+      T3 = T2 * kMinExp;
+      T1 = T3 * diffVds;
+      dT1_dVg = -T3 * dVdseff_dVg;
+      dT1_dVd = T3 * (1.0 - dVdseff_dVd);
+      dT1_dVb = -T3 * dVdseff_dVb;
+    }
+    T4 = Idsa * Vdseff_;
+    Isub = T1 * T4;
+    Gbg = T1 * (dIdsa_dVg * Vdseff_ + Idsa * dVdseff_dVg) + T4 * dT1_dVg;
+    Gbd = T1 * (dIdsa_dVd * Vdseff_ + Idsa * dVdseff_dVd) + T4 * dT1_dVd;
+    Gbb = T1 * (dIdsa_dVb * Vdseff_ + Idsa * dVdseff_dVb) + T4 * dT1_dVb;
+    Gbd += Gbg * dVgsteff_dVd;
+    Gbb += Gbg * dVgsteff_dVb;
+    Gbg *= dVgsteff_dVg;
+    Gbb *= dVbseff_dVb;
+  }
+  csub = Isub;
+  gbbs = Gbb;
+  gbgs = Gbg;
+  gbds = Gbd;
+
+  /* drain current with SCBE (:9880-9902) */
+  T9 = diffVds / VASCBE;
+  T0 = 1.0 + T9;
+  Ids = Idsa * T0;
+  Gm_ = T0 * dIdsa_dVg - Idsa * (dVdseff_dVg + T9 * dVASCBE_dVg) / VASCBE;
+  Gds = T0 * dIdsa_dVd + Idsa * (1.0 - dVdseff_dVd - T9 * dVASCBE_dVd) / VASCBE;
+  Gmb = T0 * dIdsa_dVb - Idsa * (dVdseff_dVb + T9 * dVASCBE_dVb) / VASCBE;
+  tmp1 = Gds + Gm_ * dVgsteff_dVd;
+  tmp2 = Gmb + Gm_ * dVgsteff_dVb;
+  tmp3 = Gm_;
+  Gm_ = (Ids * dVdseff_dVg + Vdseff_ * tmp3) * dVgsteff_dVg;
+  Gds = Ids * (dVdseff_dVd + dVdseff_dVg * dVgsteff_dVd) + Vdseff_ * tmp1;
+  Gmb = (Ids * (dVdseff_dVb + dVdseff_dVg * dVgsteff_dVb) + Vdseff_ * tmp2) * dVbseff_dVb;
+  cdrain = Ids * Vdseff_;
+
+  /* source-end velocity limit (:9905-9945) */
+  if ((C.vtlGiven) && (C.vtl > 0.0)) {
+    T12 = 1.0 / Leff / CoxeffWovL;
+    T11 = T12 / Vgsteff;
+    T10 = -T11 / Vgsteff;
+    vs = cdrain * T11;
+    dvs_dVg = Gm_ * T11 + cdrain * T10 * dVgsteff_dVg;
+    dvs_dVd = Gds * T11 + cdrain * T10 * dVgsteff_dVd;
+    dvs_dVb = Gmb * T11 + cdrain * T10 * dVgsteff_dVb;
+    T0 = 2 * kMM;
+    T1 = vs / (PP(vtl) * PP(tfactor));
+    if (T1 > 0.0) {
+      T2 = 1.0 + exp(T0 * log(T1));
+      T3 = (T2 - 1.0) * T0 / vs;
+      Fsevl = 1.0 / exp(log(T2) / T0);
+      dT2_dVg = T3 * dvs_dVg;
+      dT2_dVd = T3 * dvs_dVd;
+      dT2_dVb = T3 * dvs_dVb;
+      T4 = -1.0 / T0 * Fsevl / T2;
+      dFsevl_dVg = T4 * dT2_dVg;
+      dFsevl_dVd = T4 * dT2_dVd;
+      dFsevl_dVb = T4 * dT2_dVb;
+    } else {
+      Fsevl = 1.0;
+      dFsevl_dVg = 0.0;
+      dFsevl_dVd = 0.0;
+      dFsevl_dVb = 0.0;
+    }
+    Gm_ *= Fsevl;
+    Gm_ += cdrain * dFsevl_dVg;
+    Gmb *= Fsevl;
+    Gmb += cdrain * dFsevl_dVb;
+    Gds *= Fsevl;
+    Gds += cdrain * dFsevl_dVd;
+    cdrain *= Fsevl;
+  }
+  gds = Gds;
+  gm = Gm_;
+  gmbs = Gmb;
+  IdovVds = Ids;
+  if (IdovVds <= 1.0e-9) IdovVds = 1.0e-9;
+  /* rgateMod <= 1 and no NQS: no gate resistance current conductances */
+  gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+  /* rdsMod = 0 (:10063-10069) */
+  gstot = gstotd = gstotg = 0.0;
+  gstots = gstotb = 0.0;
+  gdtot = gdtotd = gdtotg = 0.0;
+  gdtots = gdtotb = 0.0;
+
+  /* GIDL (:10074-10126) */
+  T0 = 3.0 * toxe;
+  T1 = (vds - vgs_eff - PP(egidl)) / T0;
+  if ((PP(agidl) <= 0.0) || (PP(bgidl) <= 0.0) || (T1 <= 0.0) || (PP(cgidl) <= 0.0) || (vbd > 0.0)) {
+    Igidl = Ggidld = Ggidlg = Ggidlb = 0.0;
+  } else {
+    dT1_dVd = 1.0 / T0;
+    dT1_dVg = -dvgs_eff_dvg * dT1_dVd;
+    T2 = PP(bgidl) / T1;
+    if (T2 < 100.0) {
+      Igidl = PP(agidl) * PP(weffCJ) * T1 * exp(-T2);
+      T3 = Igidl * (1.0 + T2) / T1;
+      Ggidld = T3 * dT1_dVd;
+      Ggidlg = T3 * dT1_dVg;
+    } else {
+      Igidl = PP(agidl) * PP(weffCJ) * 3.720075976e-44;
+      Ggidld = Igidl * dT1_dVd;
+      Ggidlg = Igidl * dT1_dVg;
+      Igidl *= T1;
+    }
+    T4 = vbd * vbd;
+    T5 = -vbd * T4;
+    T6 = PP(cgidl) + T5;
+    T7 = T5 / T6;
+    T8 = 3.0 * PP(cgidl) * T4 / T6 / T6;
+    Ggidld = Ggidld * T7 + Igidl * T8;
+    Ggidlg = Ggidlg * T7;
+    Ggidlb = -Igidl * T8;
+    Igidl *= T7;
+  }
+  ggidld = Ggidld;
+  ggidlg = Ggidlg;
+  ggidlb = Ggidlb;
+  /* GISL (:10132-10179) */
+  T1 = (-vds - vgd_eff - PP(egisl)) / T0;
+  if ((PP(agisl) <= 0.0) || (PP(bgisl) <= 0.0) || (T1 <= 0.0) || (PP(cgisl) <= 0.0) || (vbs > 0.0)) {
+    Igisl = Ggisls = Ggislg = Ggislb = 0.0;
+  } else {
+    dT1_dVd = 1.0 / T0;
+    dT1_dVg = -dvgd_eff_dvg * dT1_dVd;
+    T2 = PP(bgisl) / T1;
+    if (T2 < 100.0) {
+      Igisl = PP(agisl) * PP(weffCJ) * T1 * exp(-T2);
+      T3 = Igisl * (1.0 + T2) / T1;
+      Ggisls = T3 * dT1_dVd;
+      Ggislg = T3 * dT1_dVg;
+    } else {
+      Igisl = PP(agisl) * PP(weffCJ) * 3.720075976e-44;
+      Ggisls = Igisl * dT1_dVd;
+      Ggislg = Igisl * dT1_dVg;
+      Igisl *= T1;
+    }
+    T4 = vbs * vbs;
+    T5 = -vbs * T4;
+    T6 = PP(cgisl) + T5;
+    T7 = T5 / T6;
+    T8 = 3.0 * PP(cgisl) * T4 / T6 / T6;
+    Ggisls = Ggisls * T7 + Igisl * T8;
+    Ggislg = Ggislg * T7;
+    Ggislb = -Igisl * T8;
+    Igisl *= T7;
+  }
+  ggisls = Ggisls;
+  ggislg = Ggislg;
+  ggislb = Ggislb;
+  /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
+  Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
+  Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
+  Igs = gIgsg = gIgss = 0.0;
+  Igd = gIgdg = gIgdd = 0.0;
+  Igb = gIgbg = gIgbd = gIgbs = gIgbb = 0.0;
+  if (nf != 1.0) {
+    cdrain *= nf;
+    gds *= nf;
+    gm *= nf;
+    gmbs *= nf;
+    IdovVds *= nf;
+    gbbs *= nf;
+    gbgs *= nf;
+    gbds *= nf;
+    csub *= nf;
+    Igidl *= nf;
+    ggidld *= nf;
+    ggidlg *= nf;
+    ggidlb *= nf;
+    Igisl *= nf;
+    ggisls *= nf;
+    ggislg *= nf;
+    ggislb *= nf;
+    Igcs *= nf;
+    gIgcsg *= nf;
+    gIgcsd *= nf;
+    gIgcsb *= nf;
+    Igcd *= nf;
+    gIgcdg *= nf;
+    gIgcdd *= nf;
+    gIgcdb *= nf;
+    Igs *= nf;
+    gIgsg *= nf;
+    gIgss *= nf;
+    Igd *= nf;
+    gIgdg *= nf;
+    gIgdd *= nf;
+    Igb *= nf;
+    gIgbg *= nf;
+    gIgbd *= nf;
+    gIgbb *= nf;
+  }
+  ggidls = -(ggidld + ggidlg + ggidlb);
+  ggisld = -(ggisls + ggislg + ggislb);
+  gIgbs = -(gIgbg + gIgbd + gIgbb);
+  gIgcss = -(gIgcsg + gIgcsd + gIgcsb);
+  gIgcds = -(gIgcdg + gIgcdd + gIgcdb);
+  cd = cdrain;
+
+  /* noise charge (:10674-10703): rewrites the Abulk and Vdseff members */
+  if (C.tnoiMod == 0) {
+    Abulk = Abulk0 * PP(abulkCVfactor);
+    Vdsat = Vgsteff / Abulk;
+    T0 = Vdsat - Vds - kDelta4;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta4 * Vdsat);
+    if (T0 >= 0.0) {
+      Vdseff = Vdsat - 0.5 * (T0 + T1);
+    } else {
+      T3 = (kDelta4 + kDelta4) / (T1 - T0);
+      T4 = 1.0 - T3;
+      T5 = Vdsat * T3 / (T1 - T0);
+      Vdseff = Vdsat * T4;
+    }
+    if (Vds == 0.0) Vdseff = 0.0;
+    T0 = Abulk * Vdseff;
+    T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1.0e-20);
+    T2 = Vdseff / T1;
+    T3 = T0 * T2;
+    qinv = Coxeff * PP(weffCV) * nf * PP(leffCV) * (Vgsteff - 0.5 * T0 + Abulk * T3);
+  }
+
+  /* intrinsic charges, capMod = 2 (:10707-11611) */
+  if ((C.xpart < 0) || (!ChargeComputationNeeded)) {
+    qgate = qdrn = qsrc = qbulk = 0.0;
+    cggb = cgsb = cgdb = 0.0;
+    cdgb = cdsb = cddb = 0.0;
+    cbgb = cbsb = cbdb = 0.0;
+    csgb = cssb = csdb = 0.0;
+    cgbb = csbb = cdbb = cbbb = 0.0;
+    cqdb = cqsb = cqgb = cqbb = 0.0;
+    gtau = 0.0;
+  } else {
+    if (Vbseff < 0.0) {
+      VbseffCV = Vbseff;
+      dVbseffCV_dVb = 1.0;
+    } else {
+      VbseffCV = PP(phi) - Phis;
+      dVbseffCV_dVb = -dPhis_dVb;
+    }
+    CoxWL = C.coxe * PP(weffCV) * PP(leffCV) * nf;
+    if (C.cvchargeMod == 0) {
+      noff = n * PP(noff);
+      dnoff_dVd = PP(noff) * dn_dVd;
+      dnoff_dVb = PP(noff) * dn_dVb;
+      T0 = Vtm * noff;
+      voffcv = PP(voffcv);
+      VgstNVt = (Vgst - voffcv) / T0;
+      if (VgstNVt > kExpThreshold) {
+        Vgsteff = Vgst - voffcv;
+        dVgsteff_dVg = dVgs_eff_dVg;
+        dVgsteff_dVd = -dVth_dVd;
+        dVgsteff_dVb = -dVth_dVb;
+      } else if (VgstNVt < -kExpThreshold) {
+        Vgsteff = T0 * log(1.0 + kMinExp);
+        dVgsteff_dVg = 0.0;
+        dVgsteff_dVd = Vgsteff / noff;
+        dVgsteff_dVb = dVgsteff_dVd * dnoff_dVb;
+        dVgsteff_dVd *= dnoff_dVd;
+      } else {
+        ExpVgst = exp(VgstNVt);
+        Vgsteff = T0 * log(1.0 + ExpVgst);
+        dVgsteff_dVg = ExpVgst / (1.0 + ExpVgst);
+        dVgsteff_dVd = -dVgsteff_dVg * (dVth_dVd + (Vgst - voffcv) / noff * dnoff_dVd)
+                       + Vgsteff / noff * dnoff_dVd;
+        dVgsteff_dVb = -dVgsteff_dVg * (dVth_dVb + (Vgst - voffcv) / noff * dnoff_dVb)
+                       + Vgsteff / noff * dnoff_dVb;
+        dVgsteff_dVg *= dVgs_eff_dVg;
+      }
+    } else {
+      T0 = n * Vtm;
+      T1 = PP(mstarcv) * Vgst;
+      T2 = T1 / T0;
+      if (T2 > kExpThreshold) {
+        T10 = T1;
+        dT10_dVg = PP(mstarcv) * dVgs_eff_dVg;
+        dT10_dVd = -dVth_dVd * PP(mstarcv);
+        dT10_dVb = -dVth_dVb * PP(mstarcv);
+      } else if (T2 < -kExpThreshold) {
+        T10 = Vtm * log(1.0 + kMinExp);
+        dT10_dVg = 0.0;
+        dT10_dVd = T10 * dn_dVd;
+        dT10_dVb = T10 * dn_dVb;
+        T10 *= n;
+      } else {
+        ExpVgst = exp(T2);
+        T3 = Vtm * log(1.0 + ExpVgst);
+        T10 = n * T3;
+        dT10_dVg = PP(mstarcv) * ExpVgst / (1.0 + ExpVgst);
+        dT10_dVb = T3 * dn_dVb - dT10_dVg * (dVth_dVb + Vgst * dn_dVb / n);
+        dT10_dVd = T3 * dn_dVd - dT10_dVg * (dVth_dVd + Vgst * dn_dVd / n);
+        dT10_dVg *= dVgs_eff_dVg;
+      }
+      T1 = PP(voffcbncv) - (1.0 - PP(mstarcv)) * Vgst;
+      T2 = T1 / T0;
+      if (T2 < -kExpThreshold) {
+        T3 = C.coxe * kMinExp / PP(cdep0);
+        T9 = PP(mstarcv) + T3 * n;
+        dT9_dVg = 0.0;
+        dT9_dVd = dn_dVd * T3;
+        dT9_dVb = dn_dVb * T3;
+      } else if (T2 > kExpThreshold) {
+        T3 = C.coxe * kMaxExp / PP(cdep0);
+        T9 = PP(mstarcv) + T3 * n;
+        dT9_dVg = 0.0;
+        dT9_dVd = dn_dVd * T3;
+        dT9_dVb = dn_dVb * T3;
+      } else {
+        ExpVgst = exp(T2);
+        T3 = C.coxe / PP(cdep0);
+        T4 = T3 * ExpVgst;
+        T5 = T1 * T4 / T0;
+        T9 = PP(mstarcv) + n * T4;
+        dT9_dVg = T3 * (PP(mstarcv) - 1.0) * ExpVgst / Vtm;
+        dT9_dVb = T4 * dn_dVb - dT9_dVg * dVth_dVb - T5 * dn_dVb;
+        dT9_dVd = T4 * dn_dVd - dT9_dVg * dVth_dVd - T5 * dn_dVd;
+        dT9_dVg *= dVgs_eff_dVg;
+      }
+      Vgsteff = T10 / T9;
+      T11 = T9 * T9;
+      dVgsteff_dVg = (T9 * dT10_dVg - T10 * dT9_dVg) / T11;
+      dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
+      dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
+    }
+    /* capMod = 2 (:11321-11602) */
+    V3 = IV(vfbzb) - Vgs_eff + VbseffCV - kDelta3;
+    if (IV(vfbzb) <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * IV(vfbzb));
+    else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * IV(vfbzb));
+    T1 = 0.5 * (1.0 + V3 / T0);
+    Vfbeff = IV(vfbzb) - 0.5 * (V3 + T0);
+    dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+    Cox = C.coxp;
+    Tox = 1.0e8 * C.toxp;
+    T0 = (Vgs_eff - VbseffCV - IV(vfbzb)) / Tox;
+    dT0_dVg = dVgs_eff_dVg / Tox;
+    dT0_dVb = -dVbseffCV_dVb / Tox;
+    tmp = T0 * PP(acde);
+    if ((-kExpThreshold < tmp) && (tmp < kExpThreshold)) {
+      Tcen = PP(ldeb) * exp(tmp);
+      dTcen_dVg = PP(acde) * Tcen;
+      dTcen_dVb = dTcen_dVg * dT0_dVb;
+      dTcen_dVg *= dT0_dVg;
+    } else if (tmp <= -kExpThreshold) {
+      Tcen = PP(ldeb) * kMinExp;
+      dTcen_dVg = dTcen_dVb = 0.0;
+    } else {
+      Tcen = PP(ldeb) * kMaxExp;
+      dTcen_dVg = dTcen_dVb = 0.0;
+    }
+    LINK = 1.0e-3 * C.toxp;
+    V3 = PP(ldeb) - Tcen - LINK;
+    V4 = sqrt(V3 * V3 + 4.0 * LINK * PP(ldeb));
+    Tcen = PP(ldeb) - 0.5 * (V3 + V4);
+    T1 = 0.5 * (1.0 + V3 / V4);
+    dTcen_dVg *= T1;
+    dTcen_dVb *= T1;
+    Ccen = epssub / Tcen;
+    T2 = Cox / (Cox + Ccen);
+    Coxeff = T2 * Ccen;
+    T3 = -Ccen / Tcen;
+    dCoxeff_dVg = T2 * T2 * T3;
+    dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
+    dCoxeff_dVg *= dTcen_dVg;
+    CoxWLcen = CoxWL * Coxeff / C.coxe;
+    Qac0 = CoxWLcen * (Vfbeff - IV(vfbzb));
+    QovCox = Qac0 / Coxeff;
+    dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
+    dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
+    T0 = 0.5 * PP(k1ox);
+    T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
+    if (PP(k1ox) == 0.0) {
+      T1 = 0.0;
+      T2 = 0.0;
+    } else if (T3 < 0.0) {
+      T1 = T0 + T3 / PP(k1ox);
+      T2 = CoxWLcen;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = CoxWLcen * T0 / T1;
+    }
+    Qsub0 = CoxWLcen * PP(k1ox) * (T1 - T0);
+    QovCox = Qsub0 / Coxeff;
+    dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg) + QovCox * dCoxeff_dVg;
+    dQsub0_dVd = -T2 * dVgsteff_dVd;
+    dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb) + QovCox * dCoxeff_dVb;
+    if (PP(k1ox) <= 0.0) {
+      Denomi = 0.25 * PP(moin) * Vtm;
+      T0 = 0.5 * PP(sqrtPhi);
+    } else {
+      Denomi = PP(moin) * Vtm * PP(k1ox) * PP(k1ox);
+      T0 = PP(k1ox) * PP(sqrtPhi);
+    }
+    T1 = 2.0 * T0 + Vgsteff;
+    DeltaPhi = Vtm * log(1.0 + T1 * Vgsteff / Denomi);
+    dDeltaPhi_dVg = 2.0 * Vtm * (T1 - T0) / (Denomi + T1 * Vgsteff);
+    T0 = Vgsteff - DeltaPhi - 0.001;
+    dT0_dVg = 1.0 - dDeltaPhi_dVg;
+    T1 = sqrt(T0 * T0 + Vgsteff * 0.004);
+    VgDP = 0.5 * (T0 + T1);
+    dVgDP_dVg = 0.5 * (dT0_dVg + (T0 * dT0_dVg + 0.002) / T1);
+    Tox += Tox;
+    T0 = (Vgsteff + IV(vtfbphi2)) / Tox;
+    tmp = exp(C.bdos * 0.7 * log(T0));
+    T1 = 1.0 + tmp;
+    T2 = C.bdos * 0.7 * tmp / (T0 * Tox);
+    Tcen = C.ados * 1.9e-9 / T1;
+    dTcen_dVg = -Tcen * T2 / T1;
+    dTcen_dVd = dTcen_dVg * dVgsteff_dVd;
+    dTcen_dVb = dTcen_dVg * dVgsteff_dVb;
+    dTcen_dVg *= dVgsteff_dVg;
+    Ccen = epssub / Tcen;
+    T0 = Cox / (Cox + Ccen);
+    Coxeff = T0 * Ccen;
+    T1 = -Ccen / Tcen;
+    dCoxeff_dVg = T0 * T0 * T1;
+    dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
+    dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
+    dCoxeff_dVg *= dTcen_dVg;
+    CoxWLcen = CoxWL * Coxeff / C.coxe;
+    AbulkCV = Abulk0 * PP(abulkCVfactor);
+    dAbulkCV_dVb = PP(abulkCVfactor) * dAbulk0_dVb;
+    VdsatCV = VgDP / AbulkCV;
+    T0 = VdsatCV - Vds - kDelta4;
+    dT0_dVg = dVgDP_dVg / AbulkCV;
+    dT0_dVb = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta4 * VdsatCV);
+    dT1_dVg = (T0 + kDelta4 + kDelta4) / T1;
+    dT1_dVd = -T0 / T1;
+    dT1_dVb = dT1_dVg * dT0_dVb;
+    dT1_dVg *= dT0_dVg;
+    if (T0 >= 0.0) {
+      VdseffCV = VdsatCV - 0.5 * (T0 + T1);
+      dVdseffCV_dVg = 0.5 * (dT0_dVg - dT1_dVg);
+      dVdseffCV_dVd = 0.5 * (1.0 - dT1_dVd);
+      dVdseffCV_dVb = 0.5 * (dT0_dVb - dT1_dVb);
+    } else {
+      T3 = (kDelta4 + kDelta4) / (T1 - T0);
+      T4 = 1.0 - T3;
+      T5 = VdsatCV * T3 / (T1 - T0);
+      VdseffCV = VdsatCV * T4;
+      dVdseffCV_dVg = dT0_dVg * T4 + T5 * (dT1_dVg - dT0_dVg);
+      dVdseffCV_dVd = T5 * (dT1_dVd + 1.0);
+      dVdseffCV_dVb = dT0_dVb * (T4 - T5) + T5 * dT1_dVb;
+    }
+    if (Vds == 0.0) {
+      VdseffCV = 0.0;
+      dVdseffCV_dVg = 0.0;
+      dVdseffCV_dVb = 0.0;
+    }
+    T0 = AbulkCV * VdseffCV;
+    T1 = VgDP;
+    T2 = 12.0 * (T1 - 0.5 * T0 + 1.0e-20);
+    T3 = T0 / T2;
+    T4 = 1.0 - 12.0 * T3 * T3;
+    T5 = AbulkCV * (6.0 * T0 * (4.0 * T1 - T0) / (T2 * T2) - 0.5);
+    T6 = T5 * VdseffCV / AbulkCV;
+    qgate = CoxWLcen * (T1 - T0 * (0.5 - T3));
+    QovCox = qgate / Coxeff;
+    Cgg1 = CoxWLcen * (T4 * dVgDP_dVg + T5 * dVdseffCV_dVg);
+    Cgd1 = CoxWLcen * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
+    Cgb1 = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteff_dVb + QovCox * dCoxeff_dVb;
+    Cgg1 = Cgg1 * dVgsteff_dVg + QovCox * dCoxeff_dVg;
+    T7 = 1.0 - AbulkCV;
+    T8 = T2 * T2;
+    T9 = 12.0 * T7 * T0 * T0 / (T8 * AbulkCV);
+    T10 = T9 * dVgDP_dVg;
+    T11 = -T7 * T5 / AbulkCV;
+    T12 = -(T9 * T1 / AbulkCV + VdseffCV * (0.5 - T0 / T2));
+    qbulk = CoxWLcen * T7 * (0.5 * VdseffCV - T0 * VdseffCV / T2);
+    QovCox = qbulk / Coxeff;
+    Cbg1 = CoxWLcen * (T10 + T11 * dVdseffCV_dVg);
+    Cbd1 = CoxWLcen * T11 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
+    Cbb1 = CoxWLcen * (T11 * dVdseffCV_dVb + T12 * dAbulkCV_dVb) + Cbg1 * dVgsteff_dVb + QovCox * dCoxeff_dVb;
+    Cbg1 = Cbg1 * dVgsteff_dVg + QovCox * dCoxeff_dVg;
+    if (C.xpart > 0.5) {
+      qsrc = -CoxWLcen * (T1 / 2.0 + T0 / 4.0 - 0.5 * T0 * T0 / T2);
+      QovCox = qsrc / Coxeff;
+      T2 += T2;
+      T3 = T2 * T2;
+      T7 = -(0.25 - 12.0 * T0 * (4.0 * T1 - T0) / T3);
+      T4 = -(0.5 + 24.0 * T0 * T0 / T3) * dVgDP_dVg;
+      T5 = T7 * AbulkCV;
+      T6 = T7 * VdseffCV;
+      Csg = CoxWLcen * (T4 + T5 * dVdseffCV_dVg);
+      Csd = CoxWLcen * T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd + QovCox * dCoxeff_dVd;
+      Csb = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb + QovCox * dCoxeff_dVb;
+      Csg = Csg * dVgsteff_dVg + QovCox * dCoxeff_dVg;
+    } else if (C.xpart < 0.5) {
+      T2 = T2 / 12.0;
+      T3 = 0.5 * CoxWLcen / (T2 * T2);
+      T4 = T1 * (2.0 * T0 * T0 / 3.0 + T1 * (T1 - 4.0 * T0 / 3.0)) - 2.0 * T0 * T0 * T0 / 15.0;
+      qsrc = -T3 * T4;
+      QovCox = qsrc / Coxeff;
+      T8 = 4.0 / 3.0 * T1 * (T1 - T0) + 0.4 * T0 * T0;
+      T5 = -2.0 * qsrc / T2 - T3 * (T1 * (3.0 * T1 - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
+      T6 = AbulkCV * (qsrc / T2 + T3 * T8);
+      T7 = T6 * VdseffCV / AbulkCV;
+      Csg = T5 * dVgDP_dVg + T6 * dVdseffCV_dVg;
+      Csd = Csg * dVgsteff_dVd + T6 * dVdseffCV_dVd + QovCox * dCoxeff_dVd;
+      Csb = Csg * dVgsteff_dVb + T6 * dVdseffCV_dVb + T7 * dAbulkCV_dVb + QovCox * dCoxeff_dVb;
+      Csg = Csg * dVgsteff_dVg + QovCox * dCoxeff_dVg;
+    } else {
+      qsrc = -0.5 * qgate;
+      Csg = -0.5 * Cgg1;
+      Csd = -0.5 * Cgd1;
+      Csb = -0.5 * Cgb1;
+    }
+    qgate += Qac0 + Qsub0 - qbulk;
+    qbulk -= (Qac0 + Qsub0);
+    qdrn = -(qgate + qbulk + qsrc);
+    Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+    Cbd = Cbd1 - dQsub0_dVd;
+    Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+    Cgg = Cgg1 - Cbg;
+    Cgd = Cgd1 - Cbd;
+    Cgb = Cgb1 - Cbb;
+    Cgb *= dVbseff_dVb;
+    Cbb *= dVbseff_dVb;
+    Csb *= dVbseff_dVb;
+    cggb = Cgg;
+    cgsb = -(Cgg + Cgd + Cgb);
+    cgdb = Cgd;
+    cdgb = -(Cgg + Cbg + Csg);
+    cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+    cddb = -(Cgd + Cbd + Csd);
+    cbgb = Cbg;
+    cbsb = -(Cbg + Cbd + Cbb);
+    cbdb = Cbd;
+  }
+  csgb = -cggb - cdgb - cbgb;
+  csdb = -cgdb - cddb - cbdb;
+  cssb = -cgsb - cdsb - cbsb;
+  cgbb = -cgdb - cggb - cgsb;
+  cdbb = -cddb - cdgb - cdsb;
+  cbbb = -cbgb - cbdb - cbsb;
+  csbb = -cgbb - cdbb - cbbb;
+
+  /* junction charges (:11662-11810) */
+  if (ChargeComputationNeeded) {
+    czbd = C.DunitAreaTempJctCap * IV(Adeff);
+    czbs = C.SunitAreaTempJctCap * IV(Aseff);
+    czbdsw = C.DunitLengthSidewallTempJctCap * IV(Pdeff);
+    czbdswg = C.DunitLengthGateSidewallTempJctCap * PP(weffCJ) * nf;
+    czbssw = C.SunitLengthSidewallTempJctCap * IV(Pseff);
+    czbsswg = C.SunitLengthGateSidewallTempJctCap * PP(weffCJ) * nf;
+    MJS = C.SbulkJctBotGradingCoeff;
+    MJSWS = C.SbulkJctSideGradingCoeff;
+    MJSWGS = C.SbulkJctGateSideGradingCoeff;
+    MJD = C.DbulkJctBotGradingCoeff;
+    MJSWD = C.DbulkJctSideGradingCoeff;
+    MJSWGD = C.DbulkJctGateSideGradingCoeff;
+    if (vbs_jct == 0.0) {
+      qbs = 0.0;
+      capbs = czbs + czbssw + czbsswg;
+    } else if (vbs_jct < 0.0) {
+      if (czbs > 0.0) {
+        arg = 1.0 - vbs_jct / C.PhiBS;
+        if (MJS == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJS * log(arg));
+        qbs = C.PhiBS * czbs * (1.0 - arg * sarg) / (1.0 - MJS);
+        capbs = czbs * sarg;
+      } else {
+        qbs = 0.0;
+        capbs = 0.0;
+      }
+      if (czbssw > 0.0) {
+        arg = 1.0 - vbs_jct / C.PhiBSWS;
+        if (MJSWS == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJSWS * log(arg));
+        qbs += C.PhiBSWS * czbssw * (1.0 - arg * sarg) / (1.0 - MJSWS);
+        capbs += czbssw * sarg;
+      }
+      if (czbsswg > 0.0) {
+        arg = 1.0 - vbs_jct / C.PhiBSWGS;
+        if (MJSWGS == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJSWGS * log(arg));
+        qbs += C.PhiBSWGS * czbsswg * (1.0 - arg * sarg) / (1.0 - MJSWGS);
+        capbs += czbsswg * sarg;
+      }
+    } else {
+      T0 = czbs + czbssw + czbsswg;
+      T1 = vbs_jct * (czbs * MJS / C.PhiBS + czbssw * MJSWS / C.PhiBSWS + czbsswg * MJSWGS / C.PhiBSWGS);
+      qbs = vbs_jct * (T0 + 0.5 * T1);
+      capbs = T0 + T1;
+    }
+    if (vbd_jct == 0.0) {
+      qbd = 0.0;
+      capbd = czbd + czbdsw + czbdswg;
+    } else if (vbd_jct < 0.0) {
+      if (czbd > 0.0) {
+        arg = 1.0 - vbd_jct / C.PhiBD;
+        if (MJD == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJD * log(arg));
+        qbd = C.PhiBD * czbd * (1.0 - arg * sarg) / (1.0 - MJD);
+        capbd = czbd * sarg;
+      } else {
+        qbd = 0.0;
+        capbd = 0.0;
+      }
+      if (czbdsw > 0.0) {
+        arg = 1.0 - vbd_jct / C.PhiBSWD;
+        if (MJSWD == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJSWD * log(arg));
+        qbd += C.PhiBSWD * czbdsw * (1.0 - arg * sarg) / (1.0 - MJSWD);
+        capbd += czbdsw * sarg;
+      }
+      if (czbdswg > 0.0) {
+        arg = 1.0 - vbd_jct / C.PhiBSWGD;
+        if (MJSWGD == 0.5) sarg = 1.0 / sqrt(arg);
+        else sarg = exp(-MJSWGD * log(arg));
+        qbd += C.PhiBSWGD * czbdswg * (1.0 - arg * sarg) / (1.0 - MJSWGD);
+        capbd += czbdswg * sarg;
+      }
+    } else {
+      T0 = czbd + czbdsw + czbdswg;
+      T1 = vbd_jct * (czbd * MJD / C.PhiBD + czbdsw * MJSWD / C.PhiBSWD + czbdswg * MJSWGD / C.PhiBSWGD);
+      qbd = vbd_jct * (T0 + 0.5 * T1);
+      capbd = T0 + T1;
+    }
+  }
+
+  /* overlap capacitances, capMod != 0 (:11812-11883) */
+  vgdx = vgd;
+  vgsx = vgs;
+  Igate = IgateMid = 0.0;
+  Igate_Jdxp = IgateMid_Jdxp = 0.0;
+  T0 = vgdx + kDelta1;
+  T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+  T2 = 0.5 * (T0 - T1);
+  T3 = PP(weffCV) * PP(cgdl);
+  T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappad));
+  cgdo = PP(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+  qgdo = (PP(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PP(ckappad) * (T4 - 1.0));
+  T0 = vgsx + kDelta1;
+  T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+  T2 = 0.5 * (T0 - T1);
+  T3 = PP(weffCV) * PP(cgsl);
+  T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappas));
+  cgso = PP(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+  qgso = (PP(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PP(ckappas) * (T4 - 1.0));
+  if (nf != 1.0) {
+    cgdo *= nf;
+    cgso *= nf;
+    qgdo *= nf;
+    qgso *= nf;
+  }
+
+  // ---- setupCapacitors_oldDAE / setupCapacitors_newDAE ----
+  {
+    double ag0 = ss.pdt;
+    if (!(ss.dcopFlag) && ss.initTranFlag && ss.newtonIter == 0) ag0 = 0.0;
+    if (mode > 0) {
+      qdrn -= qgdo;
+      gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+      gcgdb = (cgdb - cgdo) * ag0;
+      gcgsb = (cgsb - cgso) * ag0;
+      gcgbb = -(gcggb + gcgdb + gcgsb);
+      gcdgb = (cdgb - cgdo) * ag0;
+      gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
+      gcbgb = (cbgb - PP(cgbo)) * ag0;
+      gcdgmb = gcsgmb = gcbgmb = 0.0;
+      qgb = PP(cgbo) * vgb;
+      qgate += qgdo + qgso + qgb;
+      qbulk -= qgb;
+      qsrc = -(qgate + qbulk + qdrn);
+      gcddb = (cddb + capbd + cgdo) * ag0;
+      gcdsb = cdsb * ag0;
+      gcsdb = -(cgdb + cbdb + cddb) * ag0;
+      gcssb = (capbs + cgso - (cgsb + cbsb + cdsb)) * ag0;
+      gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+      gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+      gcbdb = (cbdb - capbd) * ag0;
+      gcbsb = (cbsb - capbs) * ag0;
+      gcdbdb = 0.0;
+      gcsbsb = 0.0;
+      gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
+      ggtg = ggtd = ggtb = ggts = 0.0;
+      sxpart = 0.6;
+      dxpart = 0.4;
+      ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+      dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+    } else {
+      qsrc = qdrn - qgso;
+      gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+      gcgdb = (cgsb - cgdo) * ag0;
+      gcgsb = (cgdb - cgso) * ag0;
+      gcgbb = -(gcggb + gcgdb + gcgsb);
+      gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
+      gcsgb = (cdgb - cgso) * ag0;
+      gcbgb = (cbgb - PP(cgbo)) * ag0;
+      gcdgmb = gcsgmb = gcbgmb = 0.0;
+      qgb = PP(cgbo) * vgb;
+      qgate += qgdo + qgso + qgb;
+      qbulk -= qgb;
+      qdrn = -(qgate + qbulk + qsrc);
+      gcddb = (capbd + cgdo - (cgsb + cbsb + cdsb)) * ag0;
+      gcdsb = -(cgdb + cbdb + cddb) * ag0;
+      gcsdb = cdsb * ag0;
+      gcssb = (cddb + capbs + cgso) * ag0;
+      gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+      gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+      gcbdb = (cbsb - capbd) * ag0;
+      gcbsb = (cbdb - capbs) * ag0;
+      gcdbdb = 0.0;
+      gcsbsb = 0.0;
+      gcbbb = -(gcbgb + gcbdb + gcbsb + gcbgmb);
+      ggtg = ggtd = ggtb = ggts = 0.0;
+      sxpart = 0.4;
+      dxpart = 0.6;
+      ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+      dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+    }
+  }
+  {
+    if (mode > 0) {
+      CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+      CAPcgdb = (cgdb - cgdo);
+      CAPcgsb = (cgsb - cgso);
+      CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+      CAPcdgb = (cdgb - cgdo);
+      CAPcsgb = -(cggb + cbgb + cdgb + cgso);
+      CAPcbgb = (cbgb - PP(cgbo));
+      CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+      CAPcddb = (cddb + capbd + cgdo);
+      CAPcdsb = cdsb;
+      CAPcsdb = -(cgdb + cbdb + cddb);
+      CAPcssb = (capbs + cgso - (cgsb + cbsb + cdsb));
+      CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+      CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+      CAPcbdb = (cbdb - capbd);
+      CAPcbsb = (cbsb - capbs);
+      CAPcdbdb = 0.0;
+      CAPcsbsb = 0.0;
+      CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+    } else {
+      CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+      CAPcgdb = (cgsb - cgdo);
+      CAPcgsb = (cgdb - cgso);
+      CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+      CAPcdgb = -(cggb + cbgb + cdgb + cgdo);
+      CAPcsgb = (cdgb - cgso);
+      CAPcbgb = (cbgb - PP(cgbo));
+      CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+      CAPcddb = (capbd + cgdo - (cgsb + cbsb + cdsb));
+      CAPcdsb = -(cgdb + cbdb + cddb);
+      CAPcsdb = cdsb;
+      CAPcssb = (cddb + capbs + cgso);
+      CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+      CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+      CAPcbdb = (cbsb - capbd);
+      CAPcbsb = (cbdb - capbs);
+      CAPcdbdb = 0.0;
+      CAPcsbsb = 0.0;
+      CAPcbbb = -(CAPcbgb + CAPcbdb + CAPcbsb + CAPcbgmb);
+    }
+  }
+  Idrain = IV(drainConductance) * Vddp;
+  Isource = IV(sourceConductance) * Vssp;
+
+  // ---- Master::updateState: stored voltages and charges ----
+  {
+    double* sto = a.v.nextSto;
+    const int32_t sl[12] = {li_store_vbd, li_store_vbs, li_store_vgs, li_store_vds,
+                            li_store_vges, li_store_vgms, li_store_vdes, li_store_vses,
+                            li_store_vdbs, li_store_vsbs, li_store_vdbd, li_store_von};
+    const double sv[12] = {vbd, vbs, vgs, vds, vges, vgms, vdes, vses, vdbs, vsbs, vdbd, von};
+    for (int k = 0; k < 12; ++k)
+      if (sl[k] >= 0) sto[sl[k]] = sv[k];
+  }
+  qg = qgate;
+  qd = qdrn - qbd;
+  qb = qbulk + qbd + qbs;
+  {
+    const int32_t lqb = stalid(a, A_qb, i), lqg = stalid(a, A_qg, i), lqd = stalid(a, A_qd, i);
+    if (double* sta = a.v.nextSta) {
+      if (lqg >= 0) sta[lqg] = qg;
+      if (lqd >= 0) sta[lqd] = qd;
+      if (lqb >= 0) sta[lqb] = qb;
+    }
+    if (!ss.dcopFlag && ss.initTranFlag && ss.newtonIter == 0) {
+      if (double* cs = a.v.currSta) {
+        if (lqg >= 0) cs[lqg] = qgate;
+        if (lqd >= 0) cs[lqd] = qdrn - qbd;
+        if (lqb >= 0) cs[lqb] = qbulk + qbd + qbs;
+      }
+    }
+  }
+  st(a, S_VGB, i) = vgb;
+
+  // ---- Master::loadDAEVectors ----
+  Sink<FUSED> s(a, i);
+  const double Mp = IV(numberParallel);
+  {
+    const double dt = C.dtype;
+    ceqdrn_Jdxp = 0.0; ceqbd_Jdxp = 0.0; ceqbs_Jdxp = 0.0;
+    Istoteq_Jdxp = 0.0; Idtoteq_Jdxp = 0.0;
+    Ibtoteq_Jdxp = 0.0; Igtoteq_Jdxp = 0.0;
+    ceqgcrg_Jdxp = 0.0; ceqgstot_Jdxp = 0.0; ceqgdtot_Jdxp = 0.0;
+    ceqjs_Jdxp = 0.0; ceqjd_Jdxp = 0.0;
+    if (mode >= 0) {
+      Gm = gm;
+      Gmbs = gmbs;
+      FwdSum = Gm + Gmbs;
+      RevSum = 0.0;
+      ceqdrn = dt * cdrain;
+      ceqdrn_Jdxp = dt * (-gds * (vds - vds_orig) - Gm * (vgs - vgs_orig)
+                             - Gmbs * (vbs - vbs_orig));
+      ceqbd = dt * (csub + Igidl);
+      ceqbd_Jdxp = dt * (-(gbds + ggidld) * (vds - vds_orig)
+                            - (gbgs + ggidlg) * (vgs - vgs_orig)
+                            - (gbbs + ggidlb) * (vbs - vbs_orig));
+      ceqbs = dt * Igisl;
+      ceqbs_Jdxp = dt * (+ggisls * (vds - vds_orig) - ggislg * (vgd - vgd_orig)
+                            - ggislb * (vbd - vbd_orig));
+      gbbdp = -(gbds);
+      gbbsp = gbds + gbgs + gbbs;
+      gbdpg = gbgs;
+      gbdpdp = gbds;
+      gbdpb = gbbs;
+      gbdpsp = -(gbdpg + gbdpdp + gbdpb);
+      gbspg = 0.0;
+      gbspdp = 0.0;
+      gbspb = 0.0;
+      gbspsp = 0.0;
+    } else {
+      Gm = -gm;
+      Gmbs = -gmbs;
+      FwdSum = 0.0;
+      RevSum = -(Gm + Gmbs);
+      ceqdrn = -dt * cdrain;
+      ceqdrn_Jdxp = -dt * (+gds * (vds - vds_orig) + Gm * (vgd - vgd_orig)
+                              + Gmbs * (vbd - vbd_orig));
+      ceqbs = dt * (csub + Igisl);
+      ceqbs_Jdxp = dt * (+(gbds + ggisls) * (vds - vds_orig)
+                            - (gbgs + ggislg) * (vgd - vgd_orig)
+                            - (gbbs + ggislb) * (vbd - vbd_orig));
+      ceqbd = dt * Igidl;
+      ceqbd_Jdxp = dt * (-ggidld * (vds - vds_orig) - ggidlg * (vgs - vgs_orig)
+                            - ggidlb * (vbs - vbs_orig));
+      gbbsp = -(gbds);
+      gbbdp = gbds + gbgs + gbbs;
+      gbdpg = 0.0;
+      gbdpsp = 0.0;
+      gbdpb = 0.0;
+      gbdpdp = 0.0;
+      gbspg = gbgs;
+      gbspsp = gbds;
+      gbspb = gbbs;
+      gbspdp = -(gbspg + gbspsp + gbspb);
+    }
+    gIstotg = gIstotd = gIstots = gIstotb = Istoteq = 0.0;
+    gIdtotg = gIdtotd = gIdtots = gIdtotb = Idtoteq = 0.0;
+    gIbtotg = gIbtotd = gIbtots = gIbtotb = Ibtoteq = 0.0;
+    gIgtotg = gIgtotd = gIgtots = gIgtotb = Igtoteq = 0.0;
+    ceqgcrg = gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+    gstot = gstotd = gstotg = gstots = gstotb = ceqgstot = 0.0;
+    gdtot = gdtotd = gdtotg = gdtots = gdtotb = ceqgdtot = 0.0;
+    if (C.dtype > 0) {
+      ceqjs = (cbs);
+      ceqjs_Jdxp = (-gbs * (vbs_jct - vbs_jct_orig));
+      ceqjd = (cbd);
+      ceqjd_Jdxp = (-gbd * (vbd_jct - vbd_jct_orig));
+    } else {
+      ceqjs = -(cbs);
+      ceqjs_Jdxp = (gbs * (vbs_jct - vbs_jct_orig));
+      ceqjd = -(cbd);
+      ceqjd_Jdxp = (gbd * (vbd_jct - vbd_jct_orig));
+      ceqgcrg = -ceqgcrg;
+      ceqgcrg_Jdxp = -ceqgcrg_Jdxp;
+    }
+  }
+  s.put(K_F, 0, -(ceqjd - ceqbd - ceqdrn + Idtoteq) * Mp);
+  s.put(K_F, 1, -(-(-ceqgcrg + Igtoteq) * Mp));
+  s.put(K_F, 2, -(ceqbd + ceqbs - ceqjd - ceqjs + Ibtoteq) * Mp);
+  s.put(K_F, 3, -(ceqdrn - ceqbs + ceqjs + Istoteq) * Mp);
+  if (ifld(a, F_DEXIST, i)) {
+    s.put(K_F, 4, -(-Idrain) * Mp);
+    s.put(K_F, 5, -(Idrain) * Mp);
+  }
+  if (ifld(a, F_SEXIST, i)) {
+    s.put(K_F, 6, -(-Isource) * Mp);
+    s.put(K_F, 7, -(+Isource) * Mp);
+  }
+  // Limiter corrections are loaded only when limiting moved a voltage; an
+  // instance that did not limit adds -0.0, which leaves every sum unchanged.
+  if (opt.voltageLimiterFlag) {
+    const bool lim = !origFlag;
+    s.put(K_FLIM, 0, lim ? (ceqjd_Jdxp - ceqbd_Jdxp - ceqdrn_Jdxp + Idtoteq_Jdxp) * Mp : -0.0);
+    s.put(K_FLIM, 1, lim ? -((-ceqgcrg_Jdxp + Igtoteq_Jdxp) * Mp) : -0.0);
+    s.put(K_FLIM, 2, lim ? (ceqbd_Jdxp + ceqbs_Jdxp - ceqjd_Jdxp - ceqjs_Jdxp + Ibtoteq_Jdxp) * Mp : -0.0);
+    s.put(K_FLIM, 3, lim ? (ceqdrn_Jdxp - ceqbs_Jdxp + ceqjs_Jdxp + Istoteq_Jdxp) * Mp : -0.0);
+  }
+  {
+    if (!ChargeComputationNeeded) {
+      sxpart = (1.0 - (dxpart = (mode > 0) ? 0.4 : 0.6));
+      ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+      dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      Qeqqg_Jdxp = 0.0;
+      Qeqqd_Jdxp = 0.0;
+      Qeqqb_Jdxp = 0.0;
+      Qeqqgmid_Jdxp = 0.0;
+      Qeqqjs_Jdxp = 0.0;
+      Qeqqjd_Jdxp = 0.0;
+      gtau = 0.0;
+    } else {
+      Qeqqg_Jdxp = 0.0;
+      if (!origFlag)
+        Qeqqg_Jdxp = -CAPcggb * (vgb - vgb_orig) + CAPcgdb * (vbd - vbd_orig)
+                        + CAPcgsb * (vbs - vbs_orig);
+      Qeqqd_Jdxp = 0.0;
+      if (!origFlag)
+        Qeqqd_Jdxp = -CAPcdgb * (vgb - vgb_orig) - CAPcdgmb * (vgmb - vgmb_orig)
+                        + (CAPcddb + CAPcdbdb) * (vbd - vbd_orig)
+                        - CAPcdbdb * (vbd_jct - vbd_jct_orig) + CAPcdsb * (vbs - vbs_orig);
+      Qeqqb_Jdxp = 0.0;
+      if (!origFlag)
+        Qeqqb_Jdxp = -CAPcbgb * (vgb - vgb_orig) - CAPcbgmb * (vgmb - vgmb_orig)
+                        + CAPcbdb * (vbd - vbd_orig) + CAPcbsb * (vbs - vbs_orig);
+      Qeqqgmid_Jdxp = 0.0;
+    }
+  }
+  double Qeqqg, Qeqqb, Qeqqd;
+  const double Qeqqgmid = 0.0;
+  if (C.dtype > 0) {
+    Qeqqg = qg;
+    Qeqqd = qd;
+    Qeqqb = qb;
+  } else {
+    Qeqqg = -qg;
+    Qeqqd = -qd;
+    Qeqqb = -qb;
+  }
+  s.put(K_Q, 0, -(-Qeqqd) * Mp);
+  s.put(K_Q, 1, -(-(Qeqqg) * Mp));
+  s.put(K_Q, 2, -(-Qeqqb) * Mp);
+  s.put(K_Q, 3, -(+Qeqqg + Qeqqb + Qeqqd + Qeqqgmid) * Mp);
+  if (opt.voltageLimiterFlag) {
+    const bool lim = !origFlag;
+    s.put(K_QLIM, 0, lim ? (-Qeqqd_Jdxp) * Mp : -0.0);
+    s.put(K_QLIM, 1, lim ? -((Qeqqg_Jdxp) * Mp) : -0.0);
+    s.put(K_QLIM, 2, lim ? (-Qeqqb_Jdxp) * Mp : -0.0);
+    s.put(K_QLIM, 3, lim ? (+Qeqqg_Jdxp + Qeqqb_Jdxp + Qeqqd_Jdxp + Qeqqgmid_Jdxp) * Mp : -0.0);
+  }
+
+  // ---- Master::loadDAEMatrices ----
+  gjbd = gbd;
+  gjbs = gbs;
+  gdpr = IV(drainConductance);
+  gspr = IV(sourceConductance);
+  T1 = qdef * gtau;
+    s.put(K_DFDX, 0, (-ggtg + gIgtotg) * Mp);
+    s.put(K_DFDX, 1, (-ggtd + gIgtotd) * Mp);
+    s.put(K_DFDX, 2, (-ggts + gIgtots) * Mp);
+    s.put(K_DFDX, 3, (-ggtb + gIgtotb) * Mp);
+    s.put(K_DFDX, 4, (gdpr + gds + gbd + T1 * ddxpart_dVd - gdtotd + RevSum + gbdpdp + dxpart * ggtd - gIdtotd) * Mp);
+    s.put(K_DFDX, 5, -((gdpr + gdtot) * Mp));
+    s.put(K_DFDX, 6, (Gm - gdtotg + gbdpg - gIdtotg + dxpart * ggtg + T1 * ddxpart_dVg) * Mp);
+    s.put(K_DFDX, 7, -((gds + gdtots - dxpart * ggts + gIdtots - T1 * ddxpart_dVs + FwdSum - gbdpsp) * Mp));
+    s.put(K_DFDX, 8, -((gjbd + gdtotb - Gmbs - gbdpb + gIdtotb - T1 * ddxpart_dVb - dxpart * ggtb) * Mp));
+    s.put(K_DFDX, 9, -((gdpr - gdtotd) * Mp));
+    s.put(K_DFDX, 10, (gdpr + gdtot) * Mp);
+    s.put(K_DFDX, 11, -((gds + gstotd + RevSum - gbspdp - T1 * dsxpart_dVd - sxpart * ggtd + gIstotd) * Mp));
+    s.put(K_DFDX, 12, (-Gm - gstotg + gbspg + sxpart * ggtg + T1 * dsxpart_dVg - gIstotg) * Mp);
+    s.put(K_DFDX, 13, (gspr + gds + gbs + T1 * dsxpart_dVs - gstots + FwdSum + gbspsp + sxpart * ggts - gIstots) * Mp);
+    s.put(K_DFDX, 14, -((gspr + gstot) * Mp));
+    s.put(K_DFDX, 15, -((gjbs + gstotb + Gmbs - gbspb - sxpart * ggtb - T1 * dsxpart_dVb + gIstotb) * Mp));
+    s.put(K_DFDX, 16, -((gspr - gstots) * Mp));
+    s.put(K_DFDX, 17, (gspr + gstot) * Mp);
+    s.put(K_DFDX, 18, (-gjbd + gbbdp - gIbtotd) * Mp);
+    s.put(K_DFDX, 19, (-gbgs - gIbtotg) * Mp);
+    s.put(K_DFDX, 20, (-gjbs + gbbsp - gIbtots) * Mp);
+    s.put(K_DFDX, 21, (gjbd + gjbs - gbbs - gIbtotb) * Mp);
+    s.put(K_DFDX, 22, (ggidld) * Mp);
+    s.put(K_DFDX, 23, (ggidlg) * Mp);
+    s.put(K_DFDX, 24, -(((ggidlg + ggidld + ggidlb)) * Mp));
+    s.put(K_DFDX, 25, (ggidlb) * Mp);
+    s.put(K_DFDX, 26, -((ggidld) * Mp));
+    s.put(K_DFDX, 27, -((ggidlg) * Mp));
+    s.put(K_DFDX, 28, ((ggidlg + ggidld + ggidlb)) * Mp);
+    s.put(K_DFDX, 29, -((ggidlb) * Mp));
+    s.put(K_DFDX, 30, -(((ggisls + ggislg + ggislb)) * Mp));
+    s.put(K_DFDX, 31, (ggislg) * Mp);
+    s.put(K_DFDX, 32, (ggisls) * Mp);
+    s.put(K_DFDX, 33, (ggislb) * Mp);
+    s.put(K_DFDX, 34, ((ggislg + ggisls + ggislb)) * Mp);
+    s.put(K_DFDX, 35, -((ggislg) * Mp));
+    s.put(K_DFDX, 36, -((ggisls) * Mp));
+    s.put(K_DFDX, 37, -((ggislb) * Mp));
+    s.put(K_DQDX, 0, (CAPcggb) * Mp);
+    s.put(K_DQDX, 1, (CAPcgdb) * Mp);
+    s.put(K_DQDX, 2, (CAPcgsb) * Mp);
+    s.put(K_DQDX, 3, (CAPcgbb) * Mp);
+    s.put(K_DQDX, 4, (CAPcddb) * Mp);
+    s.put(K_DQDX, 5, (+CAPcdgb) * Mp);
+    s.put(K_DQDX, 6, -((-CAPcdsb) * Mp));
+    s.put(K_DQDX, 7, -((-CAPcdbb) * Mp));
+    s.put(K_DQDX, 8, -((-CAPcsdb) * Mp));
+    s.put(K_DQDX, 9, (CAPcsgb) * Mp);
+    s.put(K_DQDX, 10, (CAPcssb) * Mp);
+    s.put(K_DQDX, 11, -((-CAPcsbb) * Mp));
+    s.put(K_DQDX, 12, (CAPcbdb) * Mp);
+    s.put(K_DQDX, 13, (CAPcbgb) * Mp);
+    s.put(K_DQDX, 14, (CAPcbsb) * Mp);
+    s.put(K_DQDX, 15, (CAPcbbb) * Mp);
+}
+
+#undef PP
+#undef IV
+
+static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
+  if (!v.nextSol || !v.nextSto) return fail(XA_ERR_ARG, "bsim4 update: nextSol/nextSto required");
+  if (ss.newtonIter == 0 && (!ss.dcopFlag || (ss.locaEnabledFlag && ss.dcopFlag)) && !v.currSto)
+    return fail(XA_ERR_ARG, "bsim4 update: currSto required");
+  if (ss.inputOPFlag && ss.initJctFlag && !v.flagSol)
+    return fail(XA_ERR_ARG, "bsim4 update: flagSol required with inputOPFlag");
+  KArgs a;
+  fill_kargs(g, ss, v, fused, a);
+  if (fused)
+    hipLaunchKernelGGL(k_update<true>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_update<false>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
+// Algorithmic bytes per instance: the SoA columns and card index read once,
+// node voltages (6 LIDs + values), 14 store LIDs, 12 store writes, 3 state
+// LIDs + writes, vgb read + write; the loads write value + target per entry.
+static double traffic(const Group& g, int phase) {
+  const double cols = ND, lids = 6 + NSTO + NSTA + 1 /*model index*/;
+  if (phase == 0) return 8.0 * (cols + 6 /*node values*/ + 12 /*sto*/ + 3 /*sta*/ + 2 /*vgb*/) + 4.0 * lids;
+  const bool lim = g.opt.voltageLimiterFlag != 0;
+  if (phase == 1) return 16.0 * (4 /*F*/ + 4 /*Q*/ + (lim ? 8 : 0));
+  return 16.0 * (38 + 16);
+}
+
+}  // namespace bsim4
+
+const ModelImpl* model_bsim4() {
+  static ModelImpl m = [] {
+    using namespace bsim4;
+    for (auto& d : g_mparams) d.is_temp = std::strcmp(d.name, "TNOM") == 0;
+    for (auto& d : g_iparams) d.is_temp = std::strcmp(d.name, "TEMP") == 0;
+    ModelImpl x{};
+    x.type = XA_BSIM4;
+    x.name = "bsim4";
+    x.mparams = g_mparams;
+    x.n_mparams = sizeof(g_mparams) / sizeof(ParamDef);
+    x.n_mslots = NM;
+    x.iparams = g_iparams;
+    x.n_iparams = sizeof(g_iparams) / sizeof(ParamDef);
+    x.n_islots = NI;
+    x.max_nodes = NNODE;
+    x.max_sto = NSTO;
+    x.max_sta = NSTA;
+    const int E[NK] = {8, 4, 4, 4, 38, 16, 0, 0};
+    for (int k = 0; k < NK; ++k) x.max_entries[k] = E[k];
+    x.setup = setup;
+    x.resolve_nodes = resolve_nodes;
+    x.stamps = stamps;
+    x.launch_update = launch_update;
+    x.launch_secondary = nullptr;
+    x.skip_kind = skip_kind;
+    x.state_names = kStateNames;
+    x.n_state = NS;
+    x.traffic = traffic;
+    return x;
+  }();
+  return &m;
+}
+
+}  // namespace xa

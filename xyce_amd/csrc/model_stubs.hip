@@ -4,5 +4,4 @@
 namespace xa {
 const ModelImpl* model_mosfet1() { return nullptr; }
 const ModelImpl* model_bjt() { return nullptr; }
-const ModelImpl* model_bsim4() { return nullptr; }
 }  // namespace xa

@@ -420,6 +420,7 @@ int xa_group_jac_pairs(xa_group gh, int64_t* count, int32_t* rows, int32_t* cols
     for (int64_t i = 0; i < n; ++i) {
       g->impl->stamps(*g, i, k, st);
       for (const Stamp& s : st) {
+        if (s.row < 0 || s.col < 0) continue;   // entry not loaded by this instance
         const int32_t r = g->nodes[s.row * n + i], q = g->nodes[s.col * n + i];
         if (r < 0 || q < 0) continue;
         if (rows) { rows[c] = r; cols[c] = q; }
@@ -447,7 +448,9 @@ int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
       if ((int)st.size() > E) return fail(XA_ERR_STATE, "stamp list exceeds max_entries");
       for (size_t e = 0; e < st.size(); ++e) {
         int32_t t = -1;
-        if (kind_is_assign(k)) {
+        if (st[e].row < 0 || (kind_is_matrix(k) && st[e].col < 0)) {
+          // entry not loaded by this instance: no target
+        } else if (kind_is_assign(k)) {
           t = (st[e].row < m->max_sto) ? g->sto_lid[st[e].row * n + i] : -1;
         } else if (!kind_is_matrix(k)) {
           t = g->nodes[st[e].row * n + i];
