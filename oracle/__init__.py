@@ -77,6 +77,9 @@ def lib():
             ("or_set_threads", None, [C.c_int]),
             ("or_default_options", None, [P(DeviceOptions)]),
             ("or_set_magnitude_mode", None, [C.c_int]),
+            ("or_set_libm_perturbation", None, [C.c_int, C.c_uint64]),
+            ("or_group_save_state", C.c_int, [vp]),
+            ("or_group_restore_state", C.c_int, [vp]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -202,6 +205,13 @@ class Master:
 
     def loadDAEMatrices(self, ss, v):
         check(lib().or_load_matrices(self.h, C.byref(ss), C.byref(v)), "loadDAEMatrices")
+
+    def save_state(self):
+        """Snapshot the instance records (values kept between calls)."""
+        check(lib().or_group_save_state(self.h), "save_state")
+
+    def restore_state(self):
+        check(lib().or_group_restore_state(self.h), "restore_state")
 
     def read(self, name):
         out = np.zeros(self._n, np.float64)

@@ -5,6 +5,7 @@
  * mtrlMod = 0 (others are rejected at setup); every other branch of the
  * reference is restated. */
 #include "or_bsim4.h"
+#include "or_eval_libm.h"
 
 #define B4_DEXP(A, B, C) do { \
     if ((A) > OR_EXP_THRESHOLD) { (B) = OR_MAX_EXP * (1.0 + (A) - OR_EXP_THRESHOLD); (C) = OR_MAX_EXP; } \

@@ -617,7 +617,7 @@ static int setup_entry(struct or_group_s* g) {
 const or_model_ops or_bsim4_ops = {
   "bsim4", mpar_t, sizeof(mpar) / sizeof(mpar[0]), B4S_NM, ipar_t,
   sizeof(ipar) / sizeof(ipar[0]), B4IS_NI, setup_entry, register_lids, jac_pairs, register_jac,
-  update_state, NULL, load_vectors, load_matrices, read_var, destroy,
+  update_state, NULL, load_vectors, load_matrices, read_var, destroy, sizeof(b4inst),
 };
 
 /* tables must be ready before the first add_model, which precedes setup */

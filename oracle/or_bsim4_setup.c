@@ -332,6 +332,9 @@ int b4_model_init(b4model* m, const or_card* c, const xa_device_options* opt) {
   if (M(DsidewallJctPotential) < 0.1) M(DsidewallJctPotential) = 0.1;
   if (M(DGatesidewallJctPotential) < 0.1) M(DGatesidewallJctPotential) = 0.1;
   /* option subset implemented by this restatement */
+  /* updateTemperature rejects other junction models (N_DEV_MOSFET_B4.C:7515) */
+  if (M(dioMod) != 0 && M(dioMod) != 1 && M(dioMod) != 2)
+    return or_fail(XA_ERR_ARG, "bsim4: specified dioMod not matched");
   if (M(capMod) != 2) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only capMod = 2");
   if (M(mtrlMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
   if (M(igcMod) != 0 || M(igbMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only igcMod = igbMod = 0");

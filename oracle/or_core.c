@@ -11,6 +11,49 @@ static __thread char or_err[512];
 int or_threads = 1;
 int or_mag = 0;
 void or_set_magnitude_mode(int on) { or_mag = on != 0; }
+int or_libm_ulps = 0;
+uint64_t or_libm_seed = 0;
+void or_set_libm_perturbation(int ulps, uint64_t seed) {
+  or_libm_ulps = ulps > 0 ? ulps : 0;
+  or_libm_seed = seed;
+}
+
+/* SplitMix64 finaliser over the argument bits and the seed */
+double or_perturb(double r, double a, double b) {
+  uint64_t ua, ub, h;
+  int64_t k;
+  int j;
+  if (!isfinite(r) || r == 0.0) return r;
+  memcpy(&ua, &a, 8);
+  memcpy(&ub, &b, 8);
+  h = ua ^ (ub * 0x9E3779B97F4A7C15ull) ^ or_libm_seed;
+  h += 0x9E3779B97F4A7C15ull;
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  /* a nonzero offset: every call moves, so every sensitive call shows */
+  k = (int64_t)(h % (uint64_t)(2 * or_libm_ulps)) - or_libm_ulps;
+  if (k >= 0) ++k;
+  for (j = 0; j < k; ++j) r = nextafter(r, INFINITY);
+  for (j = 0; j > k; --j) r = nextafter(r, -INFINITY);
+  return r;
+}
+
+/* Snapshot / restore of every instance record (the values the reference
+ * keeps in its Instance objects between calls). */
+int or_group_save_state(or_group g) {
+  size_t bytes;
+  if (!g || !g->inst || !g->ops->inst_size) return or_fail(XA_ERR_STATE, "oracle: nothing to save");
+  bytes = g->ops->inst_size * (size_t)(g->n > 0 ? g->n : 1);
+  if (!g->saved) g->saved = malloc(bytes);
+  memcpy(g->saved, g->inst, bytes);
+  return XA_OK;
+}
+int or_group_restore_state(or_group g) {
+  if (!g || !g->saved) return or_fail(XA_ERR_STATE, "oracle: no saved state");
+  memcpy(g->inst, g->saved, g->ops->inst_size * (size_t)(g->n > 0 ? g->n : 1));
+  return XA_OK;
+}
 
 int or_fail(int code, const char* msg) {
   snprintf(or_err, sizeof(or_err), "%s", msg);
@@ -72,7 +115,7 @@ int or_group_destroy(or_group g) {
   for (k = 0; k < g->ops->n_islots; ++k) free(g->ival[k]);
   free(g->ival); free(g->igiven); free(g->inst_model); free(g->lead);
   free(g->n_ext); free(g->n_int); free(g->n_sta); free(g->n_sto);
-  free(g->inst); free(g->mdata);
+  free(g->inst); free(g->mdata); free(g->saved);
   free(g);
   return XA_OK;
 }

@@ -10,6 +10,7 @@
  *   Master::loadDAEVectors :2078-2122; Master::loadDAEMatrices :2132-2171.
  */
 #include "or_internal.h"
+#include "or_eval_libm.h"
 
 enum { IS_, RS_, N_, ISR_, NR_, IKF_, TT_, CJO_, VJ_, M_, EG_, XTI_, TIKF_, TBV1_, TBV2_,
        TRS1_, TRS2_, FC_, BV_, IBV_, IRF_, NBV_, IBVL_, NBVL_, TNOM_, KF_, AF_, NMS_ };
@@ -529,5 +530,5 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
 const or_model_ops or_diode_ops = {
   "diode", mpar, sizeof(mpar) / sizeof(mpar[0]), NMS_, ipar, sizeof(ipar) / sizeof(ipar[0]), NIS_,
   setup, register_lids, jac_pairs, register_jac, update_state, NULL, load_vectors,
-  load_matrices, read_var, NULL,
+  load_matrices, read_var, NULL, sizeof(dinst),
 };

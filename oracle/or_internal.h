@@ -53,6 +53,7 @@ struct or_group_s {
   int32_t* lead;
   int* n_ext; int* n_int; int* n_sta; int* n_sto;
   void* inst;         /* model-specific instance array */
+  void* saved;        /* snapshot of inst (or_group_save_state) */
   void* mdata;        /* model-specific processed model array */
   int phase;
   int n_rows;
@@ -77,6 +78,7 @@ typedef struct or_model_ops {
   int (*load_matrices)(struct or_group_s*, const xa_solver_state*, const xa_vectors*);
   int (*read)(struct or_group_s*, const char* name, double* out);
   void (*destroy)(struct or_group_s*);
+  size_t inst_size;   /* bytes per instance record, for state snapshots */
 } or_model_ops;
 
 extern const or_model_ops or_diode_ops;
@@ -107,6 +109,19 @@ extern int or_mag;
 #define OR_ADD(vec, i, val) do { if ((i) >= 0) (vec)[i] += or_mag ? fabs(val) : (val); } while (0)
 #define OR_SUB(vec, i, val) do { if ((i) >= 0) (vec)[i] -= or_mag ? -fabs(val) : (val); } while (0)
 #define OR_SET(vec, i, val) do { if ((i) >= 0) (vec)[i] = or_mag ? fabs(val) : (val); } while (0)
+/* Libm perturbation, for the parity tests' noise floor: with
+ * or_libm_ulps > 0 every exp/log/pow result of the evaluation moves by a
+ * pseudo-random nonzero whole number of ulps in [-or_libm_ulps, or_libm_ulps], drawn
+ * from a hash of its arguments and or_libm_seed.  The spread this causes in
+ * an output entry is how far that entry can move between two conforming
+ * libm implementations (a few ulps apart), i.e. the precision to which the
+ * reference itself determines it.  Off (0) by default. */
+extern int or_libm_ulps;
+extern uint64_t or_libm_seed;
+double or_perturb(double r, double a, double b);
+static inline double or_exp(double a) { double r = exp(a); return or_libm_ulps ? or_perturb(r, a, 0.0) : r; }
+static inline double or_log(double a) { double r = log(a); return or_libm_ulps ? or_perturb(r, a, 1.0) : r; }
+static inline double or_pow(double a, double b) { double r = pow(a, b); return or_libm_ulps ? or_perturb(r, a, b) : r; }
 static inline double or_min(double a, double b) { return a < b ? a : b; }
 static inline double or_max(double a, double b) { return a > b ? a : b; }
 

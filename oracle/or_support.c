@@ -3,6 +3,7 @@
  * lambertw :105-176, limvds :186-203, pnjlim :214-252, fetlim :337-420,
  * qmeyer :545-618, qmeyerderivs :633-742. */
 #include "or_internal.h"
+#include "or_eval_libm.h"
 
 /* N_DEV_DeviceSupport.C:105-176 */
 void or_lambertw(double x, double* w, int* ierr, double* xi) {

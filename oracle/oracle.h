@@ -51,6 +51,10 @@ void or_set_threads(int n);
 void or_default_options(xa_device_options* o);
 /* 1: loads accumulate |contribution| (error scale of each entry). */
 void or_set_magnitude_mode(int on);
+/* exp/log/pow results of the evaluation moved by up to `ulps` ulps (0: off) */
+void or_set_libm_perturbation(int ulps, uint64_t seed);
+int  or_group_save_state(or_group g);
+int  or_group_restore_state(or_group g);
 
 #ifdef __cplusplus
 }

@@ -97,3 +97,39 @@ def bsim4_spec(n, seed=20261203, cards=None, geometry=True, temp_spread=False, n
     if temp_spread:
         params["TEMP"] = rng.uniform(-20.0, 120.0, n)
     return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
+
+
+# Level-1 MOSFET cards: the microbenchmark NMOS/PMOS pair, drain/source
+# resistances (internal nodes) with a zero-bias junction model, NSUB-derived
+# threshold and body factor with lateral diffusion, explicit zero-bias
+# junction capacitances and unequal grading, and sheet resistance with
+# saturation-current density.
+MOSFET1_CARDS = [
+    (1, {"TYPE": 1.0, "VTO": 0.45, "KP": 2e-4, "GAMMA": 0.4, "PHI": 0.7, "LAMBDA": 0.05,
+         "TOX": 2e-9, "CGSO": 3e-10, "CGDO": 3e-10, "CJ": 1e-3, "MJ": 0.5, "CJSW": 3e-10,
+         "MJSW": 0.33}),
+    (1, {"TYPE": -1.0, "VTO": -0.45, "KP": 1e-4, "GAMMA": 0.4, "PHI": 0.7, "LAMBDA": 0.05,
+         "TOX": 2e-9, "CGSO": 3e-10, "CGDO": 3e-10, "CJ": 1e-3, "MJ": 0.5, "CJSW": 3e-10,
+         "MJSW": 0.33}),
+    (1, {"TYPE": 1.0, "VTO": 0.5, "KP": 1e-4, "RD": 50.0, "RS": 30.0, "IS": 1e-15, "PB": 0.9,
+         "TOX": 3e-9, "CGBO": 1e-10, "LAMBDA": 0.1, "FC": 0.4}),
+    (1, {"TYPE": -1.0, "NSUB": 1e17, "TOX": 4e-9, "LD": 2e-9, "UO": 250.0, "CBD": 2e-15,
+         "CBS": 3e-15, "MJ": 0.45, "MJSW": 0.3, "CJSW": 1e-10, "TNOM": 40.0}),
+    (1, {"TYPE": 1.0, "VTO": 0.4, "KP": 3e-4, "GAMMA": 0.3, "RSH": 25.0, "JS": 1e-4,
+         "CJ": 5e-4, "MJ": 0.4, "CJSW": 2e-10, "MJSW": 0.4, "TOX": 2.5e-9, "CGSO": 2e-10}),
+]
+
+
+def mosfet1_spec(n, seed=20261203, cards=None, geometry=True, temp_spread=False):
+    rng = splitmix64(seed)
+    cards = cards if cards is not None else MOSFET1_CARDS
+    mi = rng.integers(0, len(cards), size=n).astype(np.int32)
+    params = {"W": lognormal(rng, 90e-9, 0.2, n), "L": lognormal(rng, 45e-9, 0.2, n)}
+    if geometry:
+        w = params["W"]
+        params.update({"AD": w * 120e-9, "AS": w * 100e-9, "PD": 2 * (w + 120e-9),
+                       "PS": 2 * (w + 100e-9), "NRD": rng.uniform(0.5, 2.0, n),
+                       "NRS": rng.uniform(0.5, 2.0, n), "M": rng.integers(1, 3, size=n).astype(float)})
+    if temp_spread:
+        params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return {"models": cards, "n": n, "model_index": mi, "inst_params": params}

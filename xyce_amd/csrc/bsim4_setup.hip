@@ -340,6 +340,9 @@ int model_init(Model& mref, const Card& c, const xa_device_options& opt) {
   if (m->p.DsidewallJctPotential < 0.1) m->p.DsidewallJctPotential = 0.1;
   if (m->p.DGatesidewallJctPotential < 0.1) m->p.DGatesidewallJctPotential = 0.1;
   /* option subset implemented by this restatement */
+  // updateTemperature rejects other junction models (N_DEV_MOSFET_B4.C:7515)
+  if (m->p.dioMod != 0 && m->p.dioMod != 1 && m->p.dioMod != 2)
+    return fail(XA_ERR_ARG, "bsim4: specified dioMod not matched");
   if (m->p.capMod != 2) return fail(XA_ERR_UNSUPPORTED, "bsim4: only capMod = 2");
   if (m->p.mtrlMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
   if (m->p.igcMod != 0 || m->p.igbMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only igcMod = igbMod = 0");

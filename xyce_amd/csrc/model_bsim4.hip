@@ -260,8 +260,12 @@ enum DField {
   ND
 };
 enum IField { F_MODEL, F_DEXIST, F_SEXIST, NIF };
-enum State { S_VGB, NS };
-static const char* const kStateNames[NS] = {"vgb"};
+// Values the reference's Instance keeps from one evaluation to the next and
+// reads before writing: vgb (as vgb_orig), and the junction charges and
+// capacitances, which are only recomputed when charges are needed (transient,
+// tranop, acop) and are otherwise the last computed ones.
+enum State { S_VGB, S_QBS, S_QBD, S_CAPBS, S_CAPBD, NS };
+static const char* const kStateNames[NS] = {"vgb", "qbs", "qbd", "capbs", "capbd"};
 // Local nodes.  GatePrime, GateMid and GateExt are one node, as are
 // BodyPrime, DrainBody, SourceBody and Body (rgateMod = rbodyMod = 0);
 // DP/SP are the drain/source nodes themselves when no internal node exists.
@@ -542,6 +546,12 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   double cgdo, cgso;
   // vgb keeps the previous evaluation's value until it is recomputed
   vgb = st(a, S_VGB, i);
+  if (!(ss.tranopFlag || ss.acopFlag || ss.transientFlag)) {
+    qbs = st(a, S_QBS, i);
+    qbd = st(a, S_QBD, i);
+    capbs = st(a, S_CAPBS, i);
+    capbd = st(a, S_CAPBD, i);
+  }
 
   // ---- Instance::updateIntermediateVars ----
 
@@ -2580,6 +2590,12 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
     }
   }
   st(a, S_VGB, i) = vgb;
+  if (ChargeComputationNeeded) {
+    st(a, S_QBS, i) = qbs;
+    st(a, S_QBD, i) = qbd;
+    st(a, S_CAPBS, i) = capbs;
+    st(a, S_CAPBD, i) = capbd;
+  }
 
   // ---- Master::loadDAEVectors ----
   Sink<FUSED> s(a, i);
@@ -2819,10 +2835,12 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
 
 // Algorithmic bytes per instance: the SoA columns and card index read once,
 // node voltages (6 LIDs + values), 14 store LIDs, 12 store writes, 3 state
-// LIDs + writes, vgb read + write; the loads write value + target per entry.
+// LIDs + writes, vgb read + write, junction charge state written; the loads
+// write value + target per entry.
 static double traffic(const Group& g, int phase) {
   const double cols = ND, lids = 6 + NSTO + NSTA + 1 /*model index*/;
-  if (phase == 0) return 8.0 * (cols + 6 /*node values*/ + 12 /*sto*/ + 3 /*sta*/ + 2 /*vgb*/) + 4.0 * lids;
+  if (phase == 0)   // transient: the junction charge state is written, not read
+    return 8.0 * (cols + 6 /*node values*/ + 12 /*sto*/ + 3 /*sta*/ + 2 /*vgb*/ + 4 /*qb,cap*/) + 4.0 * lids;
   const bool lim = g.opt.voltageLimiterFlag != 0;
   if (phase == 1) return 16.0 * (4 /*F*/ + 4 /*Q*/ + (lim ? 8 : 0));
   return 16.0 * (38 + 16);

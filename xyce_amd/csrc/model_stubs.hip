@@ -2,6 +2,5 @@
 // with XA_ERR_UNSUPPORTED.
 #include "xa_internal.h"
 namespace xa {
-const ModelImpl* model_mosfet1() { return nullptr; }
 const ModelImpl* model_bjt() { return nullptr; }
 }  // namespace xa
