@@ -111,7 +111,8 @@ typedef struct xa_vectors {
   const double* flagSol;        /* optional: inputOP flag vector            */
   double*       nextSta;
   double*       currSta;        /* written on the first initTran iteration  */
-  const double* lastSta;
+  double*       lastSta;        /* written by the BJT excess phase when
+                                   integration begins                       */
   const double* nextStaDeriv;   /* updateSecondaryState                     */
   double*       nextSto;
   double*       currSto;

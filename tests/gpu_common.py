@@ -12,7 +12,7 @@ ITERS = [dict(newtonIter=0, dcopFlag=1, initJctFlag=1),
          dict(newtonIter=0, transientFlag=1, initTranFlag=1, pdt=PDT),
          dict(newtonIter=1, transientFlag=1, pdt=PDT),
          dict(newtonIter=2, transientFlag=1, pdt=PDT)]
-NOISE_RUNS = 4
+NOISE_RUNS = 6
 
 
 def run_parity(p, rng, iters=ITERS, fused=False):

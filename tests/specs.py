@@ -133,3 +133,35 @@ def mosfet1_spec(n, seed=20261203, cards=None, geometry=True, temp_spread=False)
     if temp_spread:
         params["TEMP"] = rng.uniform(-20.0, 120.0, n)
     return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
+
+
+# Gummel-Poon BJT cards: the microbenchmark NPN (Early effect, high-current
+# roll-off, B-E leakage, series resistances, junction and transit-time
+# charges); a PNP with reverse Early voltage, B-C leakage, base-resistance
+# current crowding (IRB), substrate capacitance and forward-transit-time
+# bias dependence; an NPN through the alias names with temperature exponents
+# and the leakage set by C2/C4; and an NPN with excess phase.
+BJT_CARDS = [
+    (1, {"TYPE": 1.0, "IS": 1e-16, "BF": 100.0, "BR": 1.0, "VAF": 50.0, "IKF": 0.01,
+         "ISE": 1e-14, "NE": 1.5, "RB": 10.0, "RC": 1.0, "RE": 0.5, "CJE": 1e-12,
+         "CJC": 5e-13, "TF": 1e-10, "TR": 1e-8}),
+    (1, {"TYPE": -1.0, "IS": 2e-16, "BF": 60.0, "BR": 3.0, "VAR": 10.0, "IKR": 5e-3,
+         "ISC": 1e-14, "NC": 1.8, "RB": 50.0, "IRB": 1e-4, "RBM": 5.0, "CJS": 2e-13,
+         "MJS": 0.4, "VJS": 0.6, "TF": 2e-10, "XTF": 2.0, "VTF": 3.0, "ITF": 0.05,
+         "CJE": 8e-13, "MJE": 0.4, "CJC": 3e-13, "XCJC": 0.6, "FC": 0.6}),
+    (1, {"TYPE": 1.0, "BFM": 150.0, "VA": 80.0, "IK": 0.02, "JLE": 1e-14, "NLE": 1.6,
+         "XTB": 1.5, "XTI": 3.5, "EG": 1.15, "TNOM": 40.0, "C4": 10.0, "PE": 0.8, "ME": 0.4,
+         "CJE": 1e-12, "NK": 0.6}),
+    (1, {"TYPE": 1.0, "IS": 5e-17, "BF": 80.0, "VAF": 40.0, "TF": 3e-10, "PTF": 30.0,
+         "CJE": 5e-13, "CJC": 2e-13, "RC": 2.0}),
+]
+
+
+def bjt_spec(n, seed=20261203, cards=None, temp_spread=False):
+    rng = splitmix64(seed)
+    cards = cards if cards is not None else BJT_CARDS
+    mi = rng.integers(0, len(cards), size=n).astype(np.int32)
+    params = {"AREA": lognormal(rng, 1.0, 0.2, n)}
+    if temp_spread:
+        params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
