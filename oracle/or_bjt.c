@@ -408,9 +408,10 @@ static int update_intermediate(struct or_group_s* g, binst* I, const xa_solver_s
       I->origFlag = 0;
     } else if (ss->inputOPFlag) {
       const double* fl = v->flagSol;
-      if (fl[I->li_Emit] == 0 || fl[I->li_EmitP] == 0 || fl[I->li_Base] == 0 ||
-          fl[I->li_BaseP] == 0 || fl[I->li_Coll] == 0 || fl[I->li_CollP] == 0 ||
-          fl[I->li_Subst] == 0) {
+      /* a ground node reads its flag as 0 */
+      if (sol(fl, I->li_Emit) == 0 || sol(fl, I->li_EmitP) == 0 || sol(fl, I->li_Base) == 0 ||
+          sol(fl, I->li_BaseP) == 0 || sol(fl, I->li_Coll) == 0 || sol(fl, I->li_CollP) == 0 ||
+          sol(fl, I->li_Subst) == 0) {
         I->vBC = 0;
         I->vBE = I->tVCrit;
         I->vBX = I->vBC;
@@ -802,7 +803,14 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
     }
     if (g->opt.voltageLimiterFlag) {
       double Cp_f = 0.0, Bp_f = 0.0, Ep_f = 0.0, Cp_q = 0.0, Ep_q = 0.0, Bp_q = 0.0;
-      if (!bi->origFlag || bi->offFlag) {
+      if (or_mag && (!bi->origFlag || bi->offFlag)) {
+        /* magnitude mode: the rounding scale of each correction is the sum
+         * of its terms' magnitudes (they cancel near convergence) */
+        Cp_f = fabs(bi->diCEdvBp * vbe_diff) + fabs(bi->diCEdvCp * vce_diff) +
+               fabs(bi->gBCtot * vbc_diff);
+        Bp_f = fabs(bi->gBEtot * vbe_diff) + fabs(bi->gBCtot * vbc_diff);
+        Ep_f = fabs(bi->diCEdvCp * vce_diff) + fabs((bi->diCEdvBp + bi->gBEtot) * vbe_diff);
+      } else if (!bi->origFlag || bi->offFlag) {
         Cp_f = +bi->diCEdvBp * vbe_diff + bi->diCEdvCp * vce_diff - bi->gBCtot * vbc_diff;
         Cp_f *= T;
         Bp_f = bi->gBEtot * vbe_diff + bi->gBCtot * vbc_diff;
@@ -815,7 +823,12 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
         OR_ADD(v->dFdxdVp, bi->li_BaseP, Bp_f);
         OR_ADD(v->dFdxdVp, bi->li_EmitP, Ep_f);
       }
-      if (!bi->origFlag || bi->offFlag) {
+      if (or_mag && (!bi->origFlag || bi->offFlag)) {
+        Cp_q = fabs((bi->capBCdep + bi->capBCdiff) * vbc_diff);
+        Bp_q = fabs((bi->capBEdep + bi->capBEdiff) * vbe_diff) +
+               fabs((bi->capBCdiff + bi->capBCdep + bi->capeqCB) * vbc_diff);
+        Ep_q = fabs(bi->capeqCB * vbc_diff) + fabs((bi->capBEdiff + bi->capBEdep) * vbe_diff);
+      } else if (!bi->origFlag || bi->offFlag) {
         Cp_q = -(bi->capBCdep + bi->capBCdiff) * vbc_diff;
         Cp_q *= T;
         Bp_q = (bi->capBEdep + bi->capBEdiff) * vbe_diff +

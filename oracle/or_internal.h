@@ -122,6 +122,7 @@ double or_perturb(double r, double a, double b);
 static inline double or_exp(double a) { double r = exp(a); return or_libm_ulps ? or_perturb(r, a, 0.0) : r; }
 static inline double or_log(double a) { double r = log(a); return or_libm_ulps ? or_perturb(r, a, 1.0) : r; }
 static inline double or_pow(double a, double b) { double r = pow(a, b); return or_libm_ulps ? or_perturb(r, a, b) : r; }
+static inline double or_tan(double a) { double r = tan(a); return or_libm_ulps ? or_perturb(r, a, 2.0) : r; }
 static inline double or_min(double a, double b) { return a < b ? a : b; }
 static inline double or_max(double a, double b) { return a > b ? a : b; }
 

@@ -432,8 +432,10 @@ static int update_intermediate(struct or_group_s* g, m1inst* I, const xa_solver_
         /* the reference tests drain, gate and source for == 0 but the
          * internal nodes and the bulk for != 0 (:2696-2698) */
         const double* fl = v->flagSol;
-        set = fl[I->li_Drain] == 0 || fl[I->li_Gate] == 0 || fl[I->li_Source] == 0 ||
-              fl[I->li_SourcePrime] != 0 || fl[I->li_DrainPrime] != 0 || fl[I->li_Bulk] != 0;
+        /* a ground node reads its flag as 0 */
+        set = sol(fl, I->li_Drain) == 0 || sol(fl, I->li_Gate) == 0 || sol(fl, I->li_Source) == 0 ||
+              sol(fl, I->li_SourcePrime) != 0 || sol(fl, I->li_DrainPrime) != 0 ||
+              sol(fl, I->li_Bulk) != 0;
       }
       if (set) {
         I->vbs = -1;

@@ -19,7 +19,8 @@ def run_parity(p, rng, iters=ITERS, fused=False):
     """Load `p` on both back ends over `iters` with fresh random voltages and
     check every output with assert_parity; prints a per-output summary."""
     p.set_inputs(currSto=rng.uniform(-0.5, 1.5, p.topo.n_sto),
-                 currSta=rng.uniform(-1e-15, 1e-15, p.topo.n_sta))
+                 currSta=rng.uniform(-1e-15, 1e-15, p.topo.n_sta),
+                 lastSta=rng.uniform(-1e-15, 1e-15, p.topo.n_sta))
     summary = {}
     for it, flags in enumerate(iters):
         print(f"  iteration {it}: {flags}", flush=True)

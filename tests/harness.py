@@ -9,7 +9,7 @@ loadDAEVectors; zero dFdx/dQdx -> loadDAEMatrices.
 import numpy as np
 
 OUT_VECS = ("f", "q", "dFdxdVp", "dQdxdVp", "storeLeadF", "storeLeadQ")
-STATE_VECS = ("nextSto", "nextSta", "currSta")
+STATE_VECS = ("nextSto", "nextSta", "currSta", "lastSta")
 
 
 def splitmix64(seed):
@@ -120,6 +120,7 @@ class Pair:
         t = self.topo
         self.h = {
             "nextSol": np.zeros(t.n_nodes), "currSol": np.zeros(t.n_nodes),
+            "flagSol": np.zeros(t.n_nodes),
             "nextSta": np.zeros(t.n_sta), "currSta": np.zeros(t.n_sta), "lastSta": np.zeros(t.n_sta),
             "nextStaDeriv": np.zeros(t.n_sta),
             "nextSto": np.zeros(t.n_sto), "currSto": np.zeros(t.n_sto),

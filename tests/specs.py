@@ -157,11 +157,19 @@ BJT_CARDS = [
 ]
 
 
-def bjt_spec(n, seed=20261203, cards=None, temp_spread=False):
+def bjt_spec(n, seed=20261203, cards=None, temp_spread=False, lambertw_frac=0.0,
+             off_frac=0.0, ic=False):
     rng = splitmix64(seed)
     cards = cards if cards is not None else BJT_CARDS
     mi = rng.integers(0, len(cards), size=n).astype(np.int32)
     params = {"AREA": lognormal(rng, 1.0, 0.2, n)}
     if temp_spread:
         params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    if lambertw_frac:
+        params["LAMBERTW"] = (rng.random(n) < lambertw_frac).astype(np.float64)
+    if off_frac:
+        params["OFF"] = (rng.random(n) < off_frac).astype(np.float64)
+    if ic:
+        params["IC1"] = rng.uniform(0.3, 0.8, n)
+        params["IC2"] = rng.uniform(0.0, 3.0, n)
     return {"models": cards, "n": n, "model_index": mi, "inst_params": params}

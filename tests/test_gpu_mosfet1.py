@@ -1,7 +1,7 @@
 """GPU parity of the Level-1 MOSFET load against the oracle, through the C ABI.
 
 Bar: harness.assert_parity (1e-12 of each entry's error scale, or twice its
-libm noise; at most 0.01% outliers, none beyond 1e-8).  Covered: the parity
+libm noise; at most 0.05% outliers, none beyond 1e-6).  Covered: the parity
 cards (internal drain/source nodes, NSUB-derived parameters, explicit and
 area-scaled junction capacitances), lead currents, Newton iterations of DC and
 transient analysis, private and shared topologies with ground nodes, limiter

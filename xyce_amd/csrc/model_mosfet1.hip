@@ -380,10 +380,10 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
       bool set = true;
       if (ss.inputOPFlag) {
         // drain, gate, source tested for == 0; the internal nodes and the
-        // bulk for != 0 (:2696-2698)
+        // bulk for != 0 (:2696-2698); a ground node reads its flag as 0
         const double* fl = a.v.flagSol;
-        set = fl[lD] == 0 || fl[lG] == 0 || fl[lS] == 0 || fl[lSP] != 0 || fl[lDP] != 0 ||
-              fl[lB] != 0;
+        set = solv(fl, lD) == 0 || solv(fl, lG) == 0 || solv(fl, lS) == 0 || solv(fl, lSP) != 0 ||
+              solv(fl, lDP) != 0 || solv(fl, lB) != 0;
       }
       if (set) {
         vbs = -1;

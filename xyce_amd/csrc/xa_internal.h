@@ -52,9 +52,9 @@ struct Plan {
   int32_t* d_target = nullptr;
   int64_t* d_seg = nullptr;
   int64_t* d_src = nullptr;
-  double* d_vals = nullptr;        // n * E, instance-major
+  double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
   bool private_targets = true;     // every target written by one instance only
-  int32_t* d_direct = nullptr;     // n * E targets (-1 none) for the fused path
+  int32_t* d_direct = nullptr;     // E * n targets (-1 none) for the fused path, entry-major
 };
 
 struct Group;

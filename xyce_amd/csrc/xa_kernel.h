@@ -21,16 +21,18 @@ struct KArgs {
   xa_solver_state ss;
   xa_device_options opt;
   xa_vectors v;
-  double* vals[NK];                    // instance-major value buffers
+  double* vals[NK];                    // value buffers, entry-major [e][inst]
   int E[NK];
-  const int32_t* direct[NK];           // fused mode: target per (inst, entry)
+  const int32_t* direct[NK];           // fused mode: target per [e][inst]
   double* out[NK];                     // fused mode: output arrays
   unsigned kmask;                      // kinds to emit
 };
 
 // Stamp sink: an instance hands its contributions over in stamp order.
 // Stored mode keeps them for the fixed-order scatter; fused mode adds them
-// straight into outputs whose targets belong to this instance alone.
+// straight into outputs whose targets belong to this instance alone.  Both
+// buffers are entry-major, so one put across a wavefront touches 64
+// consecutive slots.
 template <bool FUSED>
 struct Sink {
   const KArgs& a;
@@ -38,7 +40,7 @@ struct Sink {
   XA_HD Sink(const KArgs& args, int64_t inst) : a(args), i(inst) {}
   XA_HD void put(int k, int e, double val) const {
     if (!(a.kmask & (1u << k))) return;
-    const int64_t idx = i * a.E[k] + e;
+    const int64_t idx = (int64_t)e * a.n + i;
     if (FUSED) {
       const int32_t t = a.direct[k][idx];
       if (t >= 0) {

@@ -516,12 +516,17 @@ static int build_plan(Group& g, int k) {
     target.push_back(t);
     seg.push_back(acc);
   }
+  // sources are slots of the entry-major value buffer: e * n + instance
+  const int64_t n = g.n;
   std::vector<int64_t> src(acc);
   std::vector<int64_t> fillp(start);
+  std::vector<int32_t> direct(tg.size(), -1);
   for (int64_t idx = 0; idx < (int64_t)tg.size(); ++idx) {
     const int32_t t = tg[idx];
+    const int64_t slot = (idx % E) * n + idx / E;
+    direct[slot] = t;
     if (t < 0) continue;
-    src[fillp[t]++] = idx;
+    src[fillp[t]++] = slot;
   }
   p.n_targets = (int64_t)target.size();
   p.n_src = acc;
@@ -529,7 +534,7 @@ static int build_plan(Group& g, int k) {
   if ((rc = upload(&p.d_target, target)) != XA_OK) return rc;
   if ((rc = upload(&p.d_seg, seg)) != XA_OK) return rc;
   if ((rc = upload(&p.d_src, src)) != XA_OK) return rc;
-  if ((rc = upload(&p.d_direct, tg)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_direct, direct)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
   XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
   return XA_OK;
