@@ -74,6 +74,26 @@ def cpu_baseline(model, n_sample, seconds):
     return n_sample * reps / el, f"{reps} load passes over {n_sample} instances in {el:.1f} s"
 
 
+def measured_traffic(model, kernel, n):
+    """HBM bytes per launch of `kernel` as the committed rocprofv3 summary of
+    this workload measured them (FETCH_SIZE + WRITE_SIZE scaled by the copy
+    calibration), rescaled to n instances; None when there is no such row."""
+    path = os.path.join(ROOT, "profiles", f"{model}_load_pass_1M.md")
+    if not os.path.exists(path):
+        return None
+    in_table = False
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == "kernel":
+            in_table = len(cells) > 1 and cells[1].startswith("FETCH")
+        if in_table and len(cells) == 6 and kernel in cells[0]:
+            try:
+                return float(cells[5]) * 1e6 * n / (1 << 20)
+            except ValueError:
+                return None
+    return None
+
+
 def copy_calibration(mib):
     import torch
     n = mib * (1 << 20) // 8
@@ -84,6 +104,56 @@ def copy_calibration(mib):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+
+
+def ring_transient(args):
+    """The plain ring-oscillator workload: one 51-stage BSIM4 ring, DC operating
+    point then backward Euler at 1 ps to 2 ns, every Newton iteration one load
+    pass on the GPU.  The oracle runs the same transient as the CPU baseline
+    and as the reference waveform for the period error."""
+    import numpy as np
+    from xyce_amd.ring import GpuLoader, Ring, period, transient
+    ring = Ring()
+    print("[bench] ring transient on the GPU", file=sys.stderr, flush=True)
+    loader = GpuLoader(ring)
+    t0 = time.perf_counter()
+    t, V, st = transient(loader, ring, t_stop=2e-9, h=1e-12)
+    wall = time.perf_counter() - t0
+    per_gpu = period(t, V[:, 0], ring.vdd / 2)
+    out = {
+        "metric": "BSIM4 ring-oscillator transient: device load passes/s",
+        "value": st["loads"] / st["load_seconds"],
+        "unit": "load passes/s",
+        "n_gpus": 1,
+        "steps": st["steps"],
+        "warmup": 0,
+        "ms_per_step": wall / st["steps"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "51-stage CMOS ring oscillator, BSIM4 level 54 default cards, "
+                               "L=45nm Wn=90nm Wp=180nm, 1fF/stage, VDD=1V, IC node1=0, "
+                               "backward Euler 1ps to 2ns", "instances": ring.n_inst},
+        "newton_iterations": st["newton_iterations"],
+        "us_per_load": st["load_seconds"] / st["loads"] * 1e6,
+        "period_s": per_gpu,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from ring_oracle import OracleLoader
+        print("[bench] ring transient on the oracle", file=sys.stderr, flush=True)
+        tc, Vc, sc = transient(OracleLoader(ring), ring, t_stop=2e-9, h=1e-12)
+        per_cpu = period(tc, Vc[:, 0], ring.vdd / 2)
+        out["cpu_baseline"] = {"value": sc["loads"] / sc["load_seconds"], "unit": "load passes/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the same transient, {sc['loads']} load passes"}
+        out["period_rel_err_vs_oracle"] = (abs(per_gpu - per_cpu) / per_cpu
+                                           if per_gpu and per_cpu else None)
+        out["max_wave_err_vs_oracle"] = float(np.max(np.abs(V - Vc)) / ring.vdd)
+    print(json.dumps(out))
 
 
 def main():
@@ -97,6 +167,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fused", action="store_true", help="time the one-call fused load")
+    ap.add_argument("--ring", action="store_true",
+                    help="run the 51-stage BSIM4 ring-oscillator transient (1 ps steps to 2 ns) "
+                         "instead of the microbenchmark")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -106,6 +179,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ring:
+        ring_transient(args)
         return
 
     import torch
@@ -165,7 +241,9 @@ def main():
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": dom_bytes},
+                     "traffic": None if args.fused else
+                     measured_traffic(args.model, f"{args.model}::k_update<false>", args.n),
+                     "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,46 @@
+"""Ring-oscillator transient with the GPU loads against the same transient
+with the oracle's loads (same driver, same linear algebra).
+
+Bar: every node voltage at every time point within 1e-6 of the supply
+(1e-6 relative), the same Newton iteration count, and the oscillation period
+within 1e-6 relative.  The 3-stage ring is the small brute-force case; the
+51-stage ring at 1 ps steps to 2 ns is the plain ring-oscillator workload.
+"""
+import numpy as np
+import pytest
+
+from ring_oracle import OracleLoader
+from xyce_amd.ring import GpuLoader, Ring, crossings, period, transient
+
+pytestmark = pytest.mark.gpu
+WAVE_RTOL = 1e-6
+
+
+def _compare(stages, t_stop):
+    ring = Ring(stages=stages)
+    print(f"ring {stages} stages to {t_stop:g} s", flush=True)
+    tg, Vg, sg = transient(GpuLoader(ring), ring, t_stop=t_stop)
+    tc, Vc, sc = transient(OracleLoader(ring), ring, t_stop=t_stop)
+    err = np.max(np.abs(Vg - Vc)) / ring.vdd
+    print(f"  newton {sg['newton_iterations']} vs {sc['newton_iterations']}, max |dV|/VDD {err:.3e}, "
+          f"gpu load {sg['load_seconds'] / sg['loads'] * 1e6:.1f} us", flush=True)
+    assert sg["newton_iterations"] == sc["newton_iterations"]
+    assert err <= WAVE_RTOL
+    return ring, tg, Vg, Vc
+
+
+def test_ring3_brute_force():
+    _compare(3, 2e-10)
+
+
+def test_ring51_transient():
+    ring, t, Vg, Vc = _compare(51, 2e-9)
+    # the crossing times of every stage (a period needs two rising edges of
+    # one node; in 2 ns each node crosses about twice)
+    for k in range(ring.N):
+        cg = crossings(t, Vg[:, k], ring.vdd / 2)
+        cc = crossings(t, Vc[:, k], ring.vdd / 2)
+        assert len(cg) == len(cc)
+        np.testing.assert_allclose(cg, cc, rtol=WAVE_RTOL)
+    pg, pc = period(t, Vg[:, 0], 0.5), period(t, Vc[:, 0], 0.5)
+    assert pg is not None and abs(pg - pc) <= WAVE_RTOL * pc

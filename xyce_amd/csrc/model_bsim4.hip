@@ -519,7 +519,8 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   double Igs, gIgsg, gIgss, Igd, gIgdg, gIgdd, Igb, gIgbg, gIgbd, gIgbs, gIgbb, cd;
   double qinv, qgate, qdrn, qsrc, qbulk, qgmid, qgdo, qgso, qgb, qgmb, qchqs, qcheq;
   double cggb, cgsb, cgdb, cdgb, cdsb, cddb, cbgb, cbsb, cbdb, csgb, cssb, csdb;
-  double cgbb, csbb, cdbb, cbbb, cqdb, cqsb, cqgb, cqbb, gtau, CoxWL, taunet;
+  double cgbb, csbb, cdbb, cbbb, cqdb, cqsb, cqgb, cqbb, CoxWL, taunet;
+  double gtau = 0.0;   // trnqsMod = 0: no NQS time constant
   double qbs, qbd, capbs, capbd, Igate, IgateMid, Igate_Jdxp, IgateMid_Jdxp;
   double Idrain, Isource, Idbb, Idbbp, Isbb, Isbbp, Ibpb;
   double gcggb, gcgdb, gcgsb, gcgbb, gcdgb, gcsgb, gcbgb, gcddb, gcdsb, gcsdb, gcssb;
@@ -870,7 +871,7 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
           cbs = IV(IVjsmFwd) + IV(SslpFwd) * (vbs_jct - IV(vjsmFwd)) + gmin * vbs_jct;
         }
         break;
-      default: break;
+      default: gbs = cbs = 0.0; break;   // dioMod is checked at setup
     }
   }
   Nvtmd = C.vtm * C.DjctEmissionCoeff;
@@ -941,7 +942,7 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
           cbd = IV(IVjdmFwd) + IV(DslpFwd) * (vbd_jct - IV(vjdmFwd)) + gmin * vbd_jct;
         }
         break;
-      default: break;
+      default: gbd = cbd = 0.0; break;
     }
   }
 

@@ -49,9 +49,11 @@ struct Card {
 struct Plan {
   int E = 0;                       // value slots per instance
   int64_t n_targets = 0, n_src = 0;
-  int32_t* d_target = nullptr;
-  int64_t* d_seg = nullptr;
-  int64_t* d_src = nullptr;
+  int32_t* d_target = nullptr;     // null when the targets are t0, t0+1, ... (contiguous)
+  int32_t t0 = 0;
+  bool idx32 = false;              // seg/src held as int32 (every index < 2^31)
+  void* d_seg = nullptr;           // n_targets + 1 segment starts (int32 or int64)
+  void* d_src = nullptr;           // n_src value-buffer slots (int32 or int64)
   double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
   bool private_targets = true;     // every target written by one instance only
   int32_t* d_direct = nullptr;     // E * n targets (-1 none) for the fused path, entry-major

@@ -61,26 +61,41 @@ static int upload(T** dst, const std::vector<T>& src) {
 // the output, which reproduces the sequential `*ptr += value` of the
 // reference bit for bit.
 // ---------------------------------------------------------------------------
+template <typename I, bool CONTIG>
 __global__ void __launch_bounds__(256)
-k_scatter_add(int64_t nt, const int32_t* __restrict__ target,
-              const int64_t* __restrict__ seg, const int64_t* __restrict__ src,
+k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
+              const I* __restrict__ seg, const I* __restrict__ src,
               const double* __restrict__ vals, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  const int32_t o = target[t];
-  const int64_t s1 = seg[t + 1];
+  const int64_t o = CONTIG ? t0 + t : target[t];
+  const I s1 = seg[t + 1];
   double acc = out[o];
-  for (int64_t j = seg[t]; j < s1; ++j) acc += vals[src[j]];
+  for (I j = seg[t]; j < s1; ++j) acc += vals[src[j]];
   out[o] = acc;
 }
 
+template <typename I, bool CONTIG>
 __global__ void __launch_bounds__(256)
-k_scatter_assign(int64_t nt, const int32_t* __restrict__ target,
-                 const int64_t* __restrict__ seg, const int64_t* __restrict__ src,
+k_scatter_assign(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
+                 const I* __restrict__ seg, const I* __restrict__ src,
                  const double* __restrict__ vals, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  out[target[t]] = vals[src[seg[t + 1] - 1]];
+  const int64_t o = CONTIG ? t0 + t : target[t];
+  out[o] = vals[src[seg[t + 1] - 1]];
+}
+
+template <typename I, bool CONTIG>
+static void launch_scatter(const Plan& p, bool assign, hipStream_t st, double* out) {
+  const I* seg = static_cast<const I*>(p.d_seg);
+  const I* src = static_cast<const I*>(p.d_src);
+  if (assign)
+    hipLaunchKernelGGL((k_scatter_assign<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
+                       st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
+  else
+    hipLaunchKernelGGL((k_scatter_add<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
+                       st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
 }
 
 static double* kind_out(const xa_vectors& v, int k) {
@@ -103,14 +118,15 @@ static int run_scatter(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   double* out = kind_out(v, k);
   if (!out || p.n_targets == 0) return XA_OK;
   if (g.impl->skip_kind && g.impl->skip_kind(g, ss, k)) return XA_OK;
-  if (kind_is_assign(k))
-    hipLaunchKernelGGL(k_scatter_assign, dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
-                       g.ctx->stream, p.n_targets, p.d_target, p.d_seg, p.d_src,
-                       p.d_vals, out);
-  else
-    hipLaunchKernelGGL(k_scatter_add, dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
-                       g.ctx->stream, p.n_targets, p.d_target, p.d_seg, p.d_src,
-                       p.d_vals, out);
+  const bool assign = kind_is_assign(k);
+  const bool contig = p.d_target == nullptr;
+  if (p.idx32) {
+    if (contig) launch_scatter<int32_t, true>(p, assign, g.ctx->stream, out);
+    else launch_scatter<int32_t, false>(p, assign, g.ctx->stream, out);
+  } else {
+    if (contig) launch_scatter<int64_t, true>(p, assign, g.ctx->stream, out);
+    else launch_scatter<int64_t, false>(p, assign, g.ctx->stream, out);
+  }
   XA_HIP(hipGetLastError());
   return XA_OK;
 }
@@ -531,9 +547,28 @@ static int build_plan(Group& g, int k) {
   p.n_targets = (int64_t)target.size();
   p.n_src = acc;
   int rc;
-  if ((rc = upload(&p.d_target, target)) != XA_OK) return rc;
-  if ((rc = upload(&p.d_seg, seg)) != XA_OK) return rc;
-  if ((rc = upload(&p.d_src, src)) != XA_OK) return rc;
+  // contiguous targets need no target array; 32-bit indices when they fit
+  bool contig = true;
+  for (size_t t = 0; t < target.size() && contig; ++t) contig = target[t] == target[0] + (int32_t)t;
+  p.t0 = target.empty() ? 0 : target[0];
+  if (!contig && (rc = upload(&p.d_target, target)) != XA_OK) return rc;
+  p.idx32 = acc < ((int64_t)1 << 31) && (int64_t)E * n < ((int64_t)1 << 31);
+  if (p.idx32) {
+    std::vector<int32_t> s32(seg.begin(), seg.end()), r32(src.begin(), src.end());
+    int32_t* ds = nullptr;
+    int32_t* dr = nullptr;
+    if ((rc = upload(&ds, s32)) != XA_OK) return rc;
+    if ((rc = upload(&dr, r32)) != XA_OK) return rc;
+    p.d_seg = ds;
+    p.d_src = dr;
+  } else {
+    int64_t* ds = nullptr;
+    int64_t* dr = nullptr;
+    if ((rc = upload(&ds, seg)) != XA_OK) return rc;
+    if ((rc = upload(&dr, src)) != XA_OK) return rc;
+    p.d_seg = ds;
+    p.d_src = dr;
+  }
   if ((rc = upload(&p.d_direct, direct)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
   XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
