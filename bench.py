@@ -109,13 +109,14 @@ def copy_calibration(mib):
 def ring_transient(args):
     """The plain ring-oscillator workload: one 51-stage BSIM4 ring, DC operating
     point then backward Euler at 1 ps to 2 ns, every Newton iteration one load
-    pass on the GPU.  The oracle runs the same transient as the CPU baseline
+    pass on the GPU (the one-call fused load: all 102 instances share one
+    block, so the pass is one kernel).  The oracle runs the same transient as the CPU baseline
     and as the reference waveform for the period error."""
     import numpy as np
     from xyce_amd.ring import GpuLoader, Ring, period, transient
     ring = Ring()
     print("[bench] ring transient on the GPU", file=sys.stderr, flush=True)
-    loader = GpuLoader(ring)
+    loader = GpuLoader(ring, fused=True)
     t0 = time.perf_counter()
     t, V, st = transient(loader, ring, t_stop=2e-9, h=1e-12)
     wall = time.perf_counter() - t0
@@ -156,6 +157,86 @@ def ring_transient(args):
     print(json.dumps(out))
 
 
+def mc_transient(args):
+    """The batched Monte Carlo workload: n rings (odd stage counts 11..101,
+    per-ring |VTH0| and TOXE samples) through one transient on the GPU -- one
+    device load of all rings and one device Newton solve per iteration.  A
+    seeded sample of the same rings runs one by one through the plain drop-in
+    (GPU loads, host Newton) to price the plain form, and through the oracle
+    for the period error."""
+    import numpy as np
+    import torch
+    from xyce_amd.mc import BatchTransient, RingBatch
+    from xyce_amd.ring import GpuLoader, period, transient
+    batch = RingBatch(n_rings=args.mc_rings)
+    h = 1e-12
+    print(f"[bench] building {batch.n_rings} rings", file=sys.stderr, flush=True)
+    bt = BatchTransient(batch)
+    print("[bench] batched transient", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, V0 = bt.run(t_stop=args.mc_tstop, h=h)
+    batched = time.perf_counter() - t0
+    rng = np.random.default_rng(np.random.SeedSequence(7))
+    sample = sorted(rng.choice(batch.n_rings, size=min(args.mc_sample, batch.n_rings),
+                               replace=False).tolist())
+    plain_times, per_err = [], []
+    for r in sample:
+        ring = batch.ring(r)
+        print(f"[bench] plain drop-in, ring {r} ({ring.N} stages)", file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        transient(GpuLoader(ring), ring, t_stop=args.mc_tstop, h=h)
+        plain_times.append(time.perf_counter() - t1)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from ring_oracle import OracleLoader
+        cpu_t = 0.0
+        for r in sample:
+            ring = batch.ring(r)
+            print(f"[bench] oracle, ring {r}", file=sys.stderr, flush=True)
+            t1 = time.perf_counter()
+            to, Vo, _ = transient(OracleLoader(ring), ring, t_stop=args.mc_tstop, h=h)
+            cpu_t += time.perf_counter() - t1
+            pb = period(t, V0[:, r], ring.vdd / 2)
+            po = period(to, Vo[:, ring.ic_node], ring.vdd / 2)
+            if pb is not None and po is not None:
+                per_err.append(abs(pb - po) / po)
+        cpu = {"value": len(sample) / cpu_t, "unit": "ring transients/s", "cores": 1,
+               "kind": "port", "sample": f"{len(sample)} rings {sample} one by one on the oracle"}
+    plain_total = float(np.mean(plain_times)) * batch.n_rings
+    periods = [period(t, V0[:, r], 0.5) for r in range(batch.n_rings)]
+    measured = [p for p in periods if p is not None]
+    out = {
+        "metric": "batched-vs-plain MC transient speedup",
+        "value": plain_total / batched,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": int(round(args.mc_tstop / h)),
+        "warmup": 0,
+        "ms_per_step": batched / round(args.mc_tstop / h) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{batch.n_rings} CMOS ring oscillators, stages uniform over odd "
+                               "11..101, BSIM4 level 54 default cards with per-ring |VTH0| "
+                               "N(1,0.03) (DELVTO) and TOXE N(1,0.02), 1fF/stage, VDD=1V, "
+                               f"backward Euler 1ps to {args.mc_tstop:g}s",
+                   "rings": batch.n_rings, "instances": int(2 * batch.stages.sum())},
+        "batched_seconds": batched,
+        "newton_iterations": bt.newton_iterations,
+        "plain_seconds_per_ring": plain_times,
+        "plain_rings_sampled": sample,
+        "plain_seconds_extrapolated": plain_total,
+        "rings_with_period": len(measured),
+        "period_rel_err_vs_oracle_max": max(per_err) if per_err else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +251,13 @@ def main():
     ap.add_argument("--ring", action="store_true",
                     help="run the 51-stage BSIM4 ring-oscillator transient (1 ps steps to 2 ns) "
                          "instead of the microbenchmark")
+    ap.add_argument("--mc", action="store_true",
+                    help="run the batched Monte Carlo ring-oscillator transient (4096 rings) and "
+                         "its speedup over the plain drop-in")
+    ap.add_argument("--mc-rings", type=int, default=4096)
+    ap.add_argument("--mc-tstop", type=float, default=6e-9)
+    ap.add_argument("--mc-sample", type=int, default=4,
+                    help="rings run through the plain drop-in and the oracle")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -182,6 +270,9 @@ def main():
         return
     if args.ring:
         ring_transient(args)
+        return
+    if args.mc:
+        mc_transient(args)
         return
 
     import torch
@@ -206,7 +297,10 @@ def main():
     ms_step = (t1 - t0) / args.steps * 1e3
     value = args.n / (ms_step * 1e-3)
     bu, bv, bm = wl.master.traffic_bytes()
-    if args.fused:
+    # xa_load_all runs one fused kernel (all phases timed as update) or, where
+    # the model is faster split at this size, the four calls
+    one_kernel = args.fused and ms_vec == 0.0 and ms_mat == 0.0
+    if one_kernel:
         dom_bytes, dom_ms, dom_name = bu + bv + bm, ms_up, "fused load"
     else:
         dom_bytes, dom_ms, dom_name = bu, ms_up, "update_state"
@@ -236,12 +330,13 @@ def main():
         "config": {"workload": f"device-eval microbenchmark, {args.n} {args.model} instances, "
                                "private nodes, V~U[-0.5,1.5], one load pass per step",
                    "instances": args.n, "device": args.model,
-                   "path": "fused load_all" if args.fused else
-                           "updateState+loadDAEVectors+loadDAEMatrices"},
+                   "path": "fused load_all" if one_kernel else
+                           ("load_all (split at this size)" if args.fused else
+                            "updateState+loadDAEVectors+loadDAEMatrices")},
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.fused else
+                     "traffic": None if one_kernel else
                      measured_traffic(args.model, f"{args.model}::k_update<false>", args.n),
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,

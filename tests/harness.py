@@ -30,6 +30,20 @@ class Topology:
                 k = n_ext[i] + n_int[i]
                 nodes.append(np.arange(next_lid, next_lid + k, dtype=np.int32))
                 next_lid += k
+        elif mode == "blocked":
+            # nodes shared only within groups of 64 consecutive instances
+            # (so every target is written by one block of the fused load)
+            pool = n_shared_nodes or 24
+            groups = (n + 63) // 64
+            next_lid = groups * pool
+            for i in range(n):
+                g = i // 64
+                ext = (g * pool + rng.integers(0, pool, size=n_ext[i])).astype(np.int32)
+                if ground_frac > 0:
+                    ext[rng.random(n_ext[i]) < ground_frac] = -1
+                it = np.arange(next_lid, next_lid + n_int[i], dtype=np.int32)
+                next_lid += n_int[i]
+                nodes.append(np.concatenate([ext, it]))
         elif mode == "shared":
             pool = n_shared_nodes or max(2, n // 2)
             next_lid = pool

@@ -77,11 +77,12 @@ def test_bjt_parity_limiter_off():
     run_parity(p, np.random.default_rng(412), iters=BJT_ITERS)
 
 
-def test_bjt_fused_equals_split():
+@pytest.mark.parametrize("topology", ["private", "blocked"])
+def test_bjt_fused_equals_split(topology):
     spec = bjt_spec(6000, seed=413)
-    print("bjt fused vs split n=6000", flush=True)
+    print(f"bjt fused vs split {topology} n=6000", flush=True)
     lead = (np.arange(6000) % 3 == 0).astype(np.int32)
-    twin_drive("bjt", spec, 414, np.random.default_rng(415), fused_b=True, lead=lead,
+    twin_drive("bjt", spec, 414, np.random.default_rng(415), fused_b=True, topology=topology, lead=lead,
                iters=BJT_ITERS)
 
 

@@ -58,10 +58,11 @@ def test_bsim4_parity_limiter_off():
     run_parity(p, np.random.default_rng(212))
 
 
-def test_bsim4_fused_equals_split():
+@pytest.mark.parametrize("topology", ["private", "blocked"])
+def test_bsim4_fused_equals_split(topology):
     spec = bsim4_spec(4000, seed=213, cards=BSIM4_CARDS[:2], geometry=False)
-    print("bsim4 fused vs split n=4000", flush=True)
-    twin_drive("bsim4", spec, 214, np.random.default_rng(215), fused_b=True)
+    print(f"bsim4 fused vs split {topology} n=4000", flush=True)
+    twin_drive("bsim4", spec, 214, np.random.default_rng(215), fused_b=True, topology=topology)
 
 
 def test_bsim4_run_to_run_bit_identical():

@@ -65,6 +65,16 @@ def test_diode_fused_equals_split():
     _run(p, np.random.default_rng(109), fused=True)
 
 
+def test_diode_fused_equals_split_blocked():
+    """Nodes shared within blocks of instances: the fused load writes them
+    from LDS in the scatter's order."""
+    from gpu_common import twin_drive
+    spec = diode_spec(10000, seed=113)
+    lead = (np.arange(10000) % 3 == 0).astype(np.int32)
+    twin_drive("diode", spec, 114, np.random.default_rng(115), fused_b=True, topology="blocked",
+               lead=lead, iters=ITERS)
+
+
 def test_diode_run_to_run_bit_identical():
     n = 30000
     spec = diode_spec(n, seed=110)

@@ -1,0 +1,49 @@
+"""Batched Monte Carlo ring oscillators: the batch (device loads of every
+ring together, device Newton solve per ring) against each ring run alone
+through the plain drop-in (GPU loads, host Newton) and through the oracle.
+
+Bar: every ring's node-0 waveform within 1e-6 of the supply of the plain
+run and of the oracle run (the two Newton loops stop on the same 1e-9 V
+update bound, the batch when every ring meets it), and the periods within
+1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from ring_oracle import OracleLoader
+from xyce_amd.mc import BatchTransient, RingBatch
+from xyce_amd.ring import GpuLoader, period, transient
+
+pytestmark = pytest.mark.gpu
+WAVE_RTOL = 1e-6
+
+
+def test_batch_matches_plain_and_oracle():
+    batch = RingBatch(stages=[3, 5, 11, 7, 3, 9, 13, 5], seed=31)
+    print(f"batch of {batch.n_rings} rings, stages {batch.stages.tolist()}", flush=True)
+    bt = BatchTransient(batch)
+    t, V0 = bt.run(t_stop=3e-10)
+    print(f"  batched: {bt.newton_iterations} Newton iterations", flush=True)
+    for r in range(batch.n_rings):
+        ring = batch.ring(r)
+        tg, Vg, _ = transient(GpuLoader(ring), ring, t_stop=3e-10)
+        to, Vo, _ = transient(OracleLoader(ring), ring, t_stop=3e-10)
+        eg = np.max(np.abs(V0[:, r] - Vg[:, ring.ic_node])) / ring.vdd
+        eo = np.max(np.abs(V0[:, r] - Vo[:, ring.ic_node])) / ring.vdd
+        print(f"  ring {r} ({ring.N} stages): vs plain {eg:.2e}, vs oracle {eo:.2e}", flush=True)
+        assert eg <= WAVE_RTOL and eo <= WAVE_RTOL
+
+
+def test_batch_periods_match_oracle():
+    """Rings long enough to oscillate for two periods within the run."""
+    batch = RingBatch(stages=[11, 15, 21], seed=32)
+    bt = BatchTransient(batch)
+    t, V0 = bt.run(t_stop=1.6e-9)
+    for r in range(batch.n_rings):
+        ring = batch.ring(r)
+        to, Vo, _ = transient(OracleLoader(ring), ring, t_stop=1.6e-9)
+        pb = period(t, V0[:, r], ring.vdd / 2)
+        po = period(to, Vo[:, ring.ic_node], ring.vdd / 2)
+        print(f"  ring {r} ({ring.N} stages): period {pb:.6e} vs oracle {po:.6e}", flush=True)
+        assert pb is not None and po is not None
+        assert abs(pb - po) <= WAVE_RTOL * po

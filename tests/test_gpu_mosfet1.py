@@ -39,11 +39,12 @@ def test_mosfet1_parity_limiter_off_plain():
     run_parity(p, np.random.default_rng(306))
 
 
-def test_mosfet1_fused_equals_split():
+@pytest.mark.parametrize("topology", ["private", "blocked"])
+def test_mosfet1_fused_equals_split(topology):
     spec = mosfet1_spec(6000, seed=307)
-    print("mosfet1 fused vs split n=6000", flush=True)
+    print(f"mosfet1 fused vs split {topology} n=6000", flush=True)
     lead = (np.arange(6000) % 3 == 0).astype(np.int32)
-    twin_drive("mosfet1", spec, 308, np.random.default_rng(309), fused_b=True, lead=lead)
+    twin_drive("mosfet1", spec, 308, np.random.default_rng(309), fused_b=True, topology=topology, lead=lead)
 
 
 def test_mosfet1_run_to_run_bit_identical():

@@ -16,10 +16,10 @@ pytestmark = pytest.mark.gpu
 WAVE_RTOL = 1e-6
 
 
-def _compare(stages, t_stop):
+def _compare(stages, t_stop, fused=False):
     ring = Ring(stages=stages)
-    print(f"ring {stages} stages to {t_stop:g} s", flush=True)
-    tg, Vg, sg = transient(GpuLoader(ring), ring, t_stop=t_stop)
+    print(f"ring {stages} stages to {t_stop:g} s, fused={fused}", flush=True)
+    tg, Vg, sg = transient(GpuLoader(ring, fused=fused), ring, t_stop=t_stop)
     tc, Vc, sc = transient(OracleLoader(ring), ring, t_stop=t_stop)
     err = np.max(np.abs(Vg - Vc)) / ring.vdd
     print(f"  newton {sg['newton_iterations']} vs {sc['newton_iterations']}, max |dV|/VDD {err:.3e}, "
@@ -29,12 +29,14 @@ def _compare(stages, t_stop):
     return ring, tg, Vg, Vc
 
 
-def test_ring3_brute_force():
-    _compare(3, 2e-10)
+@pytest.mark.parametrize("fused", [False, True])
+def test_ring3_brute_force(fused):
+    _compare(3, 2e-10, fused)
 
 
-def test_ring51_transient():
-    ring, t, Vg, Vc = _compare(51, 2e-9)
+@pytest.mark.parametrize("fused", [False, True])
+def test_ring51_transient(fused):
+    ring, t, Vg, Vc = _compare(51, 2e-9, fused)
     # the crossing times of every stage (a period needs two rising edges of
     # one node; in 2 ns each node crosses about twice)
     for k in range(ring.N):

@@ -397,8 +397,7 @@ XA_HD void depletion(double v, double cz, double pot, double mj, double fcp, dou
 
 // ---- device evaluation ---------------------------------------------------
 template <bool FUSED>
-__global__ void __launch_bounds__(256) k_update(KArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   if (i >= a.n) return;
   const DevCard& mc = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
   const xa_solver_state& ss = a.ss;
@@ -719,6 +718,12 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   s.put(K_DQDX, 13, capBEdiff + capBEdep);
 }
 
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_update(KArgs a) {
+  eval<FUSED>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (FUSED) fused_writeout(a);
+}
+
 static bool has_excess_phase(const Group& g) {
   for (size_t k = 0; k < g.models.size(); ++k)
     if (reinterpret_cast<const DevCard*>(&g.mcard[k * g.mcard_stride])->tdExcess != 0) return true;
@@ -738,12 +743,7 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
     return fail(XA_ERR_ARG, "bjt update: flagSol required with inputOPFlag");
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
-  if (fused)
-    hipLaunchKernelGGL(k_update<true>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  XA_HIP(hipGetLastError());
-  return XA_OK;
+  return launch_model_kernel(g, a, fused, (const void*)k_update<false>, (const void*)k_update<true>);
 }
 
 // Algorithmic bytes per instance (no internal nodes, transient, no excess
@@ -784,6 +784,8 @@ const ModelImpl* model_bjt() {
     x.state_names = nullptr;
     x.n_state = 0;
     x.traffic = bjt::traffic;
+    x.fused_block = 64;
+    x.fused_at_scale = true;
     return x;
   }();
   return &m;

@@ -478,8 +478,7 @@ XA_HD double stv(const double* s, int32_t lid) { return lid >= 0 ? s[lid] : 0.0;
 #define IV(x) dfld(a, V_##x, i)
 
 template <bool FUSED>
-__global__ void __launch_bounds__(256) k_update(KArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   if (i >= a.n) return;
   const DevCard& C = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
   const xa_solver_state& ss = a.ss;
@@ -2815,6 +2814,12 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
     s.put(K_DQDX, 15, (CAPcbbb) * Mp);
 }
 
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_update(KArgs a) {
+  eval<FUSED>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (FUSED) fused_writeout(a);
+}
+
 #undef PP
 #undef IV
 
@@ -2826,12 +2831,7 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
     return fail(XA_ERR_ARG, "bsim4 update: flagSol required with inputOPFlag");
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
-  if (fused)
-    hipLaunchKernelGGL(k_update<true>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  XA_HIP(hipGetLastError());
-  return XA_OK;
+  return launch_model_kernel(g, a, fused, (const void*)k_update<false>, (const void*)k_update<true>);
 }
 
 // Algorithmic bytes per instance: the SoA columns and card index read once,
@@ -2877,6 +2877,8 @@ const ModelImpl* model_bsim4() {
     x.state_names = kStateNames;
     x.n_state = NS;
     x.traffic = traffic;
+    x.fused_block = 128;
+    x.fused_at_scale = false;
     return x;
   }();
   return &m;

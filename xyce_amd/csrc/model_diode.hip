@@ -231,8 +231,7 @@ static void stamps(const Group& g, int64_t i, int kind, std::vector<Stamp>& out)
 
 // ---- device evaluation ---------------------------------------------------
 template <bool FUSED>
-__global__ void __launch_bounds__(256) k_update(KArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   if (i >= a.n) return;
   const DevCard mc = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
   const int32_t lPos = node(a, N_POS, i), lNeg = node(a, N_NEG, i), lPri = node(a, N_PRI, i);
@@ -435,17 +434,18 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   s.put(K_DQDX, 3, Cd);
 }
 
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_update(KArgs a) {
+  eval<FUSED>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (FUSED) fused_writeout(a);
+}
+
 static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
   if (!v.nextSol || !v.nextSto) return fail(XA_ERR_ARG, "diode update: nextSol/nextSto required");
   if (ss.newtonIter == 0 && !v.currSto) return fail(XA_ERR_ARG, "diode update: currSto required");
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
-  if (fused)
-    hipLaunchKernelGGL(k_update<true>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  XA_HIP(hipGetLastError());
-  return XA_OK;
+  return launch_model_kernel(g, a, fused, (const void*)k_update<false>, (const void*)k_update<true>);
 }
 
 // Algorithmic bytes per instance (default card: RS = 0, limiter on).
@@ -483,6 +483,8 @@ const ModelImpl* model_diode() {
     x.state_names = diode::kStateNames;
     x.n_state = diode::NS;
     x.traffic = diode::traffic;
+    x.fused_block = 256;
+    x.fused_at_scale = true;
     return x;
   }();
   return &m;

@@ -342,8 +342,7 @@ static void stamps(const Group& g, int64_t i, int kind, std::vector<Stamp>& out)
 
 // ---- device evaluation ---------------------------------------------------
 template <bool FUSED>
-__global__ void __launch_bounds__(256) k_update(KArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   if (i >= a.n) return;
   const DevCard mc = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
   const xa_solver_state& ss = a.ss;
@@ -559,8 +558,14 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   // Meyer gate capacitances (:3153-3268)
   double capgs, capgd, capgb;
   const double OxideCap = dfld(a, D_OXCAP, i);
-  if (mode > 0) qmeyer(vgs, vgd, Von, Vdsat, capgs, capgd, capgb, tPhi, OxideCap);
-  else qmeyer(vgd, vgs, Von, Vdsat, capgd, capgs, capgb, tPhi, OxideCap);
+  {
+    // one call with the operands swapped by mode keeps the outputs in registers
+    double c_src, c_drn;
+    qmeyer(mode > 0 ? vgs : vgd, mode > 0 ? vgd : vgs, Von, Vdsat, c_src, c_drn, capgb, tPhi,
+           OxideCap);
+    capgs = mode > 0 ? c_src : c_drn;
+    capgd = mode > 0 ? c_drn : c_src;
+  }
   const int32_t acgs = stalid(a, A_capgs, i), acgd = stalid(a, A_capgd, i), acgb = stalid(a, A_capgb, i);
   double Capgs, Capgd, Capgb;
   if (ss.dcopFlag) {
@@ -707,6 +712,12 @@ __global__ void __launch_bounds__(256) k_update(KArgs a) {
   s.put(K_DQDX, 13, (+gcbs + gcgs) * Mp);
 }
 
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_update(KArgs a) {
+  eval<FUSED>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (FUSED) fused_writeout(a);
+}
+
 static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
   if (!v.nextSol || !v.nextSto || !v.nextSta)
     return fail(XA_ERR_ARG, "mosfet1 update: nextSol/nextSto/nextSta required");
@@ -718,12 +729,7 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
     return fail(XA_ERR_ARG, "mosfet1 update: flagSol required with inputOPFlag");
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
-  if (fused)
-    hipLaunchKernelGGL(k_update<true>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(grid_for(g.n)), dim3(kBlock), 0, g.ctx->stream, a);
-  XA_HIP(hipGetLastError());
-  return XA_OK;
+  return launch_model_kernel(g, a, fused, (const void*)k_update<false>, (const void*)k_update<true>);
 }
 
 // Algorithmic bytes per instance (no internal nodes, transient): 31 SoA
@@ -763,6 +769,8 @@ const ModelImpl* model_mosfet1() {
     x.state_names = nullptr;
     x.n_state = 0;
     x.traffic = mosfet1::traffic;
+    x.fused_block = 64;
+    x.fused_at_scale = false;
     return x;
   }();
   return &m;

@@ -55,8 +55,16 @@ struct Plan {
   void* d_seg = nullptr;           // n_targets + 1 segment starts (int32 or int64)
   void* d_src = nullptr;           // n_src value-buffer slots (int32 or int64)
   double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
-  bool private_targets = true;     // every target written by one instance only
-  int32_t* d_direct = nullptr;     // E * n targets (-1 none) for the fused path, entry-major
+  // Block plan of the fused load: the same targets grouped by the block of
+  // instances that writes them (defined when every target's contributions
+  // come from one block).  Sources are LDS slots (loff + e) * B + lane.
+  bool block_private = true;
+  int loff = 0;                    // first LDS row of this kind
+  int32_t* d_bblk = nullptr;       // blocks + 1 offsets into the block's targets
+  int32_t* d_btgt = nullptr;       // target of each entry (null: contiguous from bt0)
+  int32_t bt0 = 0;
+  int32_t* d_bseg = nullptr;       // n_targets + 1 segment starts
+  uint16_t* d_bsrc = nullptr;      // n_src LDS slots
 };
 
 struct Group;
@@ -82,6 +90,8 @@ struct ModelImpl {
   bool (*skip_kind)(const Group&, const xa_solver_state&, int kind);  // may be null
   const char* const* state_names; int n_state;  // persistent per-instance doubles
   double (*traffic)(const Group&, int phase);   // algorithmic bytes per instance
+  int fused_block;          // threads per block of the fused load (0: no fused load)
+  bool fused_at_scale;      // fused load also when the launch fills the GPU
 };
 
 const ModelImpl* model_diode();
@@ -134,6 +144,8 @@ struct Group {
   double* d_state = nullptr;
   int32_t* d_fail = nullptr;
   Plan plan[NK];
+  bool fused_ok = false;     // the fused load's block plans exist
+  size_t fused_lds = 0;      // its LDS bytes per block
 
   // per-phase HIP event pairs recorded while timing is on
   bool timing = false;
@@ -154,6 +166,11 @@ int hip_check(hipError_t e, const char* what);
 
 // block size used by the per-instance kernels
 constexpr int kBlock = 256;
+// LDS of one workgroup on gfx950
+constexpr size_t kMaxLds = 160 * 1024;
+// A load of at most this many blocks is launch-latency bound: xa_load_all
+// then takes the one-kernel fused form whatever the model prefers at scale.
+constexpr int64_t kSmallLaunchBlocks = 512;
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   return (unsigned)((n + block - 1) / block);
 }

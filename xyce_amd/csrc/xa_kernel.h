@@ -23,35 +23,93 @@ struct KArgs {
   xa_vectors v;
   double* vals[NK];                    // value buffers, entry-major [e][inst]
   int E[NK];
-  const int32_t* direct[NK];           // fused mode: target per [e][inst]
   double* out[NK];                     // fused mode: output arrays
   unsigned kmask;                      // kinds to emit
+  // fused mode: LDS row of each kind and its block plan
+  int loff[NK];
+  const int32_t* bblk[NK];
+  const int32_t* btgt[NK];
+  int32_t bt0[NK];
+  const int32_t* bseg[NK];
+  const uint16_t* bsrc[NK];
 };
 
+// LDS of the fused load: row (loff[k] + e) holds entry e of kind k for every
+// lane of the block.
+extern __shared__ double xa_lds[];
+
 // Stamp sink: an instance hands its contributions over in stamp order.
-// Stored mode keeps them for the fixed-order scatter; fused mode adds them
-// straight into outputs whose targets belong to this instance alone.  Both
-// buffers are entry-major, so one put across a wavefront touches 64
-// consecutive slots.
+// Stored mode keeps them in the entry-major value buffers for the fixed-order
+// scatter (one put across a wavefront is 64 consecutive slots); fused mode
+// keeps them in LDS for the block's ordered write-out (fused_writeout).
 template <bool FUSED>
 struct Sink {
   const KArgs& a;
   int64_t i;
-  XA_HD Sink(const KArgs& args, int64_t inst) : a(args), i(inst) {}
-  XA_HD void put(int k, int e, double val) const {
+  __device__ Sink(const KArgs& args, int64_t inst) : a(args), i(inst) {}
+  __device__ void put(int k, int e, double val) const {
     if (!(a.kmask & (1u << k))) return;
-    const int64_t idx = (int64_t)e * a.n + i;
-    if (FUSED) {
-      const int32_t t = a.direct[k][idx];
-      if (t >= 0) {
-        if (k == K_LEADF || k == K_LEADQ) a.out[k][t] = val;
-        else a.out[k][t] += val;
-      }
-    } else {
-      a.vals[k][idx] = val;
-    }
+    if (FUSED) xa_lds[(a.loff[k] + e) * (int)blockDim.x + (int)threadIdx.x] = val;
+    else a.vals[k][(int64_t)e * a.n + i] = val;
   }
 };
+
+// Fused write-out: after every lane of the block has put its values, the
+// block adds them into its targets in the plan's order -- instance then stamp
+// order, the reference's `+=` order -- so the result equals the scatter's bit
+// for bit.  Each thread takes kUnroll targets at a time and issues their
+// segment, first-source and output loads together, so a batch costs a few
+// memory round trips instead of three per target.  Every thread of the block
+// must call it (it synchronises).
+constexpr int kUnroll = 8;
+__device__ __forceinline__ void fused_writeout(const KArgs& a) {
+  __syncthreads();
+  const int b = blockIdx.x;
+  const int32_t B = (int32_t)blockDim.x;
+  for (int k = 0; k < NK; ++k) {
+    if (!(a.kmask & (1u << k))) continue;
+    const int32_t tb = a.bblk[k][b], te = a.bblk[k][b + 1];
+    double* out = a.out[k];
+    const int32_t* seg = a.bseg[k];
+    const uint16_t* src = a.bsrc[k];
+    const int32_t* tgt = a.btgt[k];
+    const bool assign = k == K_LEADF || k == K_LEADQ;
+    for (int32_t t0 = tb + (int32_t)threadIdx.x; t0 < te; t0 += kUnroll * B) {
+      int32_t j0[kUnroll], j1[kUnroll];
+      int64_t o[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int32_t t = t0 + u * B;
+        if (t < te) {
+          j0[u] = seg[t];
+          j1[u] = seg[t + 1];
+          o[u] = tgt ? tgt[t] : (int64_t)a.bt0[k] + t;
+        }
+      }
+      double acc[kUnroll];
+      uint16_t s0[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (t0 + u * B < te) {
+          s0[u] = src[assign ? j1[u] - 1 : j0[u]];
+          acc[u] = assign ? 0.0 : out[o[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (t0 + u * B < te) {
+          if (assign) {
+            out[o[u]] = xa_lds[s0[u]];
+          } else {
+            double v = acc[u] + xa_lds[s0[u]];
+            for (int32_t j = j0[u] + 1; j < j1[u]; ++j) v += xa_lds[src[j]];
+            out[o[u]] = v;
+          }
+        }
+      }
+    }
+  }
+}
 
 XA_HD double dfld(const KArgs& a, int f, int64_t i) { return a.dfield[f * a.n + i]; }
 XA_HD int32_t ifld(const KArgs& a, int f, int64_t i) { return a.ifield[f * a.n + i]; }
@@ -64,5 +122,10 @@ XA_HD double solv(const double* x, int32_t lid) { return lid >= 0 ? x[lid] : 0.0
 
 void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
                 bool fused, KArgs& a);
+
+// Launch a model's k_update<false> (stored) or k_update<true> (fused, block
+// size and LDS from the group) on the context stream.
+int launch_model_kernel(Group& g, const KArgs& a, bool fused, const void* k_stored,
+                        const void* k_fused);
 
 }  // namespace xa

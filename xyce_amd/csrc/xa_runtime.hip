@@ -149,19 +149,42 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   a.v = v;
   a.kmask = 0;
   for (int k = 0; k < NK; ++k) {
-    a.vals[k] = g.plan[k].d_vals;
-    a.E[k] = g.plan[k].E;
-    a.direct[k] = g.plan[k].d_direct;
+    const Plan& p = g.plan[k];
+    a.vals[k] = p.d_vals;
+    a.E[k] = p.E;
     double* o = kind_out(v, k);
     a.out[k] = o;
-    bool on = g.plan[k].E > 0;
-    // fused mode writes straight to outputs: needs an output and a target map
-    // (a kind with no live target at all has no map and is skipped)
-    if (fused)
-      on = on && o != nullptr && g.plan[k].d_direct != nullptr &&
+    bool on = p.E > 0;
+    // fused mode stages into LDS and writes out by the block plan: needs an
+    // output, a live target and the kind not skipped on this call
+    if (fused) {
+      on = on && o != nullptr && p.n_targets > 0 &&
            !(g.impl->skip_kind && g.impl->skip_kind(g, ss, k));
+      a.loff[k] = p.loff;
+      a.bblk[k] = p.d_bblk;
+      a.btgt[k] = p.d_btgt;
+      a.bt0[k] = p.bt0;
+      a.bseg[k] = p.d_bseg;
+      a.bsrc[k] = p.d_bsrc;
+    }
     if (on) a.kmask |= 1u << k;
   }
+}
+
+int launch_model_kernel(Group& g, const KArgs& a, bool fused, const void* k_stored,
+                        const void* k_fused) {
+  void* args[] = {(void*)&a};
+  if (!fused) {
+    XA_HIP(hipLaunchKernel(k_stored, dim3(grid_for(g.n)), dim3(kBlock), args, 0, g.ctx->stream));
+    return XA_OK;
+  }
+  const int B = g.impl->fused_block;
+  if (g.fused_lds > 65536)
+    XA_HIP(hipFuncSetAttribute(k_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)g.fused_lds));
+  XA_HIP(hipLaunchKernel(k_fused, dim3(grid_for(g.n, B)), dim3(B), args, g.fused_lds,
+                         g.ctx->stream));
+  return XA_OK;
 }
 
 }  // namespace xa
@@ -264,7 +287,8 @@ int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
 
 static void free_plan(Plan& p) {
   (void)hipFree(p.d_target); (void)hipFree(p.d_seg); (void)hipFree(p.d_src);
-  (void)hipFree(p.d_vals); (void)hipFree(p.d_direct);
+  (void)hipFree(p.d_vals); (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt);
+  (void)hipFree(p.d_bseg); (void)hipFree(p.d_bsrc);
   p = Plan();
 }
 
@@ -493,6 +517,55 @@ int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
   return XA_OK;
 }
 
+// Block plan of the fused load.  The fused kernel runs blocks of B
+// instances; when every target of this kind receives contributions from the
+// instances of one block only, the block can add them itself, in the global
+// plan's order, from LDS.  Targets are grouped by block (ascending within a
+// block); a source becomes the LDS slot (loff + e) * B + lane.  loff is set
+// by the caller before (the kind's first LDS row).
+static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& target,
+                            const std::vector<int64_t>& seg, const std::vector<int64_t>& src) {
+  const int B = g.impl->fused_block;
+  const int64_t n = g.n, nt = (int64_t)target.size();
+  p.block_private = B > 0;
+  if (!p.block_private || nt == 0) return XA_OK;
+  const int64_t nb = (n + B - 1) / B;
+  std::vector<int32_t> tblock(nt);
+  for (int64_t t = 0; t < nt && p.block_private; ++t) {
+    const int64_t b = (src[seg[t]] % n) / B;
+    for (int64_t j = seg[t]; j < seg[t + 1]; ++j)
+      if ((src[j] % n) / B != b) { p.block_private = false; break; }
+    tblock[t] = (int32_t)b;
+  }
+  if (!p.block_private) return XA_OK;
+  std::vector<int32_t> bblk(nb + 1, 0);
+  for (int64_t t = 0; t < nt; ++t) bblk[tblock[t] + 1]++;
+  for (int64_t b = 0; b < nb; ++b) bblk[b + 1] += bblk[b];
+  std::vector<int32_t> pos(bblk.begin(), bblk.end() - 1);
+  std::vector<int64_t> order(nt);
+  for (int64_t t = 0; t < nt; ++t) order[pos[tblock[t]]++] = t;   // stable per block
+  std::vector<int32_t> btgt(nt), bseg(nt + 1, 0);
+  std::vector<uint16_t> bsrc;
+  bsrc.reserve(src.size());
+  for (int64_t u = 0; u < nt; ++u) {
+    const int64_t t = order[u];
+    btgt[u] = target[t];
+    for (int64_t j = seg[t]; j < seg[t + 1]; ++j) {
+      const int64_t inst = src[j] % n, e = src[j] / n;
+      bsrc.push_back((uint16_t)((p.loff + e) * B + inst % B));
+    }
+    bseg[u + 1] = (int32_t)bsrc.size();
+  }
+  bool contig = true;
+  for (int64_t u = 0; u < nt && contig; ++u) contig = btgt[u] == btgt[0] + (int32_t)u;
+  p.bt0 = btgt[0];
+  int rc;
+  if ((rc = upload(&p.d_bblk, bblk)) != XA_OK) return rc;
+  if (!contig && (rc = upload(&p.d_btgt, btgt)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_bseg, bseg)) != XA_OK) return rc;
+  return upload(&p.d_bsrc, bsrc);
+}
+
 // Counting sort of (target, instance, entry) triples by target; stable, so
 // contributions of one target stay in instance order then stamp order.
 static int build_plan(Group& g, int k) {
@@ -508,15 +581,10 @@ static int build_plan(Group& g, int k) {
     return XA_OK;
   }
   std::vector<int64_t> cnt((size_t)tmax + 2, 0);
-  std::vector<int32_t> owner((size_t)tmax + 1, -1);
-  p.private_targets = true;
   for (int64_t idx = 0; idx < (int64_t)tg.size(); ++idx) {
     const int32_t t = tg[idx];
     if (t < 0) continue;
     cnt[t + 1]++;
-    const int32_t inst = (int32_t)(idx / E);
-    if (owner[t] < 0) owner[t] = inst;
-    else if (owner[t] != inst) p.private_targets = false;
   }
   std::vector<int32_t> target;
   std::vector<int64_t> seg;
@@ -536,13 +604,10 @@ static int build_plan(Group& g, int k) {
   const int64_t n = g.n;
   std::vector<int64_t> src(acc);
   std::vector<int64_t> fillp(start);
-  std::vector<int32_t> direct(tg.size(), -1);
   for (int64_t idx = 0; idx < (int64_t)tg.size(); ++idx) {
     const int32_t t = tg[idx];
-    const int64_t slot = (idx % E) * n + idx / E;
-    direct[slot] = t;
     if (t < 0) continue;
-    src[fillp[t]++] = slot;
+    src[fillp[t]++] = (idx % E) * n + idx / E;
   }
   p.n_targets = (int64_t)target.size();
   p.n_src = acc;
@@ -569,7 +634,7 @@ static int build_plan(Group& g, int k) {
     p.d_seg = ds;
     p.d_src = dr;
   }
-  if ((rc = upload(&p.d_direct, direct)) != XA_OK) return rc;
+  if ((rc = build_block_plan(g, p, target, seg, src)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
   XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
   return XA_OK;
@@ -590,8 +655,20 @@ int xa_group_finalize(xa_group gh) {
   if ((rc = upload(&g->d_state, g->state_init)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&g->d_fail, sizeof(int32_t)));
   XA_HIP(hipMemset(g->d_fail, 0, sizeof(int32_t)));
+  // LDS rows of the fused load: the kinds with a live target, in kind order
+  int rows = 0;
+  for (int k = 0; k < NK; ++k) {
+    bool live = false;
+    for (int32_t t : g->tgt[k]) if (t >= 0) { live = true; break; }
+    g->plan[k].loff = rows;
+    if (live) rows += g->impl->max_entries[k];
+  }
   for (int k = 0; k < NK; ++k)
     if ((rc = build_plan(*g, k)) != XA_OK) return rc;
+  const int B = g->impl->fused_block;
+  g->fused_lds = (size_t)rows * (size_t)B * sizeof(double);
+  g->fused_ok = B > 0 && g->fused_lds <= kMaxLds && (int64_t)rows * B <= 65536;
+  for (int k = 0; k < NK; ++k) g->fused_ok = g->fused_ok && g->plan[k].block_private;
   g->phase = PH_FINAL;
   return XA_OK;
 }
@@ -673,11 +750,12 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   if (rc != XA_OK) return rc;
   if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
   if (g->n == 0) return XA_OK;
-  bool all_private = true;
-  for (int k = 0; k < NK; ++k) all_private = all_private && g->plan[k].private_targets;
-  // the fused kernel needs the secondary state folded in; models with a
-  // secondary pass take the split route
-  if (all_private && !g->impl->launch_secondary) {
+  // the fused kernel needs every target block-private and no secondary pass;
+  // it is taken for launch-bound sizes and, at scale, where the model runs
+  // faster fused; otherwise the four calls run in sequence (same results)
+  const int64_t blocks = (g->n + g->impl->fused_block - 1) / std::max(g->impl->fused_block, 1);
+  if (g->fused_ok && !g->impl->launch_secondary &&
+      (g->impl->fused_at_scale || blocks <= kSmallLaunchBlocks)) {
     tic(g, 0);
     rc = g->impl->launch_update(*g, *ss, *v, true);
     toc(g, 0);

@@ -37,7 +37,10 @@ class Ring:
     """Netlist of the ring: models, instance parameters and terminal nodes."""
 
     def __init__(self, stages=51, vdd=1.0, c_load=1e-15, l=45e-9, wn=90e-9, wp=180e-9,
-                 ic_node=0, ic_value=0.0):
+                 ic_node=0, ic_value=0.0, toxe=None, vth_scale=1.0):
+        """toxe: oxide thickness of both cards (default: the card default);
+        vth_scale: |VTH0| scaled by this factor through each transistor's
+        DELVTO (VTH0 defaults +-0.7 V)."""
         if stages < 3 or stages % 2 == 0:
             raise ValueError("a ring oscillator needs an odd number of stages >= 3")
         self.N = stages
@@ -46,11 +49,17 @@ class Ring:
         self.ic_node = ic_node
         self.ic_value = ic_value
         self.supply = stages                     # node index of the supply
-        self.models = [(54, {"TYPE": 1.0}), (54, {"TYPE": -1.0})]
+        cards = [{"TYPE": 1.0}, {"TYPE": -1.0}]
+        if toxe is not None:
+            for c in cards:
+                c["TOXE"] = toxe
+        self.models = [(54, c) for c in cards]
         n = 2 * stages
         self.n_inst = n
         self.model_index = np.tile(np.array([0, 1], np.int32), stages)
         self.params = {"W": np.tile(np.array([wn, wp]), stages), "L": np.full(n, l)}
+        if vth_scale != 1.0:
+            self.params["DELVTO"] = np.tile(np.array([0.7, -0.7]) * (vth_scale - 1.0), stages)
 
     def terminals(self, inst):
         """External nodes (D, G, S, B) of instance `inst`; ground is -1."""
@@ -104,9 +113,11 @@ class Topology:
 
 class GpuLoader:
     """The ring's BSIM4 group on one GPU: one load = the four Master calls on
-    device vectors, outputs copied back to the host."""
+    device vectors (or, fused, the one-call xa_load_all with the same
+    results), outputs copied back to the host."""
 
-    def __init__(self, ring, device=0, options=None):
+    def __init__(self, ring, device=0, options=None, fused=False):
+        self.fused = fused
         import torch
         from . import Context, Master, SolverState, make_vectors
         self.torch = torch
@@ -141,10 +152,13 @@ class GpuLoader:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         m = self.master
-        m.updateState(ss, self.vectors)
-        m.updateSecondaryState(ss, self.vectors)
-        m.loadDAEVectors(ss, self.vectors)
-        m.loadDAEMatrices(ss, self.vectors)
+        if self.fused:
+            m.loadAll(ss, self.vectors)
+        else:
+            m.updateState(ss, self.vectors)
+            m.updateSecondaryState(ss, self.vectors)
+            m.loadDAEVectors(ss, self.vectors)
+            m.loadDAEMatrices(ss, self.vectors)
         torch.cuda.synchronize()
         self.load_seconds += time.perf_counter() - t0
         self.loads += 1
@@ -171,10 +185,11 @@ def _newton(loader, ring, x, fixed, flags_of, residual, vtol, max_iter):
         out = loader.load(x, **flags_of(it))
         rhs, jac = residual(out, x)
         dx = np.linalg.solve(jac[np.ix_(free, free)], rhs[free])
+        x_load = x
         x = x.copy()
         x[free] += dx
         if it > 0 and np.max(np.abs(dx)) <= vtol:
-            return x, out, it + 1
+            return x, out, x_load, it + 1
     raise NewtonFailure(f"Newton did not converge in {max_iter} iterations")
 
 
@@ -199,11 +214,13 @@ def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record
         return dict(newtonIter=it, dcopFlag=1, initJctFlag=1 if it == 0 else 0)
 
     newton_total = 0
-    x, out, its = _newton(loader, ring, x, {ring.supply, ring.ic_node}, dc_flags, dc_residual,
-                          vtol, max_iter)
+    x, out, x_load, its = _newton(loader, ring, x, {ring.supply, ring.ic_node}, dc_flags,
+                                  dc_residual, vtol, max_iter)
     newton_total += its
     loader.accept()
-    q_n = out["q"] + C * x
+    # the charge history is the last load's: device Q and capacitor charge at
+    # the voltages that load saw
+    q_n = out["q"] + C * x_load
     times = [0.0]
     V = [x.copy()]
     steps = int(round(t_stop / h))
@@ -222,11 +239,11 @@ def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record
             jac = t.dense(out["dFdx"]) + t.dense(out["dQdx"]) / h + np.diag(C / h)
             return rhs, jac
 
-        x, out, its = _newton(loader, ring, x_n, {ring.supply}, tr_flags, tr_residual, vtol,
-                              max_iter)
+        x, out, x_load, its = _newton(loader, ring, x_n, {ring.supply}, tr_flags, tr_residual,
+                                      vtol, max_iter)
         newton_total += its
         loader.accept()
-        q_n = out["q"] + C * x
+        q_n = out["q"] + C * x_load
         times.append((n + 1) * h)
         V.append(x.copy())
     stats = {"steps": steps, "newton_iterations": newton_total, "loads": loader.loads,
