@@ -232,24 +232,29 @@ int xa_group_last_times(xa_group g, float* ms_update, float* ms_vectors,
  * 215-265 (obtainResidual) and the nonlinear solver form for one circuit, for
  * a batch of circuits (Monte Carlo / .STEP copies) loaded together.
  *
- * Circuit c owns the unknowns [c_start[c], c_start[c+1]); no CSR entry may
- * couple two circuits, and no circuit may have more than max_unknowns
- * unknowns (its dense system is factored in LDS: max_unknowns <= 140).
+ * Circuit c owns the unknowns [c_start[c], c_start[c+1]) (at most
+ * max_unknowns); no CSR entry may couple two circuits.  pos[g] is unknown g's
+ * position in its circuit's elimination order (NULL: natural order); in that
+ * order every assembled entry (i, j) must lie within the band: i - j <= kl
+ * below and j - i <= ku above the diagonal.  The band, with
+ * kl rows of pivoting fill, is factored in LDS: (2kl+ku+2)*max_unknowns
+ * doubles must fit in 159 KB.
  * With dc = 0:  q_now = q + c_lin .* x;
  *   rhs = -((q_now - q_prev)/h + f) + dFdxdVp + dQdxdVp/h,
  *   J = dFdx + dQdx/h + diag(c_lin/h);
  * with dc = 1:  rhs = -f + dFdxdVp, J = dFdx.
  * Rows and columns of held nodes (held[i] != 0) are left out.  x += J^-1 rhs
- * on the free nodes; dx_max[c] = the circuit's largest |update| (NaN for a
- * circuit outside the limits).  All pointers are device pointers; f, q, the
+ * on the free nodes (banded LU with partial pivoting); dx_max[c] = the
+ * circuit's largest |update| (NaN for a circuit outside the limits or with an
+ * entry outside the band).  All pointers are device pointers; f, q, the
  * limiter vectors and the CSR values are those of a load of the batch. */
-int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns,
-                         const int32_t* c_start, const int32_t* row_ptr,
-                         const int32_t* col_idx, const double* dFdx, const double* dQdx,
-                         const double* f, const double* q, const double* dFdxdVp,
-                         const double* dQdxdVp, const double* q_prev, const double* c_lin,
-                         const uint8_t* held, double h, int dc, double* x, double* q_now,
-                         double* dx_max);
+int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
+                         const int32_t* c_start, const int32_t* pos,
+                         const int32_t* row_ptr, const int32_t* col_idx, const double* dFdx,
+                         const double* dQdx, const double* f, const double* q,
+                         const double* dFdxdVp, const double* dQdxdVp, const double* q_prev,
+                         const double* c_lin, const uint8_t* held, double h, int dc,
+                         double* x, double* q_now, double* dx_max);
 
 #ifdef __cplusplus
 }

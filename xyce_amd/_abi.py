@@ -81,7 +81,7 @@ def _proto(lib):
         "xa_group_traffic_bytes": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_double)]),
         "xa_group_enable_timing": (C.c_int, [vp, C.c_int]),
         "xa_group_last_times": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
-        "xa_batch_newton_step": (C.c_int, [vp, C.c_int, C.c_int] + [vp] * 12 +
+        "xa_batch_newton_step": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int] + [vp] * 13 +
                                  [C.c_double, C.c_int, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():

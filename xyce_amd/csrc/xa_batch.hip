@@ -13,9 +13,15 @@
 //
 // or, at a DC operating point, rhs = -F + dFdxdVp and J = dF/dx.  Rows of
 // held nodes (a supply, an initial condition) are identity rows with zero
-// right-hand side.  One workgroup factors one circuit's dense matrix in LDS
-// by LU with partial pivoting (LAPACK getrf order of operations, column by
-// column), solves, updates x and reports the circuit's largest update.
+// right-hand side and their columns are left out.
+//
+// The matrix is banded once the circuit's unknowns are taken in the caller's
+// elimination order (pos[g]: position of unknown g within its circuit; for a
+// ring, a reverse Cuthill-McKee order gives two bands either side).  One
+// wavefront factors one circuit in LDS by banded LU with partial pivoting in
+// LAPACK dgbtf2's order of operations (band storage with kl extra rows for
+// the pivoting fill), forward-eliminates the right-hand side alongside,
+// back-substitutes, updates x and reports the circuit's largest update.
 #include <cmath>
 #include "xa_internal.h"
 
@@ -24,6 +30,7 @@ namespace xa {
 struct BatchArgs {
   int n_circ;
   const int32_t* c_start;
+  const int32_t* pos;
   const int32_t* row_ptr;
   const int32_t* col_idx;
   const double* dFdx;
@@ -38,18 +45,18 @@ struct BatchArgs {
   double h;
   int dc;
   int max_n;
+  int kl, ku;
   double* x;
   double* q_now;
   double* dx_max;
 };
 
-constexpr int kBatchThreads = 256;
+constexpr int kBatchThreads = 64;   // one wavefront per circuit
 
 __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
   extern __shared__ double sm[];
-  __shared__ int s_piv;
-  __shared__ double s_red[kBatchThreads / 64];
-  __shared__ int s_redi[kBatchThreads / 64];
+  __shared__ int s_jp;
+  __shared__ int s_bad;
   const int c = blockIdx.x;
   const int tid = threadIdx.x;
   const int r0 = b.c_start[c], n = b.c_start[c + 1] - r0;
@@ -57,17 +64,20 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
     if (tid == 0) b.dx_max[c] = NAN;
     return;
   }
-  double* A = sm;            // n x n, row-major
-  double* rhs = sm + n * n;  // n
-  for (int idx = tid; idx < n * n; idx += kBatchThreads) A[idx] = 0.0;
+  const int kl = b.kl, ku = b.ku, kv = kl + ku, L = 2 * kl + ku + 1;
+  double* AB = sm;             // L x n band: A(i, j) at AB[(kv + i - j) * n + j]
+  double* rhs = sm + L * n;    // n, in elimination order
+  for (int idx = tid; idx < L * n + n; idx += kBatchThreads) sm[idx] = 0.0;
+  if (tid == 0) s_bad = 0;
   __syncthreads();
   for (int i = tid; i < n; i += kBatchThreads) {
     const int gi = r0 + i;
+    const int ip = b.pos ? b.pos[gi] : i;
     const double qn = b.q[gi] + b.c_lin[gi] * b.x[gi];
     b.q_now[gi] = qn;
+    if (ip < 0 || ip >= n) { s_bad = 1; continue; }
     if (b.held[gi]) {
-      A[i * n + i] = 1.0;
-      rhs[i] = 0.0;
+      AB[kv * n + ip] = 1.0;
       continue;
     }
     for (int p = b.row_ptr[gi]; p < b.row_ptr[gi + 1]; ++p) {
@@ -75,108 +85,112 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
       // held columns multiply a zero update; entries outside the circuit
       // (none in a valid batch) are left out
       if (j < 0 || j >= n || b.held[gj]) continue;
-      A[i * n + j] += b.dc ? b.dFdx[p] : b.dFdx[p] + b.dQdx[p] / b.h;
+      const int jp = b.pos ? b.pos[gj] : j;
+      if (jp < 0 || jp >= n || ip - jp > kl || jp - ip > ku) { s_bad = 1; continue; }
+      AB[(kv + ip - jp) * n + jp] += b.dc ? b.dFdx[p] : b.dFdx[p] + b.dQdx[p] / b.h;
     }
     if (b.dc) {
-      rhs[i] = -b.f[gi] + b.dFdxdVp[gi];
+      rhs[ip] = -b.f[gi] + b.dFdxdVp[gi];
     } else {
-      A[i * n + i] += b.c_lin[gi] / b.h;
-      rhs[i] = -((qn - b.q_prev[gi]) / b.h + b.f[gi]) + b.dFdxdVp[gi] + b.dQdxdVp[gi] / b.h;
+      AB[kv * n + ip] += b.c_lin[gi] / b.h;
+      rhs[ip] = -((qn - b.q_prev[gi]) / b.h + b.f[gi]) + b.dFdxdVp[gi] + b.dQdxdVp[gi] / b.h;
     }
   }
   __syncthreads();
-  const int lane = tid & 63, wave = tid >> 6;
-  for (int k = 0; k < n; ++k) {
-    // pivot: largest |A[i][k]|, i >= k (first such i on ties)
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + tid; i < n; i += kBatchThreads) {
-      const double v = fabs(A[i * n + k]);
-      if (v > best) { best = v; bi = i; }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_down(best, off);
-      const int oi = __shfl_down(bi, off);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    if (lane == 0) { s_red[wave] = best; s_redi[wave] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      double bv = s_red[0];
-      int bj = s_redi[0];
-      for (int w = 1; w < kBatchThreads / 64; ++w)
-        if (s_red[w] > bv || (s_red[w] == bv && s_redi[w] < bj)) { bv = s_red[w]; bj = s_redi[w]; }
-      s_piv = bj;
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (p != k && p < n) {
-      for (int j = tid; j <= n; j += kBatchThreads) {
-        double* a = (j < n) ? &A[k * n + j] : &rhs[k];
-        double* bb = (j < n) ? &A[p * n + j] : &rhs[p];
-        const double t = *a;
-        *a = *bb;
-        *bb = t;
-      }
-    }
-    __syncthreads();
-    const double akk = A[k * n + k];
-    for (int i = k + 1 + tid; i < n; i += kBatchThreads) A[i * n + k] /= akk;
-    __syncthreads();
-    const int m = n - k - 1;
-    for (int idx = tid; idx < m * (m + 1); idx += kBatchThreads) {
-      const int i = k + 1 + idx / (m + 1), jj = idx % (m + 1);
-      if (jj < m) A[i * n + k + 1 + jj] -= A[i * n + k] * A[k * n + k + 1 + jj];
-      else rhs[i] -= A[i * n + k] * rhs[k];
-    }
-    __syncthreads();
+  if (s_bad) {   // an entry outside the band or a bad position: reported, not solved
+    if (tid == 0) b.dx_max[c] = NAN;
+    return;
   }
-  // back substitution, column by column
-  for (int k = n - 1; k >= 0; --k) {
-    if (tid == 0) rhs[k] /= A[k * n + k];
+  int ju = 0;   // last column touched by the row interchanges so far
+  for (int j = 0; j < n; ++j) {
+    const int km = min(kl, n - 1 - j);
+    if (tid == 0) {
+      int jp = 0;
+      double best = fabs(AB[kv * n + j]);
+      for (int i = 1; i <= km; ++i) {
+        const double v = fabs(AB[(kv + i) * n + j]);
+        if (v > best) { best = v; jp = i; }
+      }
+      s_jp = jp;
+    }
     __syncthreads();
-    for (int i = tid; i < k; i += kBatchThreads) rhs[i] -= A[i * n + k] * rhs[k];
+    const int jp = s_jp;
+    if (AB[(kv + jp) * n + j] == 0.0) continue;   // singular column: no elimination
+    ju = max(ju, min(j + ku + jp, n - 1));
+    if (jp != 0) {
+      for (int col = j + tid; col <= ju; col += kBatchThreads) {
+        double* a = &AB[(kv + j - col) * n + col];
+        double* e = &AB[(kv + j + jp - col) * n + col];
+        const double t = *a;
+        *a = *e;
+        *e = t;
+      }
+      if (tid == 0) {
+        const double t = rhs[j];
+        rhs[j] = rhs[j + jp];
+        rhs[j + jp] = t;
+      }
+      __syncthreads();
+    }
+    if (km > 0) {
+      const double piv = AB[kv * n + j];
+      for (int i = 1 + tid; i <= km; i += kBatchThreads) AB[(kv + i) * n + j] /= piv;
+      __syncthreads();
+      // rank-1 update of the km x (ju - j) block and of the right-hand side
+      const int w = ju - j + 1;
+      for (int idx = tid; idx < km * w; idx += kBatchThreads) {
+        const int i = 1 + idx / w, d = idx % w;
+        const double l = AB[(kv + i) * n + j];
+        if (d == 0) rhs[j + i] -= l * rhs[j];
+        else AB[(kv + i - d) * n + j + d] -= l * AB[(kv - d) * n + j + d];
+      }
+      __syncthreads();
+    }
+  }
+  // back substitution with U (bandwidth kl + ku)
+  for (int j = n - 1; j >= 0; --j) {
+    if (tid == 0) rhs[j] /= AB[kv * n + j];
+    __syncthreads();
+    for (int i = max(0, j - kv) + tid; i < j; i += kBatchThreads)
+      rhs[i] -= AB[(kv + i - j) * n + j] * rhs[j];
     __syncthreads();
   }
   double mx = 0.0;
   for (int i = tid; i < n; i += kBatchThreads) {
     const int gi = r0 + i;
     if (!b.held[gi]) {
-      b.x[gi] += rhs[i];
-      mx = fmax(mx, fabs(rhs[i]));
+      const double d = rhs[b.pos ? b.pos[gi] : i];
+      b.x[gi] += d;
+      mx = fmax(mx, fabs(d));
     }
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off));
-  if (lane == 0) s_red[wave] = mx;
-  __syncthreads();
-  if (tid == 0) {
-    for (int w = 1; w < kBatchThreads / 64; ++w) mx = fmax(mx, s_red[w]);
-    b.dx_max[c] = mx;
-  }
+  if (tid == 0) b.dx_max[c] = mx;
 }
 
 }  // namespace xa
 
 using namespace xa;
 
-extern "C" int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns,
-                                    const int32_t* c_start, const int32_t* row_ptr,
-                                    const int32_t* col_idx, const double* dFdx,
-                                    const double* dQdx, const double* f, const double* q,
-                                    const double* dFdxdVp, const double* dQdxdVp,
-                                    const double* q_prev, const double* c_lin,
-                                    const uint8_t* held, double h, int dc, double* x,
-                                    double* q_now, double* dx_max) {
+extern "C" int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
+                                    const int32_t* c_start, const int32_t* pos,
+                                    const int32_t* row_ptr, const int32_t* col_idx,
+                                    const double* dFdx, const double* dQdx, const double* f,
+                                    const double* q, const double* dFdxdVp,
+                                    const double* dQdxdVp, const double* q_prev,
+                                    const double* c_lin, const uint8_t* held, double h, int dc,
+                                    double* x, double* q_now, double* dx_max) {
   Ctx* cx = reinterpret_cast<Ctx*>(ctx);
   if (!cx) return fail(XA_ERR_ARG, "null context");
   if (n_circ <= 0) return XA_OK;
-  if (max_unknowns <= 0) return fail(XA_ERR_ARG, "batch: max_unknowns must be positive");
-  const size_t lds = ((size_t)max_unknowns * max_unknowns + max_unknowns) * sizeof(double);
+  if (max_unknowns <= 0 || kl < 0 || ku < 0)
+    return fail(XA_ERR_ARG, "batch: max_unknowns must be positive and kl, ku >= 0");
+  const size_t lds = ((size_t)(2 * kl + ku + 2) * (size_t)max_unknowns) * sizeof(double);
   if (lds > kMaxLds - 1024)
-    return fail(XA_ERR_ARG, "batch: a circuit's dense system does not fit in LDS");
+    return fail(XA_ERR_ARG, "batch: a circuit's band does not fit in LDS");
   if (!dc && !(h > 0.0)) return fail(XA_ERR_ARG, "batch: transient step needs h > 0");
-  BatchArgs a{n_circ, c_start, row_ptr, col_idx, dFdx, dQdx, f, q, dFdxdVp, dQdxdVp,
-              q_prev, c_lin, held, h, dc, max_unknowns, x, q_now, dx_max};
+  BatchArgs a{n_circ, c_start, pos, row_ptr, col_idx, dFdx, dQdx, f, q, dFdxdVp, dQdxdVp,
+              q_prev, c_lin, held, h, dc, max_unknowns, kl, ku, x, q_now, dx_max};
   if (lds > 65536)
     XA_HIP(hipFuncSetAttribute((const void*)k_batch_newton,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -3,8 +3,8 @@
 Thousands of ring oscillators (odd stage counts, per-ring VTH0 and TOXE
 samples) run as one circuit batch: every ring's BSIM4 instances are one
 group, loaded together by the device loads, and every ring's Newton system
-is formed and solved on the device by xa_batch_newton_step (one workgroup
-per ring).  The host only launches and checks convergence, once per Newton
+is formed and solved on the device by xa_batch_newton_step (one wavefront
+per ring, banded LU in the ring's reverse Cuthill-McKee order).  The host only launches and checks convergence, once per Newton
 iteration.  Each ring is the circuit of xyce_amd.ring.Ring with its sample's
 TOXE and |VTH0| scale, so the plain drop-in (that module, one ring at a
 time) runs exactly the same circuits.
@@ -101,6 +101,31 @@ class BatchTopology:
         master.register_csr(self.row_ptr, self.col_idx)
         master.finalize()
         self.max_unknowns = int(np.max(np.diff(self.base)))
+        self._order(rings)
+
+    def _order(self, rings):
+        """Elimination order of each ring: reverse Cuthill-McKee over the
+        couplings of its free nodes (the supply is held), which puts a
+        ring's cycle in two bands either side of the diagonal."""
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        self.pos = np.zeros(self.n_nodes, np.int32)
+        kl = ku = 0
+        rows = np.repeat(np.arange(self.n_nodes), np.diff(self.row_ptr))
+        for r, ring in enumerate(rings):
+            b0, n = int(self.base[r]), ring.N + 1
+            sel = (rows >= b0) & (rows < b0 + ring.N) & (self.col_idx >= b0) & \
+                  (self.col_idx < b0 + ring.N)
+            i, j = rows[sel] - b0, self.col_idx[sel] - b0
+            g = csr_matrix((np.ones(len(i)), (i, j)), shape=(n, n))
+            perm = reverse_cuthill_mckee(g + g.T, symmetric_mode=True)
+            p = np.empty(n, np.int32)
+            p[perm] = np.arange(n, dtype=np.int32)
+            self.pos[b0:b0 + n] = p
+            d = p[i] - p[j]
+            kl = max(kl, int(d.max(initial=0)))
+            ku = max(ku, int((-d).max(initial=0)))
+        self.kl, self.ku = kl, ku
 
 
 class BatchTransient:
@@ -130,6 +155,7 @@ class BatchTransient:
             c_lin[t.base[r]:t.base[r] + ring.N] = ring.c_load
         i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
         self.c_start = i32(t.base)
+        self.pos = i32(t.pos)
         self.row_ptr = i32(t.row_ptr)
         self.col_idx = i32(t.col_idx)
         self.c_lin = torch.from_numpy(c_lin).to(dev)
@@ -179,8 +205,8 @@ class BatchTransient:
         d, t = self.d, self.topo
         P = self._ptr
         check(lib().xa_batch_newton_step(
-            self.ctx.h, self.batch.n_rings, t.max_unknowns, P(self.c_start), P(self.row_ptr),
-            P(self.col_idx), P(d["dFdx"]), P(d["dQdx"]), P(d["f"]), P(d["q"]), P(d["dFdxdVp"]),
+            self.ctx.h, self.batch.n_rings, t.max_unknowns, t.kl, t.ku, P(self.c_start),
+            P(self.pos), P(self.row_ptr), P(self.col_idx), P(d["dFdx"]), P(d["dQdx"]), P(d["f"]), P(d["q"]), P(d["dFdxdVp"]),
             P(d["dQdxdVp"]), P(self.q_prev), P(self.c_lin),
             P(self.held_dc if dc else self.held_tr), C.c_double(h), 1 if dc else 0, P(self.x),
             P(self.q_now), P(self.dx_max)), "xa_batch_newton_step")
