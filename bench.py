@@ -20,8 +20,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(model, n_sample, seconds):
-    """The oracle's serial load pass on a bounded sample of the workload."""
+def cpu_threads():
+    """Host cores this process may use, at most 16 (one GPU's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(model, n_sample, seconds, threads=1):
+    """The oracle's load pass on a bounded sample of the workload: updateState
+    over `threads` OpenMP threads (the reference's _OMP instance loop), the
+    loads serial, in the reference's order."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle
@@ -60,6 +71,7 @@ def cpu_baseline(model, n_sample, seconds):
     ss.newtonIter = 1
     ss.gainScale = 1.0
     ss.nltermScale = 1.0
+    oracle.set_threads(threads)
     reps = 0
     t0 = time.perf_counter()
     while True:
@@ -71,7 +83,9 @@ def cpu_baseline(model, n_sample, seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return n_sample * reps / el, f"{reps} load passes over {n_sample} instances in {el:.1f} s"
+    oracle.set_threads(1)
+    return n_sample * reps / el, (f"{reps} load passes over {n_sample} instances in {el:.1f} s, "
+                                  f"updateState on {threads} thread(s)")
 
 
 def measured_traffic(model, kernel, n):
@@ -171,7 +185,7 @@ def mc_transient(args):
     batch = RingBatch(n_rings=args.mc_rings)
     h = 1e-12
     print(f"[bench] building {batch.n_rings} rings", file=sys.stderr, flush=True)
-    bt = BatchTransient(batch)
+    bt = BatchTransient(batch, use_graph=True)
     print("[bench] batched transient", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -226,6 +240,7 @@ def mc_transient(args):
                                f"backward Euler 1ps to {args.mc_tstop:g}s",
                    "rings": batch.n_rings, "instances": int(2 * batch.stages.sum())},
         "batched_seconds": batched,
+        "hip_graphs": len(bt.graphs),
         "newton_iterations": bt.newton_iterations,
         "plain_seconds_per_ring": plain_times,
         "plain_rings_sampled": sample,
@@ -245,12 +260,16 @@ def main():
     ap.add_argument("--model", default="bsim4", choices=["diode", "mosfet1", "bjt", "bsim4"])
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fused", action="store_true", help="time the one-call fused load")
     ap.add_argument("--ring", action="store_true",
                     help="run the 51-stage BSIM4 ring-oscillator transient (1 ps steps to 2 ns) "
                          "instead of the microbenchmark")
+    ap.add_argument("--netlist", choices=["logic", "hb"],
+                    help="load passes over a shared-node netlist instead of private nodes: "
+                         "'logic' = 200k-transistor random INV/NAND2/NOR2 DAG, 'hb' = the 128 "
+                         "time samples of a 2k-transistor differential amplifier")
     ap.add_argument("--mc", action="store_true",
                     help="run the batched Monte Carlo ring-oscillator transient (4096 rings) and "
                          "its speedup over the plain drop-in")
@@ -278,8 +297,26 @@ def main():
     import torch
     from xyce_amd.workloads import Workload
 
-    print(f"[bench] building {args.model} workload n={args.n}", file=sys.stderr, flush=True)
-    wl = Workload(args.model, args.n)
+    netlist_desc = None
+    if args.netlist:
+        from xyce_amd import netlists
+        gen = netlists.logic_dag if args.netlist == "logic" else netlists.hb_diffamp
+        cards, mi, params, lids, n_nodes, volts = gen()
+        args.model, args.n = "bsim4", len(mi)
+        print(f"[bench] building {args.netlist} netlist, {args.n} BSIM4 instances, "
+              f"{n_nodes} nodes", file=sys.stderr, flush=True)
+        wl = Workload("bsim4", args.n, cards=cards, model_index=mi, params=params,
+                      netlist=(lids, n_nodes, volts))
+        netlist_desc = {
+            "logic": f"flat logic netlist: {args.n} BSIM4 transistors in a seeded random DAG "
+                     "of INV/NAND2/NOR2 cells, fanout 1-4, 20 primary inputs, two shared cards "
+                     "and sizes, one VDD rail; node voltages U[0,1]; one load pass per step",
+            "hb": f"harmonic-balance samples: a {args.n // 128}-transistor BSIM4 differential "
+                  "amplifier x 128 time samples of a 1.00/1.01 GHz 10 mV two-tone drive, all "
+                  "samples' instances loaded together; one load pass per step"}[args.netlist]
+    else:
+        print(f"[bench] building {args.model} workload n={args.n}", file=sys.stderr, flush=True)
+        wl = Workload(args.model, args.n)
     step = wl.load_fused if args.fused else wl.load_pass
     for _ in range(args.warmup):
         step()
@@ -307,13 +344,17 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.netlist:
         n_sample = args.cpu_sample or {"diode": 1 << 17, "mosfet1": 1 << 15, "bjt": 1 << 15,
                                        "bsim4": 1 << 13}[args.model]
         print(f"[bench] cpu baseline on {n_sample} instances", file=sys.stderr, flush=True)
-        cv, desc = cpu_baseline(args.model, n_sample, args.cpu_seconds)
-        cpu = {"value": cv, "unit": "instance loads/s", "cores": 1, "kind": "port",
-               "sample": desc}
+        threads = cpu_threads()
+        c1, d1 = cpu_baseline(args.model, n_sample, args.cpu_seconds / 2)
+        cpu = {"value": c1, "unit": "instance loads/s", "cores": 1, "kind": "port", "sample": d1}
+        if threads > 1:
+            cn, dn = cpu_baseline(args.model, n_sample, args.cpu_seconds / 2, threads)
+            cpu = {"value": cn, "unit": "instance loads/s", "cores": threads, "kind": "port",
+                   "sample": dn, "single_core_value": c1, "single_core_sample": d1}
     out = {
         "metric": f"{args.model.upper()} instance loads/s (eval+stamp)",
         "value": value,
@@ -327,8 +368,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"device-eval microbenchmark, {args.n} {args.model} instances, "
-                               "private nodes, V~U[-0.5,1.5], one load pass per step",
+        "config": {"workload": netlist_desc or
+                   f"device-eval microbenchmark, {args.n} {args.model} instances, "
+                   "private nodes, V~U[-0.5,1.5], one load pass per step",
                    "instances": args.n, "device": args.model,
                    "path": "fused load_all" if one_kernel else
                            ("load_all (split at this size)" if args.fused else
@@ -336,7 +378,7 @@ def main():
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if one_kernel else
+                     "traffic": None if (one_kernel or args.netlist) else
                      measured_traffic(args.model, f"{args.model}::k_update<false>", args.n),
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,

@@ -139,6 +139,17 @@ int         xa_abi_version(void);
 void        xa_default_options(xa_device_options* opt);
 void        xa_default_solver_state(xa_solver_state* ss);
 
+/* ---- device memory for callers with host vectors ------------------------ */
+/* No reference counterpart: plumbing for a caller whose circuit vectors live
+ * in host memory (a plain Xyce build keeps them in host Epetra vectors).
+ * Device buffers on the context's device, and copies ordered on the
+ * context's stream that return once the copy is complete (host memory may be
+ * pageable). */
+int xa_device_alloc(xa_ctx ctx, size_t bytes, void** out);
+int xa_device_free(xa_ctx ctx, void* ptr);
+int xa_copy_to_device(xa_ctx ctx, void* dst, const void* src, size_t bytes);
+int xa_copy_to_host(xa_ctx ctx, void* dst, const void* src, size_t bytes);
+
 /* ---- group setup (one group = all instances of one model type) ---------- */
 /* Replaces Config<Traits>::registerDevice + Traits::factory. */
 int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,

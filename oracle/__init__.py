@@ -88,6 +88,11 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """OpenMP threads for the updateState loops (1: the serial order)."""
+    lib().or_set_threads(int(n))
+
+
 def check(rc, what=""):
     if rc != 0:
         raise RuntimeError(f"oracle {what}: rc={rc}: {lib().or_last_error().decode()}")

@@ -380,6 +380,9 @@ static int update_state(struct or_group_s* g, const xa_solver_state* ss, const x
   int64_t i;
   b4inst* in = (b4inst*)g->inst;
   int ok = 1;
+  /* instances are independent here (the reference's _OMP loop,
+   * N_DEV_MOSFET_B4.C:16184); one thread is its serial order */
+#pragma omp parallel for schedule(static) num_threads(or_threads) if (or_threads > 1) reduction(&& : ok)
   for (i = 0; i < g->n; ++i) {
     b4inst* I = &in[i];
     double* sto = v->nextSto;

@@ -431,6 +431,8 @@ static void update_intermediate(struct or_group_s* g, dinst* d, const xa_solver_
 static int update_state(struct or_group_s* g, const xa_solver_state* ss, const xa_vectors* v) {
   int64_t i;
   dinst* in = (dinst*)g->inst;
+  /* instances are independent here; one thread is the serial order */
+#pragma omp parallel for schedule(static) num_threads(or_threads) if (or_threads > 1)
   for (i = 0; i < g->n; ++i) {
     dinst* d = &in[i];
     if (d->li_storevd >= 0) v->nextSto[d->li_storevd] = d->Vd;

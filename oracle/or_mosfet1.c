@@ -667,6 +667,8 @@ static int update_state(struct or_group_s* g, const xa_solver_state* ss, const x
   int64_t i;
   m1inst* in = (m1inst*)g->inst;
   int ok = 1;
+  /* instances are independent here; one thread is the serial order */
+#pragma omp parallel for schedule(static) num_threads(or_threads) if (or_threads > 1) reduction(&& : ok)
   for (i = 0; i < g->n; ++i) {
     m1inst* I = &in[i];
     double* sto = v->nextSto;

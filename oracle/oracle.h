@@ -46,7 +46,8 @@ int  or_load_vectors(or_group g, const xa_solver_state* ss, const xa_vectors* v)
 int  or_load_matrices(or_group g, const xa_solver_state* ss, const xa_vectors* v);
 int  or_read_intermediate(or_group g, const char* name, double* out);
 const char* or_last_error(void);
-/* OpenMP threads for the load loops (1 = the reference's serial order). */
+/* OpenMP threads for the updateState loops (instances are independent there;
+ * 1 = the serial order).  The loads that add into shared rows stay serial. */
 void or_set_threads(int n);
 void or_default_options(xa_device_options* o);
 /* 1: loads accumulate |contribution| (error scale of each entry). */

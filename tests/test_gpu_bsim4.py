@@ -48,6 +48,17 @@ def test_bsim4_shared_size_records():
     run_parity(p, np.random.default_rng(209))
 
 
+def test_bsim4_parity_rails():
+    """24 shared nodes for 3000 instances: every target collects hundreds of
+    contributions (the one-wave long-segment scatter; the rows past its
+    in-order limit take the chunked sum)."""
+    n = 3000
+    spec = bsim4_spec(n, seed=219, cards=BSIM4_CARDS, geometry=False)
+    print(f"bsim4 rails n={n}", flush=True)
+    p = Pair("bsim4", spec, topology="shared", seed=220, n_shared_nodes=24, noise_runs=NOISE_RUNS)
+    run_parity(p, np.random.default_rng(221))
+
+
 def test_bsim4_parity_limiter_off():
     import oracle
     n = 3000

@@ -58,6 +58,26 @@ def test_diode_parity_limiter_off_lambertw(lambertw):
     _run(p, np.random.default_rng(106))
 
 
+def test_diode_parity_rails():
+    """128 shared nodes for 20000 diodes: segments of a few hundred
+    contributions, added in instance order by one wavefront each."""
+    n = 20000
+    spec = diode_spec(n, seed=116)
+    p = Pair("diode", spec, topology="shared", seed=117, n_shared_nodes=128)
+    print("diode rails: worst rel err", _run(p, np.random.default_rng(118)))
+
+
+def test_diode_parity_supply_rail():
+    """Two shared nodes for 40000 diodes: each row collects more contributions
+    than the in-order limit, so it is summed in fixed 256-term chunks whose
+    sums combine in a fixed tree; the result stays within 1e-12 of the
+    contributions' magnitude sum of the one-by-one order."""
+    n = 40000
+    spec = diode_spec(n, seed=119)
+    p = Pair("diode", spec, topology="shared", seed=120, n_shared_nodes=2)
+    print("diode supply rail: worst rel err", _run(p, np.random.default_rng(121)))
+
+
 def test_diode_fused_equals_split():
     n = 10000
     spec = diode_spec(n, seed=107)

@@ -34,6 +34,18 @@ def test_batch_matches_plain_and_oracle():
         assert eg <= WAVE_RTOL and eo <= WAVE_RTOL
 
 
+def test_graph_replays_equal_eager():
+    """The Newton iteration captured as HIP graphs (one per flag set, replayed
+    every iteration with eager copies between) gives the eager run's bits."""
+    batch = RingBatch(stages=[11, 5, 21, 7], seed=33)
+    ta, Va = BatchTransient(batch, use_graph=False).run(t_stop=2e-10)
+    bt = BatchTransient(batch, use_graph=True)
+    tb, Vb = bt.run(t_stop=2e-10)
+    print(f"  graphs captured: {len(bt.graphs)}", flush=True)
+    assert len(bt.graphs) >= 4
+    np.testing.assert_array_equal(Va.view(np.int64), Vb.view(np.int64))
+
+
 def test_batch_periods_match_oracle():
     """Rings long enough to oscillate for two periods within the run."""
     batch = RingBatch(stages=[11, 15, 21], seed=32)

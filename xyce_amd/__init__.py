@@ -74,6 +74,11 @@ class Context:
             s = torch.cuda.current_stream(device)
             check(lib().xa_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream)), "set_stream")
 
+    def set_stream(self, stream):
+        """Order the library's work on `stream` (a torch.cuda.Stream), e.g. the
+        stream a HIP graph is being captured on."""
+        check(lib().xa_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)), "set_stream")
+
     def sync(self):
         check(lib().xa_ctx_sync(self.h), "xa_ctx_sync")
 

@@ -53,6 +53,10 @@ def _proto(lib):
         "xa_ctx_sync": (C.c_int, [vp]),
         "xa_ctx_set_stream": (C.c_int, [vp, vp]),
         "xa_ctx_stream": (vp, [vp]),
+        "xa_device_alloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
+        "xa_device_free": (C.c_int, [vp, vp]),
+        "xa_copy_to_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "xa_copy_to_host": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "xa_last_error": (C.c_char_p, []),
         "xa_abi_version": (C.c_int, []),
         "xa_default_options": (None, [P(DeviceOptions)]),

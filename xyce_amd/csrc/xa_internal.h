@@ -54,6 +54,15 @@ struct Plan {
   bool idx32 = false;              // seg/src held as int32 (every index < 2^31)
   void* d_seg = nullptr;           // n_targets + 1 segment starts (int32 or int64)
   void* d_src = nullptr;           // n_src value-buffer slots (int32 or int64)
+  int32_t* d_long = nullptr;       // targets whose segments go to the one-wave kernel
+  int64_t n_long = 0;
+  // rails: segments summed in fixed blocks, then the block sums in order
+  int32_t* d_rail_t = nullptr;     // rail targets
+  int32_t* d_rail_c = nullptr;     // rails + 1 offsets into the blocks
+  void* d_chunk_j0 = nullptr;      // block source ranges (plan index type)
+  void* d_chunk_j1 = nullptr;
+  double* d_partial = nullptr;     // block sums
+  int64_t n_rails = 0, n_chunks = 0;
   double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
   // Block plan of the fused load: the same targets grouped by the block of
   // instances that writes them (defined when every target's contributions

@@ -61,6 +61,23 @@ static int upload(T** dst, const std::vector<T>& src) {
 // the output, which reproduces the sequential `*ptr += value` of the
 // reference bit for bit.
 // ---------------------------------------------------------------------------
+// Segments are read eight sources at a time, all loads issued before the
+// adds, which stay in segment order.  Segments longer than kLongSegment (a
+// supply rail's diagonal collects every device on it) are left to
+// k_scatter_long.
+constexpr int kLongSegment = 64;
+// Segments longer than kRailSegment (a supply rail of a large netlist) are
+// not added one by one: an in-order FP64 sum is one dependent add (about 16
+// cycles) per term, which would hold one wavefront for most of a pass.  They
+// are summed in fixed chunks of kRailChunk contributions, each chunk in
+// order; lane l of one wavefront adds chunk sums l, l + 64, ... in order;
+// the 64 lane sums combine in a fixed pairwise tree and the total is added
+// onto the output.  Deterministic (the same bits on every run), and within
+// (kRailChunk + chunks / 64 + 7) rounding steps of the contributions'
+// magnitude sum of the one-by-one order.
+constexpr int64_t kRailSegment = 512;
+constexpr int kRailChunk = 256;
+
 template <typename I, bool CONTIG>
 __global__ void __launch_bounds__(256)
 k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
@@ -68,11 +85,143 @@ k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
               const double* __restrict__ vals, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  const int64_t o = CONTIG ? t0 + t : target[t];
   const I s1 = seg[t + 1];
+  I j = seg[t];
+  if (s1 - j > kLongSegment) return;   // k_scatter_long adds this one
+  const int64_t o = CONTIG ? t0 + t : target[t];
   double acc = out[o];
-  for (I j = seg[t]; j < s1; ++j) acc += vals[src[j]];
+  constexpr int U = 8;
+  for (; j + U <= s1; j += U) {
+    I ix[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) ix[u] = src[j + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = vals[ix[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u];
+  }
+  for (; j < s1; ++j) acc += vals[src[j]];
   out[o] = acc;
+}
+
+// In-order sums of long runs of terms by one wavefront.  A round of ROUND
+// terms is gathered (ROUND / 64 per lane), staged in LDS, and every lane adds
+// the staged terms in order, each read as an LDS broadcast; the next round's
+// gathers are in flight meanwhile.  The sum is the one-by-one sum (the same
+// on every lane; lane 0 writes it).
+
+// Orders the wavefront's LDS accesses on either side (one-wave blocks: no
+// hardware barrier is needed, only the compiler's and the memory model's).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Term indices of the round at base (lanes past the end repeat the last
+// term; the adds below stop at the end).
+template <typename I, bool GATHER, int PER>
+__device__ __forceinline__ void round_index(I (&ix)[PER], I base, I end,
+                                            const I* __restrict__ src, int lane) {
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    I j = base + r * 64 + lane;
+    j = j < end ? j : end - 1;
+    ix[r] = GATHER ? src[j] : j;
+  }
+}
+
+// acc + t[s0] + t[s0 + 1] + ... + t[s1 - 1] in order, t[j] = vals[src[j]]
+// (GATHER) or vals[j].  The next round's terms and the index of the round
+// after are loaded while a round is added.  stage: ROUND doubles of LDS.
+template <typename I, bool GATHER, int ROUND>
+__device__ __forceinline__ double ordered_sum(double acc, I s0, I s1, const I* __restrict__ src,
+                                              const double* __restrict__ vals, double* stage) {
+  constexpr int PER = ROUND / 64;
+  if (s1 <= s0) return acc;
+  const int lane = threadIdx.x;
+  I ix[PER];
+  double v[PER];
+  round_index<I, GATHER, PER>(ix, s0, s1, src, lane);
+#pragma unroll
+  for (int r = 0; r < PER; ++r) v[r] = vals[ix[r]];
+  if (s0 + ROUND < s1) round_index<I, GATHER, PER>(ix, s0 + ROUND, s1, src, lane);
+  for (I base = s0; base < s1; base += ROUND) {
+    wave_sync();   // the previous round's reads of stage are done
+#pragma unroll
+    for (int r = 0; r < PER; ++r) stage[r * 64 + lane] = v[r];
+    wave_sync();
+    if (base + ROUND < s1) {
+#pragma unroll
+      for (int r = 0; r < PER; ++r) v[r] = vals[ix[r]];
+      if (base + 2 * ROUND < s1) round_index<I, GATHER, PER>(ix, base + 2 * ROUND, s1, src, lane);
+    }
+    const int cnt = (s1 - base) < ROUND ? (int)(s1 - base) : ROUND;
+    int u = 0;
+    for (; u + 64 <= cnt; u += 64) {
+#pragma unroll
+      for (int k = 0; k < 64; ++k) acc += stage[u + k];
+    }
+    for (; u < cnt; ++u) acc += stage[u];
+  }
+  return acc;
+}
+
+// One wavefront per long segment, added onto the output in segment order.
+template <typename I, bool CONTIG>
+__global__ void __launch_bounds__(64)
+k_scatter_long(const int32_t* __restrict__ long_t, const int32_t* __restrict__ target, int32_t t0,
+               const I* __restrict__ seg, const I* __restrict__ src,
+               const double* __restrict__ vals, double* __restrict__ out) {
+  constexpr int ROUND = (int)kRailSegment;   // a long segment is one round
+  __shared__ double stage[ROUND];
+  const int32_t t = long_t[blockIdx.x];
+  const int64_t o = CONTIG ? (int64_t)t0 + t : target[t];
+  const double acc = ordered_sum<I, true, ROUND>(out[o], seg[t], seg[t + 1], src, vals, stage);
+  if (threadIdx.x == 0) out[o] = acc;
+}
+
+// One wavefront per rail chunk of at most kRailChunk contributions, summed
+// in order (from -0.0, the exact identity of IEEE addition, so the sum is
+// the first term plus the rest).
+template <typename I>
+__global__ void __launch_bounds__(64)
+k_rail_partial(const I* __restrict__ j0s, const I* __restrict__ j1s, const I* __restrict__ src,
+               const double* __restrict__ vals, double* __restrict__ partial) {
+  __shared__ double stage[kRailChunk];
+  const int64_t c = blockIdx.x;
+  const double acc = ordered_sum<I, true, kRailChunk>(-0.0, j0s[c], j1s[c], src, vals, stage);
+  if (threadIdx.x == 0) partial[c] = acc;
+}
+
+// One wavefront per rail: lane l adds chunk sums l, l + 64, ... in order,
+// then a fixed pairwise tree over the lanes; the total goes onto the output.
+template <bool CONTIG>
+__global__ void __launch_bounds__(64)
+k_rail_sum(const int32_t* __restrict__ rail_t, const int32_t* __restrict__ rail_c,
+           const int32_t* __restrict__ target, int32_t t0, const double* __restrict__ partial,
+           double* __restrict__ out) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t t = rail_t[r];
+  const int32_t c0 = rail_c[r], c1 = rail_c[r + 1];
+  double s = -0.0;   // the identity of IEEE addition
+  constexpr int R = 4;
+  int32_t c = c0 + lane;
+  for (; c + 64 * (R - 1) < c1; c += 64 * R) {
+    double v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = partial[c + 64 * k];
+#pragma unroll
+    for (int k = 0; k < R; ++k) s += v[k];
+  }
+  for (; c < c1; c += 64) s += partial[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) {
+    const int64_t o = CONTIG ? (int64_t)t0 + t : target[t];
+    out[o] = out[o] + s;
+  }
 }
 
 template <typename I, bool CONTIG>
@@ -93,9 +242,20 @@ static void launch_scatter(const Plan& p, bool assign, hipStream_t st, double* o
   if (assign)
     hipLaunchKernelGGL((k_scatter_assign<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
-  else
+  else {
     hipLaunchKernelGGL((k_scatter_add<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
+    if (p.n_long > 0)
+      hipLaunchKernelGGL((k_scatter_long<I, CONTIG>), dim3((unsigned)p.n_long), dim3(64), 0, st,
+                         p.d_long, p.d_target, p.t0, seg, src, p.d_vals, out);
+    if (p.n_rails > 0) {
+      hipLaunchKernelGGL((k_rail_partial<I>), dim3((unsigned)p.n_chunks), dim3(64), 0, st,
+                         static_cast<const I*>(p.d_chunk_j0),
+                         static_cast<const I*>(p.d_chunk_j1), src, p.d_vals, p.d_partial);
+      hipLaunchKernelGGL((k_rail_sum<CONTIG>), dim3((unsigned)p.n_rails), dim3(64), 0, st,
+                         p.d_rail_t, p.d_rail_c, p.d_target, p.t0, p.d_partial, out);
+    }
+  }
 }
 
 static double* kind_out(const xa_vectors& v, int k) {
@@ -267,6 +427,41 @@ void* xa_ctx_stream(xa_ctx ctx) {
   return c ? (void*)c->stream : nullptr;
 }
 
+int xa_device_alloc(xa_ctx ctx, size_t bytes, void** out) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !out) return fail(XA_ERR_ARG, "xa_device_alloc: null argument");
+  *out = nullptr;
+  if (bytes == 0) return XA_OK;
+  XA_HIP(hipSetDevice(c->device));
+  XA_HIP(hipMalloc(out, bytes));
+  return XA_OK;
+}
+
+int xa_device_free(xa_ctx ctx, void* ptr) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(XA_ERR_ARG, "xa_device_free: null context");
+  if (!ptr) return XA_OK;
+  XA_HIP(hipStreamSynchronize(c->stream));
+  XA_HIP(hipFree(ptr));
+  return XA_OK;
+}
+
+static int copy_sync(Ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (!c || ((!dst || !src) && bytes)) return fail(XA_ERR_ARG, "xa_copy: null argument");
+  if (bytes == 0) return XA_OK;
+  XA_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+  XA_HIP(hipStreamSynchronize(c->stream));
+  return XA_OK;
+}
+
+int xa_copy_to_device(xa_ctx ctx, void* dst, const void* src, size_t bytes) {
+  return copy_sync(reinterpret_cast<Ctx*>(ctx), dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+int xa_copy_to_host(xa_ctx ctx, void* dst, const void* src, size_t bytes) {
+  return copy_sync(reinterpret_cast<Ctx*>(ctx), dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
 // ---------------------------------------------------------------------------
 // group setup
 // ---------------------------------------------------------------------------
@@ -287,6 +482,8 @@ int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
 
 static void free_plan(Plan& p) {
   (void)hipFree(p.d_target); (void)hipFree(p.d_seg); (void)hipFree(p.d_src);
+  (void)hipFree(p.d_long); (void)hipFree(p.d_rail_t); (void)hipFree(p.d_rail_c);
+  (void)hipFree(p.d_chunk_j0); (void)hipFree(p.d_chunk_j1); (void)hipFree(p.d_partial);
   (void)hipFree(p.d_vals); (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt);
   (void)hipFree(p.d_bseg); (void)hipFree(p.d_bsrc);
   p = Plan();
@@ -612,6 +809,32 @@ static int build_plan(Group& g, int k) {
   p.n_targets = (int64_t)target.size();
   p.n_src = acc;
   int rc;
+  std::vector<int32_t> long_t, rail_t, rail_c(1, 0);
+  std::vector<int64_t> chunk_j0, chunk_j1;
+  for (int64_t t = 0; t < p.n_targets; ++t) {
+    const int64_t len = seg[t + 1] - seg[t];
+    if (len > kRailSegment) {
+      rail_t.push_back((int32_t)t);
+      for (int64_t j = seg[t]; j < seg[t + 1]; j += kRailChunk) {
+        chunk_j0.push_back(j);
+        chunk_j1.push_back(std::min<int64_t>(j + kRailChunk, seg[t + 1]));
+      }
+      rail_c.push_back((int32_t)chunk_j0.size());
+    } else if (len > kLongSegment) {
+      long_t.push_back((int32_t)t);
+    }
+  }
+  p.n_long = (int64_t)long_t.size();
+  p.n_rails = (int64_t)rail_t.size();
+  p.n_chunks = (int64_t)chunk_j0.size();
+  if (p.n_chunks > ((int64_t)1 << 25))   // one 64-lane launch slot per chunk
+    return fail(XA_ERR_ARG, "plan: supply-rail contributions exceed the chunked launch");
+  if ((rc = upload(&p.d_long, long_t)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_rail_t, rail_t)) != XA_OK) return rc;
+  if (p.n_rails > 0) {
+    if ((rc = upload(&p.d_rail_c, rail_c)) != XA_OK) return rc;
+    XA_HIP(hipMalloc((void**)&p.d_partial, sizeof(double) * (size_t)p.n_chunks));
+  }
   // contiguous targets need no target array; 32-bit indices when they fit
   bool contig = true;
   for (size_t t = 0; t < target.size() && contig; ++t) contig = target[t] == target[0] + (int32_t)t;
@@ -626,6 +849,13 @@ static int build_plan(Group& g, int k) {
     if ((rc = upload(&dr, r32)) != XA_OK) return rc;
     p.d_seg = ds;
     p.d_src = dr;
+    std::vector<int32_t> a0(chunk_j0.begin(), chunk_j0.end()), a1(chunk_j1.begin(), chunk_j1.end());
+    int32_t* c0 = nullptr;
+    int32_t* c1 = nullptr;
+    if ((rc = upload(&c0, a0)) != XA_OK) return rc;
+    if ((rc = upload(&c1, a1)) != XA_OK) return rc;
+    p.d_chunk_j0 = c0;
+    p.d_chunk_j1 = c1;
   } else {
     int64_t* ds = nullptr;
     int64_t* dr = nullptr;
@@ -633,6 +863,12 @@ static int build_plan(Group& g, int k) {
     if ((rc = upload(&dr, src)) != XA_OK) return rc;
     p.d_seg = ds;
     p.d_src = dr;
+    int64_t* c0 = nullptr;
+    int64_t* c1 = nullptr;
+    if ((rc = upload(&c0, chunk_j0)) != XA_OK) return rc;
+    if ((rc = upload(&c1, chunk_j1)) != XA_OK) return rc;
+    p.d_chunk_j0 = c0;
+    p.d_chunk_j1 = c1;
   }
   if ((rc = build_block_plan(g, p, target, seg, src)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));

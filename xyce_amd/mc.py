@@ -131,8 +131,10 @@ class BatchTopology:
 class BatchTransient:
     """The whole batch on one GPU."""
 
-    def __init__(self, batch, device=0, options=None):
+    def __init__(self, batch, device=0, options=None, use_graph=False):
         import torch
+        self.use_graph = use_graph
+        self.graphs = {}
         self.torch = torch
         self.batch = batch
         self.ctx = Context(device)
@@ -211,12 +213,44 @@ class BatchTransient:
             P(self.held_dc if dc else self.held_tr), C.c_double(h), 1 if dc else 0, P(self.x),
             P(self.q_now), P(self.dx_max)), "xa_batch_newton_step")
 
+    def _iteration(self, flags, h, dc):
+        """One Newton iteration's device work: zero, load, solve, reduce."""
+        self._load(**flags)
+        self._solve(h, dc)
+        return self.dx_max.max()
+
+    def _graph(self, key, flags, h, dc):
+        """The iteration captured as a HIP graph.  The solver-state flags are
+        baked into the captured kernel arguments, so one graph serves every
+        iteration with the same flags; the loads test newtonIter only for
+        == 0, so one graph stands for every later iteration."""
+        torch = self.torch
+        g = self.graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            main = torch.cuda.current_stream()
+            with torch.cuda.graph(g):
+                self.ctx.set_stream(torch.cuda.current_stream())
+                try:
+                    out = self._iteration(flags, h, dc)
+                finally:
+                    self.ctx.set_stream(main)
+            self.graphs[key] = g = (g, out)
+        return g
+
     def _newton(self, flags_of, h, dc, vtol, max_iter):
         for it in range(max_iter):
-            self._load(**flags_of(it))
-            self._solve(h, dc)
+            flags = flags_of(it)
+            if self.use_graph:
+                key = (dc, h, it == 0, tuple(sorted((k, v) for k, v in flags.items()
+                                                    if k != "newtonIter")))
+                g, worst_t = self._graph(key, dict(flags, newtonIter=min(it, 1)), h, dc)
+                g.replay()
+                self.loads += 1
+            else:
+                worst_t = self._iteration(flags, h, dc)
             self.newton_iterations += 1
-            worst = float(self.dx_max.max().item())
+            worst = float(worst_t.item())
             if not np.isfinite(worst):
                 raise RuntimeError("batched Newton: a circuit's update is not finite")
             if it > 0 and worst <= vtol:

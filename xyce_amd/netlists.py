@@ -1,0 +1,116 @@
+"""Seeded synthetic netlists for load-pass measurements on shared nodes.
+
+* ``logic_dag``: a flat logic netlist -- a random DAG of INV / NAND2 / NOR2
+  cells of BSIM4 transistors (two shared cards, two shared sizes), each net a
+  driver node and a far-end node joined by a wire resistor, fanout 1-4, 20
+  primary inputs.  The wire resistors and capacitors are linear devices
+  outside these groups; here they only shape the node structure.
+* ``hb_diffamp``: the time samples of a harmonic-balance analysis of a
+  differential amplifier: a 2k-transistor amplifier replicated once per time
+  sample, every copy with its own nodes (the HB loader evaluates each
+  sample's devices at that sample's voltages).
+
+Each returns (models, model_index, params, node_lids, n_nodes, voltages):
+instance-major external node LIDs (D, G, S, B; ground -1) and one voltage
+per node, ready for :class:`xyce_amd.workloads.Workload`.
+"""
+import numpy as np
+
+SEED = 20261203
+CARDS = [(54, {"TYPE": 1.0}), (54, {"TYPE": -1.0})]
+
+
+def logic_dag(n_transistors=200_000, n_inputs=20, seed=SEED, vdd=1.0, window=2000,
+              wn=90e-9, wp=180e-9, l=45e-9):
+    rng = np.random.default_rng(np.random.SeedSequence(seed))
+    VDD = 0
+    nxt = 1
+    pool = []           # (far-end node, remaining fanout)
+    for _ in range(n_inputs):
+        pool.append([nxt, int(rng.integers(1, 5))])
+        nxt += 1
+    mi, w, lids = [], [], []
+
+    def fet(pmos, d, g, s):
+        mi.append(1 if pmos else 0)
+        w.append(wp if pmos else wn)
+        lids.extend([d, g, s, VDD if pmos else -1])
+
+    count = 0
+    while count < n_transistors:
+        kind = int(rng.integers(0, 3))          # 0 INV, 1 NAND2, 2 NOR2
+        k = 1 if kind == 0 else 2
+        lo = max(0, len(pool) - window)
+        ins = []
+        for _ in range(k):
+            j = int(rng.integers(lo, len(pool)))
+            ins.append(pool[j][0])
+            pool[j][1] -= 1
+            if pool[j][1] == 0:
+                pool[j] = pool[-1]
+                pool.pop()
+                lo = max(0, min(lo, len(pool) - 1))
+        out, far = nxt, nxt + 1
+        nxt += 2
+        if kind == 0:
+            fet(False, out, ins[0], -1)
+            fet(True, out, ins[0], VDD)
+        elif kind == 1:                          # NAND2: series NMOS, parallel PMOS
+            x = nxt
+            nxt += 1
+            fet(False, out, ins[0], x)
+            fet(False, x, ins[1], -1)
+            fet(True, out, ins[0], VDD)
+            fet(True, out, ins[1], VDD)
+        else:                                    # NOR2: series PMOS, parallel NMOS
+            y = nxt
+            nxt += 1
+            fet(True, y, ins[0], VDD)
+            fet(True, out, ins[1], y)
+            fet(False, out, ins[0], -1)
+            fet(False, out, ins[1], -1)
+        count += 2 if kind == 0 else 4
+        pool.append([far, int(rng.integers(1, 5))])
+    n = len(mi)
+    params = {"W": np.asarray(w), "L": np.full(n, l)}
+    v = rng.uniform(0.0, vdd, nxt)
+    v[VDD] = vdd
+    return CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt, v
+
+
+def hb_diffamp(samples=128, n_transistors=2000, seed=SEED, vdd=1.0, l=45e-9):
+    """One copy per time sample of an amplifier of n_transistors/5 cells.
+    Cell: differential pair (NMOS, gates on the two inputs), PMOS
+    current-mirror load, NMOS tail source gated by a shared bias node; cell
+    outputs feed the next cell's inputs.  Copy s sees the inputs of sample s
+    of a two-tone (1.00/1.01 GHz, 10 mV) drive around mid-rail."""
+    rng = np.random.default_rng(np.random.SeedSequence(seed))
+    cells = n_transistors // 5
+    mi, w, lids, volts = [], [], [], []
+    nxt = 0
+    t = np.arange(samples) / samples * 100e-9        # 100 ns: one beat period
+    drive = 0.01 * (np.sin(2 * np.pi * 1.00e9 * t) + np.sin(2 * np.pi * 1.01e9 * t))
+    for s in range(samples):
+        VDD, BIAS, INP, INN = nxt, nxt + 1, nxt + 2, nxt + 3
+        nxt += 4
+        node_v = [vdd, 0.45, 0.5 + drive[s], 0.5 - drive[s]]
+        a, b = INP, INN
+        for c in range(cells):
+            tail, outp, outn = nxt, nxt + 1, nxt + 2
+            nxt += 3
+            node_v += [0.05 + 0.01 * rng.standard_normal(),
+                       0.6 + 0.05 * rng.standard_normal(), 0.6 + 0.05 * rng.standard_normal()]
+            for pmos, d, g, src, width in ((False, outn, a, tail, 180e-9),
+                                           (False, outp, b, tail, 180e-9),
+                                           (True, outn, outn, VDD, 360e-9),
+                                           (True, outp, outn, VDD, 360e-9),
+                                           (False, tail, BIAS, -1, 360e-9)):
+                mi.append(1 if pmos else 0)
+                w.append(width)
+                lids.extend([d, g, src, VDD if pmos else -1])
+            a, b = outp, outn
+        volts.extend(node_v)
+    n = len(mi)
+    params = {"W": np.asarray(w), "L": np.full(n, l)}
+    return CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt, \
+        np.asarray(volts)

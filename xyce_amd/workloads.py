@@ -45,7 +45,9 @@ class Workload:
     """A built group plus its device vectors."""
 
     def __init__(self, model, n, seed=SEED, device=0, options=None, cards=None,
-                 model_index=None, params=None):
+                 model_index=None, params=None, netlist=None):
+        """netlist: (node_lids, n_nodes, voltages) of a shared-node netlist
+        (xyce_amd.netlists); default: private nodes, V ~ U[-0.5, 1.5]."""
         self.model = model
         self.n = n
         rng = np.random.default_rng(np.random.SeedSequence(seed))
@@ -60,8 +62,18 @@ class Workload:
         self.master.setup()
         n_ext, n_int, n_sta, n_sto = self.master.counts()
         per = (n_ext + n_int).astype(np.int64)
-        self.n_nodes = int(per.sum())
-        node_lids = np.arange(self.n_nodes, dtype=np.int32)      # private nodes
+        if netlist is None:
+            self.n_nodes = int(per.sum())
+            node_lids = np.arange(self.n_nodes, dtype=np.int32)      # private nodes
+            volts = rng.uniform(-0.5, 1.5, self.n_nodes)
+        else:
+            ext_lids, n_nodes, volts = netlist
+            if n_int.any():
+                raise ValueError("netlist workloads assume cards without internal nodes")
+            if len(ext_lids) != int(n_ext.sum()):
+                raise ValueError("netlist gives the wrong number of terminal nodes")
+            node_lids = np.ascontiguousarray(ext_lids, np.int32)
+            self.n_nodes = int(n_nodes)
         self.n_sta = int(n_sta.sum())
         self.n_sto = int(n_sto.sum())
         self.master.register_lids(node_lids, np.arange(self.n_sta, dtype=np.int32),
@@ -80,7 +92,7 @@ class Workload:
         dev = torch.device("cuda", device)
         z = lambda m: torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
         self.vec = {
-            "nextSol": torch.from_numpy(rng.uniform(-0.5, 1.5, self.n_nodes)).to(dev),
+            "nextSol": torch.from_numpy(np.ascontiguousarray(volts, np.float64)).to(dev),
             "currSol": z(self.n_nodes),
             "nextSta": z(self.n_sta), "currSta": z(self.n_sta), "lastSta": z(self.n_sta),
             "nextStaDeriv": z(self.n_sta),
