@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fused", action="store_true", help="time the one-call fused load")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the block shapes on the workload first, then time xa_load_all")
     ap.add_argument("--ring", action="store_true",
                     help="run the 51-stage BSIM4 ring-oscillator transient (1 ps steps to 2 ns) "
                          "instead of the microbenchmark")
@@ -317,6 +319,13 @@ def main():
     else:
         print(f"[bench] building {args.model} workload n={args.n}", file=sys.stderr, flush=True)
         wl = Workload(args.model, args.n)
+    tuned = None
+    if args.autotune:
+        route, block = wl.master.autotune(wl.ss, wl.vectors, reps=3)
+        wl.zero_outputs()
+        tuned = {"route": route, "block": block}
+        args.fused = True
+        print(f"[bench] autotune: {route}, block {block}", file=sys.stderr, flush=True)
     step = wl.load_fused if args.fused else wl.load_pass
     for _ in range(args.warmup):
         step()
@@ -376,6 +385,7 @@ def main():
                            ("load_all (split at this size)" if args.fused else
                             "updateState+loadDAEVectors+loadDAEMatrices")},
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if (one_kernel or args.netlist) else

@@ -216,6 +216,18 @@ int xa_load_matrices(xa_group g, const xa_solver_state* ss, const xa_vectors* v)
  * calls in sequence). */
 int xa_load_all(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
 
+/* Autotuner of the block shapes (no reference counterpart).  Times, on the
+ * caller's vectors and flags, the four-call path with update-kernel blocks of
+ * 64, 128 and 256 threads, and the fused one-kernel load with blocks of 64,
+ * 128 and 256 instances where its block plans exist and its LDS fits; then
+ * keeps the fastest: the update-kernel block for the four calls, and the path
+ * (and fused block) for xa_load_all.  Every choice gives the same bits.  It
+ * runs up to 6 * (reps + 1) load passes that add into the output vectors (zero them
+ * afterwards) and rewrites store/state as a load does.  *route: 1 four calls,
+ * 2 fused; *block: its block size.  Synchronizes. */
+int xa_group_autotune(xa_group g, const xa_solver_state* ss, const xa_vectors* v, int reps,
+                      int* route, int* block);
+
 /* Device-side status of the last update: number of instances whose
  * evaluation returned false (0 = all succeeded). Synchronizes. */
 int xa_group_failures(xa_group g, int64_t* n_failed);
@@ -249,7 +261,7 @@ int xa_group_last_times(xa_group g, float* ms_update, float* ms_vectors,
  * order every assembled entry (i, j) must lie within the band: i - j <= kl
  * below and j - i <= ku above the diagonal.  The band, with
  * kl rows of pivoting fill, is factored in LDS: (2kl+ku+2)*max_unknowns
- * doubles must fit in 159 KB.
+ * doubles must fit in 159 KB.  One wavefront per circuit.
  * With dc = 0:  q_now = q + c_lin .* x;
  *   rhs = -((q_now - q_prev)/h + f) + dFdxdVp + dQdxdVp/h,
  *   J = dFdx + dQdx/h + diag(c_lin/h);

@@ -59,3 +59,4 @@ def test_batch_periods_match_oracle():
         print(f"  ring {r} ({ring.N} stages): period {pb:.6e} vs oracle {po:.6e}", flush=True)
         assert pb is not None and po is not None
         assert abs(pb - po) <= WAVE_RTOL * po
+

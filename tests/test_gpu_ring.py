@@ -46,3 +46,27 @@ def test_ring51_transient(fused):
         np.testing.assert_allclose(cg, cc, rtol=WAVE_RTOL)
     pg, pc = period(t, Vg[:, 0], 0.5), period(t, Vc[:, 0], 0.5)
     assert pg is not None and abs(pg - pc) <= WAVE_RTOL * pc
+
+
+def _cross_any(t, v, level):
+    """Crossings of `level` in either direction, in time order (.MEASURE's
+    CROSS= counting)."""
+    return np.sort(np.concatenate([crossings(t, v, level, rising=True),
+                                   crossings(t, v, level, rising=False)]))
+
+
+def test_ring17_measured_period():
+    """The shape of the reference's 17-stage ring example
+    (utils/ADMS/examples/bsimcmg_107.0.0/benchmark_test/ringosc_17stg.sp):
+    .IC on node 1 = 1 V, 1 ps steps, period = (7th - 1st crossing of
+    v(1) = 0.5 V) / 3.  Here with BSIM4 default cards, run to 2 ns so seven
+    crossings fall inside; GPU (fused loads) and oracle periods within 1e-6."""
+    ring = Ring(stages=17, ic_node=0, ic_value=1.0)
+    tg, Vg, _ = transient(GpuLoader(ring, fused=True), ring, t_stop=2e-9)
+    tc, Vc, _ = transient(OracleLoader(ring), ring, t_stop=2e-9)
+    cg, cc = _cross_any(tg, Vg[:, 0], 0.5), _cross_any(tc, Vc[:, 0], 0.5)
+    assert len(cg) >= 7 and len(cg) == len(cc)
+    pg, pc = (cg[6] - cg[0]) / 3, (cc[6] - cc[0]) / 3
+    print(f"  17 stages: period {pg:.6e} s (oracle {pc:.6e}), {pg / 34:.3e} s per stage",
+          flush=True)
+    assert abs(pg - pc) <= WAVE_RTOL * pc

@@ -197,6 +197,15 @@ class Master:
     def loadAll(self, ss, vectors):
         check(lib().xa_load_all(self.h, C.byref(ss), C.byref(vectors)), "loadAll")
 
+    def autotune(self, ss, vectors, reps=5):
+        """Time the block shapes on these vectors and keep the fastest
+        (xa_group_autotune); returns ("split" | "fused", block).  The tuning
+        loads add into the output vectors: zero them afterwards."""
+        route, block = C.c_int(), C.c_int()
+        check(lib().xa_group_autotune(self.h, C.byref(ss), C.byref(vectors), int(reps),
+                                      C.byref(route), C.byref(block)), "autotune")
+        return ("split" if route.value == 1 else "fused"), block.value
+
     def failures(self):
         c = C.c_int64()
         check(lib().xa_group_failures(self.h, C.byref(c)), "failures")

@@ -474,13 +474,29 @@ XA_HD void polyDepletion(double phi, double ngate, double epsgate, double coxe, 
 // store-vector value, 0 for an unregistered slot
 XA_HD double stv(const double* s, int32_t lid) { return lid >= 0 ? s[lid] : 0.0; }
 
-#define PP(x) dfld(a, P_##x, i)
+// per-instance parameter columns, read through `dfb` (see reload_params)
+#define PP(x) (dfb[(int64_t)(P_##x) * a.n + i])
 #define IV(x) dfld(a, V_##x, i)
 
-template <bool FUSED>
-__device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
+// A section boundary for the per-instance parameter columns: the compiler
+// takes the column base as changed, so a parameter used in a later section is
+// read again there (from L1/L2) instead of being held in registers across
+// sections.  The eval's register peak (the DC model around Vasat) holds
+// fewer long-lived parameters, and spills less.
+__device__ __forceinline__ void reload_params(const double*& dfb) {
+  asm volatile("" : "+s"(dfb));
+}
+
+// LDSCARD: the model cards are staged in LDS by the kernel (s_card, the
+// group's cards in model order); otherwise they are read from global memory.
+template <bool FUSED, bool LDSCARD>
+__device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
-  const DevCard& C = *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
+  const int32_t model_ix = ifld(a, F_MODEL, i);
+  const DevCard& C =
+      LDSCARD ? *reinterpret_cast<const DevCard*>(s_card + (int64_t)model_ix * a.mstride)
+              : *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)model_ix * a.mstride);
+  const double* dfb = a.dfield;
   const xa_solver_state& ss = a.ss;
   const xa_device_options& opt = a.opt;
   const double gmin = opt.gmin;
@@ -602,6 +618,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   ChargeComputationNeeded = (ss.tranopFlag || ss.acopFlag || ss.transientFlag) ? 1 : 0;
   limitedFlag = 0;
 
+  reload_params(dfb);
   /* node voltages (:7919-7974) */
   Vd = solv(a.v.nextSol, li_Drain);
   Vs = solv(a.v.nextSol, li_Source);
@@ -667,6 +684,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   Vgegm_orig = Vgegm;
   Vgmgp_orig = Vgmgp;
 
+  reload_params(dfb);
   /* initial conditions (:8052-8097) */
   if (ss.initJctFlag && !IV(OFF) && opt.voltageLimiterFlag) {
     int setv = 1;
@@ -697,6 +715,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     vds = vsbs = vdes = vses = qdef = 0.0;
   }
 
+  reload_params(dfb);
   /* previous iteration's junction voltages (:8100-8158) */
   if (ss.newtonIter == 0) {
     newtonIterOld = 0;
@@ -753,6 +772,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   (void)vged_old; (void)vgmd_old; (void)vges_old; (void)vgms_old; (void)vdbs_old;
   (void)vsbs_old; (void)vdbd_old; (void)vbd_old;
 
+  reload_params(dfb);
   /* voltage limiting (:8165-8359), rgateMod = rbodyMod = rdsMod = 0 */
   if (opt.voltageLimiterFlag && !(ss.initFixFlag && IV(OFF))) {
     if (ss.newtonIter >= 0 && !(ss.initJctFlag)) {
@@ -801,6 +821,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   vbs_jct = (!irbodyMod) ? vbs : vsbs;
   vbd_jct = (!irbodyMod) ? vbd : vdbd;
 
+  reload_params(dfb);
   /* source/drain junction diodes (:8425-8769) */
   Nvtms = C.vtm * C.SjctEmissionCoeff;
   if ((IV(Aseff) <= 0.0) && (IV(Pseff) <= 0.0)) SourceSatCurrent = 1.0e-14;
@@ -945,6 +966,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     }
   }
 
+  reload_params(dfb);
   /* trap-assisted tunneling and recombination (:8655-8769) */
   Nvtmrssws = C.vtm0 * C.njtsswstemp;
   Nvtmrsswgs = C.vtm0 * C.njtsswgstemp;
@@ -1033,6 +1055,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   cbd -= IV(DjctTempRevSatCur) * (T2 - 1.0) + IV(DswTempRevSatCur) * (T4 - 1.0)
             + IV(DswgTempRevSatCur) * (T6 - 1.0);
 
+  reload_params(dfb);
   /* mode (:8773-8788) */
   if (vds >= 0.0) {
     mode = 1;
@@ -1051,6 +1074,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   epsrox = C.epsrox;
   toxe = C.toxe;
   epssub = kEpsSi;
+  reload_params(dfb);
   /* MOSFET homotopy (:8818-8838), one gain-scale block, no staggering */
   if (ss.artParameterFlag) {
     const double alpha = ss.gainScale;
@@ -1059,6 +1083,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     Vgs = (alpha) * Vgs + (1.0 - alpha) * vgstConst;
   }
 
+  reload_params(dfb);
   /* Vbseff (:8849-8868) */
   T0 = Vbs - IV(vbsc) - 0.001;
   T1 = sqrt(T0 * T0 - 0.004 * IV(vbsc));
@@ -1085,6 +1110,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   Vtm = C.vtm;
   Vtm0 = C.vtm0;
 
+  reload_params(dfb);
   /* threshold voltage (:8883-9052) */
   T3 = sqrt(Xdep);
   V0 = PP(vbi) - PP(phi);
@@ -1204,6 +1230,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   }
   von = Vth;
 
+  reload_params(dfb);
   /* poly gate depletion (:9055-9080) */
   T0 = IV(vfb) + PP(phi);
   T1 = kEpsSi;
@@ -1218,6 +1245,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   }
   Vgst = Vgs_eff - Vth;
 
+  reload_params(dfb);
   /* effective Vgst (:9085-9150) */
   T0 = n * Vtm;
   T1 = PP(mstar) * Vgst;
@@ -1273,6 +1301,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
   dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
 
+  reload_params(dfb);
   /* effective channel geometry and series resistance (:9153-9198) */
   T9 = sqrtPhis - PP(sqrtPhi);
   Weff = PP(weff) - 2.0 * (PP(dwg) * Vgsteff + PP(dwb) * T9);
@@ -1305,6 +1334,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     else grdsw = 0.0;
   }
 
+  reload_params(dfb);
   /* bulk charge effect (:9201-9258) */
   T9 = 0.5 * PP(k1ox) * Lpe_Vb / sqrtPhis;
   T1 = T9 + IV(k2ox) - PP(k3b) * Vth_NarrowW;
@@ -1352,6 +1382,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   Abulk *= T0;
   Abulk0 *= T0;
 
+  reload_params(dfb);
   /* mobility (:9261-9341) */
   T14 = 0.0;
   if (C.mobMod == 0) {
@@ -1424,6 +1455,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   dueff_dVd = T9 * dDenomi_dVd;
   dueff_dVb = T9 * dDenomi_dVb;
 
+  reload_params(dfb);
   /* saturation field and Vdsat (:9344-9450) */
   WVCox = Weff * IV(vsattemp) * C.coxe;
   WVCoxRds = WVCox * Rds;
@@ -1500,6 +1532,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   }
   vdsat = Vdsat;
 
+  reload_params(dfb);
   /* effective Vds (:9453-9497) */
   T1 = Vdsat - Vds - PP(delta);
   dT1_dVg = dVdsat_dVg;
@@ -1535,6 +1568,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   diffVds = Vds - Vdseff_;
   Vdseff = Vdseff_;
 
+  reload_params(dfb);
   /* velocity overshoot (:9500-9540) */
   if ((C.lambdaGiven) && (C.lambda > 0.0)) {
     T1 = Leff * ueff;
@@ -1572,6 +1606,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     EsatL *= T10;
   }
 
+  reload_params(dfb);
   /* Vasat (:9543-9565) */
   tmp4 = 1.0 - 0.5 * Abulk * Vdsat / Vgst2Vtm;
   T9 = WVCoxRds * Vgsteff;
@@ -1592,6 +1627,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   dVasat_dVb = (dT0_dVb - Vasat * dT1_dVb) / T1;
   dVasat_dVd = dT0_dVd / T1;
 
+  reload_params(dfb);
   /* effective oxide capacitance, beta, Idl (:9568-9622) */
   tmp1 = IV(vtfbphi2);
   tmp2 = 2.0e8 * C.toxp;
@@ -1636,6 +1672,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   dIdl_dVd = T1 * dgche_dVd;
   dIdl_dVb = T1 * dgche_dVb - T2 * dRds_dVb;
 
+  reload_params(dfb);
   /* output resistance: CLM, DIBL, DITS, SCBE (:9625-9832) */
   if (PP(fprout) <= 0.0) {
     FP = 1.0;
@@ -1787,6 +1824,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   dIdsa_dVd = dIdsa_dVd * T9 + Idsa * dT9_dVd;
   Idsa *= T9;
 
+  reload_params(dfb);
   /* substrate current (:9835-9877) */
   tmp = PP(alpha0) + PP(alpha1) * Leff;
   if ((tmp <= 0.0) || (PP(beta0) <= 0.0)) {
@@ -1822,6 +1860,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   gbgs = Gbg;
   gbds = Gbd;
 
+  reload_params(dfb);
   /* drain current with SCBE (:9880-9902) */
   T9 = diffVds / VASCBE;
   T0 = 1.0 + T9;
@@ -1837,6 +1876,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   Gmb = (Ids * (dVdseff_dVb + dVdseff_dVg * dVgsteff_dVb) + Vdseff_ * tmp2) * dVbseff_dVb;
   cdrain = Ids * Vdseff_;
 
+  reload_params(dfb);
   /* source-end velocity limit (:9905-9945) */
   if ((C.vtlGiven) && (C.vtl > 0.0)) {
     T12 = 1.0 / Leff / CoxeffWovL;
@@ -1878,14 +1918,17 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   gmbs = Gmb;
   IdovVds = Ids;
   if (IdovVds <= 1.0e-9) IdovVds = 1.0e-9;
+  reload_params(dfb);
   /* rgateMod <= 1 and no NQS: no gate resistance current conductances */
   gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+  reload_params(dfb);
   /* rdsMod = 0 (:10063-10069) */
   gstot = gstotd = gstotg = 0.0;
   gstots = gstotb = 0.0;
   gdtot = gdtotd = gdtotg = 0.0;
   gdtots = gdtotb = 0.0;
 
+  reload_params(dfb);
   /* GIDL (:10074-10126) */
   T0 = 3.0 * toxe;
   T1 = (vds - vgs_eff - PP(egidl)) / T0;
@@ -1919,6 +1962,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   ggidld = Ggidld;
   ggidlg = Ggidlg;
   ggidlb = Ggidlb;
+  reload_params(dfb);
   /* GISL (:10132-10179) */
   T1 = (-vds - vgd_eff - PP(egisl)) / T0;
   if ((PP(agisl) <= 0.0) || (PP(bgisl) <= 0.0) || (T1 <= 0.0) || (PP(cgisl) <= 0.0) || (vbs > 0.0)) {
@@ -1951,6 +1995,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   ggisls = Ggisls;
   ggislg = Ggislg;
   ggislb = Ggislb;
+  reload_params(dfb);
   /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
   Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
   Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
@@ -2001,6 +2046,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   gIgcds = -(gIgcdg + gIgcdd + gIgcdb);
   cd = cdrain;
 
+  reload_params(dfb);
   /* noise charge (:10674-10703): rewrites the Abulk and Vdseff members */
   if (C.tnoiMod == 0) {
     Abulk = Abulk0 * PP(abulkCVfactor);
@@ -2023,6 +2069,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     qinv = Coxeff * PP(weffCV) * nf * PP(leffCV) * (Vgsteff - 0.5 * T0 + Abulk * T3);
   }
 
+  reload_params(dfb);
   /* intrinsic charges, capMod = 2 (:10707-11611) */
   if ((C.xpart < 0) || (!ChargeComputationNeeded)) {
     qgate = qdrn = qsrc = qbulk = 0.0;
@@ -2339,6 +2386,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   cbbb = -cbgb - cbdb - cbsb;
   csbb = -cgbb - cdbb - cbbb;
 
+  reload_params(dfb);
   /* junction charges (:11662-11810) */
   if (ChargeComputationNeeded) {
     czbd = C.DunitAreaTempJctCap * IV(Adeff);
@@ -2423,6 +2471,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     }
   }
 
+  reload_params(dfb);
   /* overlap capacitances, capMod != 0 (:11812-11883) */
   vgdx = vgd;
   vgsx = vgs;
@@ -2815,13 +2864,28 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
 }
 
 // The stored-mode kernel is held to two waves per SIMD: the eval's peak
-// (about 135 live doubles in the DC model) then spills about 400 bytes per
-// lane to scratch, and the second wave hides more memory latency than the
-// spills cost (update_state 7% faster on 2^20 instances).  The fused kernel
-// serves small latency-bound launches and keeps every value in registers.
-template <bool FUSED>
+// (about 135 live doubles in the DC model) then spills to scratch, and the
+// second wave hides more memory latency than the spills cost (update_state 7%
+// faster on 2^20 instances).  The fused kernel serves small latency-bound
+// launches and keeps every value in registers.
+//
+// The stored kernel with LDSCARD stages all of the group's model cards in
+// LDS first (groups of at most kLdsCards models), so the eval's card reads
+// are LDS reads instead of L1/L2 hits.
+constexpr int kLdsCards = 16;
+
+template <bool FUSED, bool LDSCARD>
 __global__ void __launch_bounds__(256, FUSED ? 1 : 2) k_update(KArgs a) {
-  eval<FUSED>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (LDSCARD) {
+    __shared__ double s_card[kLdsCards * (sizeof(DevCard) / sizeof(double))];
+    const int total = a.n_models * a.mstride;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) s_card[k] = a.mcard[k];
+    __syncthreads();
+    eval<FUSED, true>(a, i, s_card);
+  } else {
+    eval<FUSED, false>(a, i, nullptr);
+  }
   if (FUSED) fused_writeout(a);
 }
 
@@ -2836,7 +2900,12 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
     return fail(XA_ERR_ARG, "bsim4 update: flagSol required with inputOPFlag");
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
-  return launch_model_kernel(g, a, fused, (const void*)k_update<false>, (const void*)k_update<true>);
+  static_assert(sizeof(DevCard) % sizeof(double) == 0, "cards are whole doubles");
+  const bool lds_cards = a.n_models <= kLdsCards && a.mstride == (int)(sizeof(DevCard) / sizeof(double));
+  return launch_model_kernel(g, a, fused,
+                             lds_cards ? (const void*)k_update<false, true>
+                                       : (const void*)k_update<false, false>,
+                             (const void*)k_update<true, false>);
 }
 
 // Algorithmic bytes per instance: the SoA columns and card index read once,

@@ -18,10 +18,11 @@
 // The matrix is banded once the circuit's unknowns are taken in the caller's
 // elimination order (pos[g]: position of unknown g within its circuit; for a
 // ring, a reverse Cuthill-McKee order gives two bands either side).  One
-// wavefront factors one circuit in LDS by banded LU with partial pivoting in
-// LAPACK dgbtf2's order of operations (band storage with kl extra rows for
-// the pivoting fill), forward-eliminates the right-hand side alongside,
-// back-substitutes, updates x and reports the circuit's largest update.
+// wavefront per circuit assembles the band in LDS and factors it by banded LU
+// with partial pivoting in LAPACK dgbtf2's order of operations (kl extra
+// rows for the pivoting fill), forward-eliminates the right-hand side
+// alongside, back-substitutes, updates x and reports the circuit's largest
+// update.
 #include <cmath>
 #include "xa_internal.h"
 

@@ -64,6 +64,14 @@ struct Plan {
   double* d_partial = nullptr;     // block sums
   int64_t n_rails = 0, n_chunks = 0;
   double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
+  // Pattern plan: every instance stamps its own block of S consecutive
+  // targets (instance-major from t0) and entry e of every instance goes to
+  // local slot pslot[e] (-1: none).  The scatter then needs no segment or
+  // source arrays.
+  bool pattern = false;
+  int pS = 0;
+  int pblock = 0;                  // instances per block of k_scatter_pattern
+  int32_t* d_pslot = nullptr;      // E local slots
   // Block plan of the fused load: the same targets grouped by the block of
   // instances that writes them (defined when every target's contributions
   // come from one block).  Sources are LDS slots (loff + e) * B + lane.
@@ -155,6 +163,10 @@ struct Group {
   Plan plan[NK];
   bool fused_ok = false;     // the fused load's block plans exist
   size_t fused_lds = 0;      // its LDS bytes per block
+  int fused_rows = 0;        // LDS rows of the fused load (stamp rows of live kinds)
+  int fused_B = 0;           // instances per block the block plans were built for
+  int split_block = 256;     // threads per block of the stored-mode update kernel
+  int route = 0;             // xa_load_all: 0 size/model policy, 1 four calls, 2 fused
 
   // per-phase HIP event pairs recorded while timing is on
   bool timing = false;

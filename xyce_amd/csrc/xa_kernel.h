@@ -13,6 +13,7 @@ struct KArgs {
   const int32_t* __restrict__ ifield;  // [field][inst]
   const double* __restrict__ mcard;    // [model][mstride]
   int mstride;
+  int n_models;                        // cards in mcard
   const int32_t* __restrict__ nodes;   // [local node][inst]
   const int32_t* __restrict__ sto;     // [local sto][inst]
   const int32_t* __restrict__ sta;     // [local sta][inst]

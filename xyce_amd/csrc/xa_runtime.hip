@@ -224,6 +224,47 @@ k_rail_sum(const int32_t* __restrict__ rail_t, const int32_t* __restrict__ rail_
   }
 }
 
+// Pattern plan (see Plan::pattern): one thread per instance.  A block stages
+// its instances' target blocks (contiguous in the output) in LDS, each
+// instance's row padded to S + 1 doubles against bank conflicts; every
+// thread adds its instance's entries in entry order (the plan order of a
+// private target) from the entry-major value rows, four rows' loads issued
+// before their adds; the block writes its targets back.  Same bits as the
+// segmented scatter, with coalesced reads and writes and no index arrays.
+__global__ void __launch_bounds__(256)
+k_scatter_pattern(int64_t n, int S, int E, int32_t t0, const int32_t* __restrict__ pslot,
+                  const double* __restrict__ vals, double* __restrict__ out) {
+  extern __shared__ double blk[];   // blockDim.x * (S + 1)
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  const int m = (int)((n - i0) < (int64_t)blockDim.x ? (n - i0) : (int64_t)blockDim.x);
+  const int W = S + 1;
+  double* o = out + t0 + i0 * S;
+  for (int k = threadIdx.x; k < m * S; k += blockDim.x) blk[(k / S) * W + k % S] = o[k];
+  __syncthreads();
+  const int il = threadIdx.x;
+  if (il < m) {
+    const double* vi = vals + i0 + il;
+    double* row = blk + il * W;
+    int e = 0;
+    for (; e + 4 <= E; e += 4) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = vi[(int64_t)(e + u) * n];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int sl = pslot[e + u];
+        if (sl >= 0) row[sl] += v[u];
+      }
+    }
+    for (; e < E; ++e) {
+      const int sl = pslot[e];
+      if (sl >= 0) row[sl] += vi[(int64_t)e * n];
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < m * S; k += blockDim.x) o[k] = blk[(k / S) * W + k % S];
+}
+
 template <typename I, bool CONTIG>
 __global__ void __launch_bounds__(256)
 k_scatter_assign(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
@@ -242,7 +283,12 @@ static void launch_scatter(const Plan& p, bool assign, hipStream_t st, double* o
   if (assign)
     hipLaunchKernelGGL((k_scatter_assign<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
-  else {
+  else if (p.pattern) {
+    const size_t lds = sizeof(double) * (size_t)p.pblock * (size_t)(p.pS + 1);
+    hipLaunchKernelGGL(k_scatter_pattern, dim3(grid_for(p.n_targets / p.pS, p.pblock)),
+                       dim3(p.pblock), lds, st, p.n_targets / p.pS, p.pS, p.E, p.t0, p.d_pslot,
+                       p.d_vals, out);
+  } else {
     hipLaunchKernelGGL((k_scatter_add<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
     if (p.n_long > 0)
@@ -299,6 +345,7 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   a.ifield = g.d_ifield;
   a.mcard = g.d_mcard;
   a.mstride = g.mcard_stride;
+  a.n_models = (int)g.models.size();
   a.nodes = g.d_nodes;
   a.sto = g.d_sto;
   a.sta = g.d_sta;
@@ -335,10 +382,11 @@ int launch_model_kernel(Group& g, const KArgs& a, bool fused, const void* k_stor
                         const void* k_fused) {
   void* args[] = {(void*)&a};
   if (!fused) {
-    XA_HIP(hipLaunchKernel(k_stored, dim3(grid_for(g.n)), dim3(kBlock), args, 0, g.ctx->stream));
+    XA_HIP(hipLaunchKernel(k_stored, dim3(grid_for(g.n, g.split_block)), dim3(g.split_block), args,
+                           0, g.ctx->stream));
     return XA_OK;
   }
-  const int B = g.impl->fused_block;
+  const int B = g.fused_B;
   if (g.fused_lds > 65536)
     XA_HIP(hipFuncSetAttribute(k_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)g.fused_lds));
@@ -485,6 +533,7 @@ static void free_plan(Plan& p) {
   (void)hipFree(p.d_long); (void)hipFree(p.d_rail_t); (void)hipFree(p.d_rail_c);
   (void)hipFree(p.d_chunk_j0); (void)hipFree(p.d_chunk_j1); (void)hipFree(p.d_partial);
   (void)hipFree(p.d_vals); (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt);
+  (void)hipFree(p.d_pslot);
   (void)hipFree(p.d_bseg); (void)hipFree(p.d_bsrc);
   p = Plan();
 }
@@ -720,9 +769,18 @@ int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
 // plan's order, from LDS.  Targets are grouped by block (ascending within a
 // block); a source becomes the LDS slot (loff + e) * B + lane.  loff is set
 // by the caller before (the kind's first LDS row).
+static void free_block_plan(Plan& p) {
+  (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt); (void)hipFree(p.d_bseg);
+  (void)hipFree(p.d_bsrc);
+  p.d_bblk = p.d_btgt = p.d_bseg = nullptr;
+  p.d_bsrc = nullptr;
+  p.bt0 = 0;
+}
+
 static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& target,
-                            const std::vector<int64_t>& seg, const std::vector<int64_t>& src) {
-  const int B = g.impl->fused_block;
+                            const std::vector<int64_t>& seg, const std::vector<int64_t>& src,
+                            int B) {
+  free_block_plan(p);
   const int64_t n = g.n, nt = (int64_t)target.size();
   p.block_private = B > 0;
   if (!p.block_private || nt == 0) return XA_OK;
@@ -761,6 +819,49 @@ static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& targe
   if (!contig && (rc = upload(&p.d_btgt, btgt)) != XA_OK) return rc;
   if ((rc = upload(&p.d_bseg, bseg)) != XA_OK) return rc;
   return upload(&p.d_bsrc, bsrc);
+}
+
+// A pattern plan applies when the n instances own consecutive blocks of S
+// targets in instance order (contiguous from the first target) and every
+// instance's slot s is fed by the same entries.  Sources are e * n + i.
+static int detect_pattern(const Group& g, Plan& p, const std::vector<int32_t>& target,
+                          const std::vector<int64_t>& seg, const std::vector<int64_t>& src) {
+  p.pattern = false;
+  const int64_t n = g.n, nt = (int64_t)target.size();
+  if (n == 0 || nt == 0 || nt % n != 0 || nt / n > 127) return XA_OK;
+  const int S = (int)(nt / n), E = p.E;
+  for (int64_t t = 0; t < nt; ++t)
+    if (target[t] != target[0] + (int32_t)t) return XA_OK;
+  std::vector<int32_t> pslot((size_t)E, -1);
+  for (int s = 0; s < S; ++s)   // instance 0 defines the pattern
+    for (int64_t j = seg[s]; j < seg[s + 1]; ++j) {
+      if (src[j] % n != 0) return XA_OK;
+      pslot[src[j] / n] = s;
+    }
+  // every instance: slot s holds exactly the entries with pslot == s, in
+  // entry order (the plan's order within a private target)
+  std::vector<int32_t> cnt((size_t)S, 0);
+  for (int e = 0; e < E; ++e) if (pslot[e] >= 0) cnt[pslot[e]]++;
+  for (int64_t i = 0; i < n; ++i)
+    for (int s = 0; s < S; ++s) {
+      const int64_t t = i * S + s;
+      if (seg[t + 1] - seg[t] != cnt[s]) return XA_OK;
+      int64_t prev = -1;
+      for (int64_t j = seg[t]; j < seg[t + 1]; ++j) {
+        const int64_t e = src[j] / n;
+        if (src[j] % n != i || pslot[e] != s || e <= prev) return XA_OK;
+        prev = e;
+      }
+    }
+  int B = 256;   // LDS of the staged blocks within 64 KB
+  while (B > 64 && (size_t)B * (S + 1) * sizeof(double) > 65536) B /= 2;
+  if ((size_t)B * (S + 1) * sizeof(double) > 65536) return XA_OK;
+  int rc;
+  if ((rc = upload(&p.d_pslot, pslot)) != XA_OK) return rc;
+  p.pattern = true;
+  p.pS = S;
+  p.pblock = B;
+  return XA_OK;
 }
 
 // Counting sort of (target, instance, entry) triples by target; stable, so
@@ -809,6 +910,7 @@ static int build_plan(Group& g, int k) {
   p.n_targets = (int64_t)target.size();
   p.n_src = acc;
   int rc;
+  if ((rc = detect_pattern(g, p, target, seg, src)) != XA_OK) return rc;
   std::vector<int32_t> long_t, rail_t, rail_c(1, 0);
   std::vector<int64_t> chunk_j0, chunk_j1;
   for (int64_t t = 0; t < p.n_targets; ++t) {
@@ -870,10 +972,23 @@ static int build_plan(Group& g, int k) {
     p.d_chunk_j0 = c0;
     p.d_chunk_j1 = c1;
   }
-  if ((rc = build_block_plan(g, p, target, seg, src)) != XA_OK) return rc;
+  if ((rc = build_block_plan(g, p, target, seg, src, g.impl->fused_block)) != XA_OK) return rc;
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
   XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
   return XA_OK;
+}
+
+// The fused load's geometry for blocks of B instances: its LDS and whether
+// every kind's block plan (built for B) exists.
+static bool fused_fits(const Group& g, int B) {
+  const size_t lds = (size_t)g.fused_rows * (size_t)B * sizeof(double);
+  return B > 0 && B <= kBlock && lds <= kMaxLds && (int64_t)g.fused_rows * B <= 65536;
+}
+static void set_fused_block(Group& g, int B) {
+  g.fused_B = B;
+  g.fused_lds = (size_t)g.fused_rows * (size_t)std::max(B, 0) * sizeof(double);
+  g.fused_ok = fused_fits(g, B);
+  for (int k = 0; k < NK; ++k) g.fused_ok = g.fused_ok && g.plan[k].block_private;
 }
 
 int xa_group_finalize(xa_group gh) {
@@ -901,10 +1016,8 @@ int xa_group_finalize(xa_group gh) {
   }
   for (int k = 0; k < NK; ++k)
     if ((rc = build_plan(*g, k)) != XA_OK) return rc;
-  const int B = g->impl->fused_block;
-  g->fused_lds = (size_t)rows * (size_t)B * sizeof(double);
-  g->fused_ok = B > 0 && g->fused_lds <= kMaxLds && (int64_t)rows * B <= 65536;
-  for (int k = 0; k < NK; ++k) g->fused_ok = g->fused_ok && g->plan[k].block_private;
+  g->fused_rows = rows;
+  set_fused_block(*g, g->impl->fused_block);
   g->phase = PH_FINAL;
   return XA_OK;
 }
@@ -989,9 +1102,10 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   // the fused kernel needs every target block-private and no secondary pass;
   // it is taken for launch-bound sizes and, at scale, where the model runs
   // faster fused; otherwise the four calls run in sequence (same results)
-  const int64_t blocks = (g->n + g->impl->fused_block - 1) / std::max(g->impl->fused_block, 1);
+  const int64_t blocks = (g->n + g->fused_B - 1) / std::max(g->fused_B, 1);
+  const bool fused_policy = g->impl->fused_at_scale || blocks <= kSmallLaunchBlocks;
   if (g->fused_ok && !g->impl->launch_secondary &&
-      (g->impl->fused_at_scale || blocks <= kSmallLaunchBlocks)) {
+      (g->route == 2 || (g->route == 0 && fused_policy))) {
     tic(g, 0);
     rc = g->impl->launch_update(*g, *ss, *v, true);
     toc(g, 0);
@@ -1001,6 +1115,159 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   if ((rc = xa_update_secondary_state(gh, ss, v)) != XA_OK) return rc;
   if ((rc = xa_load_vectors(gh, ss, v)) != XA_OK) return rc;
   return xa_load_matrices(gh, ss, v);
+}
+
+// ---------------------------------------------------------------------------
+// autotuning of the block shapes
+// ---------------------------------------------------------------------------
+// Host copy of one kind's global plan: targets, segment starts, sources.
+static int download_plan(const Plan& p, std::vector<int32_t>& target, std::vector<int64_t>& seg,
+                         std::vector<int64_t>& src) {
+  const int64_t nt = p.n_targets;
+  target.assign((size_t)nt, 0);
+  seg.assign((size_t)nt + 1, 0);
+  src.assign((size_t)p.n_src, 0);
+  if (nt == 0) return XA_OK;
+  if (p.d_target)
+    XA_HIP(hipMemcpy(target.data(), p.d_target, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
+  else
+    for (int64_t t = 0; t < nt; ++t) target[t] = p.t0 + (int32_t)t;
+  if (p.idx32) {
+    std::vector<int32_t> s32((size_t)nt + 1), r32((size_t)p.n_src);
+    XA_HIP(hipMemcpy(s32.data(), p.d_seg, sizeof(int32_t) * (nt + 1), hipMemcpyDeviceToHost));
+    if (p.n_src)
+      XA_HIP(hipMemcpy(r32.data(), p.d_src, sizeof(int32_t) * p.n_src, hipMemcpyDeviceToHost));
+    seg.assign(s32.begin(), s32.end());
+    src.assign(r32.begin(), r32.end());
+  } else {
+    XA_HIP(hipMemcpy(seg.data(), p.d_seg, sizeof(int64_t) * (nt + 1), hipMemcpyDeviceToHost));
+    if (p.n_src)
+      XA_HIP(hipMemcpy(src.data(), p.d_src, sizeof(int64_t) * p.n_src, hipMemcpyDeviceToHost));
+  }
+  return XA_OK;
+}
+
+// Block plans of every kind for blocks of B instances (one kind's host copy
+// at a time), then the fused geometry for B.
+static int build_block_plans(Group& g, int B) {
+  std::vector<int32_t> target;
+  std::vector<int64_t> seg, src;
+  int rc;
+  for (int k = 0; k < NK; ++k) {
+    Plan& p = g.plan[k];
+    if (p.E == 0) continue;
+    if ((rc = download_plan(p, target, seg, src)) != XA_OK) return rc;
+    if ((rc = build_block_plan(g, p, target, seg, src, B)) != XA_OK) return rc;
+  }
+  set_fused_block(g, B);
+  return XA_OK;
+}
+
+// One load pass as xa_load_all would run it on the fused or the split path.
+static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
+  int rc;
+  if (fused) return g.impl->launch_update(g, ss, v, true);
+  if ((rc = g.impl->launch_update(g, ss, v, false)) != XA_OK) return rc;
+  if (g.impl->launch_secondary && (rc = g.impl->launch_secondary(g, ss, v)) != XA_OK) return rc;
+  for (int k : {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ, K_DFDX, K_DQDX})
+    if ((rc = run_scatter(g, ss, v, k)) != XA_OK) return rc;
+  return XA_OK;
+}
+
+// Mean milliseconds of one pass over `reps` passes after one untimed pass.
+static int time_passes(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused,
+                       int reps, float* ms) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  XA_HIP(hipEventCreate(&e0));
+  XA_HIP(hipEventCreate(&e1));
+  int rc = load_pass(g, ss, v, fused);
+  if (rc == XA_OK) (void)hipEventRecord(e0, g.ctx->stream);
+  for (int r = 0; r < reps && rc == XA_OK; ++r) rc = load_pass(g, ss, v, fused);
+  if (rc == XA_OK) {
+    (void)hipEventRecord(e1, g.ctx->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (e != hipSuccess) rc = hip_check(e, "autotune timing");
+    else *ms /= (float)reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// The search: the four-call path with update-kernel blocks of 64, 128 and
+// 256 threads, then the fused load with blocks of 64, 128 and 256 instances
+// where it exists; the fastest wins (the split block is kept for the four
+// calls in any case).
+static int tune(Group& g, const xa_solver_state& ss, const xa_vectors& v, int reps, int* route,
+                int* block) {
+  const int candidates[] = {64, 128, 256};
+  int rc;
+  float best_split = 0.0f;
+  int split_B = g.split_block;
+  for (int c = 0; c < 3; ++c) {
+    g.split_block = candidates[c];
+    float ms = 0.0f;
+    if ((rc = time_passes(g, ss, v, false, reps, &ms)) != XA_OK) return rc;
+    if (c == 0 || ms < best_split) { best_split = ms; split_B = candidates[c]; }
+  }
+  g.split_block = split_B;
+  float best = best_split;
+  int best_route = 1, best_B = split_B;
+  if (!g.impl->launch_secondary && g.impl->fused_block > 0) {
+    int built = g.fused_B;
+    for (int B : candidates) {
+      if (!fused_fits(g, B)) continue;
+      if ((rc = build_block_plans(g, B)) != XA_OK) return rc;
+      built = B;
+      if (!g.fused_ok) continue;
+      float ms = 0.0f;
+      if ((rc = time_passes(g, ss, v, true, reps, &ms)) != XA_OK) return rc;
+      if (ms < best) { best = ms; best_route = 2; best_B = B; }
+    }
+    const int keep = best_route == 2 ? best_B : g.impl->fused_block;
+    if (built != keep && (rc = build_block_plans(g, keep)) != XA_OK) return rc;
+  }
+  g.route = best_route;
+  if (route) *route = best_route;
+  if (block) *block = best_B;
+  return XA_OK;
+}
+
+int xa_group_autotune(xa_group gh, const xa_solver_state* ss, const xa_vectors* v, int reps,
+                      int* route, int* block) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0) {
+    if (route) *route = 1;
+    if (block) *block = g->split_block;
+    return XA_OK;
+  }
+  // the tuning loads must not advance the instances' own state (the values
+  // the reference keeps in its Instance objects) or the failure count
+  const size_t state_bytes = sizeof(double) * (size_t)g->n_state * (size_t)g->n;
+  double* saved = nullptr;
+  int32_t fail0 = 0;
+  if (state_bytes) XA_HIP(hipMalloc((void**)&saved, state_bytes));
+  hipError_t e = hipSuccess;
+  if (state_bytes)
+    e = hipMemcpyAsync(saved, g->d_state, state_bytes, hipMemcpyDeviceToDevice, g->ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&fail0, g->d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+  rc = e == hipSuccess ? tune(*g, *ss, *v, std::max(reps, 1), route, block)
+                       : hip_check(e, "autotune: saving the instance state");
+  e = hipSuccess;
+  if (state_bytes)
+    e = hipMemcpyAsync(g->d_state, saved, state_bytes, hipMemcpyDeviceToDevice, g->ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(g->d_fail, &fail0, sizeof(int32_t), hipMemcpyHostToDevice, g->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+  (void)hipFree(saved);
+  if (rc == XA_OK && e != hipSuccess) rc = hip_check(e, "autotune: restoring the instance state");
+  return rc;
 }
 
 int xa_group_failures(xa_group gh, int64_t* n_failed) {

@@ -4,8 +4,8 @@ Thousands of ring oscillators (odd stage counts, per-ring VTH0 and TOXE
 samples) run as one circuit batch: every ring's BSIM4 instances are one
 group, loaded together by the device loads, and every ring's Newton system
 is formed and solved on the device by xa_batch_newton_step (one wavefront
-per ring, banded LU in the ring's reverse Cuthill-McKee order).  The host only launches and checks convergence, once per Newton
-iteration.  Each ring is the circuit of xyce_amd.ring.Ring with its sample's
+per ring, banded LU in the ring's reverse Cuthill-McKee order).  The host
+only launches and checks convergence, once per Newton iteration.  Each ring is the circuit of xyce_amd.ring.Ring with its sample's
 TOXE and |VTH0| scale, so the plain drop-in (that module, one ring at a
 time) runs exactly the same circuits.
 
