@@ -228,6 +228,13 @@ int xa_load_all(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
 int xa_group_autotune(xa_group g, const xa_solver_state* ss, const xa_vectors* v, int reps,
                       int* route, int* block);
 
+/* Shape of the scatter plan of one output kind (XA_KIND_*), for diagnostics
+ * and tests (no reference counterpart): targets, contributions, and the
+ * targets per instance of a pattern plan (0 when the kind uses the general
+ * segmented plan). */
+int xa_group_plan_info(xa_group g, int kind, int64_t* n_targets, int64_t* n_sources,
+                       int* pattern_slots);
+
 /* Device-side status of the last update: number of instances whose
  * evaluation returned false (0 = all succeeded). Synchronizes. */
 int xa_group_failures(xa_group g, int64_t* n_failed);

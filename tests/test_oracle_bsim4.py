@@ -162,3 +162,45 @@ def test_drain_source_symmetry():
     scale = np.maximum(np.abs(r1["f"][d]), np.abs(r1["f"][s])) + 1e-18
     np.testing.assert_array_less(np.abs(r2["f"][s] - r1["f"][d]) / scale, 1e-9)
     np.testing.assert_array_less(np.abs(r2["f"][d] - r1["f"][s]) / scale, 1e-9)
+
+
+@pytest.mark.parametrize("bin_unit", [1, 0])
+def test_binning_closed_form(bin_unit):
+    """Size-dependent parameter binning (the reference's size-dependent block
+    in updateTemperature, N_DEV_MOSFET_B4.C:5640-7734): P = P0 + PL/Leff +
+    PW/Weff + PP/(Leff Weff), lengths in microns with BINUNIT = 1; here VTH0
+    and its L/W/P coefficients, plus an instance DELVTO.  With LINT = WINT =
+    LL = WW = XL = XW = 0 and NF = 1, Leff = L and Weff = W."""
+    n = 32
+    rng = splitmix64(31)
+    w = 90e-9 * np.exp(0.3 * rng.standard_normal(n))
+    l = 45e-9 * np.exp(0.3 * rng.standard_normal(n))
+    dv = rng.uniform(-0.05, 0.05, n)
+    s = 1e-6 if bin_unit == 1 else 1.0
+    card = {"TYPE": 1.0, "VTH0": 0.4, "LVTH0": 0.01 / s, "WVTH0": 0.02 / s,
+            "PVTH0": 0.003 / (s * s), "BINUNIT": float(bin_unit)}
+    spec = {"models": [(54, card)], "n": n, "model_index": np.zeros(n, np.int32),
+            "inst_params": {"W": w, "L": l, "DELVTO": dv}}
+    p = _pair(spec)
+    np.testing.assert_allclose(p.orc.read("leff"), l, rtol=1e-15)
+    np.testing.assert_allclose(p.orc.read("weff"), w, rtol=1e-15)
+    want = 0.4 + (0.01 / s) * (s / l) + (0.02 / s) * (s / w) + (0.003 / (s * s)) * (s * s / (l * w)) + dv
+    np.testing.assert_allclose(p.orc.read("vth0"), want, rtol=1e-14)
+
+
+@pytest.mark.parametrize("u0,up", [(0.03, 0.0), (400.0, 0.1)])
+def test_mobility_temperature_law(u0, up):
+    """tempMod 0 low-field mobility: u0temp = U0 (1 - UP exp(-Leff/LP))
+    (T/TNOM)^UTE, U0 above 1 taken in cm^2/Vs (divided by 1e4), at an
+    instance TEMP of 85 C against TNOM = 27 C."""
+    n = 16
+    rng = splitmix64(32)
+    w = 90e-9 * np.exp(0.2 * rng.standard_normal(n))
+    l = 45e-9 * np.exp(0.2 * rng.standard_normal(n))
+    card = {"TYPE": 1.0, "U0": u0, "UTE": -1.3, "UP": up, "LP": 1e-8, "TNOM": 27.0}
+    spec = {"models": [(54, card)], "n": n, "model_index": np.zeros(n, np.int32),
+            "inst_params": {"W": w, "L": l, "TEMP": np.full(n, 85.0)}}
+    p = _pair(spec)
+    u = u0 / 1e4 if u0 > 1.0 else u0
+    want = u * (1.0 - up * np.exp(-l / 1e-8)) * ((85.0 + 273.15) / (27.0 + 273.15)) ** -1.3
+    np.testing.assert_allclose(p.orc.read("u0temp"), want, rtol=1e-13)

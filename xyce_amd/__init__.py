@@ -197,6 +197,15 @@ class Master:
     def loadAll(self, ss, vectors):
         check(lib().xa_load_all(self.h, C.byref(ss), C.byref(vectors)), "loadAll")
 
+    def plan_info(self, kind):
+        """(targets, contributions, pattern slots per instance or 0) of the
+        scatter plan of output kind `kind` (0 F, 1 Q, 2 dFdxdVp, 3 dQdxdVp,
+        4 dFdx, 5 dQdx)."""
+        nt, ns, ps = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().xa_group_plan_info(self.h, int(kind), C.byref(nt), C.byref(ns), C.byref(ps)),
+              "plan_info")
+        return nt.value, ns.value, ps.value
+
     def autotune(self, ss, vectors, reps=5):
         """Time the block shapes on these vectors and keep the fastest
         (xa_group_autotune); returns ("split" | "fused", block).  The tuning

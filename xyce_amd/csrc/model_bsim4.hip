@@ -487,15 +487,15 @@ __device__ __forceinline__ void reload_params(const double*& dfb) {
   asm volatile("" : "+s"(dfb));
 }
 
-// LDSCARD: the model cards are staged in LDS by the kernel (s_card, the
-// group's cards in model order); otherwise they are read from global memory.
+// LDSCARD: the block's model cards are staged in LDS by the kernel (s_card,
+// in the order of the block's card table; card_slot gives the instance's);
+// otherwise they are read from global memory.
 template <bool FUSED, bool LDSCARD>
 __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
-  const int32_t model_ix = ifld(a, F_MODEL, i);
   const DevCard& C =
-      LDSCARD ? *reinterpret_cast<const DevCard*>(s_card + (int64_t)model_ix * a.mstride)
-              : *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)model_ix * a.mstride);
+      LDSCARD ? *reinterpret_cast<const DevCard*>(s_card + (int)a.card_slot[i] * a.mstride)
+              : *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
   const double* dfb = a.dfield;
   const xa_solver_state& ss = a.ss;
   const xa_device_options& opt = a.opt;
@@ -2869,18 +2869,19 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
 // faster on 2^20 instances).  The fused kernel serves small latency-bound
 // launches and keeps every value in registers.
 //
-// The stored kernel with LDSCARD stages all of the group's model cards in
-// LDS first (groups of at most kLdsCards models), so the eval's card reads
-// are LDS reads instead of L1/L2 hits.
-constexpr int kLdsCards = 16;
-
+// The stored kernel with LDSCARD first stages the cards of its block's table
+// (the distinct models of its instances, at most kBlockCards) in LDS, so the
+// eval's card reads are LDS reads instead of L1/L2 hits.
 template <bool FUSED, bool LDSCARD>
 __global__ void __launch_bounds__(256, FUSED ? 1 : 2) k_update(KArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (LDSCARD) {
-    __shared__ double s_card[kLdsCards * (sizeof(DevCard) / sizeof(double))];
-    const int total = a.n_models * a.mstride;
-    for (int k = threadIdx.x; k < total; k += blockDim.x) s_card[k] = a.mcard[k];
+    constexpr int kStride = (int)(sizeof(DevCard) / sizeof(double));
+    __shared__ double s_card[kBlockCards * kStride];
+    const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
+    const int used = tab[0];   // 1 <= used <= kBlockCards
+    for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+      s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
     __syncthreads();
     eval<FUSED, true>(a, i, s_card);
   } else {
@@ -2901,7 +2902,10 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
   KArgs a;
   fill_kargs(g, ss, v, fused, a);
   static_assert(sizeof(DevCard) % sizeof(double) == 0, "cards are whole doubles");
-  const bool lds_cards = a.n_models <= kLdsCards && a.mstride == (int)(sizeof(DevCard) / sizeof(double));
+  // the tables exist for the current update block when no block uses more
+  // than kBlockCards models (built at finalize and by the autotuner)
+  const bool lds_cards = g.card_tables && g.card_block == g.split_block &&
+                         a.mstride == (int)(sizeof(DevCard) / sizeof(double));
   return launch_model_kernel(g, a, fused,
                              lds_cards ? (const void*)k_update<false, true>
                                        : (const void*)k_update<false, false>,

@@ -64,14 +64,16 @@ struct Plan {
   double* d_partial = nullptr;     // block sums
   int64_t n_rails = 0, n_chunks = 0;
   double* d_vals = nullptr;        // E * n, entry-major: slot e of instance i at e*n + i
-  // Pattern plan: every instance stamps its own block of S consecutive
-  // targets (instance-major from t0) and entry e of every instance goes to
-  // local slot pslot[e] (-1: none).  The scatter then needs no segment or
-  // source arrays.
+  // Pattern plan: instance i feeds S targets at t0 + i * pP + offset (the
+  // same offsets for every instance, the last at plast < pP), and entry e of
+  // every instance goes to offset pslot[e] (-1: none).  The scatter then
+  // needs no segment or source arrays.
   bool pattern = false;
   int pS = 0;
+  int pP = 0;
+  int plast = 0;
   int pblock = 0;                  // instances per block of k_scatter_pattern
-  int32_t* d_pslot = nullptr;      // E local slots
+  int32_t* d_pslot = nullptr;      // E offsets
   // Block plan of the fused load: the same targets grouped by the block of
   // instances that writes them (defined when every target's contributions
   // come from one block).  Sources are LDS slots (loff + e) * B + lane.
@@ -167,6 +169,14 @@ struct Group {
   int fused_B = 0;           // instances per block the block plans were built for
   int split_block = 256;     // threads per block of the stored-mode update kernel
   int route = 0;             // xa_load_all: 0 size/model policy, 1 four calls, 2 fused
+  // Per-block card tables of the stored-mode update: for every block of
+  // card_block instances, the number of distinct models it uses and the
+  // models (at most kBlockCards; kBlockCards + 1 ints per block), and each
+  // instance's slot in its block's table.
+  int card_block = 0;        // block size the tables were built for (0: none)
+  bool card_tables = false;  // every block uses at most kBlockCards models
+  int32_t* d_blk_cards = nullptr;
+  uint8_t* d_card_slot = nullptr;
 
   // per-phase HIP event pairs recorded while timing is on
   bool timing = false;
@@ -187,6 +197,8 @@ int hip_check(hipError_t e, const char* what);
 
 // block size used by the per-instance kernels
 constexpr int kBlock = 256;
+// distinct model cards a block of the stored-mode update stages in LDS
+constexpr int kBlockCards = 16;
 // LDS of one workgroup on gfx950
 constexpr size_t kMaxLds = 160 * 1024;
 // A load of at most this many blocks is launch-latency bound: xa_load_all

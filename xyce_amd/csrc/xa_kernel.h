@@ -14,6 +14,8 @@ struct KArgs {
   const double* __restrict__ mcard;    // [model][mstride]
   int mstride;
   int n_models;                        // cards in mcard
+  const int32_t* blk_cards;            // [block][1 + kBlockCards]: count, models to stage
+  const uint8_t* card_slot;            // [inst] slot of its model in its block's table
   const int32_t* __restrict__ nodes;   // [local node][inst]
   const int32_t* __restrict__ sto;     // [local sto][inst]
   const int32_t* __restrict__ sta;     // [local sta][inst]

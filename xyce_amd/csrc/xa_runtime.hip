@@ -224,22 +224,25 @@ k_rail_sum(const int32_t* __restrict__ rail_t, const int32_t* __restrict__ rail_
   }
 }
 
-// Pattern plan (see Plan::pattern): one thread per instance.  A block stages
-// its instances' target blocks (contiguous in the output) in LDS, each
-// instance's row padded to S + 1 doubles against bank conflicts; every
+// Pattern plan (see Plan::pattern): one thread per instance.  Instance i's
+// targets lie at t0 + i * P + offset; a block stages the output range from
+// its first instance's first target to its last instance's last target in
+// LDS (instance rows padded to P + 1 doubles against bank conflicts), every
 // thread adds its instance's entries in entry order (the plan order of a
 // private target) from the entry-major value rows, four rows' loads issued
-// before their adds; the block writes its targets back.  Same bits as the
+// before their adds, and the block writes the range back (slots of the range
+// that no entry feeds are written back unchanged).  Same bits as the
 // segmented scatter, with coalesced reads and writes and no index arrays.
 __global__ void __launch_bounds__(256)
-k_scatter_pattern(int64_t n, int S, int E, int32_t t0, const int32_t* __restrict__ pslot,
+k_scatter_pattern(int64_t n, int P, int last, int E, int32_t t0, const int32_t* __restrict__ poff,
                   const double* __restrict__ vals, double* __restrict__ out) {
-  extern __shared__ double blk[];   // blockDim.x * (S + 1)
+  extern __shared__ double blk[];   // blockDim.x * (P + 1)
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
   const int m = (int)((n - i0) < (int64_t)blockDim.x ? (n - i0) : (int64_t)blockDim.x);
-  const int W = S + 1;
-  double* o = out + t0 + i0 * S;
-  for (int k = threadIdx.x; k < m * S; k += blockDim.x) blk[(k / S) * W + k % S] = o[k];
+  const int W = P + 1;
+  const int span = (m - 1) * P + last + 1;
+  double* o = out + t0 + i0 * P;
+  for (int k = threadIdx.x; k < span; k += blockDim.x) blk[(k / P) * W + k % P] = o[k];
   __syncthreads();
   const int il = threadIdx.x;
   if (il < m) {
@@ -252,17 +255,17 @@ k_scatter_pattern(int64_t n, int S, int E, int32_t t0, const int32_t* __restrict
       for (int u = 0; u < 4; ++u) v[u] = vi[(int64_t)(e + u) * n];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int sl = pslot[e + u];
-        if (sl >= 0) row[sl] += v[u];
+        const int off = poff[e + u];
+        if (off >= 0) row[off] += v[u];
       }
     }
     for (; e < E; ++e) {
-      const int sl = pslot[e];
-      if (sl >= 0) row[sl] += vi[(int64_t)e * n];
+      const int off = poff[e];
+      if (off >= 0) row[off] += vi[(int64_t)e * n];
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < m * S; k += blockDim.x) o[k] = blk[(k / S) * W + k % S];
+  for (int k = threadIdx.x; k < span; k += blockDim.x) o[k] = blk[(k / P) * W + k % P];
 }
 
 template <typename I, bool CONTIG>
@@ -284,10 +287,10 @@ static void launch_scatter(const Plan& p, bool assign, hipStream_t st, double* o
     hipLaunchKernelGGL((k_scatter_assign<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
   else if (p.pattern) {
-    const size_t lds = sizeof(double) * (size_t)p.pblock * (size_t)(p.pS + 1);
-    hipLaunchKernelGGL(k_scatter_pattern, dim3(grid_for(p.n_targets / p.pS, p.pblock)),
-                       dim3(p.pblock), lds, st, p.n_targets / p.pS, p.pS, p.E, p.t0, p.d_pslot,
-                       p.d_vals, out);
+    const int64_t n = p.n_targets / p.pS;
+    const size_t lds = sizeof(double) * (size_t)p.pblock * (size_t)(p.pP + 1);
+    hipLaunchKernelGGL(k_scatter_pattern, dim3(grid_for(n, p.pblock)), dim3(p.pblock), lds, st, n,
+                       p.pP, p.plast, p.E, p.t0, p.d_pslot, p.d_vals, out);
   } else {
     hipLaunchKernelGGL((k_scatter_add<I, CONTIG>), dim3(grid_for(p.n_targets)), dim3(kBlock), 0,
                        st, p.n_targets, p.d_target, p.t0, seg, src, p.d_vals, out);
@@ -346,6 +349,8 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   a.mcard = g.d_mcard;
   a.mstride = g.mcard_stride;
   a.n_models = (int)g.models.size();
+  a.blk_cards = g.d_blk_cards;
+  a.card_slot = g.d_card_slot;
   a.nodes = g.d_nodes;
   a.sto = g.d_sto;
   a.sta = g.d_sta;
@@ -542,6 +547,7 @@ int xa_group_destroy(xa_group gh) {
   Group* g = reinterpret_cast<Group*>(gh);
   if (!g) return XA_OK;
   (void)hipFree(g->d_dfield); (void)hipFree(g->d_ifield); (void)hipFree(g->d_mcard);
+  (void)hipFree(g->d_blk_cards); (void)hipFree(g->d_card_slot);
   (void)hipFree(g->d_nodes); (void)hipFree(g->d_sto); (void)hipFree(g->d_sta);
   (void)hipFree(g->d_state); (void)hipFree(g->d_fail);
   for (int k = 0; k < NK; ++k) free_plan(g->plan[k]);
@@ -821,27 +827,34 @@ static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& targe
   return upload(&p.d_bsrc, bsrc);
 }
 
-// A pattern plan applies when the n instances own consecutive blocks of S
-// targets in instance order (contiguous from the first target) and every
+// A pattern plan applies when each of the n instances feeds S targets at the
+// same offsets from its own base, the bases P apart in instance order
+// (target[i * S + s] = target[0] + i * P + off[s], off[S-1] < P), and every
 // instance's slot s is fed by the same entries.  Sources are e * n + i.
 static int detect_pattern(const Group& g, Plan& p, const std::vector<int32_t>& target,
                           const std::vector<int64_t>& seg, const std::vector<int64_t>& src) {
   p.pattern = false;
   const int64_t n = g.n, nt = (int64_t)target.size();
-  if (n == 0 || nt == 0 || nt % n != 0 || nt / n > 127) return XA_OK;
+  if (n == 0 || nt == 0 || nt % n != 0) return XA_OK;
   const int S = (int)(nt / n), E = p.E;
-  for (int64_t t = 0; t < nt; ++t)
-    if (target[t] != target[0] + (int32_t)t) return XA_OK;
-  std::vector<int32_t> pslot((size_t)E, -1);
+  const int64_t last = (int64_t)target[S - 1] - target[0];
+  const int64_t P = n > 1 ? (int64_t)target[S] - target[0] : last + 1;
+  if (last >= P || P > 127) return XA_OK;
+  for (int64_t i = 0; i < n; ++i)
+    for (int s = 0; s < S; ++s)
+      if ((int64_t)target[i * S + s] != (int64_t)target[0] + i * P + (target[s] - target[0]))
+        return XA_OK;
+  std::vector<int32_t> poff((size_t)E, -1), slot_of((size_t)E, -1);
   for (int s = 0; s < S; ++s)   // instance 0 defines the pattern
     for (int64_t j = seg[s]; j < seg[s + 1]; ++j) {
       if (src[j] % n != 0) return XA_OK;
-      pslot[src[j] / n] = s;
+      slot_of[src[j] / n] = s;
+      poff[src[j] / n] = target[s] - target[0];
     }
-  // every instance: slot s holds exactly the entries with pslot == s, in
-  // entry order (the plan's order within a private target)
+  // every instance: slot s holds exactly the entries of slot s, in entry
+  // order (the plan's order within a private target)
   std::vector<int32_t> cnt((size_t)S, 0);
-  for (int e = 0; e < E; ++e) if (pslot[e] >= 0) cnt[pslot[e]]++;
+  for (int e = 0; e < E; ++e) if (slot_of[e] >= 0) cnt[slot_of[e]]++;
   for (int64_t i = 0; i < n; ++i)
     for (int s = 0; s < S; ++s) {
       const int64_t t = i * S + s;
@@ -849,17 +862,19 @@ static int detect_pattern(const Group& g, Plan& p, const std::vector<int32_t>& t
       int64_t prev = -1;
       for (int64_t j = seg[t]; j < seg[t + 1]; ++j) {
         const int64_t e = src[j] / n;
-        if (src[j] % n != i || pslot[e] != s || e <= prev) return XA_OK;
+        if (src[j] % n != i || slot_of[e] != s || e <= prev) return XA_OK;
         prev = e;
       }
     }
-  int B = 256;   // LDS of the staged blocks within 64 KB
-  while (B > 64 && (size_t)B * (S + 1) * sizeof(double) > 65536) B /= 2;
-  if ((size_t)B * (S + 1) * sizeof(double) > 65536) return XA_OK;
+  int B = 256;   // LDS of the staged ranges within 64 KB
+  while (B > 64 && (size_t)B * (P + 1) * sizeof(double) > 65536) B /= 2;
+  if ((size_t)B * (P + 1) * sizeof(double) > 65536) return XA_OK;
   int rc;
-  if ((rc = upload(&p.d_pslot, pslot)) != XA_OK) return rc;
+  if ((rc = upload(&p.d_pslot, poff)) != XA_OK) return rc;
   p.pattern = true;
   p.pS = S;
+  p.pP = (int)P;
+  p.plast = (int)last;
   p.pblock = B;
   return XA_OK;
 }
@@ -978,6 +993,40 @@ static int build_plan(Group& g, int k) {
   return XA_OK;
 }
 
+// (Re)builds the per-block card tables of the stored-mode update for blocks
+// of B instances.
+static int build_card_tables(Group& g, int B) {
+  if (g.card_block == B) return XA_OK;
+  const int64_t n = g.n, nb = (n + B - 1) / B;
+  // per block: the number of cards, then the cards (kBlockCards + 1 ints)
+  std::vector<int32_t> cards((size_t)nb * (kBlockCards + 1), 0);
+  std::vector<uint8_t> slot((size_t)n, 0);
+  bool ok = true;
+  for (int64_t b = 0; b < nb && ok; ++b) {
+    int used = 0;
+    int32_t* tab = &cards[(size_t)b * (kBlockCards + 1) + 1];
+    for (int64_t i = b * B; i < std::min<int64_t>(n, (b + 1) * B) && ok; ++i) {
+      const int32_t m = g.inst_model[i];
+      int k = 0;
+      while (k < used && tab[k] != m) ++k;
+      if (k == used) {
+        if (used == kBlockCards) { ok = false; break; }
+        tab[used++] = m;
+      }
+      slot[i] = (uint8_t)k;
+    }
+    tab[-1] = used;
+  }
+  int rc;
+  if (ok) {
+    if ((rc = upload(&g.d_blk_cards, cards)) != XA_OK) return rc;
+    if ((rc = upload(&g.d_card_slot, slot)) != XA_OK) return rc;
+  }
+  g.card_tables = ok;
+  g.card_block = B;
+  return XA_OK;
+}
+
 // The fused load's geometry for blocks of B instances: its LDS and whether
 // every kind's block plan (built for B) exists.
 static bool fused_fits(const Group& g, int B) {
@@ -1018,6 +1067,7 @@ int xa_group_finalize(xa_group gh) {
     if ((rc = build_plan(*g, k)) != XA_OK) return rc;
   g->fused_rows = rows;
   set_fused_block(*g, g->impl->fused_block);
+  if ((rc = build_card_tables(*g, g->split_block)) != XA_OK) return rc;
   g->phase = PH_FINAL;
   return XA_OK;
 }
@@ -1207,11 +1257,13 @@ static int tune(Group& g, const xa_solver_state& ss, const xa_vectors& v, int re
   int split_B = g.split_block;
   for (int c = 0; c < 3; ++c) {
     g.split_block = candidates[c];
+    if ((rc = build_card_tables(g, g.split_block)) != XA_OK) return rc;
     float ms = 0.0f;
     if ((rc = time_passes(g, ss, v, false, reps, &ms)) != XA_OK) return rc;
     if (c == 0 || ms < best_split) { best_split = ms; split_B = candidates[c]; }
   }
   g.split_block = split_B;
+  if ((rc = build_card_tables(g, g.split_block)) != XA_OK) return rc;
   float best = best_split;
   int best_route = 1, best_B = split_B;
   if (!g.impl->launch_secondary && g.impl->fused_block > 0) {
@@ -1268,6 +1320,19 @@ int xa_group_autotune(xa_group gh, const xa_solver_state* ss, const xa_vectors* 
   (void)hipFree(saved);
   if (rc == XA_OK && e != hipSuccess) rc = hip_check(e, "autotune: restoring the instance state");
   return rc;
+}
+
+int xa_group_plan_info(xa_group gh, int kind, int64_t* n_targets, int64_t* n_sources,
+                       int* pattern_slots) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (kind < 0 || kind >= XA_NKINDS) return fail(XA_ERR_ARG, "plan_info: unknown kind");
+  const Plan& p = g->plan[kind];
+  if (n_targets) *n_targets = p.n_targets;
+  if (n_sources) *n_sources = p.n_src;
+  if (pattern_slots) *pattern_slots = p.pattern ? p.pS : 0;
+  return XA_OK;
 }
 
 int xa_group_failures(xa_group gh, int64_t* n_failed) {
