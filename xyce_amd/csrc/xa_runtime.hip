@@ -229,10 +229,21 @@ k_rail_sum(const int32_t* __restrict__ rail_t, const int32_t* __restrict__ rail_
 // its first instance's first target to its last instance's last target in
 // LDS (instance rows padded to P + 1 doubles against bank conflicts), every
 // thread adds its instance's entries in entry order (the plan order of a
-// private target) from the entry-major value rows, four rows' loads issued
+// private target) from the entry-major value rows, eight rows' loads issued
 // before their adds, and the block writes the range back (slots of the range
 // that no entry feeds are written back unchanged).  Same bits as the
 // segmented scatter, with coalesced reads and writes and no index arrays.
+// LDS slot of element k of a staged range: row k / P padded to P + 1.  The
+// quotient comes from a float reciprocal and one correction, exact for
+// k < 2^16 and P < 128 (the estimate is within one of the quotient).
+__device__ __forceinline__ int staged_slot(int k, int P, float inv_p) {
+  int q = (int)((float)k * inv_p);
+  int r = k - q * P;
+  if (r < 0) { --q; r += P; }
+  if (r >= P) { ++q; r -= P; }
+  return q * (P + 1) + r;
+}
+
 __global__ void __launch_bounds__(256)
 k_scatter_pattern(int64_t n, int P, int last, int E, int32_t t0, const int32_t* __restrict__ poff,
                   const double* __restrict__ vals, double* __restrict__ out) {
@@ -241,20 +252,22 @@ k_scatter_pattern(int64_t n, int P, int last, int E, int32_t t0, const int32_t* 
   const int m = (int)((n - i0) < (int64_t)blockDim.x ? (n - i0) : (int64_t)blockDim.x);
   const int W = P + 1;
   const int span = (m - 1) * P + last + 1;
+  const float inv_p = 1.0f / (float)P;
   double* o = out + t0 + i0 * P;
-  for (int k = threadIdx.x; k < span; k += blockDim.x) blk[(k / P) * W + k % P] = o[k];
+  for (int k = threadIdx.x; k < span; k += blockDim.x) blk[staged_slot(k, P, inv_p)] = o[k];
   __syncthreads();
   const int il = threadIdx.x;
   if (il < m) {
     const double* vi = vals + i0 + il;
     double* row = blk + il * W;
     int e = 0;
-    for (; e + 4 <= E; e += 4) {
-      double v[4];
+    constexpr int U = 8;   // value rows in flight per thread
+    for (; e + U <= E; e += U) {
+      double v[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = vi[(int64_t)(e + u) * n];
+      for (int u = 0; u < U; ++u) v[u] = vi[(int64_t)(e + u) * n];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int off = poff[e + u];
         if (off >= 0) row[off] += v[u];
       }
@@ -265,7 +278,7 @@ k_scatter_pattern(int64_t n, int P, int last, int E, int32_t t0, const int32_t* 
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < span; k += blockDim.x) o[k] = blk[(k / P) * W + k % P];
+  for (int k = threadIdx.x; k < span; k += blockDim.x) o[k] = blk[staged_slot(k, P, inv_p)];
 }
 
 template <typename I, bool CONTIG>
