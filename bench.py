@@ -389,7 +389,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if (one_kernel or args.netlist) else
-                     measured_traffic(args.model, f"{args.model}::k_update<false>", args.n),
+                     measured_traffic(args.model, f"{args.model}::k_update<false", args.n),
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,
     }

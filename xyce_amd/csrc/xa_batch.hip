@@ -54,9 +54,17 @@ struct BatchArgs {
 
 constexpr int kBatchThreads = 64;   // one wavefront per circuit
 
+// Orders the wavefront's LDS accesses across lanes where no barrier stands
+// between two phases: a workgroup is one wavefront, whose LDS operations run
+// in program order; this keeps the compiler to that order.
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
   extern __shared__ double sm[];
-  __shared__ int s_jp;
   __shared__ int s_bad;
   const int c = blockIdx.x;
   const int tid = threadIdx.x;
@@ -102,23 +110,24 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
     if (tid == 0) b.dx_max[c] = NAN;
     return;
   }
+  // Every lane reads the pivot column and the pivots itself (LDS broadcast
+  // reads), so the only barriers are after the row interchange and after the
+  // update; a lane recomputes its row's multiplier a / pivot, the same
+  // division the stored multiplier would hold (the multipliers are not kept:
+  // the right-hand side is eliminated alongside).
   int ju = 0;   // last column touched by the row interchanges so far
   for (int j = 0; j < n; ++j) {
     const int km = min(kl, n - 1 - j);
-    if (tid == 0) {
-      int jp = 0;
-      double best = fabs(AB[kv * n + j]);
-      for (int i = 1; i <= km; ++i) {
-        const double v = fabs(AB[(kv + i) * n + j]);
-        if (v > best) { best = v; jp = i; }
-      }
-      s_jp = jp;
+    int jp = 0;
+    double best = fabs(AB[kv * n + j]);
+    for (int i = 1; i <= km; ++i) {
+      const double v = fabs(AB[(kv + i) * n + j]);
+      if (v > best) { best = v; jp = i; }
     }
-    __syncthreads();
-    const int jp = s_jp;
     if (AB[(kv + jp) * n + j] == 0.0) continue;   // singular column: no elimination
     ju = max(ju, min(j + ku + jp, n - 1));
     if (jp != 0) {
+      wave_fence();   // every lane has read the pivot column before the swap
       for (int col = j + tid; col <= ju; col += kBatchThreads) {
         double* a = &AB[(kv + j - col) * n + col];
         double* e = &AB[(kv + j + jp - col) * n + col];
@@ -135,25 +144,25 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
     }
     if (km > 0) {
       const double piv = AB[kv * n + j];
-      for (int i = 1 + tid; i <= km; i += kBatchThreads) AB[(kv + i) * n + j] /= piv;
-      __syncthreads();
       // rank-1 update of the km x (ju - j) block and of the right-hand side
       const int w = ju - j + 1;
       for (int idx = tid; idx < km * w; idx += kBatchThreads) {
         const int i = 1 + idx / w, d = idx % w;
-        const double l = AB[(kv + i) * n + j];
+        const double l = AB[(kv + i) * n + j] / piv;
         if (d == 0) rhs[j + i] -= l * rhs[j];
         else AB[(kv + i - d) * n + j + d] -= l * AB[(kv - d) * n + j + d];
       }
       __syncthreads();
     }
   }
-  // back substitution with U (bandwidth kl + ku)
+  // back substitution with U (bandwidth kl + ku): every lane forms x_j, the
+  // lanes update the rows above, lane 0 keeps x_j
   for (int j = n - 1; j >= 0; --j) {
-    if (tid == 0) rhs[j] /= AB[kv * n + j];
-    __syncthreads();
+    const double xj = rhs[j] / AB[kv * n + j];
     for (int i = max(0, j - kv) + tid; i < j; i += kBatchThreads)
-      rhs[i] -= AB[(kv + i - j) * n + j] * rhs[j];
+      rhs[i] -= AB[(kv + i - j) * n + j] * xj;
+    wave_fence();   // every lane has read rhs[j]
+    if (tid == 0) rhs[j] = xj;
     __syncthreads();
   }
   double mx = 0.0;
