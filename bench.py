@@ -265,6 +265,9 @@ def main():
     ap.add_argument("--fused", action="store_true", help="time the one-call fused load")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the block shapes on the workload first, then time xa_load_all")
+    ap.add_argument("--path", choices=["split", "fused", "pattern"],
+                    help="pin xa_load_all's path (with --block) and time it")
+    ap.add_argument("--block", type=int, default=128, help="block size for --path")
     ap.add_argument("--ring", action="store_true",
                     help="run the 51-stage BSIM4 ring-oscillator transient (1 ps steps to 2 ns) "
                          "instead of the microbenchmark")
@@ -320,7 +323,11 @@ def main():
         print(f"[bench] building {args.model} workload n={args.n}", file=sys.stderr, flush=True)
         wl = Workload(args.model, args.n)
     tuned = None
-    if args.autotune:
+    if args.path:
+        wl.master.set_path(args.path, args.block)
+        tuned = {"route": args.path, "block": args.block, "pinned": True}
+        args.fused = True
+    elif args.autotune:
         route, block = wl.master.autotune(wl.ss, wl.vectors, reps=3)
         wl.zero_outputs()
         tuned = {"route": route, "block": block}
@@ -381,7 +388,8 @@ def main():
                    f"device-eval microbenchmark, {args.n} {args.model} instances, "
                    "private nodes, V~U[-0.5,1.5], one load pass per step",
                    "instances": args.n, "device": args.model,
-                   "path": "fused load_all" if one_kernel else
+                   "path": (f"{tuned['route']} load_all" if tuned and one_kernel else
+                            "fused load_all") if one_kernel else
                            ("load_all (split at this size)" if args.fused else
                             "updateState+loadDAEVectors+loadDAEMatrices")},
         "phase_ms": {"update_state": ms_up, "load_vectors": ms_vec, "load_matrices": ms_mat},

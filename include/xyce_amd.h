@@ -219,14 +219,26 @@ int xa_load_all(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
 /* Autotuner of the block shapes (no reference counterpart).  Times, on the
  * caller's vectors and flags, the four-call path with update-kernel blocks of
  * 64, 128 and 256 threads, and the fused one-kernel load with blocks of 64,
- * 128 and 256 instances where its block plans exist and its LDS fits; then
+ * 128 and 256 instances where its block plans exist and its LDS fits, and
+ * the pattern-fused load likewise where every kind has a pattern plan; then
  * keeps the fastest: the update-kernel block for the four calls, and the path
  * (and fused block) for xa_load_all.  Every choice gives the same bits.  It
- * runs up to 6 * (reps + 1) load passes that add into the output vectors (zero them
+ * runs up to 9 * (reps + 1) load passes that add into the output vectors (zero them
  * afterwards) and rewrites store/state as a load does.  *route: 1 four calls,
- * 2 fused; *block: its block size.  Synchronizes. */
+ * 2 fused, 3 pattern-fused (see xa_group_set_path); *block: its block size.
+ * Synchronizes. */
 int xa_group_autotune(xa_group g, const xa_solver_state* ss, const xa_vectors* v, int reps,
                       int* route, int* block);
+
+/* Pins xa_load_all's path (no reference counterpart; for benchmarks and
+ * tests; the autotuner picks the fastest instead): 1 the four calls with
+ * update-kernel blocks of `block` threads, 2 the fused load with blocks of
+ * `block` instances, 3 the pattern-fused load (every kind a pattern plan: one
+ * kernel that stages each block's output spans in LDS and adds the stamps
+ * into them in order) with blocks of `block` instances.  block: 64, 128 or
+ * 256.  XA_ERR_UNSUPPORTED when the group has no such path.  All paths give
+ * the same bits. */
+int xa_group_set_path(xa_group g, int path, int block);
 
 /* Shape of the scatter plan of one output kind (XA_KIND_*), for diagnostics
  * and tests (no reference counterpart): targets, contributions, and the

@@ -44,3 +44,25 @@ def test_shared_nodes_tune_to_four_calls():
     p = Pair("diode", diode_spec(8000, seed=507), topology="shared", seed=508,
              n_shared_nodes=64)
     _tune(p, want="split")
+
+
+@pytest.mark.parametrize("path,block", [("pattern", 64), ("pattern", 128), ("pattern", 256),
+                                        ("fused", 64), ("split", 128)])
+def test_pinned_paths_equal_four_calls(path, block):
+    """Each pinned xa_load_all path and block size gives the four calls' bits
+    (the pattern-fused load: every stamp added into the instance's staged
+    output span in stamp order).  Private nodes without ground and one
+    node-collapse pattern, so every kind has a pattern plan."""
+    # cards without RSH: no instance has internal nodes, so every instance's
+    # stamps follow one pattern
+    spec = bsim4_spec(3000, seed=509, cards=[BSIM4_CARDS[i] for i in (0, 1, 3)])
+    print(f"bsim4 {path} {block}", flush=True)
+    twin_drive("bsim4", spec, 510, np.random.default_rng(511), fused_b=True, topology="private",
+               prepare_b=lambda p: p.gpu.set_path(path, block), ground_frac=0.0)
+
+
+def test_pattern_path_needs_pattern_plans():
+    p = Pair("bsim4", bsim4_spec(2000, seed=512, cards=BSIM4_CARDS), topology="shared", seed=513,
+             n_shared_nodes=64)
+    with pytest.raises(RuntimeError):
+        p.gpu.set_path("pattern", 64)

@@ -206,6 +206,13 @@ class Master:
               "plan_info")
         return nt.value, ns.value, ps.value
 
+    PATHS = {"split": 1, "fused": 2, "pattern": 3}
+
+    def set_path(self, path, block=256):
+        """Pin xa_load_all's path ("split", "fused" or "pattern") and block
+        size (xa_group_set_path); raises where the group has no such path."""
+        check(lib().xa_group_set_path(self.h, self.PATHS[path], int(block)), "set_path")
+
     def autotune(self, ss, vectors, reps=5):
         """Time the block shapes on these vectors and keep the fastest
         (xa_group_autotune); returns ("split" | "fused", block).  The tuning
@@ -213,7 +220,7 @@ class Master:
         route, block = C.c_int(), C.c_int()
         check(lib().xa_group_autotune(self.h, C.byref(ss), C.byref(vectors), int(reps),
                                       C.byref(route), C.byref(block)), "autotune")
-        return ("split" if route.value == 1 else "fused"), block.value
+        return {1: "split", 2: "fused", 3: "pattern"}[route.value], block.value
 
     def failures(self):
         c = C.c_int64()

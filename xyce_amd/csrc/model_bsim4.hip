@@ -490,7 +490,7 @@ __device__ __forceinline__ void reload_params(const double*& dfb) {
 // LDSCARD: the block's model cards are staged in LDS by the kernel (s_card,
 // in the order of the block's card table; card_slot gives the instance's);
 // otherwise they are read from global memory.
-template <bool FUSED, bool LDSCARD>
+template <int MODE, bool LDSCARD>
 __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
   const DevCard& C =
@@ -2647,7 +2647,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
 
   // ---- Master::loadDAEVectors ----
-  Sink<FUSED> s(a, i);
+  Sink<MODE> s(a, i);
   const double Mp = IV(numberParallel);
   {
     const double dt = C.dtype;
@@ -2883,11 +2883,34 @@ __global__ void __launch_bounds__(256, FUSED ? 1 : 2) k_update(KArgs a) {
     for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
       s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
     __syncthreads();
-    eval<FUSED, true>(a, i, s_card);
+    eval<FUSED ? 1 : 0, true>(a, i, s_card);
   } else {
-    eval<FUSED, false>(a, i, nullptr);
+    eval<FUSED ? 1 : 0, false>(a, i, nullptr);
   }
   if (FUSED) fused_writeout(a);
+}
+
+// Pattern-fused load (every kind a pattern plan): the block stages its output
+// spans in LDS, every instance adds its stamps into its own span in stamp
+// order as the eval produces them, and the block writes the spans back.  No
+// value buffers and no scatter kernels; one wave per SIMD (the spans and the
+// eval's registers).
+template <bool LDSCARD>
+__global__ void __launch_bounds__(256, 1) k_update_pattern(KArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kStride = (int)(sizeof(DevCard) / sizeof(double));
+  __shared__ double s_card[LDSCARD ? kBlockCards * kStride : 1];
+  pattern_stage(a, true);
+  if (LDSCARD) {
+    const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
+    const int used = tab[0];
+    for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+      s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+  }
+  __syncthreads();
+  eval<2, LDSCARD>(a, i, s_card);
+  __syncthreads();
+  pattern_stage(a, false);
 }
 
 #undef PP
@@ -2910,6 +2933,20 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
                              lds_cards ? (const void*)k_update<false, true>
                                        : (const void*)k_update<false, false>,
                              (const void*)k_update<true, false>);
+}
+
+static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors& v) {
+  if (!v.nextSol || !v.nextSto) return fail(XA_ERR_ARG, "bsim4 update: nextSol/nextSto required");
+  if (ss.newtonIter == 0 && (!ss.dcopFlag || (ss.locaEnabledFlag && ss.dcopFlag)) && !v.currSto)
+    return fail(XA_ERR_ARG, "bsim4 update: currSto required");
+  if (ss.inputOPFlag && ss.initJctFlag && !v.flagSol)
+    return fail(XA_ERR_ARG, "bsim4 update: flagSol required with inputOPFlag");
+  KArgs a;
+  fill_kargs(g, ss, v, true, a);   // kinds with an output, a target and not skipped
+  const bool lds_cards = g.card_tables && g.card_block == g.pattern_B &&
+                         a.mstride == (int)(sizeof(DevCard) / sizeof(double));
+  return launch_pattern_kernel(g, a, lds_cards ? (const void*)k_update_pattern<true>
+                                               : (const void*)k_update_pattern<false>);
 }
 
 // Algorithmic bytes per instance: the SoA columns and card index read once,
@@ -2956,6 +2993,7 @@ const ModelImpl* model_bsim4() {
     x.n_state = NS;
     x.traffic = traffic;
     x.fused_block = 128;
+    x.launch_pattern = launch_pattern;
     x.fused_at_scale = false;
     return x;
   }();

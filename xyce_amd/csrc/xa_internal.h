@@ -111,6 +111,9 @@ struct ModelImpl {
   double (*traffic)(const Group&, int phase);   // algorithmic bytes per instance
   int fused_block;          // threads per block of the fused load (0: no fused load)
   bool fused_at_scale;      // fused load also when the launch fills the GPU
+  // pattern-fused load (one kernel staging the outputs of pattern plans in
+  // LDS); null when the model has none
+  int (*launch_pattern)(Group&, const xa_solver_state&, const xa_vectors&);
 };
 
 const ModelImpl* model_diode();
@@ -168,7 +171,11 @@ struct Group {
   int fused_rows = 0;        // LDS rows of the fused load (stamp rows of live kinds)
   int fused_B = 0;           // instances per block the block plans were built for
   int split_block = 256;     // threads per block of the stored-mode update kernel
-  int route = 0;             // xa_load_all: 0 size/model policy, 1 four calls, 2 fused
+  int route = 0;             // xa_load_all: 0 size/model policy, 1 four calls, 2 fused,
+                             // 3 pattern-fused
+  bool pattern_ok = false;   // every kind with targets has a pattern plan, no lead kinds
+  int pattern_rows = 0;      // LDS doubles per instance of the pattern-fused load
+  int pattern_B = 0;         // its instances per block
   // Per-block card tables of the stored-mode update: for every block of
   // card_block instances, the number of distinct models it uses and the
   // models (at most kBlockCards; kBlockCards + 1 ints per block), and each

@@ -35,6 +35,12 @@ struct KArgs {
   int32_t bt0[NK];
   const int32_t* bseg[NK];
   const uint16_t* bsrc[NK];
+  // pattern-fused mode: each kind's staged output range (see pattern_stage)
+  const int32_t* poff[NK];             // offset of each entry in its instance's span
+  int pbase[NK];                       // first LDS double of the kind's range
+  int pP[NK];                          // span of one instance
+  int plast[NK];                       // offset of its last target
+  int32_t pt0[NK];                     // first target of instance 0
 };
 
 // LDS of the fused load: row (loff[k] + e) holds entry e of kind k for every
@@ -42,20 +48,62 @@ struct KArgs {
 extern __shared__ double xa_lds[];
 
 // Stamp sink: an instance hands its contributions over in stamp order.
-// Stored mode keeps them in the entry-major value buffers for the fixed-order
-// scatter (one put across a wavefront is 64 consecutive slots); fused mode
-// keeps them in LDS for the block's ordered write-out (fused_writeout).
-template <bool FUSED>
+// MODE 0 (stored) keeps them in the entry-major value buffers for the
+// fixed-order scatter (one put across a wavefront is 64 consecutive slots);
+// MODE 1 (fused) keeps them in LDS rows for the block's ordered write-out
+// (fused_writeout); MODE 2 (pattern-fused) adds each one at once into the
+// instance's staged output span in LDS (pattern_stage), which in stamp order
+// is the plan order of a private target.
+template <int MODE>
 struct Sink {
   const KArgs& a;
   int64_t i;
   __device__ Sink(const KArgs& args, int64_t inst) : a(args), i(inst) {}
   __device__ void put(int k, int e, double val) const {
     if (!(a.kmask & (1u << k))) return;
-    if (FUSED) xa_lds[(a.loff[k] + e) * (int)blockDim.x + (int)threadIdx.x] = val;
-    else a.vals[k][(int64_t)e * a.n + i] = val;
+    if (MODE == 1) {
+      xa_lds[(a.loff[k] + e) * (int)blockDim.x + (int)threadIdx.x] = val;
+    } else if (MODE == 2) {
+      const int off = a.poff[k][e];
+      if (off >= 0) xa_lds[a.pbase[k] + (int)threadIdx.x * (a.pP[k] + 1) + off] += val;
+    } else {
+      a.vals[k][(int64_t)e * a.n + i] = val;
+    }
   }
 };
+
+// LDS slot of element k of a staged range: row k / P padded to P + 1.  The
+// quotient comes from a float reciprocal and one correction, exact for
+// k < 2^16 and P < 128 (the estimate is within one of the quotient).
+__device__ __forceinline__ int staged_slot(int k, int P, float inv_p) {
+  int q = (int)((float)k * inv_p);
+  int r = k - q * P;
+  if (r < 0) { --q; r += P; }
+  if (r >= P) { ++q; r -= P; }
+  return q * (P + 1) + r;
+}
+
+// Pattern-fused staging of the block's outputs: for every kind on this call,
+// the output range from the block's first instance's first target to its
+// last instance's last target, instance rows padded to P + 1 doubles (load:
+// outputs into LDS; otherwise LDS back into the outputs, slots no entry feeds
+// unchanged).  Every thread of the block must call it.
+__device__ __forceinline__ void pattern_stage(const KArgs& a, bool load) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  const int m = (int)((a.n - i0) < (int64_t)blockDim.x ? (a.n - i0) : (int64_t)blockDim.x);
+  for (int k = 0; k < NK; ++k) {
+    if (!(a.kmask & (1u << k))) continue;
+    const int P = a.pP[k];
+    const int span = (m - 1) * P + a.plast[k] + 1;
+    const float inv_p = 1.0f / (float)P;
+    double* o = a.out[k] + a.pt0[k] + i0 * P;
+    double* base = xa_lds + a.pbase[k];
+    for (int x = threadIdx.x; x < span; x += blockDim.x) {
+      if (load) base[staged_slot(x, P, inv_p)] = o[x];
+      else o[x] = base[staged_slot(x, P, inv_p)];
+    }
+  }
+}
 
 // Fused write-out: after every lane of the block has put its values, the
 // block adds them into its targets in the plan's order -- instance then stamp
@@ -130,5 +178,9 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
 // size and LDS from the group) on the context stream.
 int launch_model_kernel(Group& g, const KArgs& a, bool fused, const void* k_stored,
                         const void* k_fused);
+
+// Pattern-fused launch: fills the pattern fields of `a` (staging layout for
+// blocks of g.pattern_B instances) and launches `kernel` with that LDS.
+int launch_pattern_kernel(Group& g, KArgs& a, const void* kernel);
 
 }  // namespace xa

@@ -233,17 +233,6 @@ k_rail_sum(const int32_t* __restrict__ rail_t, const int32_t* __restrict__ rail_
 // before their adds, and the block writes the range back (slots of the range
 // that no entry feeds are written back unchanged).  Same bits as the
 // segmented scatter, with coalesced reads and writes and no index arrays.
-// LDS slot of element k of a staged range: row k / P padded to P + 1.  The
-// quotient comes from a float reciprocal and one correction, exact for
-// k < 2^16 and P < 128 (the estimate is within one of the quotient).
-__device__ __forceinline__ int staged_slot(int k, int P, float inv_p) {
-  int q = (int)((float)k * inv_p);
-  int r = k - q * P;
-  if (r < 0) { --q; r += P; }
-  if (r >= P) { ++q; r -= P; }
-  return q * (P + 1) + r;
-}
-
 __global__ void __launch_bounds__(256)
 k_scatter_pattern(int64_t n, int P, int last, int E, int32_t t0, const int32_t* __restrict__ poff,
                   const double* __restrict__ vals, double* __restrict__ out) {
@@ -410,6 +399,26 @@ int launch_model_kernel(Group& g, const KArgs& a, bool fused, const void* k_stor
                                (int)g.fused_lds));
   XA_HIP(hipLaunchKernel(k_fused, dim3(grid_for(g.n, B)), dim3(B), args, g.fused_lds,
                          g.ctx->stream));
+  return XA_OK;
+}
+
+int launch_pattern_kernel(Group& g, KArgs& a, const void* kernel) {
+  const int B = g.pattern_B;
+  int base = 0;
+  for (int k = 0; k < NK; ++k) {
+    const Plan& p = g.plan[k];
+    a.poff[k] = p.d_pslot;
+    a.pbase[k] = base;
+    a.pP[k] = p.pP;
+    a.plast[k] = p.plast;
+    a.pt0[k] = p.t0;
+    if (p.pattern) base += B * (p.pP + 1);
+  }
+  const size_t lds = sizeof(double) * (size_t)base;
+  if (lds > 65536)
+    XA_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  void* args[] = {(void*)&a};
+  XA_HIP(hipLaunchKernel(kernel, dim3(grid_for(g.n, B)), dim3(B), args, lds, g.ctx->stream));
   return XA_OK;
 }
 
@@ -1080,6 +1089,14 @@ int xa_group_finalize(xa_group gh) {
     if ((rc = build_plan(*g, k)) != XA_OK) return rc;
   g->fused_rows = rows;
   set_fused_block(*g, g->impl->fused_block);
+  g->pattern_ok = g->impl->launch_pattern != nullptr && g->n > 0;
+  g->pattern_rows = 0;
+  for (int k = 0; k < NK; ++k) {
+    const Plan& p = g->plan[k];
+    if (p.n_targets == 0) continue;
+    if (!p.pattern || kind_is_assign(k)) g->pattern_ok = false;
+    else g->pattern_rows += p.pP + 1;
+  }
   if ((rc = build_card_tables(*g, g->split_block)) != XA_OK) return rc;
   g->phase = PH_FINAL;
   return XA_OK;
@@ -1167,6 +1184,12 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   // faster fused; otherwise the four calls run in sequence (same results)
   const int64_t blocks = (g->n + g->fused_B - 1) / std::max(g->fused_B, 1);
   const bool fused_policy = g->impl->fused_at_scale || blocks <= kSmallLaunchBlocks;
+  if (g->route == 3 && g->pattern_ok && g->pattern_B > 0) {
+    tic(g, 0);
+    rc = g->impl->launch_pattern(*g, *ss, *v);
+    toc(g, 0);
+    return rc;
+  }
   if (g->fused_ok && !g->impl->launch_secondary &&
       (g->route == 2 || (g->route == 0 && fused_policy))) {
     tic(g, 0);
@@ -1227,9 +1250,10 @@ static int build_block_plans(Group& g, int B) {
 }
 
 // One load pass as xa_load_all would run it on the fused or the split path.
-static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
+static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, int path) {
   int rc;
-  if (fused) return g.impl->launch_update(g, ss, v, true);
+  if (path == 3) return g.impl->launch_pattern(g, ss, v);
+  if (path == 2) return g.impl->launch_update(g, ss, v, true);
   if ((rc = g.impl->launch_update(g, ss, v, false)) != XA_OK) return rc;
   if (g.impl->launch_secondary && (rc = g.impl->launch_secondary(g, ss, v)) != XA_OK) return rc;
   for (int k : {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ, K_DFDX, K_DQDX})
@@ -1238,14 +1262,14 @@ static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, b
 }
 
 // Mean milliseconds of one pass over `reps` passes after one untimed pass.
-static int time_passes(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused,
+static int time_passes(Group& g, const xa_solver_state& ss, const xa_vectors& v, int path,
                        int reps, float* ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   XA_HIP(hipEventCreate(&e0));
   XA_HIP(hipEventCreate(&e1));
-  int rc = load_pass(g, ss, v, fused);
+  int rc = load_pass(g, ss, v, path);
   if (rc == XA_OK) (void)hipEventRecord(e0, g.ctx->stream);
-  for (int r = 0; r < reps && rc == XA_OK; ++r) rc = load_pass(g, ss, v, fused);
+  for (int r = 0; r < reps && rc == XA_OK; ++r) rc = load_pass(g, ss, v, path);
   if (rc == XA_OK) {
     (void)hipEventRecord(e1, g.ctx->stream);
     hipError_t e = hipEventSynchronize(e1);
@@ -1272,7 +1296,7 @@ static int tune(Group& g, const xa_solver_state& ss, const xa_vectors& v, int re
     g.split_block = candidates[c];
     if ((rc = build_card_tables(g, g.split_block)) != XA_OK) return rc;
     float ms = 0.0f;
-    if ((rc = time_passes(g, ss, v, false, reps, &ms)) != XA_OK) return rc;
+    if ((rc = time_passes(g, ss, v, 1, reps, &ms)) != XA_OK) return rc;
     if (c == 0 || ms < best_split) { best_split = ms; split_B = candidates[c]; }
   }
   g.split_block = split_B;
@@ -1287,15 +1311,55 @@ static int tune(Group& g, const xa_solver_state& ss, const xa_vectors& v, int re
       built = B;
       if (!g.fused_ok) continue;
       float ms = 0.0f;
-      if ((rc = time_passes(g, ss, v, true, reps, &ms)) != XA_OK) return rc;
+      if ((rc = time_passes(g, ss, v, 2, reps, &ms)) != XA_OK) return rc;
       if (ms < best) { best = ms; best_route = 2; best_B = B; }
     }
     const int keep = best_route == 2 ? best_B : g.impl->fused_block;
     if (built != keep && (rc = build_block_plans(g, keep)) != XA_OK) return rc;
   }
+  if (g.pattern_ok) {   // the pattern-fused load: instances per block
+    const int keep_B = g.pattern_B;
+    for (int B : candidates) {
+      if ((size_t)B * g.pattern_rows * sizeof(double) > kMaxLds - 16 * 1024) continue;
+      g.pattern_B = B;
+      float ms = 0.0f;
+      if ((rc = time_passes(g, ss, v, 3, reps, &ms)) != XA_OK) return rc;
+      if (ms < best) { best = ms; best_route = 3; best_B = B; }
+    }
+    g.pattern_B = best_route == 3 ? best_B : keep_B;
+  }
   g.route = best_route;
   if (route) *route = best_route;
   if (block) *block = best_B;
+  return XA_OK;
+}
+
+int xa_group_set_path(xa_group gh, int path, int block) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (block != 64 && block != 128 && block != 256)
+    return fail(XA_ERR_ARG, "set_path: block must be 64, 128 or 256");
+  switch (path) {
+    case 1:
+      g->split_block = block;
+      if ((rc = build_card_tables(*g, block)) != XA_OK) return rc;
+      break;
+    case 2:
+      if (g->impl->launch_secondary || g->impl->fused_block <= 0 || !fused_fits(*g, block))
+        return fail(XA_ERR_UNSUPPORTED, "set_path: no fused load of that block size");
+      if ((rc = build_block_plans(*g, block)) != XA_OK) return rc;
+      if (!g->fused_ok) return fail(XA_ERR_UNSUPPORTED, "set_path: targets shared across blocks");
+      break;
+    case 3:
+      if (!g->pattern_ok || (size_t)block * g->pattern_rows * sizeof(double) > kMaxLds - 16 * 1024)
+        return fail(XA_ERR_UNSUPPORTED, "set_path: no pattern-fused load of that block size");
+      g->pattern_B = block;
+      break;
+    default:
+      return fail(XA_ERR_ARG, "set_path: path must be 1, 2 or 3");
+  }
+  g->route = path;
   return XA_OK;
 }
 
