@@ -786,6 +786,7 @@ const ModelImpl* model_bjt() {
     x.traffic = bjt::traffic;
     x.fused_block = 64;
     x.fused_at_scale = true;
+    x.fused_small = true;
     return x;
   }();
   return &m;

@@ -2995,6 +2995,9 @@ const ModelImpl* model_bsim4() {
     x.fused_block = 128;
     x.launch_pattern = launch_pattern;
     x.fused_at_scale = false;
+    // the four calls beat the fused kernel at every size: one block's
+    // in-order sums over shared targets are serial in the fused writeout
+    x.fused_small = false;
     return x;
   }();
   return &m;

@@ -485,6 +485,7 @@ const ModelImpl* model_diode() {
     x.traffic = diode::traffic;
     x.fused_block = 256;
     x.fused_at_scale = true;
+    x.fused_small = true;
     return x;
   }();
   return &m;

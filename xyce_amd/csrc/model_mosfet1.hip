@@ -771,6 +771,7 @@ const ModelImpl* model_mosfet1() {
     x.traffic = mosfet1::traffic;
     x.fused_block = 64;
     x.fused_at_scale = false;
+    x.fused_small = true;
     return x;
   }();
   return &m;

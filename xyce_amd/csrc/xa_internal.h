@@ -111,6 +111,7 @@ struct ModelImpl {
   double (*traffic)(const Group&, int phase);   // algorithmic bytes per instance
   int fused_block;          // threads per block of the fused load (0: no fused load)
   bool fused_at_scale;      // fused load also when the launch fills the GPU
+  bool fused_small;         // fused load when the launch is a few blocks
   // pattern-fused load (one kernel staging the outputs of pattern plans in
   // LDS); null when the model has none
   int (*launch_pattern)(Group&, const xa_solver_state&, const xa_vectors&);

@@ -1180,10 +1180,12 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
   if (g->n == 0) return XA_OK;
   // the fused kernel needs every target block-private and no secondary pass;
-  // it is taken for launch-bound sizes and, at scale, where the model runs
-  // faster fused; otherwise the four calls run in sequence (same results)
+  // it is taken for launch-bound sizes where the model's fused kernel is the
+  // faster (fused_small) and at scale where fused_at_scale says so; otherwise
+  // the four calls run in sequence (same results)
   const int64_t blocks = (g->n + g->fused_B - 1) / std::max(g->fused_B, 1);
-  const bool fused_policy = g->impl->fused_at_scale || blocks <= kSmallLaunchBlocks;
+  const bool fused_policy =
+      g->impl->fused_at_scale || (g->impl->fused_small && blocks <= kSmallLaunchBlocks);
   if (g->route == 3 && g->pattern_ok && g->pattern_B > 0) {
     tic(g, 0);
     rc = g->impl->launch_pattern(*g, *ss, *v);
