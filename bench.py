@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--model", default="bsim4", choices=["diode", "mosfet1", "bjt", "bsim4"])
+    ap.add_argument("--model", default="bsim4", choices=["diode", "mosfet1", "bjt", "bsim4", "resistor", "capacitor"])
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -362,7 +362,8 @@ def main():
     cpu = None
     if not args.no_cpu and not args.netlist:
         n_sample = args.cpu_sample or {"diode": 1 << 17, "mosfet1": 1 << 15, "bjt": 1 << 15,
-                                       "bsim4": 1 << 13}[args.model]
+                                       "bsim4": 1 << 13, "resistor": 1 << 18,
+                                       "capacitor": 1 << 18}[args.model]
         print(f"[bench] cpu baseline on {n_sample} instances", file=sys.stderr, flush=True)
         threads = cpu_threads()
         c1, d1 = cpu_baseline(args.model, n_sample, args.cpu_seconds / 2)

@@ -51,6 +51,8 @@ extern "C" {
 #define XA_MOSFET1  2   /* "m", level 1          N_DEV_MOSFET1.C:4984-4996   */
 #define XA_BJT      3   /* "q", level 1          N_DEV_BJT.C:4368-4380       */
 #define XA_BSIM4    4   /* "m", levels 14,54     N_DEV_MOSFET_B4.C:17270-17280 */
+#define XA_RESISTOR 5   /* "r", level 1          N_DEV_Resistor.C:1198-1224  */
+#define XA_CAPACITOR 6  /* "c", level 1          N_DEV_Capacitor.C:2152-2177 */
 
 /* ---- output kinds of a load ------------------------------------------- */
 #define XA_KIND_F        0   /* daeFVector rows                              */

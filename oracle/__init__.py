@@ -16,7 +16,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
-MODEL_TYPES = {"diode": 1, "mosfet1": 2, "bjt": 3, "bsim4": 4}
+MODEL_TYPES = {"diode": 1, "mosfet1": 2, "bjt": 3, "bsim4": 4, "resistor": 5, "capacitor": 6}
 
 
 class SolverState(C.Structure):

@@ -78,6 +78,8 @@ void or_default_options(xa_device_options* o) {
 static const or_model_ops* ops_for(int type) {
   switch (type) {
     case XA_DIODE: return &or_diode_ops;
+    case XA_RESISTOR: return &or_resistor_ops;
+    case XA_CAPACITOR: return &or_capacitor_ops;
     case XA_MOSFET1: return &or_mosfet1_ops;
     case XA_BJT: return &or_bjt_ops;
     case XA_BSIM4: return &or_bsim4_ops;

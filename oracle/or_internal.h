@@ -85,6 +85,8 @@ extern const or_model_ops or_diode_ops;
 extern const or_model_ops or_mosfet1_ops;
 extern const or_model_ops or_bjt_ops;
 extern const or_model_ops or_bsim4_ops;
+extern const or_model_ops or_resistor_ops;
+extern const or_model_ops or_capacitor_ops;
 
 int or_fail(int code, const char* msg);
 /* Offset of column `col` in CSR row `row`, as Indexor::matrixGlobalToLocal

@@ -173,3 +173,59 @@ def bjt_spec(n, seed=20261203, cards=None, temp_spread=False, lambertw_frac=0.0,
         params["IC1"] = rng.uniform(0.3, 0.8, n)
         params["IC2"] = rng.uniform(0.0, 3.0, n)
     return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
+
+
+RESISTOR_CARDS = [
+    (1, {}),
+    (1, {"TC1": 1.5e-3, "TC2": 2e-6, "TNOM": 30.0, "R": 1.25}),
+    (1, {"TC1": -4e-4}),
+]
+
+
+def resistor_spec(n, seed=20261203, cards=None, temp_spread=False, sheet=False):
+    """Resistors with R lognormal around 1 kOhm, or (sheet) R from RSH and
+    per-instance L/W on cards that give RSH, NARROW and DEFW."""
+    rng = splitmix64(seed)
+    if sheet:
+        cards = cards if cards is not None else [
+            (1, {"RSH": 50.0, "NARROW": 1e-7}), (1, {"RSH": 120.0, "DEFW": 2e-6, "TC1": 1e-3})]
+        params = {"L": lognormal(rng, 2e-6, 0.3, n)}
+        if n:
+            params["W"] = lognormal(rng, 1e-6, 0.2, n)
+    else:
+        cards = cards if cards is not None else RESISTOR_CARDS
+        params = {"R": lognormal(rng, 1000.0, 0.5, n)}
+    mi = rng.integers(0, len(cards), size=n).astype(np.int32)
+    if temp_spread:
+        params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
+
+
+CAPACITOR_CARDS = [
+    (1, {}),
+    (1, {"C": 1.2, "TC1": 2e-4, "TC2": 1e-6, "TNOM": 20.0}),
+]
+
+
+def capacitor_spec(n, seed=20261203, cards=None, temp_spread=False, ic=False, semi=False,
+                   age=False):
+    """Capacitors with C lognormal around 1 fF (or, semi, C from CJ/CJSW and
+    per-instance L/W); ic adds IC= values (a branch unknown each); age adds
+    AGE/D."""
+    rng = splitmix64(seed)
+    if semi:
+        cards = cards if cards is not None else [
+            (1, {"CJ": 1e-3, "CJSW": 1e-10, "NARROW": 5e-8}), (1, {"CJ": 2e-3, "DEFW": 3e-6})]
+        params = {"L": lognormal(rng, 2e-6, 0.3, n), "W": lognormal(rng, 1e-6, 0.2, n)}
+    else:
+        cards = cards if cards is not None else CAPACITOR_CARDS
+        params = {"C": lognormal(rng, 1e-15, 0.3, n)}
+    mi = rng.integers(0, len(cards), size=n).astype(np.int32)
+    if ic:
+        params["IC"] = rng.uniform(-0.5, 1.5, n)
+    if age:
+        params["AGE"] = rng.uniform(0.5, 1e4, n)
+        params["D"] = rng.uniform(0.01, 0.05, n)
+    if temp_spread:
+        params["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return {"models": cards, "n": n, "model_index": mi, "inst_params": params}

@@ -11,8 +11,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxyce_amd.so")
 
 XA_OK = 0
-XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4 = 1, 2, 3, 4
-MODEL_TYPES = {"diode": XA_DIODE, "mosfet1": XA_MOSFET1, "bjt": XA_BJT, "bsim4": XA_BSIM4}
+XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4, XA_RESISTOR, XA_CAPACITOR = 1, 2, 3, 4, 5, 6
+MODEL_TYPES = {"diode": XA_DIODE, "mosfet1": XA_MOSFET1, "bjt": XA_BJT, "bsim4": XA_BSIM4,
+               "resistor": XA_RESISTOR, "capacitor": XA_CAPACITOR}
 
 
 class SolverState(C.Structure):

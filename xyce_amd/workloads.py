@@ -5,7 +5,8 @@ instances of one model type, each with private nodes (disjoint CSR blocks),
 terminal voltages i.i.d. U[-0.5, 1.5] V from a seed, previous-iteration store
 values from a second draw, and per-instance geometry spread lognormally
 (sigma 0.2) -- W/L around 90 nm / 45 nm for MOSFETs, AREA around 1 for the
-diode and BJT.  Solver state: transient, newtonIter = 1, limiter on.
+diode and BJT, R around 1 kOhm and C around 1 fF for the linear devices.
+Solver state: transient, newtonIter = 1, limiter on.
 """
 import numpy as np
 import torch
@@ -28,6 +29,8 @@ CARDS = {
                  "ISE": 1e-14, "NE": 1.5, "RB": 10.0, "RC": 1.0, "RE": 0.5, "CJE": 1e-12,
                  "CJC": 5e-13, "TF": 1e-10, "TR": 1e-8})],
     "bsim4": [(54, {"TYPE": 1.0}), (54, {"TYPE": -1.0})],
+    "resistor": [(1, {})],
+    "capacitor": [(1, {})],
 }
 
 
@@ -38,6 +41,10 @@ def lognormal(rng, median, sigma, n):
 def instance_params(model, n, rng):
     if model in ("diode", "bjt"):
         return {"AREA": lognormal(rng, 1.0, 0.2, n)}
+    if model == "resistor":
+        return {"R": lognormal(rng, 1000.0, 0.2, n)}
+    if model == "capacitor":
+        return {"C": lognormal(rng, 1e-15, 0.2, n)}
     return {"W": lognormal(rng, 90e-9, 0.2, n), "L": lognormal(rng, 45e-9, 0.2, n)}
 
 
