@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <initializer_list>
 #include <vector>
 #include "xa_internal.h"
 #include "xa_kernel.h"
@@ -80,17 +81,17 @@ constexpr int kLongSegment = 64;
 constexpr int64_t kRailSegment = 512;
 constexpr int kRailChunk = 256;
 
-template <typename I, bool CONTIG>
-__global__ void __launch_bounds__(256)
-k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
-              const I* __restrict__ seg, const I* __restrict__ src,
-              const double* __restrict__ vals, double* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
+// One thread per target: the target's terms added onto the output in order.
+template <typename I>
+__device__ __forceinline__ void add_short(int64_t t, const int32_t* __restrict__ target, int32_t t0,
+                                          bool contig, const I* __restrict__ seg,
+                                          const I* __restrict__ src,
+                                          const double* __restrict__ vals,
+                                          double* __restrict__ out) {
   const I s1 = seg[t + 1];
   I j = seg[t];
   if (s1 - j > kLongSegment) return;   // k_scatter_long adds this one
-  const int64_t o = CONTIG ? t0 + t : target[t];
+  const int64_t o = contig ? t0 + t : target[t];
   double acc = out[o];
   constexpr int U = 8;
   for (; j + U <= s1; j += U) {
@@ -105,6 +106,16 @@ k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
   }
   for (; j < s1; ++j) acc += vals[src[j]];
   out[o] = acc;
+}
+
+template <typename I, bool CONTIG>
+__global__ void __launch_bounds__(256)
+k_scatter_add(int64_t nt, const int32_t* __restrict__ target, int32_t t0,
+              const I* __restrict__ seg, const I* __restrict__ src,
+              const double* __restrict__ vals, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  add_short<I>(t, target, t0, CONTIG, seg, src, vals, out);
 }
 
 // In-order sums of long runs of terms by one wavefront.  A round of ROUND
@@ -182,6 +193,51 @@ k_scatter_long(const int32_t* __restrict__ long_t, const int32_t* __restrict__ t
   const int64_t o = CONTIG ? (int64_t)t0 + t : target[t];
   const double acc = ordered_sum<I, true, ROUND>(out[o], seg[t], seg[t + 1], src, vals, stage);
   if (threadIdx.x == 0) out[o] = acc;
+}
+
+// The segmented plans of several kinds of one call (32-bit indices, distinct
+// outputs) in one launch each for the short and the long segments: block b
+// belongs to the kind whose block range holds it and does for that kind's
+// plan what k_scatter_add / k_scatter_long do, so every sum is the same.
+struct SegKind {
+  int64_t nt;
+  const int32_t* target;   // null: contiguous targets from t0
+  int32_t t0;
+  const int32_t* seg;
+  const int32_t* src;
+  const double* vals;
+  double* out;
+  const int32_t* long_t;
+  int64_t b0, lb0;          // first short / long block of this kind
+};
+struct SegSet {
+  SegKind k[NK];
+  int n;
+};
+
+template <bool LONG>
+__device__ __forceinline__ int seg_kind(const SegSet& s) {
+  int k = 0;
+  while (k + 1 < s.n && (int64_t)blockIdx.x >= (LONG ? s.k[k + 1].lb0 : s.k[k + 1].b0)) ++k;
+  return k;
+}
+
+__global__ void __launch_bounds__(256) k_scatter_add_set(SegSet s) {
+  const SegKind& d = s.k[seg_kind<false>(s)];
+  const int64_t t = ((int64_t)blockIdx.x - d.b0) * blockDim.x + threadIdx.x;
+  if (t >= d.nt) return;
+  add_short<int32_t>(t, d.target, d.t0, d.target == nullptr, d.seg, d.src, d.vals, d.out);
+}
+
+__global__ void __launch_bounds__(64) k_scatter_long_set(SegSet s) {
+  constexpr int ROUND = (int)kRailSegment;
+  __shared__ double stage[ROUND];
+  const SegKind& d = s.k[seg_kind<true>(s)];
+  const int32_t t = d.long_t[(int64_t)blockIdx.x - d.lb0];
+  const int64_t o = d.target ? (int64_t)d.target[t] : (int64_t)d.t0 + t;
+  const double acc =
+      ordered_sum<int32_t, true, ROUND>(d.out[o], d.seg[t], d.seg[t + 1], d.src, d.vals, stage);
+  if (threadIdx.x == 0) d.out[o] = acc;
 }
 
 // One wavefront per rail chunk of at most kRailChunk contributions, summed
@@ -1151,14 +1207,82 @@ int xa_update_secondary_state(xa_group gh, const xa_solver_state* ss, const xa_v
   return g->impl->launch_secondary(*g, *ss, *v);
 }
 
+// The scatters of one call.  When every kind on the call writes its own
+// output vector, the segmented plans with 32-bit indices go in one short and
+// one long launch (k_scatter_add_set / k_scatter_long_set), then their rails;
+// the other kinds (pattern plans, lead assignments, 64-bit plans) follow one
+// by one.  Different kinds write different vectors, so the order among them
+// does not change any bit.
+static int run_scatters(Group& g, const xa_solver_state& ss, const xa_vectors& v,
+                        std::initializer_list<int> kinds) {
+  std::vector<int> live;
+  for (int k : kinds) {
+    const Plan& p = g.plan[k];
+    if (!kind_out(v, k) || p.n_targets == 0) continue;
+    if (g.impl->skip_kind && g.impl->skip_kind(g, ss, k)) continue;
+    live.push_back(k);
+  }
+  bool distinct = true;
+  for (size_t a = 0; a < live.size(); ++a)
+    for (size_t b = a + 1; b < live.size(); ++b)
+      distinct = distinct && kind_out(v, live[a]) != kind_out(v, live[b]);
+  SegSet s{};
+  int64_t blocks = 0, lblocks = 0;
+  std::vector<int> rest;
+  for (int k : live) {
+    const Plan& p = g.plan[k];
+    if (!distinct || kind_is_assign(k) || p.pattern || !p.idx32) {
+      rest.push_back(k);
+      continue;
+    }
+    SegKind& d = s.k[s.n++];
+    d.nt = p.n_targets;
+    d.target = p.d_target;
+    d.t0 = p.t0;
+    d.seg = static_cast<const int32_t*>(p.d_seg);
+    d.src = static_cast<const int32_t*>(p.d_src);
+    d.vals = p.d_vals;
+    d.out = kind_out(v, k);
+    d.long_t = p.d_long;
+    d.b0 = blocks;
+    d.lb0 = lblocks;
+    blocks += grid_for(p.n_targets);
+    lblocks += p.n_long;
+  }
+  hipStream_t st = g.ctx->stream;
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_scatter_add_set, dim3((unsigned)blocks), dim3(kBlock), 0, st, s);
+  if (lblocks > 0)
+    hipLaunchKernelGGL(k_scatter_long_set, dim3((unsigned)lblocks), dim3(64), 0, st, s);
+  for (int k : live) {
+    const Plan& p = g.plan[k];
+    if (!distinct || kind_is_assign(k) || p.pattern || !p.idx32 || p.n_rails == 0) continue;
+    const int32_t* src = static_cast<const int32_t*>(p.d_src);
+    hipLaunchKernelGGL((k_rail_partial<int32_t>), dim3((unsigned)p.n_chunks), dim3(64), 0, st,
+                       static_cast<const int32_t*>(p.d_chunk_j0),
+                       static_cast<const int32_t*>(p.d_chunk_j1), src, p.d_vals, p.d_partial);
+    if (p.d_target == nullptr)
+      hipLaunchKernelGGL((k_rail_sum<true>), dim3((unsigned)p.n_rails), dim3(64), 0, st,
+                         p.d_rail_t, p.d_rail_c, p.d_target, p.t0, p.d_partial, kind_out(v, k));
+    else
+      hipLaunchKernelGGL((k_rail_sum<false>), dim3((unsigned)p.n_rails), dim3(64), 0, st,
+                         p.d_rail_t, p.d_rail_c, p.d_target, p.t0, p.d_partial, kind_out(v, k));
+  }
+  XA_HIP(hipGetLastError());
+  int rc;
+  for (int k : rest)
+    if ((rc = run_scatter(g, ss, v, k)) != XA_OK) return rc;
+  return XA_OK;
+}
+
 int xa_load_vectors(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   Group* g = reinterpret_cast<Group*>(gh);
   int rc = ready(g);
   if (rc != XA_OK) return rc;
   if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
   tic(g, 1);
-  for (int k : {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ})
-    if ((rc = run_scatter(*g, *ss, *v, k)) != XA_OK) return rc;
+  if ((rc = run_scatters(*g, *ss, *v, {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ})) != XA_OK)
+    return rc;
   toc(g, 1);
   return XA_OK;
 }
@@ -1169,8 +1293,7 @@ int xa_load_matrices(xa_group gh, const xa_solver_state* ss, const xa_vectors* v
   if (rc != XA_OK) return rc;
   if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
   tic(g, 2);
-  for (int k : {K_DFDX, K_DQDX})
-    if ((rc = run_scatter(*g, *ss, *v, k)) != XA_OK) return rc;
+  if ((rc = run_scatters(*g, *ss, *v, {K_DFDX, K_DQDX})) != XA_OK) return rc;
   toc(g, 2);
   return XA_OK;
 }
