@@ -289,8 +289,10 @@ int xa_group_last_times(xa_group g, float* ms_update, float* ms_vectors,
  * with dc = 1:  rhs = -f + dFdxdVp, J = dFdx.
  * Rows and columns of held nodes (held[i] != 0) are left out.  x += J^-1 rhs
  * on the free nodes (banded LU with partial pivoting); dx_max[c] = the
- * circuit's largest |update| (NaN for a circuit outside the limits or with an
- * entry outside the band).  All pointers are device pointers; f, q, the
+ * circuit's largest |update| (NaN for a circuit outside the limits, with an
+ * entry outside the band, or with a zero pivot -- a singular Jacobian, such
+ * as a floating node -- in which case its x is left unchanged; a NaN update
+ * also gives NaN, never a finite maximum).  All pointers are device pointers; f, q, the
  * limiter vectors and the CSR values are those of a load of the batch. */
 int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
                          const int32_t* c_start, const int32_t* pos,

@@ -158,7 +158,14 @@ class Master {
           put(v.lastSta, ext_.lastStaVectorRawPtr, n_sta_) && put(v.nextSto, stoVec, n_sto_) &&
           put(v.currSto, ext_.currStoVectorRawPtr, n_sto_);
     if (!ok_) return false;
-    v.flagSol = ext_.flagSolVectorRawPtr ? v.flagSol : nullptr;
+    // a vector the host does not have reaches the kernels as null (never as
+    // the device buffer's values from an earlier call); the loads reject a
+    // null vector where the solver state needs it
+    if (!ext_.flagSolVectorRawPtr) v.flagSol = nullptr;
+    if (!ext_.currSolVectorRawPtr) v.currSol = nullptr;
+    if (!ext_.currStaVectorRawPtr) v.currSta = nullptr;
+    if (!ext_.lastStaVectorRawPtr) v.lastSta = nullptr;
+    if (!ext_.currStoVectorRawPtr) v.currSto = nullptr;
     if (!run(xa_update_state(g_, &ss_, &v), "xa_update_state")) return false;
     // the state slots this type writes, and the initial-transient copies
     // into currSta / lastSta (N_DEV_MOSFET_B4.C:16254-16289, N_DEV_BJT.C:3923-3933)

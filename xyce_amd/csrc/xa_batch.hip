@@ -63,6 +63,12 @@ __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Largest of two magnitudes, NaN when either is NaN (fmax would drop a NaN
+// and let a circuit whose update is not finite pass as converged).
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (a != a || b != b) ? (double)NAN : (a > b ? a : b);
+}
+
 __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
   extern __shared__ double sm[];
   __shared__ int s_bad;
@@ -124,7 +130,13 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
       const double v = fabs(AB[(kv + i) * n + j]);
       if (v > best) { best = v; jp = i; }
     }
-    if (AB[(kv + jp) * n + j] == 0.0) continue;   // singular column: no elimination
+    // A zero pivot means a singular Jacobian (a floating node, a cut-off
+    // branch): the circuit gets no update and reports NaN.  Every lane read
+    // the same column, so every lane leaves here together.
+    if (AB[(kv + jp) * n + j] == 0.0) {
+      if (tid == 0) b.dx_max[c] = NAN;
+      return;
+    }
     ju = max(ju, min(j + ku + jp, n - 1));
     if (jp != 0) {
       wave_fence();   // every lane has read the pivot column before the swap
@@ -171,10 +183,10 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
     if (!b.held[gi]) {
       const double d = rhs[b.pos ? b.pos[gi] : i];
       b.x[gi] += d;
-      mx = fmax(mx, fabs(d));
+      mx = nan_max(mx, fabs(d));
     }
   }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off));
+  for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off));
   if (tid == 0) b.dx_max[c] = mx;
 }
 

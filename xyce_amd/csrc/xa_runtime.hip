@@ -1383,9 +1383,10 @@ static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, i
   if (path == 2) return g.impl->launch_update(g, ss, v, true);
   if ((rc = g.impl->launch_update(g, ss, v, false)) != XA_OK) return rc;
   if (g.impl->launch_secondary && (rc = g.impl->launch_secondary(g, ss, v)) != XA_OK) return rc;
-  for (int k : {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ, K_DFDX, K_DQDX})
-    if ((rc = run_scatter(g, ss, v, k)) != XA_OK) return rc;
-  return XA_OK;
+  // the same merged launches as xa_load_vectors and xa_load_matrices
+  if ((rc = run_scatters(g, ss, v, {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ})) != XA_OK)
+    return rc;
+  return run_scatters(g, ss, v, {K_DFDX, K_DQDX});
 }
 
 // Mean milliseconds of one pass over `reps` passes after one untimed pass.

@@ -3,16 +3,20 @@
 * ``logic_dag``: a flat logic netlist -- a random DAG of INV / NAND2 / NOR2
   cells of BSIM4 transistors (two shared cards, two shared sizes), each net a
   driver node and a far-end node joined by a wire resistor, fanout 1-4, 20
-  primary inputs.  The wire resistors and capacitors are linear devices
-  outside these groups; here they only shape the node structure.
+  primary inputs.  ``logic_wires`` gives that netlist's wires: one resistor
+  per net from the driver to the far end (R lognormal, median 50 ohm, sigma
+  0.5) and one capacitor from the far end to ground (C uniform 0.5-5 fF).
 * ``hb_diffamp``: the time samples of a harmonic-balance analysis of a
   differential amplifier: a 2k-transistor amplifier replicated once per time
   sample, every copy with its own nodes (the HB loader evaluates each
-  sample's devices at that sample's voltages).
+  sample's devices at that sample's voltages).  ``hb_filter`` gives each
+  copy's output RC filter (a series resistor from each output of the last
+  cell to a filter node, and a capacitor from that node to ground).
 
-Each returns (models, model_index, params, node_lids, n_nodes, voltages):
-instance-major external node LIDs (D, G, S, B; ground -1) and one voltage
-per node, ready for :class:`xyce_amd.workloads.Workload`.
+Each transistor netlist returns (models, model_index, params, node_lids,
+n_nodes, voltages): instance-major external node LIDs (D, G, S, B; ground
+-1) and one voltage per node, ready for :class:`xyce_amd.workloads.Workload`.
+The wire functions return two-terminal device lists (node pairs, values).
 """
 import numpy as np
 
@@ -20,8 +24,12 @@ SEED = 20261203
 CARDS = [(54, {"TYPE": 1.0}), (54, {"TYPE": -1.0})]
 
 
-def logic_dag(n_transistors=200_000, n_inputs=20, seed=SEED, vdd=1.0, window=2000,
-              wn=90e-9, wp=180e-9, l=45e-9):
+def logic_dag(n_transistors=200_000, n_inputs=20, seed=SEED, **kw):
+    return _logic(n_transistors, n_inputs, seed, **kw)[0]
+
+
+def _logic(n_transistors, n_inputs, seed, vdd=1.0, window=2000, wn=90e-9, wp=180e-9, l=45e-9):
+    """(transistor netlist, nets[k] = (driver, far end), primary-input nodes)"""
     rng = np.random.default_rng(np.random.SeedSequence(seed))
     VDD = 0
     nxt = 1
@@ -36,6 +44,7 @@ def logic_dag(n_transistors=200_000, n_inputs=20, seed=SEED, vdd=1.0, window=200
         w.append(wp if pmos else wn)
         lids.extend([d, g, s, VDD if pmos else -1])
 
+    nets = []          # (driver node, far-end node) of every cell output
     count = 0
     while count < n_transistors:
         kind = int(rng.integers(0, 3))          # 0 INV, 1 NAND2, 2 NOR2
@@ -71,22 +80,44 @@ def logic_dag(n_transistors=200_000, n_inputs=20, seed=SEED, vdd=1.0, window=200
             fet(False, out, ins[1], -1)
         count += 2 if kind == 0 else 4
         pool.append([far, int(rng.integers(1, 5))])
+        nets.append((out, far))
     n = len(mi)
     params = {"W": np.asarray(w), "L": np.full(n, l)}
     v = rng.uniform(0.0, vdd, nxt)
     v[VDD] = vdd
-    return CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt, v
+    fets = (CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt, v)
+    return fets, np.asarray(nets, np.int32).reshape(-1, 2), np.arange(1, 1 + n_inputs,
+                                                                     dtype=np.int32)
 
 
-def hb_diffamp(samples=128, n_transistors=2000, seed=SEED, vdd=1.0, l=45e-9):
-    """One copy per time sample of an amplifier of n_transistors/5 cells.
+def logic_wires(n_transistors=200_000, n_inputs=20, seed=SEED, **kw):
+    """Wires of ``logic_dag`` with the same arguments: returns
+    (r_nodes[n, 2], R[n], c_nodes[n, 2], C[n], input_nodes): resistor k from
+    net k's driver to its far end, capacitor k from the far end to ground
+    (-1), and the primary-input nodes.  The values come from their own seeded
+    stream, so the transistors are those of ``logic_dag``."""
+    _, nets, inputs = _logic(n_transistors, n_inputs, seed, **kw)
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 1]))
+    R = 50.0 * np.exp(0.5 * rng.standard_normal(len(nets)))
+    C = rng.uniform(0.5e-15, 5e-15, len(nets))
+    c_nodes = np.stack([nets[:, 1], np.full(len(nets), -1, np.int32)], axis=1)
+    return nets, R, c_nodes, C, inputs
+
+
+def hb_diffamp(samples=128, n_transistors=2000, seed=SEED, **kw):
+    return _diffamp(samples, n_transistors, seed, **kw)[0]
+
+
+def _diffamp(samples, n_transistors, seed, vdd=1.0, l=45e-9):
+    """(transistor netlist, outputs[s] = the last cell's two outputs of copy s).
+    One copy per time sample of an amplifier of n_transistors/5 cells.
     Cell: differential pair (NMOS, gates on the two inputs), PMOS
     current-mirror load, NMOS tail source gated by a shared bias node; cell
     outputs feed the next cell's inputs.  Copy s sees the inputs of sample s
     of a two-tone (1.00/1.01 GHz, 10 mV) drive around mid-rail."""
     rng = np.random.default_rng(np.random.SeedSequence(seed))
     cells = n_transistors // 5
-    mi, w, lids, volts = [], [], [], []
+    mi, w, lids, volts, outs = [], [], [], [], []
     nxt = 0
     t = np.arange(samples) / samples * 100e-9        # 100 ns: one beat period
     drive = 0.01 * (np.sin(2 * np.pi * 1.00e9 * t) + np.sin(2 * np.pi * 1.01e9 * t))
@@ -110,7 +141,24 @@ def hb_diffamp(samples=128, n_transistors=2000, seed=SEED, vdd=1.0, l=45e-9):
                 lids.extend([d, g, src, VDD if pmos else -1])
             a, b = outp, outn
         volts.extend(node_v)
+        outs.append((a, b))
     n = len(mi)
     params = {"W": np.asarray(w), "L": np.full(n, l)}
-    return CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt, \
-        np.asarray(volts)
+    fets = (CARDS, np.asarray(mi, np.int32), params, np.asarray(lids, np.int32), nxt,
+            np.asarray(volts))
+    return fets, np.asarray(outs, np.int32).reshape(-1, 2)
+
+
+def hb_filter(samples=128, n_transistors=2000, seed=SEED, r=1000.0, c=50e-15, **kw):
+    """Output RC filter of every copy of ``hb_diffamp`` with the same
+    arguments: returns (r_nodes, R, c_nodes, C, n_nodes, voltages) -- per
+    copy a resistor from each of the last cell's two outputs to a new filter
+    node, and a capacitor from each filter node to ground; the node count and
+    voltages cover the amplifier's nodes followed by the filter nodes."""
+    (_, _, _, _, n0, v0), outs = _diffamp(samples, n_transistors, seed, **kw)
+    k = 2 * len(outs)
+    filt = n0 + np.arange(k, dtype=np.int32)
+    r_nodes = np.stack([outs.reshape(-1), filt], axis=1).astype(np.int32)
+    c_nodes = np.stack([filt, np.full(k, -1, np.int32)], axis=1)
+    volts = np.concatenate([v0, v0[outs.reshape(-1)]])
+    return r_nodes, np.full(k, r), c_nodes, np.full(k, c), n0 + k, volts
