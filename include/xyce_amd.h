@@ -218,6 +218,21 @@ int xa_load_matrices(xa_group g, const xa_solver_state* ss, const xa_vectors* v)
  * calls in sequence). */
 int xa_load_all(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
 
+/* xa_load_all without any device allocation: no reference counterpart.  For
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It runs exactly the
+ * route xa_load_all takes, with the same results, but the call's scratch --
+ * every output kind's stamp value buffer (entries x instances doubles) and
+ * the supply-rail partial sums -- lives in one static device array of
+ * 4 Mi doubles (32 MiB) per GPU, held from library load to process end,
+ * instead of the group's buffers.  XA_ERR_ARG when the group needs more than
+ * that (the message gives the size).  The array is shared by every call on
+ * one GPU and keeps the last call's contents; it is cleared by a fill kernel
+ * at the start of each call.  Calls that overlap on one GPU (different
+ * streams or threads) race on it.  Calls no hipMalloc/hipFree and creates no
+ * timing events (group timing is not recorded for this call). */
+int xa_load_all_noalloc(xa_group g, const xa_solver_state* ss, const xa_vectors* v);
+
 /* Autotuner of the block shapes (no reference counterpart).  Times, on the
  * caller's vectors and flags, the four-call path with update-kernel blocks of
  * 64, 128 and 256 threads, and the fused one-kernel load with blocks of 64,

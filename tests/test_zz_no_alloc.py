@@ -1,0 +1,69 @@
+"""xa_load_all_noalloc (the load whose scratch lives in the library's static
+device array) against xa_load_all on twin groups: bit-identical outputs on
+two calls in a row, the second on a smaller group (so the array holds the
+first call's larger contents when it starts), and a clear error for a group
+larger than the array."""
+import numpy as np
+import pytest
+
+from harness import OUT_VECS, STATE_VECS, Pair
+from specs import BSIM4_CARDS, bsim4_spec, diode_spec
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = [dict(newtonIter=0, dcopFlag=1, initJctFlag=1),
+         dict(newtonIter=1, transientFlag=1, pdt=2e9)]
+
+
+def _gpu_load(p, method, flags):
+    for k in OUT_VECS + ("dFdx", "dQdx"):
+        p.d[k].zero_()
+    getattr(p.gpu, method)(p._ss("gpu", **flags), p._vecs("gpu"))
+    p.torch.cuda.synchronize()
+    return {k: p.d[k][:len(p.h[k])].cpu().numpy().copy()
+            for k in OUT_VECS + STATE_VECS + ("dFdx", "dQdx")}
+
+
+def _twins(model, spec, seed, path, pool):
+    kw = dict(topology="shared", seed=seed, ground_frac=0.05, n_shared_nodes=pool)
+    pa = Pair(model, spec, **kw)
+    pb = Pair(model, spec, **kw)
+    if path:
+        for p in (pa, pb):
+            p.gpu.set_path(path, 256)
+    return pa, pb
+
+
+@pytest.mark.parametrize("model,sizes,path,pool", [
+    ("bsim4", (3000, 700), None, None),   # BSIM4 takes the four calls: value buffers in scratch
+    ("diode", (20000, 5000), "split", 16),  # rows of >512 terms: rail partial sums in scratch too
+    ("diode", (4000, 1000), None, None),  # the fused route: no scratch used
+])
+def test_noalloc_equals_load_all(model, sizes, path, pool):
+    rng = np.random.default_rng(11)
+    for call, n in enumerate(sizes):
+        spec = (bsim4_spec(n, seed=40 + call, cards=BSIM4_CARDS) if model == "bsim4"
+                else diode_spec(n, seed=40 + call))
+        pa, pb = _twins(model, spec, 50 + call, path, pool)
+        print(f"{model} call {call}: {n} instances, path {path or 'default'}", flush=True)
+        cs = rng.uniform(-0.5, 1.5, pa.topo.n_sto)
+        for p in (pa, pb):
+            p.set_inputs(currSto=cs)
+        for flags in FLAGS:
+            x = rng.uniform(-0.5, 1.5, pa.topo.n_nodes)
+            pa.set_inputs(nextSol=x)
+            pb.set_inputs(nextSol=x)
+            ra = _gpu_load(pa, "loadAll", flags)
+            rb = _gpu_load(pb, "loadAllNoAlloc", flags)
+            for k in ra:
+                np.testing.assert_array_equal(ra[k].view(np.int64), rb[k].view(np.int64),
+                                              err_msg=f"call {call} {flags} {k}")
+
+
+def test_noalloc_rejects_group_beyond_scratch():
+    import xyce_amd
+    n = 60000   # 74 stamp rows per BSIM4 instance: 4.4 Mi doubles > 4 Mi
+    p = Pair("bsim4", bsim4_spec(n, seed=3, cards=BSIM4_CARDS), topology="private", seed=4)
+    with pytest.raises(xyce_amd.XaError, match="scratch"):
+        p.gpu.loadAllNoAlloc(p._ss("gpu", newtonIter=1, transientFlag=1, pdt=2e9),
+                             p._vecs("gpu"))

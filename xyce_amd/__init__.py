@@ -197,6 +197,12 @@ class Master:
     def loadAll(self, ss, vectors):
         check(lib().xa_load_all(self.h, C.byref(ss), C.byref(vectors)), "loadAll")
 
+    def loadAllNoAlloc(self, ss, vectors):
+        """xa_load_all_noalloc: loadAll with its scratch in the library's
+        static device array (no allocation; groups up to its size)."""
+        check(lib().xa_load_all_noalloc(self.h, C.byref(ss), C.byref(vectors)),
+              "loadAllNoAlloc")
+
     def plan_info(self, kind):
         """(targets, contributions, pattern slots per instance or 0) of the
         scatter plan of output kind `kind` (0 F, 1 Q, 2 dFdxdVp, 3 dQdxdVp,

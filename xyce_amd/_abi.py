@@ -81,6 +81,7 @@ def _proto(lib):
         "xa_load_vectors": (C.c_int, [vp, P(SolverState), P(Vectors)]),
         "xa_load_matrices": (C.c_int, [vp, P(SolverState), P(Vectors)]),
         "xa_load_all": (C.c_int, [vp, P(SolverState), P(Vectors)]),
+        "xa_load_all_noalloc": (C.c_int, [vp, P(SolverState), P(Vectors)]),
         "xa_group_failures": (C.c_int, [vp, P(i64)]),
         "xa_group_plan_info": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(C.c_int)]),
         "xa_group_set_path": (C.c_int, [vp, C.c_int, C.c_int]),

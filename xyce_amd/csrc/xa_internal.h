@@ -198,6 +198,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  double* scratch = nullptr;   // this device's address of the no-alloc load's scratch
 };
 
 // error reporting

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 #include <initializer_list>
+#include <mutex>
 #include <vector>
 #include "xa_internal.h"
 #include "xa_kernel.h"
@@ -15,6 +16,33 @@
 namespace xa {
 
 static thread_local std::string g_err;
+
+// Scratch of the allocation-free load (xa_load_all_noalloc): the stamp value
+// buffers and supply-rail partial sums of one call, in a file-scope device
+// array that every GPU the library runs on holds until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it names a
+// stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once per device at context creation, and only
+// that address is used.
+constexpr size_t kScratchDoubles = (size_t)4 << 20;   // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Resolves (once) the scratch address on `device`, which must be current.
+static int resolve_scratch(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(XA_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return XA_OK;
+}
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -522,8 +550,12 @@ int xa_ctx_create(int device_ordinal, xa_ctx* out) {
   if (device_ordinal < 0 || device_ordinal >= ndev)
     return fail(XA_ERR_ARG, "xa_ctx_create: no such device");
   XA_HIP(hipSetDevice(device_ordinal));
+  double* scr = nullptr;
+  int rc = resolve_scratch(device_ordinal, &scr);
+  if (rc != XA_OK) return rc;
   Ctx* c = new Ctx();
   c->device = device_ordinal;
+  c->scratch = scr;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_check(e, "hipStreamCreate"); }
   c->own_stream = true;
@@ -1298,12 +1330,10 @@ int xa_load_matrices(xa_group gh, const xa_solver_state* ss, const xa_vectors* v
   return XA_OK;
 }
 
-int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
-  Group* g = reinterpret_cast<Group*>(gh);
-  int rc = ready(g);
-  if (rc != XA_OK) return rc;
-  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
-  if (g->n == 0) return XA_OK;
+// The route xa_load_all takes (timed when the group's timing is on).
+static int load_all(Group* g, const xa_solver_state* ss, const xa_vectors* v) {
+  int rc;
+  xa_group gh = reinterpret_cast<xa_group>(g);
   // the fused kernel needs every target block-private and no secondary pass;
   // it is taken for launch-bound sizes where the model's fused kernel is the
   // faster (fused_small) and at scale where fused_at_scale says so; otherwise
@@ -1328,6 +1358,76 @@ int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
   if ((rc = xa_update_secondary_state(gh, ss, v)) != XA_OK) return rc;
   if ((rc = xa_load_vectors(gh, ss, v)) != XA_OK) return rc;
   return xa_load_matrices(gh, ss, v);
+}
+
+int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0) return XA_OK;
+  return load_all(g, ss, v);
+}
+
+__global__ void k_zero(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// xa_load_all with the call's scratch -- every kind's stamp value buffer
+// (E doubles per instance) and supply-rail partial sums -- laid out in the
+// device's `scratch` array instead of the group's own buffers.  The plans'
+// pointers are swapped for the call and restored before returning (the
+// kernels take them by value at launch).  The used range is cleared first by
+// a fill kernel, so no entry starts from an earlier call's contents.
+int xa_load_all_noalloc(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (!ss || !v) return fail(XA_ERR_ARG, "null solver state / vectors");
+  if (g->n == 0) return XA_OK;
+  double* base = g->ctx->scratch;
+  if (!base) return fail(XA_ERR_STATE, "no-alloc load: scratch address not resolved");
+  size_t need = 0;
+  for (int k = 0; k < NK; ++k) {
+    const Plan& p = g->plan[k];
+    if (p.d_vals) need += (size_t)p.E * (size_t)g->n;
+    if (p.d_partial) need += (size_t)p.n_chunks;
+  }
+  if (need > kScratchDoubles)
+    return fail(XA_ERR_ARG, "no-alloc load: the group needs " + std::to_string(need) +
+                                " doubles of scratch; the maximum is " +
+                                std::to_string(kScratchDoubles) +
+                                " (use xa_load_all for larger groups)");
+  double* saved_vals[NK];
+  double* saved_part[NK];
+  size_t off = 0;
+  for (int k = 0; k < NK; ++k) {
+    Plan& p = g->plan[k];
+    saved_vals[k] = p.d_vals;
+    saved_part[k] = p.d_partial;
+    if (p.d_vals) { p.d_vals = base + off; off += (size_t)p.E * (size_t)g->n; }
+    if (p.d_partial) { p.d_partial = base + off; off += (size_t)p.n_chunks; }
+  }
+  if (off > 0) {
+    const unsigned blocks = (unsigned)std::min<size_t>((off + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(kBlock), 0, g->ctx->stream, base,
+                       (int64_t)off);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_check(e, "no-alloc load: clearing scratch");
+  }
+  if (rc == XA_OK) {
+    const bool timing = g->timing;
+    g->timing = false;   // timing records events, which are created on demand
+    rc = load_all(g, ss, v);
+    g->timing = timing;
+  }
+  for (int k = 0; k < NK; ++k) {
+    g->plan[k].d_vals = saved_vals[k];
+    g->plan[k].d_partial = saved_part[k];
+  }
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
