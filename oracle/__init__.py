@@ -16,6 +16,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# "pm": the parity build, whose evaluation takes exp/log/pow/tan from
+# xyce_amd/csrc/xa_pmath.h (the functions of the GPU library's parity build)
+VARIANT_PATHS = {"": LIB_PATH, "pm": os.path.join(_HERE, "liboracle_pm.so")}
 MODEL_TYPES = {"diode": 1, "mosfet1": 2, "bjt": 3, "bsim4": 4, "resistor": 5, "capacitor": 6}
 
 
@@ -45,15 +48,16 @@ class Vectors(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in VECTOR_FIELDS]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
-        L = C.CDLL(LIB_PATH)
+def lib(variant=""):
+    L = _libs.get(variant)
+    if L is None:
+        path = VARIANT_PATHS[variant]
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run make -C oracle")
+        L = C.CDLL(path)
         vp = C.c_void_p
         P = C.POINTER
         for name, res, args in [
@@ -75,6 +79,7 @@ def lib():
             ("or_read_intermediate", C.c_int, [vp, C.c_char_p, P(C.c_double)]),
             ("or_last_error", C.c_char_p, []),
             ("or_set_threads", None, [C.c_int]),
+            ("or_math", C.c_double, [C.c_int, C.c_double, C.c_double]),
             ("or_default_options", None, [P(DeviceOptions)]),
             ("or_set_magnitude_mode", None, [C.c_int]),
             ("or_set_libm_perturbation", None, [C.c_int, C.c_uint64]),
@@ -84,8 +89,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return L
 
 
 def set_threads(n):
@@ -93,9 +98,9 @@ def set_threads(n):
     lib().or_set_threads(int(n))
 
 
-def check(rc, what=""):
+def check(rc, what="", L=None):
     if rc != 0:
-        raise RuntimeError(f"oracle {what}: rc={rc}: {lib().or_last_error().decode()}")
+        raise RuntimeError(f"oracle {what}: rc={rc}: {(L or lib()).or_last_error().decode()}")
 
 
 def default_options(**kw):
@@ -129,11 +134,12 @@ def make_vectors(**arrays):
 class Master:
     """Oracle group with the product Master's call shape (host arrays)."""
 
-    def __init__(self, model_type, options=None):
+    def __init__(self, model_type, options=None, variant=""):
+        self.L = lib(variant)
         self.type = MODEL_TYPES[model_type] if isinstance(model_type, str) else model_type
         self.h = C.c_void_p()
         opt = options if options is not None else default_options()
-        check(lib().or_group_create(self.type, C.byref(opt), C.byref(self.h)), "create")
+        check(self.L.or_group_create(self.type, C.byref(opt), C.byref(self.h)), "create", self.L)
         self._keep = []
         self._n = 0
 
@@ -141,8 +147,8 @@ class Master:
         names = list(params.keys())
         vals = (C.c_double * max(len(names), 1))(*[float(params[k]) for k in names])
         idx = C.c_int()
-        check(lib().or_group_add_model(self.h, level, len(names), _cstrs(names), vals, C.byref(idx)),
-              "add_model")
+        check(self.L.or_group_add_model(self.h, level, len(names), _cstrs(names), vals, C.byref(idx)),
+              "add_model", self.L)
         return idx.value
 
     def add_instances(self, n, model_index=None, params=None):
@@ -157,16 +163,16 @@ class Master:
             mi_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(model_index, np.int32), (n,)))
             self._keep.append(mi_arr)
             mi = _p(mi_arr, C.c_int)
-        check(lib().or_group_add_instances(self.h, n, mi, len(names), _cstrs(names), ptrs),
-              "add_instances")
+        check(self.L.or_group_add_instances(self.h, n, mi, len(names), _cstrs(names), ptrs),
+              "add_instances", self.L)
         self._n += n
 
     def request_lead_current(self, flags):
         f = np.ascontiguousarray(flags, np.int32)
-        check(lib().or_group_request_lead_current(self.h, _p(f, C.c_int32)), "lead")
+        check(self.L.or_group_request_lead_current(self.h, _p(f, C.c_int32)), "lead", self.L)
 
     def setup(self):
-        check(lib().or_group_setup(self.h), "setup")
+        check(self.L.or_group_setup(self.h), "setup", self.L)
 
     @property
     def n(self):
@@ -174,58 +180,58 @@ class Master:
 
     def counts(self):
         out = [np.zeros(self._n, np.int32) for _ in range(4)]
-        check(lib().or_group_counts(self.h, *[_p(o, C.c_int) for o in out]), "counts")
+        check(self.L.or_group_counts(self.h, *[_p(o, C.c_int) for o in out]), "counts", self.L)
         return out
 
     def register_lids(self, node_lids, sta_lids, sto_lids):
         arrs = [np.ascontiguousarray(a, np.int32) for a in (node_lids, sta_lids, sto_lids)]
-        check(lib().or_group_register_lids(self.h, *[_p(a, C.c_int32) for a in arrs]), "lids")
+        check(self.L.or_group_register_lids(self.h, *[_p(a, C.c_int32) for a in arrs]), "lids", self.L)
 
     def jac_pairs(self):
         cnt = C.c_int64()
-        check(lib().or_group_jac_pairs(self.h, C.byref(cnt), None, None), "jac_pairs")
+        check(self.L.or_group_jac_pairs(self.h, C.byref(cnt), None, None), "jac_pairs", self.L)
         r = np.zeros(cnt.value, np.int32)
         c = np.zeros(cnt.value, np.int32)
-        check(lib().or_group_jac_pairs(self.h, C.byref(cnt), _p(r, C.c_int32), _p(c, C.c_int32)),
-              "jac_pairs")
+        check(self.L.or_group_jac_pairs(self.h, C.byref(cnt), _p(r, C.c_int32), _p(c, C.c_int32)),
+              "jac_pairs", self.L)
         return r, c
 
     def register_csr(self, row_ptr, col_idx):
         self._rp = np.ascontiguousarray(row_ptr, np.int32)
         self._ci = np.ascontiguousarray(col_idx, np.int32)
-        check(lib().or_group_register_csr(self.h, len(self._rp) - 1, _p(self._rp, C.c_int32),
-                                          _p(self._ci, C.c_int32)), "csr")
+        check(self.L.or_group_register_csr(self.h, len(self._rp) - 1, _p(self._rp, C.c_int32),
+                                          _p(self._ci, C.c_int32)), "csr", self.L)
 
     def finalize(self):
-        check(lib().or_group_finalize(self.h), "finalize")
+        check(self.L.or_group_finalize(self.h), "finalize", self.L)
 
     def updateState(self, ss, v):
-        check(lib().or_update_state(self.h, C.byref(ss), C.byref(v)), "updateState")
+        check(self.L.or_update_state(self.h, C.byref(ss), C.byref(v)), "updateState", self.L)
 
     def updateSecondaryState(self, ss, v):
-        check(lib().or_update_secondary_state(self.h, C.byref(ss), C.byref(v)), "updateSecondary")
+        check(self.L.or_update_secondary_state(self.h, C.byref(ss), C.byref(v)), "updateSecondary", self.L)
 
     def loadDAEVectors(self, ss, v):
-        check(lib().or_load_vectors(self.h, C.byref(ss), C.byref(v)), "loadDAEVectors")
+        check(self.L.or_load_vectors(self.h, C.byref(ss), C.byref(v)), "loadDAEVectors", self.L)
 
     def loadDAEMatrices(self, ss, v):
-        check(lib().or_load_matrices(self.h, C.byref(ss), C.byref(v)), "loadDAEMatrices")
+        check(self.L.or_load_matrices(self.h, C.byref(ss), C.byref(v)), "loadDAEMatrices", self.L)
 
     def save_state(self):
         """Snapshot the instance records (values kept between calls)."""
-        check(lib().or_group_save_state(self.h), "save_state")
+        check(self.L.or_group_save_state(self.h), "save_state", self.L)
 
     def restore_state(self):
-        check(lib().or_group_restore_state(self.h), "restore_state")
+        check(self.L.or_group_restore_state(self.h), "restore_state", self.L)
 
     def read(self, name):
         out = np.zeros(self._n, np.float64)
-        check(lib().or_read_intermediate(self.h, name.encode(), _p(out)), "read")
+        check(self.L.or_read_intermediate(self.h, name.encode(), _p(out)), "read", self.L)
         return out
 
     def close(self):
         if self.h:
-            lib().or_group_destroy(self.h)
+            self.L.or_group_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

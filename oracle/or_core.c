@@ -62,6 +62,19 @@ int or_fail(int code, const char* msg) {
 const char* or_last_error(void) { return or_err; }
 void or_set_threads(int n) { or_threads = n > 0 ? n : 1; }
 
+/* The evaluation's math functions as this build resolves them (the C
+ * library's, or in the parity build the portable ones): 0 exp(a), 1 log(a),
+ * 2 pow(a, b), 3 tan(a).  For the accuracy tests of the portable functions. */
+double or_math(int which, double a, double b) {
+  switch (which) {
+    case 0: return OR_EXP(a);
+    case 1: return OR_LOG(a);
+    case 2: return OR_POW(a, b);
+    case 3: return OR_TAN(a);
+  }
+  return NAN;
+}
+
 /* DeviceOptions defaults (N_DEV_DeviceOptions.C:66-140) */
 void or_default_options(xa_device_options* o) {
   memset(o, 0, sizeof(*o));

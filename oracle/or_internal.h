@@ -121,10 +121,24 @@ extern int or_mag;
 extern int or_libm_ulps;
 extern uint64_t or_libm_seed;
 double or_perturb(double r, double a, double b);
-static inline double or_exp(double a) { double r = exp(a); return or_libm_ulps ? or_perturb(r, a, 0.0) : r; }
-static inline double or_log(double a) { double r = log(a); return or_libm_ulps ? or_perturb(r, a, 1.0) : r; }
-static inline double or_pow(double a, double b) { double r = pow(a, b); return or_libm_ulps ? or_perturb(r, a, b) : r; }
-static inline double or_tan(double a) { double r = tan(a); return or_libm_ulps ? or_perturb(r, a, 2.0) : r; }
+/* The parity build (OR_PORTABLE_MATH) takes the portable functions the GPU
+ * library's parity build uses (xyce_amd/csrc/xa_pmath.h) instead of libm's. */
+#ifdef OR_PORTABLE_MATH
+#include "../xyce_amd/csrc/xa_pmath.h"
+#define OR_EXP xa_pm_exp
+#define OR_LOG xa_pm_log
+#define OR_POW xa_pm_pow
+#define OR_TAN xa_pm_tan
+#else
+#define OR_EXP exp
+#define OR_LOG log
+#define OR_POW pow
+#define OR_TAN tan
+#endif
+static inline double or_exp(double a) { double r = OR_EXP(a); return or_libm_ulps ? or_perturb(r, a, 0.0) : r; }
+static inline double or_log(double a) { double r = OR_LOG(a); return or_libm_ulps ? or_perturb(r, a, 1.0) : r; }
+static inline double or_pow(double a, double b) { double r = OR_POW(a, b); return or_libm_ulps ? or_perturb(r, a, b) : r; }
+static inline double or_tan(double a) { double r = OR_TAN(a); return or_libm_ulps ? or_perturb(r, a, 2.0) : r; }
 static inline double or_min(double a, double b) { return a < b ? a : b; }
 static inline double or_max(double a, double b) { return a > b ? a : b; }
 

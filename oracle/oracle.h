@@ -49,6 +49,7 @@ const char* or_last_error(void);
 /* OpenMP threads for the updateState loops (instances are independent there;
  * 1 = the serial order).  The loads that add into shared rows stay serial. */
 void or_set_threads(int n);
+double or_math(int which, double a, double b);
 void or_default_options(xa_device_options* o);
 /* 1: loads accumulate |contribution| (error scale of each entry). */
 void or_set_magnitude_mode(int on);

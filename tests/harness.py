@@ -97,8 +97,11 @@ class Pair:
 
     def __init__(self, model_type, spec, options=None, topology="private", seed=1,
                  lead=None, gpu=True, ground_frac=0.0, n_shared_nodes=None,
-                 noise_runs=0, noise_ulps=2):
+                 noise_runs=0, noise_ulps=2, variant=""):
+        """variant "pm": both back ends in their parity builds (shared
+        portable exp/log/pow/tan), where every output must match bit for bit."""
         import oracle
+        self.variant = variant
         self.noise_runs = noise_runs
         self.noise_ulps = noise_ulps
         self.noise = {}
@@ -106,7 +109,8 @@ class Pair:
         self.rng = splitmix64(seed)
         self.spec = spec
         self.orc = oracle.Master(model_type, _copy_opts(options, oracle.DeviceOptions)
-                                 if options else None)
+                                 if options else None, variant=variant)
+        self.olib = self.orc.L
         counts = build_group(self.orc, spec, lead)
         self.counts = counts
         self.topo = Topology(counts, topology, self.rng, n_shared_nodes, ground_frac)
@@ -119,7 +123,7 @@ class Pair:
             import torch
             import xyce_amd
             self.torch = torch
-            self.ctx = xyce_amd.Context(0)
+            self.ctx = xyce_amd.Context(0, variant=variant)
             self.gpu = xyce_amd.Master(self.ctx, model_type,
                                        _copy_opts(options, xyce_amd.DeviceOptions) if options else None)
             gcounts = build_group(self.gpu, spec, lead)
@@ -205,11 +209,11 @@ class Pair:
                 self.orc.restore_state()
                 for k, v in inputs.items():
                     self.h[k][:] = v
-                oracle.lib().or_set_libm_perturbation(self.noise_ulps, 1000003 * self._loads + r + 1)
+                self.olib.or_set_libm_perturbation(self.noise_ulps, 1000003 * self._loads + r + 1)
                 try:
                     runs.append(self._oracle_pass(so))
                 finally:
-                    oracle.lib().or_set_libm_perturbation(0, 0)
+                    self.olib.or_set_libm_perturbation(0, 0)
             self.orc.restore_state()
             for k, v in inputs.items():
                 self.h[k][:] = v
@@ -221,12 +225,12 @@ class Pair:
         mag = {k: np.zeros_like(self.h[k]) for k in OUT_VECS + ("dFdx", "dQdx")}
         vm = oracle.make_vectors(**{k: (mag[k] if k in mag else v)
                                     for k, v in self.h.items() if len(v)})
-        oracle.lib().or_set_magnitude_mode(1)
+        self.olib.or_set_magnitude_mode(1)
         try:
             self.orc.loadDAEVectors(so, vm)
             self.orc.loadDAEMatrices(so, vm)
         finally:
-            oracle.lib().or_set_magnitude_mode(0)
+            self.olib.or_set_magnitude_mode(0)
         self.mag = mag
         if self.gpu is None:
             return res_o, None

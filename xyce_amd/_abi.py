@@ -101,27 +101,33 @@ def _proto(lib):
     return lib
 
 
-_lib = None
+# Library variants: "" the production build; "pm" the parity build, whose
+# evaluation takes exp/log/pow/tan from xa_pmath.h (the functions of the
+# oracle's parity build) instead of the GPU math library.  Both may be loaded
+# in one process; each has its own kernels, state and error message.
+VARIANT_PATHS = {"": LIB_PATH, "pm": os.path.join(_HERE, "libxyce_amd_pm.so")}
+_libs = {}
 
 
-def lib():
-    """Load libxyce_amd.so (raises if it has not been built)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(variant=""):
+    """Load libxyce_amd.so (or a variant; raises if it has not been built)."""
+    L = _libs.get(variant)
+    if L is None:
+        path = VARIANT_PATHS[variant]
+        if not os.path.exists(path):
             raise RuntimeError(
-                f"{LIB_PATH} is missing: build it with __graft_entry__.build(); "
+                f"{path} is missing: build it with __graft_entry__.build(); "
                 "the device loads have no CPU fallback")
-        _lib = _proto(C.CDLL(LIB_PATH))
-    return _lib
+        L = _libs[variant] = _proto(C.CDLL(path))
+    return L
 
 
 class XaError(RuntimeError):
     pass
 
 
-def check(rc, what=""):
+def check(rc, what="", L=None):
     if rc != XA_OK:
-        msg = lib().xa_last_error().decode()
+        msg = (L or lib()).xa_last_error().decode()
         raise XaError(f"{what}: rc={rc}: {msg}")
     return rc

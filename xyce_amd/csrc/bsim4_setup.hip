@@ -16,6 +16,13 @@
 #include <cstring>
 #include "bsim4_host.h"
 
+// The host setup (processParams, updateTemperature) keeps the C library's
+// exp/log/pow in every build, as the CPU restatement's BSIM4 setup does.
+#undef exp
+#undef log
+#undef pow
+#undef tan
+
 namespace xa {
 namespace bsim4 {
 

@@ -149,10 +149,10 @@ static int setup(Group& g) {
     if (m.BVGiven) {
       // breakdown knee by fixed-count Picard iteration (N_DEV_Diode.C:1534-1581)
       const double cbv = m.v[M_IBV];
-      double a1 = 3.0 * vte / (kE * tempBV);
+      double a1 = 3.0 * vte / (XA_CONSTE * tempBV);
       a1 = a1 * a1 * a1;
       const double lo = tSatCur * tIRF * (1 - a1);
-      const double hi = tSatCur * tIRF * (1 - 1.0 / (kE * kE * kE)) * exp(-1.0 * (3.0 * vte - tempBV) / vte);
+      const double hi = tSatCur * tIRF * (1 - 1.0 / (XA_CONSTE * XA_CONSTE * XA_CONSTE)) * exp(-1.0 * (3.0 * vte - tempBV) / vte);
       if (cbv >= hi) {
         tBrkdwnV = 3.0 * vte;
       } else if (cbv <= lo) {
@@ -160,7 +160,7 @@ static int setup(Group& g) {
       } else {
         double xbv = tempBV;
         for (int it = 0; it < 8; ++it) {
-          double a2 = 3.0 * vte / (kE * xbv);
+          double a2 = 3.0 * vte / (XA_CONSTE * xbv);
           a2 = a2 * a2 * a2;
           xbv = tempBV - vte * log(cbv / (tSatCur * tIRF)) + vte * log(1 - a2);
         }
@@ -363,14 +363,14 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
         Gd = Gd1 * Khi + Inorm * DKhi + Gd2 * Kgen + Irec * DKgen;
       }
     } else if (!tBrkdwnV || Vd >= -tBrkdwnV) {
-      double ar = 3.0 * Vte / (Vd * kE);
+      double ar = 3.0 * Vte / (Vd * XA_CONSTE);
       ar = ar * ar * ar;
       Id = -Isat * (1.0 + ar) + gmin * Vd;
       Gd = Isat * 3.0 * ar / Vd + gmin;
     } else {
       double e1 = dmin(kMaxExpArg, -(tBrkdwnV + Vd) / Vte);
       const double evrev = exp(e1);
-      double a2 = 3.0 * Vte / (kE * tBrkdwnV);
+      double a2 = 3.0 * Vte / (XA_CONSTE * tBrkdwnV);
       a2 = a2 * a2 * a2;
       const double IsatB = Isat * (1 - a2);
       Id = -IsatB * evrev + gmin * Vd;

@@ -9,6 +9,18 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+// Parity build (XA_PORTABLE_MATH): exp, log, pow and tan in the evaluation
+// code from here on -- these helpers and every model file that includes
+// this header -- are the portable functions of xa_pmath.h, which the CPU
+// restatement's parity build uses too, so the two agree bit for bit.
+#ifdef XA_PORTABLE_MATH
+#include "xa_pmath.h"
+#define exp(a) xa_pm_exp(a)
+#define log(a) xa_pm_log(a)
+#define pow(a, b) xa_pm_pow(a, b)
+#define tan(a) xa_pm_tan(a)
+#endif
+
 #define XA_HD __host__ __device__ __forceinline__
 
 namespace xa {
@@ -30,9 +42,12 @@ constexpr double kMaxExp   = 5.834617425e14;
 constexpr double kMinExp   = 1.713908431e-15;
 constexpr double kExpThreshold = 34.0;
 constexpr double kMaxExpArg = 100.0;
-// e = exp(1) and sqrt(2) as correctly rounded literals.
-constexpr double kE        = 2.718281828459045;
+// sqrt(2) as a correctly rounded literal (the reference's sqrt(2.0); sqrt is
+// correctly rounded on both sides).
 constexpr double kRoot2    = 1.4142135623730951;
+// N_DEV_Const.h:60 defines CONSTe as exp(1.0), a math-library call; the loads
+// call it the same way, so e is whatever this build's exp gives.
+#define XA_CONSTE exp(1.0)
 
 XA_HD double dmin(double a, double b) { return a < b ? a : b; }
 XA_HD double dmax(double a, double b) { return a > b ? a : b; }
@@ -114,7 +129,7 @@ XA_HD void lambertw(double x, double& w, int& ierr, double& xi) {
       xi = log(1.0 - x + c1 * x * x) - w;
     } else {
       const double d = fabs(x - turnpt);
-      w = -1 + sqrt(2.0 * kE) * sqrt(x - turnpt);
+      w = -1 + sqrt(2.0 * XA_CONSTE) * sqrt(x - turnpt);
       if (d == 0.0) return;
       xi = log(x / w) - w;
     }
