@@ -928,6 +928,10 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
     else if (!strcmp(name, "origFlag")) out[i] = I->origFlag;
     else if (!strcmp(name, "tSatCur")) out[i] = I->tSatCur;
     else if (!strcmp(name, "tBetaF")) out[i] = I->tBetaF;
+    else if (!strcmp(name, "tBECap")) out[i] = I->tBECap;
+    else if (!strcmp(name, "tBEPot")) out[i] = I->tBEPot;
+    else if (!strcmp(name, "tBCCap")) out[i] = I->tBCCap;
+    else if (!strcmp(name, "tBCPot")) out[i] = I->tBCPot;
     else return or_fail(XA_ERR_ARG, "bjt: unknown intermediate");
   }
   return XA_OK;

@@ -126,6 +126,7 @@ typedef struct {
   double CAPcgmgmb, CAPcgmdb, CAPcgmsb, CAPcgmbb, CAPcdgmb, CAPcsgmb, CAPcbgmb;
   double CAPcqgb, CAPcqdb, CAPcqsb, CAPcqbb;
   double qg, qd, qb, ScalingFactor;
+  double dcVgsteff, dcAbulk, dcVdseff;   /* DC-section values, for reading only */
   /* setupFVectorVars / auxChargeCalculations products */
   double Gm, Gmbs, FwdSum, RevSum, ceqdrn, ceqbd, ceqbs, ceqjs, ceqjd, ceqgcrg;
   double ceqgstot, ceqgdtot, gbbdp, gbbsp, gbdpg, gbdpdp, gbdpb, gbdpsp;

@@ -1020,6 +1020,10 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   if (Vdseff_ > Vds) Vdseff_ = Vds;
   diffVds = Vds - Vdseff_;
   I->Vdseff = Vdseff_;
+  /* the DC values, kept for reading (the C-V section reuses these fields) */
+  I->dcVgsteff = I->Vgsteff;
+  I->dcAbulk = I->Abulk;
+  I->dcVdseff = I->Vdseff;
 
   /* velocity overshoot (:9500-9540) */
   if ((MG(lambda)) && (M(lambda) > 0.0)) {

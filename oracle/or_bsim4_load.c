@@ -590,6 +590,18 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
     else if (!strcmp(name, "vgs")) out[i] = I->vgs;
     else if (!strcmp(name, "vds")) out[i] = I->vds;
     else if (!strcmp(name, "vbs")) out[i] = I->vbs;
+    else if (!strcmp(name, "Vgsteff")) out[i] = I->dcVgsteff;
+    else if (!strcmp(name, "Abulk")) out[i] = I->dcAbulk;
+    else if (!strcmp(name, "Vdseff")) out[i] = I->dcVdseff;
+    else if (!strcmp(name, "ueff")) out[i] = I->ueff;
+    else if (!strcmp(name, "EsatL")) out[i] = I->EsatL;
+    else if (!strcmp(name, "vsattemp")) out[i] = I->vsattemp;
+    else if (!strcmp(name, "vtm")) out[i] = I->model->vtm;
+    else if (!strcmp(name, "coxe")) out[i] = I->model->coxe;
+    else if (!strcmp(name, "qsrc")) out[i] = I->qsrc;
+    else if (!strcmp(name, "qdrn")) out[i] = I->qdrn;
+    else if (!strcmp(name, "Igidl")) out[i] = I->Igidl;
+    else if (!strcmp(name, "weffCJ")) out[i] = p->weffCJ;
     else return or_fail(XA_ERR_ARG, "bsim4: unknown intermediate");
   }
   return XA_OK;

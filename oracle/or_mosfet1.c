@@ -887,6 +887,9 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
     else if (!strcmp(name, "tVto")) out[i] = I->tVto;
     else if (!strcmp(name, "tPhi")) out[i] = I->tPhi;
     else if (!strcmp(name, "OxideCap")) out[i] = I->OxideCap;
+    else if (!strcmp(name, "tCj")) out[i] = I->tCj;
+    else if (!strcmp(name, "tCjsw")) out[i] = I->tCjsw;
+    else if (!strcmp(name, "tBulkPot")) out[i] = I->tBulkPot;
     else return or_fail(XA_ERR_ARG, "mosfet1: unknown intermediate");
   }
   return XA_OK;
