@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define XA_ABI_VERSION 1
+#define XA_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define XA_OK              0
@@ -53,6 +53,7 @@ extern "C" {
 #define XA_BSIM4    4   /* "m", levels 14,54     N_DEV_MOSFET_B4.C:17270-17280 */
 #define XA_RESISTOR 5   /* "r", level 1          N_DEV_Resistor.C:1198-1224  */
 #define XA_CAPACITOR 6  /* "c", level 1          N_DEV_Capacitor.C:2152-2177 */
+#define XA_VSOURCE  7   /* "v", level 1          N_DEV_Vsrc.C:1090-1097      */
 
 /* ---- output kinds of a load ------------------------------------------- */
 #define XA_KIND_F        0   /* daeFVector rows                              */
@@ -90,6 +91,16 @@ typedef struct xa_solver_state {
   double  currTime;
   double  gainScale;            /* gainScale[0] of the reference            */
   double  nltermScale;
+  /* read by the independent sources (SourceData::initializeSource and
+     updateSource, N_DEV_SourceData.C): breakpoint tolerance, the analysis'
+     final time and first step (the PULSE/SIN/EXP defaults), and whether a
+     harmonic-balance analysis is set up (which selects the transient
+     waveform) */
+  double  bpTol;
+  double  finalTime;
+  double  startingTimeStep;
+  int32_t HBspecified;
+  int32_t _pad2;
 } xa_solver_state;
 
 /* Device options read by the loads (N_DEV_DeviceOptions.C:66-140). */

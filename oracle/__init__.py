@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 # "pm": the parity build, whose evaluation takes exp/log/pow/tan from
 # xyce_amd/csrc/xa_pmath.h (the functions of the GPU library's parity build)
 VARIANT_PATHS = {"": LIB_PATH, "pm": os.path.join(_HERE, "liboracle_pm.so")}
-MODEL_TYPES = {"diode": 1, "mosfet1": 2, "bjt": 3, "bsim4": 4, "resistor": 5, "capacitor": 6}
+MODEL_TYPES = {"diode": 1, "mosfet1": 2, "bjt": 3, "bsim4": 4, "resistor": 5, "capacitor": 6,
+               "vsource": 7}
 
 
 class SolverState(C.Structure):
@@ -28,7 +29,9 @@ class SolverState(C.Structure):
         "locaEnabledFlag", "initTranFlag", "beginIntegrationFlag", "tranopFlag",
         "acopFlag", "transientFlag", "artParameterFlag", "bjtArtParameterFlag", "_pad")] + \
         [(n, C.c_double) for n in (
-            "pdt", "currTimeStep", "lastTimeStep", "currTime", "gainScale", "nltermScale")]
+            "pdt", "currTimeStep", "lastTimeStep", "currTime", "gainScale", "nltermScale",
+            "bpTol", "finalTime", "startingTimeStep")] + \
+        [(n, C.c_int32) for n in ("HBspecified", "_pad2")]
 
 
 class DeviceOptions(C.Structure):

@@ -64,13 +64,14 @@ void or_set_threads(int n) { or_threads = n > 0 ? n : 1; }
 
 /* The evaluation's math functions as this build resolves them (the C
  * library's, or in the parity build the portable ones): 0 exp(a), 1 log(a),
- * 2 pow(a, b), 3 tan(a).  For the accuracy tests of the portable functions. */
+ * 2 pow(a, b), 3 tan(a), 4 sin(a).  For the accuracy tests of the portable functions. */
 double or_math(int which, double a, double b) {
   switch (which) {
     case 0: return OR_EXP(a);
     case 1: return OR_LOG(a);
     case 2: return OR_POW(a, b);
     case 3: return OR_TAN(a);
+    case 4: return OR_SIN(a);
   }
   return NAN;
 }
@@ -93,6 +94,7 @@ static const or_model_ops* ops_for(int type) {
     case XA_DIODE: return &or_diode_ops;
     case XA_RESISTOR: return &or_resistor_ops;
     case XA_CAPACITOR: return &or_capacitor_ops;
+    case XA_VSOURCE: return &or_vsrc_ops;
     case XA_MOSFET1: return &or_mosfet1_ops;
     case XA_BJT: return &or_bjt_ops;
     case XA_BSIM4: return &or_bsim4_ops;

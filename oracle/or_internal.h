@@ -87,6 +87,7 @@ extern const or_model_ops or_bjt_ops;
 extern const or_model_ops or_bsim4_ops;
 extern const or_model_ops or_resistor_ops;
 extern const or_model_ops or_capacitor_ops;
+extern const or_model_ops or_vsrc_ops;
 
 int or_fail(int code, const char* msg);
 /* Offset of column `col` in CSR row `row`, as Indexor::matrixGlobalToLocal
@@ -129,16 +130,19 @@ double or_perturb(double r, double a, double b);
 #define OR_LOG xa_pm_log
 #define OR_POW xa_pm_pow
 #define OR_TAN xa_pm_tan
+#define OR_SIN xa_pm_sin
 #else
 #define OR_EXP exp
 #define OR_LOG log
 #define OR_POW pow
 #define OR_TAN tan
+#define OR_SIN sin
 #endif
 static inline double or_exp(double a) { double r = OR_EXP(a); return or_libm_ulps ? or_perturb(r, a, 0.0) : r; }
 static inline double or_log(double a) { double r = OR_LOG(a); return or_libm_ulps ? or_perturb(r, a, 1.0) : r; }
 static inline double or_pow(double a, double b) { double r = OR_POW(a, b); return or_libm_ulps ? or_perturb(r, a, b) : r; }
 static inline double or_tan(double a) { double r = OR_TAN(a); return or_libm_ulps ? or_perturb(r, a, 2.0) : r; }
+static inline double or_sin(double a) { double r = OR_SIN(a); return or_libm_ulps ? or_perturb(r, a, 3.0) : r; }
 static inline double or_min(double a, double b) { return a < b ? a : b; }
 static inline double or_max(double a, double b) { return a > b ? a : b; }
 

@@ -115,6 +115,14 @@ class CircuitPair:
             self.d = {k: self.torch.zeros(max(len(v), 1), dtype=self.torch.float64, device=dev)
                       for k, v in self.h.items()}
 
+    def unknowns(self, volts, branch=0.0):
+        """A solution vector: the given node voltages followed by `branch`
+        for the unknowns the device types added (internal nodes, branch
+        currents)."""
+        x = np.full(self.n_nodes, float(branch))
+        x[:len(volts)] = volts
+        return x
+
     def set_inputs(self, **arrays):
         for k, a in arrays.items():
             self.h[k][:] = a

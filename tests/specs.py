@@ -229,3 +229,26 @@ def capacitor_spec(n, seed=20261203, cards=None, temp_spread=False, ic=False, se
     if temp_spread:
         params["TEMP"] = rng.uniform(-20.0, 120.0, n)
     return {"models": cards, "n": n, "model_index": mi, "inst_params": params}
+
+
+def vsource_spec(n, seed=20261203, kinds=(-1, 0, 1, 2, 6), defaults_frac=0.2):
+    """Voltage sources with a mix of waveforms (TRANSIENTSOURCETYPE: -1 none,
+    0 SIN, 1 EXP, 2 PULSE, 6 DC) and random parameters; a fraction of the
+    timing parameters is left not given (NaN) to take the solver-state
+    defaults."""
+    rng = splitmix64(seed)
+    kind = rng.choice(np.asarray(kinds, np.float64), size=n)
+    p = {"DCV0": rng.uniform(-1.0, 2.0, n), "TRANSIENTSOURCETYPE": kind,
+         "V0": rng.uniform(-0.5, 0.5, n), "VA": rng.uniform(0.1, 1.0, n),
+         "FREQ": rng.uniform(1e8, 5e9, n), "THETA": rng.uniform(0.0, 1e9, n),
+         "PHASE": rng.uniform(-90.0, 90.0, n), "V1": rng.uniform(-0.5, 0.5, n),
+         "V2": rng.uniform(0.5, 1.5, n), "TD": rng.uniform(0.0, 2e-10, n),
+         "TR": rng.uniform(1e-11, 1e-10, n), "TF": rng.uniform(1e-11, 1e-10, n),
+         "PW": rng.uniform(1e-10, 5e-10, n), "PER": rng.uniform(8e-10, 1.2e-9, n),
+         "TD1": rng.uniform(0.0, 1e-10, n), "TAU1": rng.uniform(1e-11, 2e-10, n),
+         "TD2": rng.uniform(3e-10, 6e-10, n), "TAU2": rng.uniform(1e-11, 2e-10, n)}
+    p["TRANSIENTSOURCETYPE"][kind == -1] = np.nan
+    for k in ("TD", "TR", "TF", "PW", "PER", "FREQ", "THETA", "TD1", "TAU1", "TD2", "TAU2"):
+        p[k][rng.random(n) < defaults_frac] = np.nan
+    return {"models": [(1, {})], "n": n, "model_index": np.zeros(n, np.int32),
+            "inst_params": p}

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     import xyce_amd
-    assert xyce_amd.lib().xa_abi_version() == 1
+    assert xyce_amd.lib().xa_abi_version() == 2
     o = xyce_amd.default_options()
     assert o.gmin == 1e-12 and o.voltageLimiterFlag == 1 and abs(o.tnom - 300.15) < 1e-12
     s = xyce_amd.default_solver_state()
@@ -33,7 +33,7 @@ def test_abi_version_and_defaults():
 
 def test_struct_layouts_match_header():
     import xyce_amd
-    assert C.sizeof(xyce_amd.SolverState) == 14 * 4 + 6 * 8
+    assert C.sizeof(xyce_amd.SolverState) == 14 * 4 + 9 * 8 + 2 * 4
     assert C.sizeof(xyce_amd.DeviceOptions) == 9 * 8 + 4 * 4
     assert C.sizeof(xyce_amd.Vectors) == 18 * 8
 

@@ -12,7 +12,7 @@ def test_reduced_netlists_load():
     for build in (lambda: logic_circuit(4000), lambda: hb_circuit(4, 200)):
         parts, n_nodes, volts = build()
         cp = CircuitPair(parts, n_nodes, gpu=False)
-        cp.set_inputs(nextSol=volts)
+        cp.set_inputs(nextSol=cp.unknowns(volts, 1e-4))
         ro, _ = cp.load(newtonIter=0, dcopFlag=1, initJctFlag=1)
         for k, v in ro.items():
             assert np.all(np.isfinite(v)), k

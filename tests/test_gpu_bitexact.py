@@ -20,7 +20,7 @@ import pytest
 from gpu_common import ITERS
 from harness import Pair
 from specs import (BSIM4_CARDS, BSIM4_VTL_CARD, bjt_spec, bsim4_spec, diode_spec,
-                   mosfet1_spec)
+                   mosfet1_spec, vsource_spec)
 from test_gpu_bjt import BJT_ITERS
 
 pytestmark = pytest.mark.gpu
@@ -138,3 +138,13 @@ def test_bsim4_bitexact_plain_limiter_off_fused():
               seed=208)
     q.gpu.set_path("fused", 128)
     run_exact(q, np.random.default_rng(209), fused=True)
+
+
+def test_vsource_bitexact():
+    """SIN and EXP waveforms through the shared portable sin/exp."""
+    p = _pair("vsource", vsource_spec(4000, seed=701), topology="shared", seed=702,
+              ground_frac=0.2)
+    times = dict(startingTimeStep=1e-12, finalTime=2e-9, bpTol=1e-18)
+    iters = [dict(newtonIter=1, transientFlag=1, currTime=float(t), **times)
+             for t in np.linspace(0.0, 2.5e-9, 9)]
+    run_exact(p, np.random.default_rng(703), iters=iters)

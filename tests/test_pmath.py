@@ -48,6 +48,17 @@ def test_tan_within_three_ulps():
         assert _ulps(_pm(3, x), math.tan(x)) <= 3.0, x
 
 
+def test_sin_within_three_ulps():
+    rng = np.random.default_rng(4)
+    for x in np.concatenate([rng.uniform(-20, 20, 40000), rng.uniform(-1.5, 1.5, 20000)]):
+        want = math.sin(x)
+        got = _pm(4, x)
+        assert _ulps(got, want) <= 3.0 or abs(got - want) <= 1e-16, x
+    for k in range(-12, 13):          # near the zeros (k pi rounded): absolute error
+        x = k * math.pi
+        assert abs(_pm(4, x) - math.sin(x)) <= 1e-17 + 3 * math.ulp(math.sin(x)), k
+
+
 def test_pow_within_bound():
     rng = np.random.default_rng(3)
     for _ in range(40000):

@@ -11,9 +11,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxyce_amd.so")
 
 XA_OK = 0
-XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4, XA_RESISTOR, XA_CAPACITOR = 1, 2, 3, 4, 5, 6
+XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4, XA_RESISTOR, XA_CAPACITOR, XA_VSOURCE = range(1, 8)
 MODEL_TYPES = {"diode": XA_DIODE, "mosfet1": XA_MOSFET1, "bjt": XA_BJT, "bsim4": XA_BSIM4,
-               "resistor": XA_RESISTOR, "capacitor": XA_CAPACITOR}
+               "resistor": XA_RESISTOR, "capacitor": XA_CAPACITOR, "vsource": XA_VSOURCE}
 
 
 class SolverState(C.Structure):
@@ -23,7 +23,9 @@ class SolverState(C.Structure):
         "locaEnabledFlag", "initTranFlag", "beginIntegrationFlag", "tranopFlag",
         "acopFlag", "transientFlag", "artParameterFlag", "bjtArtParameterFlag", "_pad")] + \
         [(n, C.c_double) for n in (
-            "pdt", "currTimeStep", "lastTimeStep", "currTime", "gainScale", "nltermScale")]
+            "pdt", "currTimeStep", "lastTimeStep", "currTime", "gainScale", "nltermScale",
+            "bpTol", "finalTime", "startingTimeStep")] + \
+        [(n, C.c_int32) for n in ("HBspecified", "_pad2")]
 
 
 class DeviceOptions(C.Structure):

@@ -22,6 +22,7 @@
 #undef log
 #undef pow
 #undef tan
+#undef sin
 
 namespace xa {
 namespace bsim4 {

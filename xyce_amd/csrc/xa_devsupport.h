@@ -9,7 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-// Parity build (XA_PORTABLE_MATH): exp, log, pow and tan in the evaluation
+// Parity build (XA_PORTABLE_MATH): exp, log, pow, tan and sin in the evaluation
 // code from here on -- these helpers and every model file that includes
 // this header -- are the portable functions of xa_pmath.h, which the CPU
 // restatement's parity build uses too, so the two agree bit for bit.
@@ -19,6 +19,7 @@
 #define log(a) xa_pm_log(a)
 #define pow(a, b) xa_pm_pow(a, b)
 #define tan(a) xa_pm_tan(a)
+#define sin(a) xa_pm_sin(a)
 #endif
 
 #define XA_HD __host__ __device__ __forceinline__

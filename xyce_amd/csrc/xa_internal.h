@@ -120,6 +120,7 @@ struct ModelImpl {
 const ModelImpl* model_diode();
 const ModelImpl* model_resistor();
 const ModelImpl* model_capacitor();
+const ModelImpl* model_vsrc();
 const ModelImpl* model_mosfet1();
 const ModelImpl* model_bjt();
 const ModelImpl* model_bsim4();

@@ -1,4 +1,4 @@
-/* xa_pmath.h -- portable exp, log, pow and tan for the parity build.
+/* xa_pmath.h -- portable exp, log, pow, tan and sin for the parity build.
  *
  * The device kernels normally call the GPU math library (ocml) and the CPU
  * loads the C library (glibc); the two differ in the last bits of exp, log,
@@ -22,14 +22,16 @@
  *            written log(1 + f) = f - s (f - z P(z)), z = s^2, with the odd
  *            series of atanh to s^23.
  *   pow(x, y) = exp(y log x) with the special values of C99 F.9.4.4.
- *   tan(x)   = sin(r) / cos(r) or -cos(r) / sin(r) for k even or odd,
+ *   sin(x) and tan(x): sin(r), cos(r) (signed by k mod 4) and
+ *            sin(r) / cos(r) or -cos(r) / sin(r) for k even or odd,
  *            k = nearest integer to x / (pi/2), r = x - k pi/2 (pi/2 in three
  *            parts), |r| <= pi/4, sine to r^19 and cosine to r^20 by Taylor
  *            series.
  * The split constants and series coefficients were computed for this file
  * to 60 digits (ln 2 and pi from their series) and rounded to double.
  * Accuracy against the C library, measured over the ranges the loads use:
- * exp and log within 1 ulp, tan within 3 ulps for |x| <= 20, pow within
+ * exp and log within 1 ulp, tan and sin within 3 ulps for |x| <= 20 (sin
+ * absolute 1e-16 near its zeros), pow within
  * 2 + 2 |y log x| ulps (the rounding of log x and of y log x, carried
  * through exp) (tests/test_pmath.py).  They are a parity tool, not
  * a replacement for the system functions in production builds.
@@ -135,10 +137,9 @@ XA_PM_FN double xa_pm_pow(double x, double y) {
   return sign * xa_pm_exp(y * xa_pm_log(x));
 }
 
-XA_PM_FN double xa_pm_tan(double x) {
-  if (x != x || x == HUGE_VAL || x == -HUGE_VAL) return (x - x) / 0.0;
-  const double kd = floor(x * XA_PM_TWO_OVER_PI + 0.5);
-  const double r = ((x - kd * XA_PM_PIO2_1) - kd * XA_PM_PIO2_2) - kd * XA_PM_PIO2_3;
+/* sin(r) and cos(r) for |r| <= pi/4 by their Taylor series (to r^19 and
+   r^20): s = r + r z S(z), c = 1 - z/2 + z^2 C(z), z = r^2 */
+XA_PM_FN void xa_pm_sincos_reduced(double r, double* sn, double* cs) {
   const double z = r * r;
   /* sin(r) = r + r z S(z), cos(r) = 1 - z/2 + z^2 C(z) */
   double S = -8.22063524662433e-18;         /* -1/19! */
@@ -150,7 +151,7 @@ XA_PM_FN double xa_pm_tan(double x) {
   S = S * z - 0.0001984126984126984;        /* -1/7! */
   S = S * z + 0.008333333333333333;         /* 1/5! */
   S = S * z - 0.16666666666666666;          /* -1/3! */
-  const double sn = r + r * (z * S);
+  *sn = r + r * (z * S);
   double Cc = 4.110317623312165e-19;        /* 1/20! */
   Cc = Cc * z - 1.5619206968586225e-16;     /* -1/18! */
   Cc = Cc * z + 4.779477332387385e-14;      /* 1/16! */
@@ -160,9 +161,31 @@ XA_PM_FN double xa_pm_tan(double x) {
   Cc = Cc * z + 2.48015873015873e-05;       /* 1/8! */
   Cc = Cc * z - 0.001388888888888889;       /* -1/6! */
   Cc = Cc * z + 0.041666666666666664;       /* 1/4! */
-  const double cs = (1.0 - 0.5 * z) + (z * z) * Cc;
+  *cs = (1.0 - 0.5 * z) + (z * z) * Cc;
+}
+
+XA_PM_FN double xa_pm_tan(double x) {
+  if (x != x || x == HUGE_VAL || x == -HUGE_VAL) return (x - x) / 0.0;
+  const double kd = floor(x * XA_PM_TWO_OVER_PI + 0.5);
+  const double r = ((x - kd * XA_PM_PIO2_1) - kd * XA_PM_PIO2_2) - kd * XA_PM_PIO2_3;
+  double sn, cs;
+  xa_pm_sincos_reduced(r, &sn, &cs);
   const double kh = kd * 0.5;
   return floor(kh) == kh ? sn / cs : -cs / sn;
+}
+
+/* sin(x) = sin(r), cos(r), -sin(r), -cos(r) for k = 0, 1, 2, 3 (mod 4) */
+XA_PM_FN double xa_pm_sin(double x) {
+  if (x != x || x == HUGE_VAL || x == -HUGE_VAL) return (x - x) / 0.0;
+  const double kd = floor(x * XA_PM_TWO_OVER_PI + 0.5);
+  const double r = ((x - kd * XA_PM_PIO2_1) - kd * XA_PM_PIO2_2) - kd * XA_PM_PIO2_3;
+  double sn, cs;
+  xa_pm_sincos_reduced(r, &sn, &cs);
+  const double q = kd - 4.0 * floor(kd * 0.25);   /* k mod 4, exact */
+  if (q == 0.0) return sn;
+  if (q == 1.0) return cs;
+  if (q == 2.0) return -sn;
+  return -cs;
 }
 
 #endif /* XA_PMATH_H */

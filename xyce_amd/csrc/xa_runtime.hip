@@ -58,6 +58,7 @@ static const ModelImpl* impl_for(int type) {
     case XA_DIODE: return model_diode();
     case XA_RESISTOR: return model_resistor();
     case XA_CAPACITOR: return model_capacitor();
+    case XA_VSOURCE: return model_vsrc();
     case XA_MOSFET1: return model_mosfet1();
     case XA_BJT: return model_bjt();
     case XA_BSIM4: return model_bsim4();

@@ -16,7 +16,9 @@
 Each transistor netlist returns (models, model_index, params, node_lids,
 n_nodes, voltages): instance-major external node LIDs (D, G, S, B; ground
 -1) and one voltage per node, ready for :class:`xyce_amd.workloads.Workload`.
-The wire functions return two-terminal device lists (node pairs, values).
+The wire functions return two-terminal device lists (node pairs, values);
+the source functions return voltage-source lists (node pairs, instance
+parameter columns for the "vsource" group).
 """
 import numpy as np
 
@@ -162,3 +164,35 @@ def hb_filter(samples=128, n_transistors=2000, seed=SEED, r=1000.0, c=50e-15, **
     c_nodes = np.stack([filt, np.full(k, -1, np.int32)], axis=1)
     volts = np.concatenate([v0, v0[outs.reshape(-1)]])
     return r_nodes, np.full(k, r), c_nodes, np.full(k, c), n0 + k, volts
+
+
+def logic_sources(n_inputs=20, vdd=1.0, period=1e-9, edge=50e-12):
+    """Sources of ``logic_dag``: the supply (node 0, DC vdd) and one PULSE
+    per primary input (nodes 1..n_inputs): 0 -> vdd at 1 GHz (PER = 1 ns),
+    edges of 50 ps, pulse width half a period less an edge, staggered delays.
+    Returns (nodes[n, 2], params)."""
+    nodes = np.array([[0, -1]] + [[k, -1] for k in range(1, n_inputs + 1)], np.int32)
+    n = len(nodes)
+    kind = np.full(n, 2.0)
+    kind[0] = np.nan                      # the supply: DC only
+    params = {"DCV0": np.concatenate([[vdd], np.zeros(n_inputs)]),
+              "TRANSIENTSOURCETYPE": kind,
+              "V1": np.zeros(n), "V2": np.full(n, vdd),
+              "TD": np.concatenate([[0.0], np.arange(n_inputs) * period / n_inputs]),
+              "TR": np.full(n, edge), "TF": np.full(n, edge),
+              "PW": np.full(n, period / 2 - edge), "PER": np.full(n, period)}
+    return nodes, params
+
+
+def hb_sources(samples=128, n_transistors=2000, seed=SEED, **kw):
+    """Sources of every sample copy of ``hb_diffamp``: VDD, the bias and the
+    two inputs, each a DC source at that sample's value (the HB loader hands
+    each sample's devices the sources' values at that sample's time).
+    Returns (nodes[n, 2], params)."""
+    (_, _, _, _, _, volts), _ = _diffamp(samples, n_transistors, seed, **kw)
+    cells = n_transistors // 5
+    stride = 4 + 3 * cells                # nodes per copy
+    base = np.arange(samples, dtype=np.int32) * stride
+    nodes = np.stack([np.repeat(base, 4) + np.tile(np.arange(4, dtype=np.int32), samples),
+                      np.full(4 * samples, -1, np.int32)], axis=1)
+    return nodes, {"DCV0": volts[nodes[:, 0]]}
