@@ -275,6 +275,13 @@ int xa_group_set_path(xa_group g, int path, int block);
 int xa_group_plan_info(xa_group g, int kind, int64_t* n_targets, int64_t* n_sources,
                        int* pattern_slots);
 
+/* Layout of the per-instance parameter data the update kernel reads (no
+ * reference counterpart; diagnostics and tests): the columns read per
+ * instance from HBM, and the columns served from the model cards because
+ * every instance of each card holds the same value (BSIM4: size-dependent
+ * parameters of binless cards and shared instance values). */
+int xa_group_param_layout(xa_group g, int* per_instance_columns, int* card_columns);
+
 /* Device-side status of the last update: number of instances whose
  * evaluation returned false (0 = all succeeded). Synchronizes. */
 int xa_group_failures(xa_group g, int64_t* n_failed);

@@ -11,7 +11,7 @@ Jacobians -- must be equal bit for bit, over the cases of the production
 parity tests: the parity cards of each model, DC / transient-OP / transient
 Newton iterations, private and shared topologies with ground, limiter on
 and off, lead currents, Lambert-W forms, IC and OFF instances, inputOP
-flags.  (Rows of more than 512 terms are summed in a fixed chunked order,
+flags.  (Rows of more than 1024 terms are summed in a fixed chunked order,
 not the reference's one-by-one order, so no test here has such rows.)
 """
 import numpy as np
@@ -148,3 +148,32 @@ def test_vsource_bitexact():
     iters = [dict(newtonIter=1, transientFlag=1, currTime=float(t), **times)
              for t in np.linspace(0.0, 2.5e-9, 9)]
     run_exact(p, np.random.default_rng(703), iters=iters)
+
+
+def _card_mode_spec(variant, n):
+    spec = bsim4_spec(n, seed=711, cards=BSIM4_CARDS[:2], geometry=False)
+    rng = np.random.default_rng(712)
+    if variant == "delvto":
+        spec["inst_params"]["DELVTO"] = rng.uniform(-0.02, 0.02, n)
+    elif variant == "temp":
+        spec["inst_params"]["TEMP"] = rng.uniform(-20.0, 120.0, n)
+    return spec
+
+
+# (per-instance columns, card columns) of each variant: binless cards with
+# W/L only serve all but the geometry-derived columns from the cards; a
+# per-instance DELVTO keeps the instance values per instance
+CARD_LAYOUTS = {"plain": (14, 123), "delvto": (51, 86)}
+
+
+@pytest.mark.parametrize("variant", ["plain", "delvto", "temp"])
+def test_bsim4_card_modes_bitexact(variant):
+    """The update kernel's card modes (parameter columns served from the
+    model cards when card-uniform) read the same bits as the columns."""
+    p = _pair("bsim4", _card_mode_spec(variant, 3000), topology="shared", seed=713,
+              ground_frac=0.05)
+    layout = p.gpu.param_layout()
+    print(f"  {variant}: per-instance columns {layout[0]}, card columns {layout[1]}")
+    if variant in CARD_LAYOUTS:
+        assert layout == CARD_LAYOUTS[variant]
+    run_exact(p, np.random.default_rng(714))

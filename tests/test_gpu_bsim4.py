@@ -80,3 +80,15 @@ def test_bsim4_run_to_run_bit_identical():
     spec = bsim4_spec(4000, seed=216, cards=ALL_CARDS)
     print("bsim4 determinism n=4000", flush=True)
     twin_drive("bsim4", spec, 217, np.random.default_rng(218), fused_b=False, topology="shared")
+
+
+@pytest.mark.parametrize("variant", ["plain", "delvto", "temp"])
+def test_bsim4_card_modes(variant):
+    """Production parity in each card mode of the update kernel (parameter
+    columns served from the model cards where card-uniform)."""
+    from test_gpu_bitexact import CARD_LAYOUTS, _card_mode_spec
+    p = Pair("bsim4", _card_mode_spec(variant, 3000), topology="shared", seed=715,
+             ground_frac=0.05, noise_runs=NOISE_RUNS)
+    if variant in CARD_LAYOUTS:
+        assert p.gpu.param_layout() == CARD_LAYOUTS[variant]
+    run_parity(p, np.random.default_rng(716))

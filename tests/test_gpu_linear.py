@@ -7,7 +7,7 @@ oracle), added into each entry in the reference's order, so every F/Q/lead/
 state/CSR entry must equal the oracle's bits, across operating-point and
 transient iterations, private, shared and grounded topologies, IC= branches
 and lead currents; the fused one-call load equals the four calls.  Rows of
-more than 512 terms (supply rails) are the exception: they are summed in a
+more than 1024 terms (supply rails) are the exception: they are summed in a
 fixed chunked order and meet the 1e-12 bar instead (test_rails)."""
 import numpy as np
 import pytest
@@ -60,7 +60,7 @@ def test_fused_equals_four_calls(model):
 
 def test_rails():
     """Rails: every resistor and capacitor between two of 3 nodes, so each row
-    sums thousands of terms.  Rows of more than 512 terms are summed as
+    sums thousands of terms.  Rows of more than 1024 terms are summed as
     256-term chunks in order and then a fixed pairwise tree over the chunk
     sums: the same bits on every run, within the 1e-12 bar of the oracle's
     sequential sum (run_parity asserts it), but not its bits."""

@@ -70,3 +70,18 @@ def test_ring17_measured_period():
     print(f"  17 stages: period {pg:.6e} s (oracle {pc:.6e}), {pg / 34:.3e} s per stage",
           flush=True)
     assert abs(pg - pc) <= WAVE_RTOL * pc
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_ring_graph_replays_equal_eager(fused):
+    """The plain loader's HIP-graph replay (one graph per solver-flag set,
+    eager host Newton work and copies between the replays) gives the eager
+    loads' waveforms bit for bit."""
+    ring = Ring(stages=11)
+    ta, Va, sa = transient(GpuLoader(ring, fused=fused), ring, t_stop=3e-10)
+    loader = GpuLoader(ring, fused=fused, graph=True)
+    tb, Vb, sb = transient(loader, ring, t_stop=3e-10)
+    print(f"  graphs {len(loader.graphs)}, eager {sa['load_seconds'] / sa['loads'] * 1e6:.1f} us, "
+          f"graph {sb['load_seconds'] / sb['loads'] * 1e6:.1f} us per load", flush=True)
+    assert len(loader.graphs) >= 3
+    np.testing.assert_array_equal(Va.view(np.int64), Vb.view(np.int64))

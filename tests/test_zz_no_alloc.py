@@ -36,7 +36,7 @@ def _twins(model, spec, seed, path, pool):
 
 @pytest.mark.parametrize("model,sizes,path,pool", [
     ("bsim4", (3000, 700), None, None),   # BSIM4 takes the four calls: value buffers in scratch
-    ("diode", (20000, 5000), "split", 16),  # rows of >512 terms: rail partial sums in scratch too
+    ("diode", (20000, 5000), "split", 16),  # rows of >1024 terms: rail partial sums in scratch too
     ("diode", (4000, 1000), None, None),  # the fused route: no scratch used
 ])
 def test_noalloc_equals_load_all(model, sizes, path, pool):

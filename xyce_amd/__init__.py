@@ -217,6 +217,14 @@ class Master:
               "plan_info", self.L)
         return nt.value, ns.value, ps.value
 
+    def param_layout(self):
+        """(per-instance columns read from HBM, columns served from the model
+        cards) of the update kernel (xa_group_param_layout)."""
+        a, b = C.c_int(), C.c_int()
+        check(self.L.xa_group_param_layout(self.h, C.byref(a), C.byref(b)), "param_layout",
+              self.L)
+        return a.value, b.value
+
     PATHS = {"split": 1, "fused": 2, "pattern": 3}
 
     def set_path(self, path, block=256):

@@ -86,6 +86,7 @@ def _proto(lib):
         "xa_load_all_noalloc": (C.c_int, [vp, P(SolverState), P(Vectors)]),
         "xa_group_failures": (C.c_int, [vp, P(i64)]),
         "xa_group_plan_info": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(C.c_int)]),
+        "xa_group_param_layout": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
         "xa_group_set_path": (C.c_int, [vp, C.c_int, C.c_int]),
         "xa_group_autotune": (C.c_int, [vp, P(SolverState), P(Vectors), C.c_int, P(C.c_int),
                                         P(C.c_int)]),

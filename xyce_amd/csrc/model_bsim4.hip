@@ -259,6 +259,51 @@ enum DField {
 #undef B4V
   ND
 };
+// A model's entry in the card table: the DevCard, then one value per
+// per-instance column.  Columns whose value is the same for every instance
+// of every card (parameters with no geometry or instance dependence, which
+// the reference also shares through its size-dependent records and model)
+// are read from there -- from the block's LDS copy of the cards -- instead
+// of from the instance's HBM column (KArgs::ucol marks them).
+constexpr int kCardBase = (int)(sizeof(DevCard) / sizeof(double));
+constexpr int kCardStride = kCardBase + ND;
+// The kernels take the card-uniform columns as a compile-time set, so that a
+// parameter read is a plain LDS or HBM load with no per-read branch (one in
+// every read measurably worsened register allocation).  CM (card mode):
+//   0: every column from HBM;
+//   1: the size-dependent columns except the geometry-derived ones (uniform
+//      for binless cards whatever the instance parameters: the Monte Carlo
+//      rings with per-instance DELVTO, the netlists);
+//   2: set 1 plus the instance-value columns except those of the drain/source
+//      perimeters and vfbzb (also uniform when instances give no AS/AD/NRD/
+//      NRS/DELVTO/TEMP/M and share a temperature: the microbenchmark).
+// The launch takes the largest set that the group's runtime mask (ucol)
+// covers, so the bits read are the same in every mode.
+constexpr int kGeomP[] = {P_abulkCVfactor, P_cgdo, P_cgso, P_leff, P_leffCV, P_rds0,
+                          P_theta0vb0, P_thetaRout, P_weff, P_weffCJ, P_weffCV};
+constexpr int kGeomV[] = {V_Pdeff, V_Pseff, V_vfbzb};
+// the size-dependent columns come first in DField, then the instance values
+enum {
+#define B4P(x) C_##x,
+#define B4V(x)
+  B4_PFIELDS
+#undef B4P
+#undef B4V
+  kNumP
+};
+__host__ __device__ constexpr bool in_list(const int* l, int m, int f) {
+  for (int k = 0; k < m; ++k)
+    if (l[k] == f) return true;
+  return false;
+}
+// column f is read from the card in card mode cm
+__host__ __device__ constexpr bool card_col(int cm, int f) {
+  return cm == 0 ? false
+                 : f < kNumP ? !in_list(kGeomP, (int)(sizeof(kGeomP) / sizeof(int)), f)
+                             : cm == 2 && !in_list(kGeomV, (int)(sizeof(kGeomV) / sizeof(int)), f);
+}
+static_assert(P_a0 < kNumP && V_vth0 >= kNumP, "size-dependent columns precede instance values");
+
 enum IField { F_MODEL, F_DEXIST, F_SEXIST, NIF };
 // Values the reference's Instance keeps from one evaluation to the next and
 // reads before writing: vgb (as vgb_orig), and the junction charges and
@@ -389,11 +434,26 @@ static int setup(Group& g) {
   }
   // cards after every instance: the model's temperature values are those
   // the last instance's updateTemperature wrote
-  g.mcard_stride = sizeof(DevCard) / sizeof(double);
+  g.mcard_stride = kCardStride;
   g.mcard.assign(g.models.size() * g.mcard_stride, 0.0);
   for (size_t k = 0; k < models.size(); ++k) {
     const DevCard c = make_card(models[k]);
     std::memcpy(&g.mcard[k * g.mcard_stride], &c, sizeof(c));
+  }
+  // card-uniform columns: every instance of each card holds the same bits
+  std::vector<int64_t> first(models.size(), -1);
+  for (int64_t i = 0; i < n; ++i)
+    if (first[g.inst_model[i]] < 0) first[g.inst_model[i]] = i;
+  g.ucol[0] = g.ucol[1] = g.ucol[2] = 0;
+  for (int f = 0; f < ND; ++f) {
+    bool uniform = true;
+    const double* col = &g.dfield[(size_t)f * n];
+    for (int64_t i = 0; i < n && uniform; ++i)
+      uniform = std::memcmp(&col[i], &col[first[g.inst_model[i]]], sizeof(double)) == 0;
+    if (!uniform) continue;
+    g.ucol[f >> 6] |= 1ull << (f & 63);
+    for (size_t k = 0; k < models.size(); ++k)
+      if (first[k] >= 0) g.mcard[k * g.mcard_stride + kCardBase + f] = col[first[k]];
   }
   return XA_OK;
 }
@@ -475,27 +535,34 @@ XA_HD void polyDepletion(double phi, double ngate, double epsgate, double coxe, 
 XA_HD double stv(const double* s, int32_t lid) { return lid >= 0 ? s[lid] : 0.0; }
 
 // per-instance parameter columns, read through `dfb` (see reload_params)
-#define PP(x) (dfb[(int64_t)(P_##x) * a.n + i])
-#define IV(x) dfld(a, V_##x, i)
+// (card-uniform columns of card mode CM from the card's copy, see card_col)
+#define PP(x) (card_col(CM, P_##x) ? Ccol[P_##x] : dfb[(int64_t)(P_##x) * a.n + i])
+#define IV(x) (card_col(CM, V_##x) ? Ccol[V_##x] : dfld(a, V_##x, i))
 
 // A section boundary for the per-instance parameter columns: the compiler
 // takes the column base as changed, so a parameter used in a later section is
-// read again there (from L1/L2) instead of being held in registers across
-// sections.  The eval's register peak (the DC model around Vasat) holds
+// read again there (from L1/L2, or from the LDS card for a card-uniform
+// column) instead of being held in registers across sections.  (The base is
+// a vector operand: the compiler may fold a column read and a card read into
+// one per-lane address.)  The eval's register peak (the DC model around Vasat) holds
 // fewer long-lived parameters, and spills less.
 __device__ __forceinline__ void reload_params(const double*& dfb) {
-  asm volatile("" : "+s"(dfb));
+  asm volatile("" : "+v"(dfb));
 }
 
 // LDSCARD: the block's model cards are staged in LDS by the kernel (s_card,
 // in the order of the block's card table; card_slot gives the instance's);
 // otherwise they are read from global memory.
-template <int MODE, bool LDSCARD>
+template <int MODE, bool LDSCARD, int CM>
 __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
+  // the fused kernel stages every card of the group (slot = model index),
+  // the stored and pattern kernels their block's card table
   const DevCard& C =
-      LDSCARD ? *reinterpret_cast<const DevCard*>(s_card + (int)a.card_slot[i] * a.mstride)
+      LDSCARD ? *reinterpret_cast<const DevCard*>(
+                    s_card + (MODE == 1 ? ifld(a, F_MODEL, i) : (int)a.card_slot[i]) * a.mstride)
               : *reinterpret_cast<const DevCard*>(a.mcard + (int64_t)ifld(a, F_MODEL, i) * a.mstride);
+  const double* Ccol = reinterpret_cast<const double*>(&C) + kCardBase;
   const double* dfb = a.dfield;
   const xa_solver_state& ss = a.ss;
   const xa_device_options& opt = a.opt;
@@ -2872,20 +2939,27 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
 // The stored kernel with LDSCARD first stages the cards of its block's table
 // (the distinct models of its instances, at most kBlockCards) in LDS, so the
 // eval's card reads are LDS reads instead of L1/L2 hits.
-template <bool FUSED, bool LDSCARD>
-__global__ void __launch_bounds__(256, FUSED ? 1 : 2) k_update(KArgs a) {
+#ifndef XA_B4_STORED_WAVES
+#define XA_B4_STORED_WAVES 2   // waves per SIMD of the stored-mode update (a tuning knob)
+#endif
+template <bool FUSED, bool LDSCARD, int CM>
+__global__ void __launch_bounds__(256, FUSED ? 1 : XA_B4_STORED_WAVES) k_update(KArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (LDSCARD) {
-    constexpr int kStride = (int)(sizeof(DevCard) / sizeof(double));
+    constexpr int kStride = kCardStride;
     __shared__ double s_card[kBlockCards * kStride];
-    const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
-    const int used = tab[0];   // 1 <= used <= kBlockCards
-    for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
-      s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+    if (FUSED) {   // every card of the group (at most kBlockCards)
+      for (int k = threadIdx.x; k < a.n_models * kStride; k += blockDim.x) s_card[k] = a.mcard[k];
+    } else {
+      const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
+      const int used = tab[0];   // 1 <= used <= kBlockCards
+      for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+        s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+    }
     __syncthreads();
-    eval<FUSED ? 1 : 0, true>(a, i, s_card);
+    eval<FUSED ? 1 : 0, true, CM>(a, i, s_card);
   } else {
-    eval<FUSED ? 1 : 0, false>(a, i, nullptr);
+    eval<FUSED ? 1 : 0, false, CM>(a, i, nullptr);
   }
   if (FUSED) fused_writeout(a);
 }
@@ -2895,10 +2969,10 @@ __global__ void __launch_bounds__(256, FUSED ? 1 : 2) k_update(KArgs a) {
 // order as the eval produces them, and the block writes the spans back.  No
 // value buffers and no scatter kernels; one wave per SIMD (the spans and the
 // eval's registers).
-template <bool LDSCARD>
+template <bool LDSCARD, int CM>
 __global__ void __launch_bounds__(256, 1) k_update_pattern(KArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  constexpr int kStride = (int)(sizeof(DevCard) / sizeof(double));
+  constexpr int kStride = kCardStride;
   __shared__ double s_card[LDSCARD ? kBlockCards * kStride : 1];
   pattern_stage(a, true);
   if (LDSCARD) {
@@ -2908,13 +2982,31 @@ __global__ void __launch_bounds__(256, 1) k_update_pattern(KArgs a) {
       s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
   }
   __syncthreads();
-  eval<2, LDSCARD>(a, i, s_card);
+  eval<2, LDSCARD, CM>(a, i, s_card);
   __syncthreads();
   pattern_stage(a, false);
 }
 
 #undef PP
 #undef IV
+
+// The largest card mode whose columns are all card-uniform in this group.
+static int card_mode(const Group& g) {
+  for (int cm = 2; cm > 0; --cm) {
+    bool ok = true;
+    for (int f = 0; f < ND && ok; ++f)
+      ok = !card_col(cm, f) || ((g.ucol[f >> 6] >> (f & 63)) & 1);
+    if (ok) return cm;
+  }
+  return 0;
+}
+
+static int card_columns(const Group& g) {
+  const int cm = card_mode(g);
+  int cols = 0;
+  for (int f = 0; f < ND; ++f) cols += card_col(cm, f) ? 1 : 0;
+  return cols;
+}
 
 static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& v, bool fused) {
   if (!v.nextSol || !v.nextSto) return fail(XA_ERR_ARG, "bsim4 update: nextSol/nextSto required");
@@ -2928,11 +3020,22 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
   // the tables exist for the current update block when no block uses more
   // than kBlockCards models (built at finalize and by the autotuner)
   const bool lds_cards = g.card_tables && g.card_block == g.split_block &&
-                         a.mstride == (int)(sizeof(DevCard) / sizeof(double));
-  return launch_model_kernel(g, a, fused,
-                             lds_cards ? (const void*)k_update<false, true>
-                                       : (const void*)k_update<false, false>,
-                             (const void*)k_update<true, false>);
+                         a.mstride == kCardStride;
+  const int cm = card_mode(g);
+  const void* stored[2][3] = {
+      {(const void*)k_update<false, false, 0>, (const void*)k_update<false, false, 1>,
+       (const void*)k_update<false, false, 2>},
+      {(const void*)k_update<false, true, 0>, (const void*)k_update<false, true, 1>,
+       (const void*)k_update<false, true, 2>}};
+  const void* fused_k[2][3] = {
+      {(const void*)k_update<true, false, 0>, (const void*)k_update<true, false, 1>,
+       (const void*)k_update<true, false, 2>},
+      {(const void*)k_update<true, true, 0>, (const void*)k_update<true, true, 1>,
+       (const void*)k_update<true, true, 2>}};
+  // the fused kernel stages the cards in LDS beside its stamp rows when both fit
+  const bool fused_lds = a.n_models <= kBlockCards && a.mstride == kCardStride &&
+                         g.fused_lds + sizeof(double) * kBlockCards * kCardStride <= kMaxLds;
+  return launch_model_kernel(g, a, fused, stored[lds_cards][cm], fused_k[fused_lds][cm]);
 }
 
 static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors& v) {
@@ -2944,9 +3047,13 @@ static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors&
   KArgs a;
   fill_kargs(g, ss, v, true, a);   // kinds with an output, a target and not skipped
   const bool lds_cards = g.card_tables && g.card_block == g.pattern_B &&
-                         a.mstride == (int)(sizeof(DevCard) / sizeof(double));
-  return launch_pattern_kernel(g, a, lds_cards ? (const void*)k_update_pattern<true>
-                                               : (const void*)k_update_pattern<false>);
+                         a.mstride == kCardStride;
+  const void* k[2][3] = {
+      {(const void*)k_update_pattern<false, 0>, (const void*)k_update_pattern<false, 1>,
+       (const void*)k_update_pattern<false, 2>},
+      {(const void*)k_update_pattern<true, 0>, (const void*)k_update_pattern<true, 1>,
+       (const void*)k_update_pattern<true, 2>}};
+  return launch_pattern_kernel(g, a, k[lds_cards][card_mode(g)]);
 }
 
 // Algorithmic bytes per instance: the SoA columns and card index read once,
@@ -2954,7 +3061,10 @@ static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors&
 // LIDs + writes, vgb read + write, junction charge state written; the loads
 // write value + target per entry.
 static double traffic(const Group& g, int phase) {
-  const double cols = ND, lids = 6 + NSTO + NSTA + 1 /*model index*/;
+  // per-instance columns read: those outside the card mode's set (the others
+  // come with the card, one copy per model)
+  const double cols = ND - card_columns(g);
+  const double lids = 6 + NSTO + NSTA + 1 /*model index*/;
   if (phase == 0)   // transient: the junction charge state is written, not read
     return 8.0 * (cols + 6 /*node values*/ + 12 /*sto*/ + 3 /*sta*/ + 2 /*vgb*/ + 4 /*qb,cap*/) + 4.0 * lids;
   const bool lim = g.opt.voltageLimiterFlag != 0;
@@ -2991,6 +3101,7 @@ const ModelImpl* model_bsim4() {
     x.skip_kind = skip_kind;
     x.state_names = kStateNames;
     x.n_state = NS;
+    x.card_columns = card_columns;
     x.traffic = traffic;
     x.fused_block = 128;
     x.launch_pattern = launch_pattern;

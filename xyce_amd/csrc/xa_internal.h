@@ -115,6 +115,8 @@ struct ModelImpl {
   // pattern-fused load (one kernel staging the outputs of pattern plans in
   // LDS); null when the model has none
   int (*launch_pattern)(Group&, const xa_solver_state&, const xa_vectors&);
+  // per-instance columns the update reads from its card (null: none)
+  int (*card_columns)(const Group&);
 };
 
 const ModelImpl* model_diode();
@@ -150,6 +152,9 @@ struct Group {
   std::vector<int32_t> ifield;   // [field][inst]
   std::vector<double> mcard;     // device model cards, model-major
   int mcard_stride = 0;
+  // per-instance columns served from the card table instead (bit f: column f
+  // holds one value per card; models that use it put the value in the card)
+  uint64_t ucol[3] = {0, 0, 0};
   int n_state = 0;
   std::vector<double> state_init;  // [state][inst]
 

@@ -14,6 +14,7 @@ struct KArgs {
   const double* __restrict__ mcard;    // [model][mstride]
   int mstride;
   int n_models;                        // cards in mcard
+  uint64_t ucol[3];                    // card-uniform per-instance columns (Group::ucol)
   const int32_t* blk_cards;            // [block][1 + kBlockCards]: count, models to stage
   const uint8_t* card_slot;            // [inst] slot of its model in its block's table
   const int32_t* __restrict__ nodes;   // [local node][inst]
@@ -113,6 +114,19 @@ __device__ __forceinline__ void pattern_stage(const KArgs& a, bool load) {
 // memory round trips instead of three per target.  Every thread of the block
 // must call it (it synchronises).
 constexpr int kUnroll = 8;
+// Segments longer than this (a shared supply or ground row of a small
+// circuit) are summed by a whole wavefront: its lanes gather 64 terms at a
+// time from the LDS rows in parallel, then every lane adds them in segment
+// order from register broadcasts -- the same additions in the same order as
+// one thread would make, without one index load per term on the chain.
+constexpr int32_t kWaveSegment = 16;
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ void fused_writeout(const KArgs& a) {
   __syncthreads();
   const int b = blockIdx.x;
@@ -125,6 +139,7 @@ __device__ __forceinline__ void fused_writeout(const KArgs& a) {
     const uint16_t* src = a.bsrc[k];
     const int32_t* tgt = a.btgt[k];
     const bool assign = k == K_LEADF || k == K_LEADQ;
+    // short segments: one thread per target, kUnroll targets at a time
     for (int32_t t0 = tb + (int32_t)threadIdx.x; t0 < te; t0 += kUnroll * B) {
       int32_t j0[kUnroll], j1[kUnroll];
       int64_t o[kUnroll];
@@ -141,14 +156,14 @@ __device__ __forceinline__ void fused_writeout(const KArgs& a) {
       uint16_t s0[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        if (t0 + u * B < te) {
+        if (t0 + u * B < te && (assign || j1[u] - j0[u] <= kWaveSegment)) {
           s0[u] = src[assign ? j1[u] - 1 : j0[u]];
           acc[u] = assign ? 0.0 : out[o[u]];
         }
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        if (t0 + u * B < te) {
+        if (t0 + u * B < te && (assign || j1[u] - j0[u] <= kWaveSegment)) {
           if (assign) {
             out[o[u]] = xa_lds[s0[u]];
           } else {
@@ -157,6 +172,37 @@ __device__ __forceinline__ void fused_writeout(const KArgs& a) {
             out[o[u]] = v;
           }
         }
+      }
+    }
+    if (assign) continue;
+    // long segments: each wavefront scans its share of the targets and sums
+    // every long one it finds with all its lanes
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = (int)threadIdx.x >> 6, nwaves = (B + 63) >> 6;
+    for (int32_t c0 = tb + wave * 64; c0 < te; c0 += nwaves * 64) {
+      const int32_t t = c0 + lane;
+      int32_t j0 = 0, j1 = 0;
+      if (t < te) {
+        j0 = seg[t];
+        j1 = seg[t + 1];
+      }
+      uint64_t m = __ballot(t < te && j1 - j0 > kWaveSegment);
+      while (m) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const int32_t tt = c0 + l;
+        const int32_t a0 = __shfl(j0, l), a1 = __shfl(j1, l);
+        const int64_t o = tgt ? tgt[tt] : (int64_t)a.bt0[k] + tt;
+        double acc = out[o];
+        for (int32_t base = a0; base < a1; base += 64) {
+          const int32_t j = base + lane;
+          const double x = j < a1 ? xa_lds[src[j]] : 0.0;
+          const int cnt = (a1 - base) < 64 ? (int)(a1 - base) : 64;
+#pragma unroll
+          for (int q = 0; q < 64; ++q)
+            if (q < cnt) acc += readlane_f64(x, q);
+        }
+        if (lane == 0) out[o] = acc;
       }
     }
   }

@@ -107,7 +107,7 @@ constexpr int kLongSegment = 64;
 // onto the output.  Deterministic (the same bits on every run), and within
 // (kRailChunk + chunks / 64 + 7) rounding steps of the contributions'
 // magnitude sum of the one-by-one order.
-constexpr int64_t kRailSegment = 512;
+constexpr int64_t kRailSegment = 1024;
 constexpr int kRailChunk = 256;
 
 // One thread per target: the target's terms added onto the output in order.
@@ -182,7 +182,7 @@ __device__ __forceinline__ double ordered_sum(double acc, I s0, I s1, const I* _
                                               const double* __restrict__ vals, double* stage) {
   constexpr int PER = ROUND / 64;
   if (s1 <= s0) return acc;
-  const int lane = threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;   // one wavefront per sum
   I ix[PER];
   double v[PER];
   round_index<I, GATHER, PER>(ix, s0, s1, src, lane);
@@ -244,29 +244,38 @@ struct SegSet {
   int n;
 };
 
+// the kind whose short-block (LONG false) or long-target range holds x
 template <bool LONG>
-__device__ __forceinline__ int seg_kind(const SegSet& s) {
+__device__ __forceinline__ int seg_kind(const SegSet& s, int64_t x) {
   int k = 0;
-  while (k + 1 < s.n && (int64_t)blockIdx.x >= (LONG ? s.k[k + 1].lb0 : s.k[k + 1].b0)) ++k;
+  while (k + 1 < s.n && x >= (LONG ? s.k[k + 1].lb0 : s.k[k + 1].b0)) ++k;
   return k;
 }
 
-__global__ void __launch_bounds__(256) k_scatter_add_set(SegSet s) {
-  const SegKind& d = s.k[seg_kind<false>(s)];
-  const int64_t t = ((int64_t)blockIdx.x - d.b0) * blockDim.x + threadIdx.x;
-  if (t >= d.nt) return;
-  add_short<int32_t>(t, d.target, d.t0, d.target == nullptr, d.seg, d.src, d.vals, d.out);
-}
-
-__global__ void __launch_bounds__(64) k_scatter_long_set(SegSet s) {
+// The short and the long segments of the set in one launch: the first
+// short_blocks blocks add the short segments (one thread per target), the
+// rest give each of their four wavefronts one long target.
+constexpr int kLongPerBlock = 4;
+__global__ void __launch_bounds__(256) k_scatter_set(SegSet s, int64_t short_blocks,
+                                                     int64_t n_long) {
   constexpr int ROUND = (int)kRailSegment;
-  __shared__ double stage[ROUND];
-  const SegKind& d = s.k[seg_kind<true>(s)];
-  const int32_t t = d.long_t[(int64_t)blockIdx.x - d.lb0];
+  __shared__ double stage[kLongPerBlock][ROUND];
+  if ((int64_t)blockIdx.x < short_blocks) {
+    const SegKind& d = s.k[seg_kind<false>(s, blockIdx.x)];
+    const int64_t t = ((int64_t)blockIdx.x - d.b0) * blockDim.x + threadIdx.x;
+    if (t >= d.nt) return;
+    add_short<int32_t>(t, d.target, d.t0, d.target == nullptr, d.seg, d.src, d.vals, d.out);
+    return;
+  }
+  const int wave = (int)threadIdx.x >> 6;
+  const int64_t L = ((int64_t)blockIdx.x - short_blocks) * kLongPerBlock + wave;
+  if (L >= n_long) return;   // the whole wavefront
+  const SegKind& d = s.k[seg_kind<true>(s, L)];
+  const int32_t t = d.long_t[L - d.lb0];
   const int64_t o = d.target ? (int64_t)d.target[t] : (int64_t)d.t0 + t;
-  const double acc =
-      ordered_sum<int32_t, true, ROUND>(d.out[o], d.seg[t], d.seg[t + 1], d.src, d.vals, stage);
-  if (threadIdx.x == 0) d.out[o] = acc;
+  const double acc = ordered_sum<int32_t, true, ROUND>(d.out[o], d.seg[t], d.seg[t + 1], d.src,
+                                                        d.vals, stage[wave]);
+  if ((threadIdx.x & 63) == 0) d.out[o] = acc;
 }
 
 // One wavefront per rail chunk of at most kRailChunk contributions, summed
@@ -438,6 +447,7 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   a.mcard = g.d_mcard;
   a.mstride = g.mcard_stride;
   a.n_models = (int)g.models.size();
+  for (int w = 0; w < 3; ++w) a.ucol[w] = g.ucol[w];
   a.blk_cards = g.d_blk_cards;
   a.card_slot = g.d_card_slot;
   a.nodes = g.d_nodes;
@@ -1241,8 +1251,8 @@ int xa_update_secondary_state(xa_group gh, const xa_solver_state* ss, const xa_v
 }
 
 // The scatters of one call.  When every kind on the call writes its own
-// output vector, the segmented plans with 32-bit indices go in one short and
-// one long launch (k_scatter_add_set / k_scatter_long_set), then their rails;
+// output vector, the segmented plans with 32-bit indices go in one launch for
+// their short and long segments (k_scatter_set), then their rails;
 // the other kinds (pattern plans, lead assignments, 64-bit plans) follow one
 // by one.  Different kinds write different vectors, so the order among them
 // does not change any bit.
@@ -1283,10 +1293,10 @@ static int run_scatters(Group& g, const xa_solver_state& ss, const xa_vectors& v
     lblocks += p.n_long;
   }
   hipStream_t st = g.ctx->stream;
-  if (blocks > 0)
-    hipLaunchKernelGGL(k_scatter_add_set, dim3((unsigned)blocks), dim3(kBlock), 0, st, s);
-  if (lblocks > 0)
-    hipLaunchKernelGGL(k_scatter_long_set, dim3((unsigned)lblocks), dim3(64), 0, st, s);
+  if (blocks + lblocks > 0)
+    hipLaunchKernelGGL(k_scatter_set,
+                       dim3((unsigned)(blocks + (lblocks + kLongPerBlock - 1) / kLongPerBlock)),
+                       dim3(kBlock), 0, st, s, blocks, lblocks);
   for (int k : live) {
     const Plan& p = g.plan[k];
     if (!distinct || kind_is_assign(k) || p.pattern || !p.idx32 || p.n_rails == 0) continue;
@@ -1357,8 +1367,13 @@ static int load_all(Group* g, const xa_solver_state* ss, const xa_vectors* v) {
   }
   if ((rc = xa_update_state(gh, ss, v)) != XA_OK) return rc;
   if ((rc = xa_update_secondary_state(gh, ss, v)) != XA_OK) return rc;
-  if ((rc = xa_load_vectors(gh, ss, v)) != XA_OK) return rc;
-  return xa_load_matrices(gh, ss, v);
+  if (g->timing) {   // the phases timed one by one
+    if ((rc = xa_load_vectors(gh, ss, v)) != XA_OK) return rc;
+    return xa_load_matrices(gh, ss, v);
+  }
+  // the vector and the matrix kinds write different vectors: their scatters
+  // share one launch (a small group's load is launch-latency bound)
+  return run_scatters(*g, *ss, *v, {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ, K_DFDX, K_DQDX});
 }
 
 int xa_load_all(xa_group gh, const xa_solver_state* ss, const xa_vectors* v) {
@@ -1484,10 +1499,8 @@ static int load_pass(Group& g, const xa_solver_state& ss, const xa_vectors& v, i
   if (path == 2) return g.impl->launch_update(g, ss, v, true);
   if ((rc = g.impl->launch_update(g, ss, v, false)) != XA_OK) return rc;
   if (g.impl->launch_secondary && (rc = g.impl->launch_secondary(g, ss, v)) != XA_OK) return rc;
-  // the same merged launches as xa_load_vectors and xa_load_matrices
-  if ((rc = run_scatters(g, ss, v, {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ})) != XA_OK)
-    return rc;
-  return run_scatters(g, ss, v, {K_DFDX, K_DQDX});
+  // the same merged launch as xa_load_all's split route
+  return run_scatters(g, ss, v, {K_F, K_Q, K_FLIM, K_QLIM, K_LEADF, K_LEADQ, K_DFDX, K_DQDX});
 }
 
 // Mean milliseconds of one pass over `reps` passes after one untimed pass.
@@ -1626,6 +1639,16 @@ int xa_group_autotune(xa_group gh, const xa_solver_state* ss, const xa_vectors* 
   (void)hipFree(saved);
   if (rc == XA_OK && e != hipSuccess) rc = hip_check(e, "autotune: restoring the instance state");
   return rc;
+}
+
+int xa_group_param_layout(xa_group gh, int* per_instance_columns, int* card_columns) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  if (!g) return fail(XA_ERR_ARG, "null group");
+  if (g->phase < PH_SETUP) return fail(XA_ERR_STATE, "param_layout before setup");
+  const int card = g->impl->card_columns ? g->impl->card_columns(*g) : 0;
+  if (per_instance_columns) *per_instance_columns = g->n_dfield - card;
+  if (card_columns) *card_columns = card;
+  return XA_OK;
 }
 
 int xa_group_plan_info(xa_group gh, int kind, int64_t* n_targets, int64_t* n_sources,

@@ -147,10 +147,16 @@ class BatchTransient:
         z = lambda m: torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
         self.d = {"nextSol": z(t.n_nodes), "currSol": z(t.n_nodes), "nextSta": z(t.n_sta),
                   "currSta": z(t.n_sta), "lastSta": z(t.n_sta), "nextStaDeriv": z(t.n_sta),
-                  "nextSto": z(t.n_sto), "currSto": z(t.n_sto), "f": z(t.n_nodes),
-                  "q": z(t.n_nodes), "b": z(t.n_nodes), "dFdxdVp": z(t.n_nodes),
-                  "dQdxdVp": z(t.n_nodes), "storeLeadF": z(t.n_sto), "storeLeadQ": z(t.n_sto),
-                  "dFdx": z(t.nnz), "dQdx": z(t.nnz)}
+                  "nextSto": z(t.n_sto), "currSto": z(t.n_sto), "b": z(t.n_nodes),
+                  "storeLeadF": z(t.n_sto), "storeLeadQ": z(t.n_sto)}
+        # the outputs zeroed before every load share one buffer: one fill
+        sizes = {"f": t.n_nodes, "q": t.n_nodes, "dFdxdVp": t.n_nodes, "dQdxdVp": t.n_nodes,
+                 "dFdx": t.nnz, "dQdx": t.nnz}
+        self.outputs = z(sum(max(m, 1) for m in sizes.values()))
+        off = 0
+        for k, m in sizes.items():
+            self.d[k] = self.outputs[off:off + max(m, 1)]
+            off += max(m, 1)
         self.vectors = make_vectors(**self.d)
         c_lin = np.zeros(t.n_nodes)
         for r, ring in enumerate(t.rings):
@@ -193,9 +199,7 @@ class BatchTransient:
         ss.nltermScale = 1.0
         for k, v in flags.items():
             setattr(ss, k, v)
-        d = self.d
-        for k in ("f", "q", "dFdxdVp", "dQdxdVp", "dFdx", "dQdx"):
-            d[k].zero_()
+        self.outputs.zero_()
         m = self.master
         m.updateState(ss, self.vectors)
         m.updateSecondaryState(ss, self.vectors)

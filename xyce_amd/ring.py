@@ -116,8 +116,16 @@ class GpuLoader:
     device vectors (or, fused, the one-call xa_load_all with the same
     results), outputs copied back to the host."""
 
-    def __init__(self, ring, device=0, options=None, fused=False):
+    def __init__(self, ring, device=0, options=None, fused=False, graph=False):
+        """graph: each load (output zeroing and the device calls) is captured
+        once per set of solver flags as a HIP graph and replayed, which
+        removes the per-kernel launch cost of this launch-bound size; the
+        flags are baked into the captured arguments, and newtonIter is only
+        tested for == 0 by the loads, so one graph serves every later
+        iteration of a flag set."""
         self.fused = fused
+        self.graph = graph
+        self.graphs = {}
         import torch
         from . import Context, Master, SolverState, make_vectors
         self.torch = torch
@@ -130,10 +138,16 @@ class GpuLoader:
         z = lambda m: torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
         self.d = {"nextSol": z(t.n_nodes), "currSol": z(t.n_nodes), "nextSta": z(t.n_sta),
                   "currSta": z(t.n_sta), "lastSta": z(t.n_sta), "nextStaDeriv": z(t.n_sta),
-                  "nextSto": z(t.n_sto), "currSto": z(t.n_sto), "f": z(t.n_nodes),
-                  "q": z(t.n_nodes), "b": z(t.n_nodes), "dFdxdVp": z(t.n_nodes),
-                  "dQdxdVp": z(t.n_nodes), "storeLeadF": z(t.n_sto), "storeLeadQ": z(t.n_sto),
-                  "dFdx": z(t.nnz), "dQdx": z(t.nnz)}
+                  "nextSto": z(t.n_sto), "currSto": z(t.n_sto), "b": z(t.n_nodes),
+                  "storeLeadF": z(t.n_sto), "storeLeadQ": z(t.n_sto)}
+        # the outputs zeroed before every load share one buffer: one fill
+        self.sizes = sizes = {"f": t.n_nodes, "q": t.n_nodes, "dFdxdVp": t.n_nodes,
+                              "dQdxdVp": t.n_nodes, "dFdx": t.nnz, "dQdx": t.nnz}
+        self.outputs = z(sum(max(m, 1) for m in sizes.values()))
+        off = 0
+        for k, m in sizes.items():
+            self.d[k] = self.outputs[off:off + max(m, 1)]
+            off += max(m, 1)
         self.vectors = make_vectors(**self.d)
         self.load_seconds = 0.0
         self.loads = 0
@@ -147,10 +161,41 @@ class GpuLoader:
             setattr(ss, k, v)
         d = self.d
         d["nextSol"].copy_(torch.from_numpy(np.ascontiguousarray(x)))
-        for k in ("f", "q", "dFdxdVp", "dQdxdVp", "dFdx", "dQdx"):
-            d[k].zero_()
+        if self.graph:
+            key = tuple(sorted((k, v) for k, v in flags.items() if k != "newtonIter")) + \
+                (flags.get("newtonIter", 0) == 0,)
+            g = self.graphs.get(key)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                main = torch.cuda.current_stream()
+                with torch.cuda.graph(g):
+                    self.ctx.set_stream(torch.cuda.current_stream())
+                    try:
+                        self._calls(ss)
+                    finally:
+                        self.ctx.set_stream(main)
+                self.graphs[key] = g
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.replay()
+        else:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self._calls(ss)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        self.load_seconds += time.perf_counter() - t0
+        self.loads += 1
+        # one device-to-host copy of the output buffer, cut into the vectors
+        host = self.outputs.cpu().numpy()
+        out, off = {}, 0
+        for k, m in self.sizes.items():
+            out[k] = host[off:off + m]
+            off += max(m, 1)
+        return out
+
+    def _calls(self, ss):
+        """Zero the outputs and load (the four Master calls, or xa_load_all)."""
+        self.outputs.zero_()
         m = self.master
         if self.fused:
             m.loadAll(ss, self.vectors)
@@ -159,12 +204,6 @@ class GpuLoader:
             m.updateSecondaryState(ss, self.vectors)
             m.loadDAEVectors(ss, self.vectors)
             m.loadDAEMatrices(ss, self.vectors)
-        torch.cuda.synchronize()
-        self.load_seconds += time.perf_counter() - t0
-        self.loads += 1
-        n, nnz = self.topo.n_nodes, self.topo.nnz
-        return {k: d[k][:(nnz if k in ("dFdx", "dQdx") else n)].cpu().numpy()
-                for k in ("f", "q", "dFdxdVp", "dQdxdVp", "dFdx", "dQdx")}
 
     def accept(self):
         """End of a time step: the converged store/state become the previous
