@@ -553,7 +553,7 @@ __device__ __forceinline__ void reload_params(const double*& dfb) {
 // LDSCARD: the block's model cards are staged in LDS by the kernel (s_card,
 // in the order of the block's card table; card_slot gives the instance's);
 // otherwise they are read from global memory.
-template <int MODE, bool LDSCARD, int CM>
+template <int MODE, bool LDSCARD, int CM, bool RELOAD = true>
 __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
   // the fused kernel stages every card of the group (slot = model index),
@@ -685,7 +685,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   ChargeComputationNeeded = (ss.tranopFlag || ss.acopFlag || ss.transientFlag) ? 1 : 0;
   limitedFlag = 0;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* node voltages (:7919-7974) */
   Vd = solv(a.v.nextSol, li_Drain);
   Vs = solv(a.v.nextSol, li_Source);
@@ -751,7 +751,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   Vgegm_orig = Vgegm;
   Vgmgp_orig = Vgmgp;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* initial conditions (:8052-8097) */
   if (ss.initJctFlag && !IV(OFF) && opt.voltageLimiterFlag) {
     int setv = 1;
@@ -782,7 +782,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     vds = vsbs = vdes = vses = qdef = 0.0;
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* previous iteration's junction voltages (:8100-8158) */
   if (ss.newtonIter == 0) {
     newtonIterOld = 0;
@@ -839,7 +839,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   (void)vged_old; (void)vgmd_old; (void)vges_old; (void)vgms_old; (void)vdbs_old;
   (void)vsbs_old; (void)vdbd_old; (void)vbd_old;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* voltage limiting (:8165-8359), rgateMod = rbodyMod = rdsMod = 0 */
   if (opt.voltageLimiterFlag && !(ss.initFixFlag && IV(OFF))) {
     if (ss.newtonIter >= 0 && !(ss.initJctFlag)) {
@@ -888,7 +888,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   vbs_jct = (!irbodyMod) ? vbs : vsbs;
   vbd_jct = (!irbodyMod) ? vbd : vdbd;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* source/drain junction diodes (:8425-8769) */
   Nvtms = C.vtm * C.SjctEmissionCoeff;
   if ((IV(Aseff) <= 0.0) && (IV(Pseff) <= 0.0)) SourceSatCurrent = 1.0e-14;
@@ -1033,7 +1033,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     }
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* trap-assisted tunneling and recombination (:8655-8769) */
   Nvtmrssws = C.vtm0 * C.njtsswstemp;
   Nvtmrsswgs = C.vtm0 * C.njtsswgstemp;
@@ -1122,7 +1122,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   cbd -= IV(DjctTempRevSatCur) * (T2 - 1.0) + IV(DswTempRevSatCur) * (T4 - 1.0)
             + IV(DswgTempRevSatCur) * (T6 - 1.0);
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* mode (:8773-8788) */
   if (vds >= 0.0) {
     mode = 1;
@@ -1141,7 +1141,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   epsrox = C.epsrox;
   toxe = C.toxe;
   epssub = kEpsSi;
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* MOSFET homotopy (:8818-8838), one gain-scale block, no staggering */
   if (ss.artParameterFlag) {
     const double alpha = ss.gainScale;
@@ -1150,7 +1150,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     Vgs = (alpha) * Vgs + (1.0 - alpha) * vgstConst;
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* Vbseff (:8849-8868) */
   T0 = Vbs - IV(vbsc) - 0.001;
   T1 = sqrt(T0 * T0 - 0.004 * IV(vbsc));
@@ -1177,7 +1177,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   Vtm = C.vtm;
   Vtm0 = C.vtm0;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* threshold voltage (:8883-9052) */
   T3 = sqrt(Xdep);
   V0 = PP(vbi) - PP(phi);
@@ -1297,7 +1297,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
   von = Vth;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* poly gate depletion (:9055-9080) */
   T0 = IV(vfb) + PP(phi);
   T1 = kEpsSi;
@@ -1312,7 +1312,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
   Vgst = Vgs_eff - Vth;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* effective Vgst (:9085-9150) */
   T0 = n * Vtm;
   T1 = PP(mstar) * Vgst;
@@ -1368,7 +1368,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
   dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* effective channel geometry and series resistance (:9153-9198) */
   T9 = sqrtPhis - PP(sqrtPhi);
   Weff = PP(weff) - 2.0 * (PP(dwg) * Vgsteff + PP(dwb) * T9);
@@ -1401,7 +1401,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     else grdsw = 0.0;
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* bulk charge effect (:9201-9258) */
   T9 = 0.5 * PP(k1ox) * Lpe_Vb / sqrtPhis;
   T1 = T9 + IV(k2ox) - PP(k3b) * Vth_NarrowW;
@@ -1449,7 +1449,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   Abulk *= T0;
   Abulk0 *= T0;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* mobility (:9261-9341) */
   T14 = 0.0;
   if (C.mobMod == 0) {
@@ -1522,7 +1522,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   dueff_dVd = T9 * dDenomi_dVd;
   dueff_dVb = T9 * dDenomi_dVb;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* saturation field and Vdsat (:9344-9450) */
   WVCox = Weff * IV(vsattemp) * C.coxe;
   WVCoxRds = WVCox * Rds;
@@ -1599,7 +1599,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
   vdsat = Vdsat;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* effective Vds (:9453-9497) */
   T1 = Vdsat - Vds - PP(delta);
   dT1_dVg = dVdsat_dVg;
@@ -1635,7 +1635,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   diffVds = Vds - Vdseff_;
   Vdseff = Vdseff_;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* velocity overshoot (:9500-9540) */
   if ((C.lambdaGiven) && (C.lambda > 0.0)) {
     T1 = Leff * ueff;
@@ -1673,7 +1673,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     EsatL *= T10;
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* Vasat (:9543-9565) */
   tmp4 = 1.0 - 0.5 * Abulk * Vdsat / Vgst2Vtm;
   T9 = WVCoxRds * Vgsteff;
@@ -1694,7 +1694,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   dVasat_dVb = (dT0_dVb - Vasat * dT1_dVb) / T1;
   dVasat_dVd = dT0_dVd / T1;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* effective oxide capacitance, beta, Idl (:9568-9622) */
   tmp1 = IV(vtfbphi2);
   tmp2 = 2.0e8 * C.toxp;
@@ -1739,7 +1739,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   dIdl_dVd = T1 * dgche_dVd;
   dIdl_dVb = T1 * dgche_dVb - T2 * dRds_dVb;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* output resistance: CLM, DIBL, DITS, SCBE (:9625-9832) */
   if (PP(fprout) <= 0.0) {
     FP = 1.0;
@@ -1891,7 +1891,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   dIdsa_dVd = dIdsa_dVd * T9 + Idsa * dT9_dVd;
   Idsa *= T9;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* substrate current (:9835-9877) */
   tmp = PP(alpha0) + PP(alpha1) * Leff;
   if ((tmp <= 0.0) || (PP(beta0) <= 0.0)) {
@@ -1927,7 +1927,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   gbgs = Gbg;
   gbds = Gbd;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* drain current with SCBE (:9880-9902) */
   T9 = diffVds / VASCBE;
   T0 = 1.0 + T9;
@@ -1943,7 +1943,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   Gmb = (Ids * (dVdseff_dVb + dVdseff_dVg * dVgsteff_dVb) + Vdseff_ * tmp2) * dVbseff_dVb;
   cdrain = Ids * Vdseff_;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* source-end velocity limit (:9905-9945) */
   if ((C.vtlGiven) && (C.vtl > 0.0)) {
     T12 = 1.0 / Leff / CoxeffWovL;
@@ -1985,17 +1985,17 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   gmbs = Gmb;
   IdovVds = Ids;
   if (IdovVds <= 1.0e-9) IdovVds = 1.0e-9;
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* rgateMod <= 1 and no NQS: no gate resistance current conductances */
   gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* rdsMod = 0 (:10063-10069) */
   gstot = gstotd = gstotg = 0.0;
   gstots = gstotb = 0.0;
   gdtot = gdtotd = gdtotg = 0.0;
   gdtots = gdtotb = 0.0;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* GIDL (:10074-10126) */
   T0 = 3.0 * toxe;
   T1 = (vds - vgs_eff - PP(egidl)) / T0;
@@ -2029,7 +2029,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   ggidld = Ggidld;
   ggidlg = Ggidlg;
   ggidlb = Ggidlb;
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* GISL (:10132-10179) */
   T1 = (-vds - vgd_eff - PP(egisl)) / T0;
   if ((PP(agisl) <= 0.0) || (PP(bgisl) <= 0.0) || (T1 <= 0.0) || (PP(cgisl) <= 0.0) || (vbs > 0.0)) {
@@ -2062,7 +2062,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   ggisls = Ggisls;
   ggislg = Ggislg;
   ggislb = Ggislb;
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
   Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
   Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
@@ -2113,7 +2113,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   gIgcds = -(gIgcdg + gIgcdd + gIgcdb);
   cd = cdrain;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* noise charge (:10674-10703): rewrites the Abulk and Vdseff members */
   if (C.tnoiMod == 0) {
     Abulk = Abulk0 * PP(abulkCVfactor);
@@ -2136,7 +2136,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     qinv = Coxeff * PP(weffCV) * nf * PP(leffCV) * (Vgsteff - 0.5 * T0 + Abulk * T3);
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* intrinsic charges, capMod = 2 (:10707-11611) */
   if ((C.xpart < 0) || (!ChargeComputationNeeded)) {
     qgate = qdrn = qsrc = qbulk = 0.0;
@@ -2453,7 +2453,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   cbbb = -cbgb - cbdb - cbsb;
   csbb = -cgbb - cdbb - cbbb;
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* junction charges (:11662-11810) */
   if (ChargeComputationNeeded) {
     czbd = C.DunitAreaTempJctCap * IV(Adeff);
@@ -2538,7 +2538,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     }
   }
 
-  reload_params(dfb);
+  if (RELOAD) reload_params(dfb);
   /* overlap capacitances, capMod != 0 (:11812-11883) */
   vgdx = vgd;
   vgsx = vgs;
@@ -2957,11 +2957,31 @@ __global__ void __launch_bounds__(256, FUSED ? 1 : XA_B4_STORED_WAVES) k_update(
         s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
     }
     __syncthreads();
-    eval<FUSED ? 1 : 0, true, CM>(a, i, s_card);
+    eval<FUSED ? 1 : 0, true, CM, !FUSED>(a, i, s_card);
   } else {
-    eval<FUSED ? 1 : 0, false, CM>(a, i, nullptr);
+    eval<FUSED ? 1 : 0, false, CM, !FUSED>(a, i, nullptr);
   }
   if (FUSED) fused_writeout(a);
+}
+
+// Small launches (at most about one wavefront per SIMD over the GPU): the
+// stored-mode update with the block's cards in LDS, compiled for one wave per
+// SIMD so that every value stays in registers -- no spills, and no section
+// reloads of the parameter columns.  Such a launch has no second wave to hide
+// a latency behind, so each scratch or column reload would sit on its
+// critical path.
+constexpr int64_t kSmallUpdate = 65536;   // instances
+template <int CM>
+__global__ void __launch_bounds__(256, 1) k_update_small(KArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kStride = kCardStride;
+  __shared__ double s_card[kBlockCards * kStride];
+  const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
+  const int used = tab[0];   // 1 <= used <= kBlockCards
+  for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+    s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+  __syncthreads();
+  eval<0, true, CM, false>(a, i, s_card);
 }
 
 // Pattern-fused load (every kind a pattern plan): the block stages its output
@@ -3035,7 +3055,10 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
   // the fused kernel stages the cards in LDS beside its stamp rows when both fit
   const bool fused_lds = a.n_models <= kBlockCards && a.mstride == kCardStride &&
                          g.fused_lds + sizeof(double) * kBlockCards * kCardStride <= kMaxLds;
-  return launch_model_kernel(g, a, fused, stored[lds_cards][cm], fused_k[fused_lds][cm]);
+  const void* small[3] = {(const void*)k_update_small<0>, (const void*)k_update_small<1>,
+                         (const void*)k_update_small<2>};
+  const void* k_stored = lds_cards && g.n <= kSmallUpdate ? small[cm] : stored[lds_cards][cm];
+  return launch_model_kernel(g, a, fused, k_stored, fused_k[fused_lds][cm]);
 }
 
 static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors& v) {

@@ -74,17 +74,24 @@ struct Plan {
   int plast = 0;
   int pblock = 0;                  // instances per block of k_scatter_pattern
   int32_t* d_pslot = nullptr;      // E offsets
-  // Block plan of the fused load: the same targets grouped by the block of
-  // instances that writes them (defined when every target's contributions
-  // come from one block).  Sources are LDS slots (loff + e) * B + lane.
+  // The fused load needs every target's contributions to come from the
+  // instances of one block (its block plan is the group's flat plan).
   bool block_private = true;
   int loff = 0;                    // first LDS row of this kind
-  int32_t* d_bblk = nullptr;       // blocks + 1 offsets into the block's targets
-  int32_t* d_btgt = nullptr;       // target of each entry (null: contiguous from bt0)
-  int32_t bt0 = 0;
-  int32_t* d_bseg = nullptr;       // n_targets + 1 segment starts
-  uint16_t* d_bsrc = nullptr;      // n_src LDS slots
 };
+
+// Flat block plan of the fused load, every kind's targets together.  For
+// each block of fused_B instances, kind by kind (targets ascending), one
+// record per target: {target | kind << kKindShift, first source}; the next
+// record's first source ends it (a last record closes the list).  Sources are
+// LDS slots (loff + e) * B + lane.  Targets of more than kWaveSegment terms
+// (lead assignments aside) are listed again per block as
+// {target | kind << kKindShift, first, end, 0}: a whole wavefront sums each.
+constexpr int kKindShift = 29;
+constexpr uint32_t kTargetMask = (1u << kKindShift) - 1;
+constexpr int32_t kWaveSegment = 16;
+struct FRec { int32_t tk, j0; };
+struct __attribute__((aligned(16))) FLong { int32_t tk, j0, j1, pad; };
 
 struct Group;
 
@@ -175,6 +182,11 @@ struct Group {
   double* d_state = nullptr;
   int32_t* d_fail = nullptr;
   Plan plan[NK];
+  int32_t* d_fblk = nullptr;   // flat block plan: blocks + 1 offsets into d_frec
+  FRec* d_frec = nullptr;      // records (+ the closing one)
+  uint16_t* d_fsrc = nullptr;  // LDS slots
+  int32_t* d_flblk = nullptr;  // blocks + 1 offsets into d_flong
+  FLong* d_flong = nullptr;    // long targets
   bool fused_ok = false;     // the fused load's block plans exist
   size_t fused_lds = 0;      // its LDS bytes per block
   int fused_rows = 0;        // LDS rows of the fused load (stamp rows of live kinds)

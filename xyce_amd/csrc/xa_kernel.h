@@ -29,13 +29,13 @@ struct KArgs {
   int E[NK];
   double* out[NK];                     // fused mode: output arrays
   unsigned kmask;                      // kinds to emit
-  // fused mode: LDS row of each kind and its block plan
+  // fused mode: LDS row of each kind and the flat block plan
   int loff[NK];
-  const int32_t* bblk[NK];
-  const int32_t* btgt[NK];
-  int32_t bt0[NK];
-  const int32_t* bseg[NK];
-  const uint16_t* bsrc[NK];
+  const int32_t* fblk;
+  const FRec* frec;
+  const uint16_t* fsrc;
+  const int32_t* flblk;
+  const FLong* flong;
   // pattern-fused mode: each kind's staged output range (see pattern_stage)
   const int32_t* poff[NK];             // offset of each entry in its instance's span
   int pbase[NK];                       // first LDS double of the kind's range
@@ -109,17 +109,17 @@ __device__ __forceinline__ void pattern_stage(const KArgs& a, bool load) {
 // Fused write-out: after every lane of the block has put its values, the
 // block adds them into its targets in the plan's order -- instance then stamp
 // order, the reference's `+=` order -- so the result equals the scatter's bit
-// for bit.  Each thread takes kUnroll targets at a time and issues their
-// segment, first-source and output loads together, so a batch costs a few
-// memory round trips instead of three per target.  Every thread of the block
-// must call it (it synchronises).
+// for bit.  The block's targets of every kind form one list (the flat block
+// plan), shared out over all threads, kUnroll targets per thread at a time:
+// the records of a batch load together, then every target's first kInline
+// source slots and its output, so a batch costs two memory round trips
+// whatever the number of kinds.  Targets of more than kWaveSegment terms are
+// summed by a whole wavefront: its lanes gather 64 terms at a time from the
+// LDS rows, then every lane adds them in segment order from register
+// broadcasts -- the same additions in the same order as one thread would
+// make.  Every thread of the block must call it (it synchronises).
 constexpr int kUnroll = 8;
-// Segments longer than this (a shared supply or ground row of a small
-// circuit) are summed by a whole wavefront: its lanes gather 64 terms at a
-// time from the LDS rows in parallel, then every lane adds them in segment
-// order from register broadcasts -- the same additions in the same order as
-// one thread would make, without one index load per term on the chain.
-constexpr int32_t kWaveSegment = 16;
+constexpr int kInline = 8;
 
 __device__ __forceinline__ double readlane_f64(double x, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
@@ -127,84 +127,95 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+// a.out[k] for a per-lane kind (a select chain, no indexed private array)
+__device__ __forceinline__ double* kind_out_ptr(const KArgs& a, int k) {
+  double* p = a.out[0];
+#pragma unroll
+  for (int kk = 1; kk < NK; ++kk) p = k == kk ? a.out[kk] : p;
+  return p;
+}
+
+__device__ __forceinline__ bool kind_assigns(int k) { return k == K_LEADF || k == K_LEADQ; }
+
 __device__ __forceinline__ void fused_writeout(const KArgs& a) {
   __syncthreads();
   const int b = blockIdx.x;
   const int32_t B = (int32_t)blockDim.x;
-  for (int k = 0; k < NK; ++k) {
-    if (!(a.kmask & (1u << k))) continue;
-    const int32_t tb = a.bblk[k][b], te = a.bblk[k][b + 1];
-    double* out = a.out[k];
-    const int32_t* seg = a.bseg[k];
-    const uint16_t* src = a.bsrc[k];
-    const int32_t* tgt = a.btgt[k];
-    const bool assign = k == K_LEADF || k == K_LEADQ;
-    // short segments: one thread per target, kUnroll targets at a time
-    for (int32_t t0 = tb + (int32_t)threadIdx.x; t0 < te; t0 += kUnroll * B) {
-      int32_t j0[kUnroll], j1[kUnroll];
-      int64_t o[kUnroll];
+  const int32_t tb = a.fblk[b], te = a.fblk[b + 1];
+  const FRec* rec = a.frec;
+  const uint16_t* src = a.fsrc;
+  for (int32_t x0 = tb + (int32_t)threadIdx.x; x0 < te; x0 += kUnroll * B) {
+    int32_t tk[kUnroll], j0[kUnroll], j1[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int32_t t = t0 + u * B;
-        if (t < te) {
-          j0[u] = seg[t];
-          j1[u] = seg[t + 1];
-          o[u] = tgt ? tgt[t] : (int64_t)a.bt0[k] + t;
-        }
-      }
-      double acc[kUnroll];
-      uint16_t s0[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        if (t0 + u * B < te && (assign || j1[u] - j0[u] <= kWaveSegment)) {
-          s0[u] = src[assign ? j1[u] - 1 : j0[u]];
-          acc[u] = assign ? 0.0 : out[o[u]];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        if (t0 + u * B < te && (assign || j1[u] - j0[u] <= kWaveSegment)) {
-          if (assign) {
-            out[o[u]] = xa_lds[s0[u]];
-          } else {
-            double v = acc[u] + xa_lds[s0[u]];
-            for (int32_t j = j0[u] + 1; j < j1[u]; ++j) v += xa_lds[src[j]];
-            out[o[u]] = v;
-          }
-        }
-      }
-    }
-    if (assign) continue;
-    // long segments: each wavefront scans its share of the targets and sums
-    // every long one it finds with all its lanes
-    const int lane = (int)threadIdx.x & 63;
-    const int wave = (int)threadIdx.x >> 6, nwaves = (B + 63) >> 6;
-    for (int32_t c0 = tb + wave * 64; c0 < te; c0 += nwaves * 64) {
-      const int32_t t = c0 + lane;
-      int32_t j0 = 0, j1 = 0;
+    for (int u = 0; u < kUnroll; ++u) {
+      const int32_t t = x0 + u * B;
+      tk[u] = j0[u] = j1[u] = 0;
       if (t < te) {
-        j0 = seg[t];
-        j1 = seg[t + 1];
-      }
-      uint64_t m = __ballot(t < te && j1 - j0 > kWaveSegment);
-      while (m) {
-        const int l = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        const int32_t tt = c0 + l;
-        const int32_t a0 = __shfl(j0, l), a1 = __shfl(j1, l);
-        const int64_t o = tgt ? tgt[tt] : (int64_t)a.bt0[k] + tt;
-        double acc = out[o];
-        for (int32_t base = a0; base < a1; base += 64) {
-          const int32_t j = base + lane;
-          const double x = j < a1 ? xa_lds[src[j]] : 0.0;
-          const int cnt = (a1 - base) < 64 ? (int)(a1 - base) : 64;
-#pragma unroll
-          for (int q = 0; q < 64; ++q)
-            if (q < cnt) acc += readlane_f64(x, q);
-        }
-        if (lane == 0) out[o] = acc;
+        const FRec r = rec[t];
+        tk[u] = r.tk;
+        j0[u] = r.j0;
+        j1[u] = rec[t + 1].j0;
       }
     }
+    // a target this pass adds: in the batch, of a kind on this call, and a
+    // lead assignment or at most kWaveSegment terms
+    bool mine[kUnroll];
+    int kind[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      kind[u] = (int)((uint32_t)tk[u] >> kKindShift);
+      mine[u] = x0 + u * B < te && (a.kmask >> kind[u] & 1u) &&
+                (kind_assigns(kind[u]) || j1[u] - j0[u] <= kWaveSegment);
+    }
+    uint16_t s[kUnroll][kInline];
+    double acc[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (!mine[u]) continue;
+      if (kind_assigns(kind[u])) {
+        s[u][0] = src[j1[u] - 1];
+      } else {
+#pragma unroll
+        for (int w = 0; w < kInline; ++w)
+          if (w < j1[u] - j0[u]) s[u][w] = src[j0[u] + w];
+        acc[u] = kind_out_ptr(a, kind[u])[(uint32_t)tk[u] & kTargetMask];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (!mine[u]) continue;
+      double* o = kind_out_ptr(a, kind[u]) + ((uint32_t)tk[u] & kTargetMask);
+      if (kind_assigns(kind[u])) {
+        *o = xa_lds[s[u][0]];
+      } else {
+        double v = acc[u];
+#pragma unroll
+        for (int w = 0; w < kInline; ++w)
+          if (w < j1[u] - j0[u]) v += xa_lds[s[u][w]];
+        for (int32_t j = j0[u] + kInline; j < j1[u]; ++j) v += xa_lds[src[j]];
+        *o = v;
+      }
+    }
+  }
+  // long targets: one per wavefront at a time
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = (int)threadIdx.x >> 6, nwaves = (B + 63) >> 6;
+  const int32_t le = a.flblk[b + 1];
+  for (int32_t L = a.flblk[b] + wave; L < le; L += nwaves) {
+    const FLong r = a.flong[L];
+    const int k = (int)((uint32_t)r.tk >> kKindShift);
+    if (!(a.kmask >> k & 1u)) continue;
+    double* o = kind_out_ptr(a, k) + ((uint32_t)r.tk & kTargetMask);
+    double acc = *o;
+    for (int32_t base = r.j0; base < r.j1; base += 64) {
+      const int32_t j = base + lane;
+      const double x = j < r.j1 ? xa_lds[src[j]] : 0.0;
+      const int cnt = (r.j1 - base) < 64 ? (int)(r.j1 - base) : 64;
+#pragma unroll
+      for (int q = 0; q < 64; ++q)
+        if (q < cnt) acc += readlane_f64(x, q);
+    }
+    if (lane == 0) *o = acc;
   }
 }
 

@@ -459,6 +459,13 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
   a.opt = g.opt;
   a.v = v;
   a.kmask = 0;
+  if (fused) {
+    a.fblk = g.d_fblk;
+    a.frec = g.d_frec;
+    a.fsrc = g.d_fsrc;
+    a.flblk = g.d_flblk;
+    a.flong = g.d_flong;
+  }
   for (int k = 0; k < NK; ++k) {
     const Plan& p = g.plan[k];
     a.vals[k] = p.d_vals;
@@ -472,11 +479,6 @@ void fill_kargs(Group& g, const xa_solver_state& ss, const xa_vectors& v,
       on = on && o != nullptr && p.n_targets > 0 &&
            !(g.impl->skip_kind && g.impl->skip_kind(g, ss, k));
       a.loff[k] = p.loff;
-      a.bblk[k] = p.d_bblk;
-      a.btgt[k] = p.d_btgt;
-      a.bt0[k] = p.bt0;
-      a.bseg[k] = p.d_bseg;
-      a.bsrc[k] = p.d_bsrc;
     }
     if (on) a.kmask |= 1u << k;
   }
@@ -656,13 +658,13 @@ int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
   return XA_OK;
 }
 
+static void free_flat_plan(Group& g);
+
 static void free_plan(Plan& p) {
   (void)hipFree(p.d_target); (void)hipFree(p.d_seg); (void)hipFree(p.d_src);
   (void)hipFree(p.d_long); (void)hipFree(p.d_rail_t); (void)hipFree(p.d_rail_c);
   (void)hipFree(p.d_chunk_j0); (void)hipFree(p.d_chunk_j1); (void)hipFree(p.d_partial);
-  (void)hipFree(p.d_vals); (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt);
-  (void)hipFree(p.d_pslot);
-  (void)hipFree(p.d_bseg); (void)hipFree(p.d_bsrc);
+  (void)hipFree(p.d_vals); (void)hipFree(p.d_pslot);
   p = Plan();
 }
 
@@ -674,6 +676,7 @@ int xa_group_destroy(xa_group gh) {
   (void)hipFree(g->d_nodes); (void)hipFree(g->d_sto); (void)hipFree(g->d_sta);
   (void)hipFree(g->d_state); (void)hipFree(g->d_fail);
   for (int k = 0; k < NK; ++k) free_plan(g->plan[k]);
+  free_flat_plan(*g);
   for (auto& v : g->ev) for (auto& e : v) (void)hipEventDestroy(e);
   delete g;
   return XA_OK;
@@ -893,26 +896,24 @@ int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
 }
 
 // Block plan of the fused load.  The fused kernel runs blocks of B
-// instances; when every target of this kind receives contributions from the
+// instances; when every target of a kind receives contributions from the
 // instances of one block only, the block can add them itself, in the global
-// plan's order, from LDS.  Targets are grouped by block (ascending within a
-// block); a source becomes the LDS slot (loff + e) * B + lane.  loff is set
-// by the caller before (the kind's first LDS row).
-static void free_block_plan(Plan& p) {
-  (void)hipFree(p.d_bblk); (void)hipFree(p.d_btgt); (void)hipFree(p.d_bseg);
-  (void)hipFree(p.d_bsrc);
-  p.d_bblk = p.d_btgt = p.d_bseg = nullptr;
-  p.d_bsrc = nullptr;
-  p.bt0 = 0;
-}
+// plan's order, from LDS.  One kind's plan on the host: its targets grouped by
+// block (ascending within a block), a source becoming the LDS slot
+// (loff + e) * B + lane; loff is set by the caller before (the kind's first
+// LDS row).  The kinds' plans are then joined into the group's flat plan.
+struct BlockHost {
+  std::vector<int32_t> bblk, btgt, bseg;
+  std::vector<uint16_t> bsrc;
+};
 
-static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& target,
-                            const std::vector<int64_t>& seg, const std::vector<int64_t>& src,
-                            int B) {
-  free_block_plan(p);
+static void build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& target,
+                             const std::vector<int64_t>& seg, const std::vector<int64_t>& src,
+                             int B, BlockHost& h) {
+  h = BlockHost();
   const int64_t n = g.n, nt = (int64_t)target.size();
   p.block_private = B > 0;
-  if (!p.block_private || nt == 0) return XA_OK;
+  if (!p.block_private || nt == 0) return;
   const int64_t nb = (n + B - 1) / B;
   std::vector<int32_t> tblock(nt);
   for (int64_t t = 0; t < nt && p.block_private; ++t) {
@@ -921,33 +922,91 @@ static int build_block_plan(Group& g, Plan& p, const std::vector<int32_t>& targe
       if ((src[j] % n) / B != b) { p.block_private = false; break; }
     tblock[t] = (int32_t)b;
   }
-  if (!p.block_private) return XA_OK;
-  std::vector<int32_t> bblk(nb + 1, 0);
-  for (int64_t t = 0; t < nt; ++t) bblk[tblock[t] + 1]++;
-  for (int64_t b = 0; b < nb; ++b) bblk[b + 1] += bblk[b];
-  std::vector<int32_t> pos(bblk.begin(), bblk.end() - 1);
+  if (!p.block_private) return;
+  h.bblk.assign(nb + 1, 0);
+  for (int64_t t = 0; t < nt; ++t) h.bblk[tblock[t] + 1]++;
+  for (int64_t b = 0; b < nb; ++b) h.bblk[b + 1] += h.bblk[b];
+  std::vector<int32_t> pos(h.bblk.begin(), h.bblk.end() - 1);
   std::vector<int64_t> order(nt);
   for (int64_t t = 0; t < nt; ++t) order[pos[tblock[t]]++] = t;   // stable per block
-  std::vector<int32_t> btgt(nt), bseg(nt + 1, 0);
-  std::vector<uint16_t> bsrc;
-  bsrc.reserve(src.size());
+  h.btgt.resize(nt);
+  h.bseg.assign(nt + 1, 0);
+  h.bsrc.reserve(src.size());
   for (int64_t u = 0; u < nt; ++u) {
     const int64_t t = order[u];
-    btgt[u] = target[t];
+    h.btgt[u] = target[t];
     for (int64_t j = seg[t]; j < seg[t + 1]; ++j) {
       const int64_t inst = src[j] % n, e = src[j] / n;
-      bsrc.push_back((uint16_t)((p.loff + e) * B + inst % B));
+      h.bsrc.push_back((uint16_t)((p.loff + e) * B + inst % B));
     }
-    bseg[u + 1] = (int32_t)bsrc.size();
+    h.bseg[u + 1] = (int32_t)h.bsrc.size();
   }
-  bool contig = true;
-  for (int64_t u = 0; u < nt && contig; ++u) contig = btgt[u] == btgt[0] + (int32_t)u;
-  p.bt0 = btgt[0];
+}
+
+static void free_flat_plan(Group& g) {
+  (void)hipFree(g.d_fblk); (void)hipFree(g.d_frec); (void)hipFree(g.d_fsrc);
+  (void)hipFree(g.d_flblk); (void)hipFree(g.d_flong);
+  g.d_fblk = g.d_flblk = nullptr;
+  g.d_frec = nullptr;
+  g.d_fsrc = nullptr;
+  g.d_flong = nullptr;
+}
+
+// Joins the kinds' block plans (blocks of B) into the flat plan: per block,
+// kind by kind, every target's record and slots, and the long targets' list.
+// Leaves no flat plan (and the fused load off) when a kind is not
+// block-private or a target index or the slot count does not fit the records.
+static int build_flat_plan(Group& g, int B, const BlockHost (&h)[NK]) {
+  free_flat_plan(g);
+  if (B <= 0 || g.n == 0) return XA_OK;
+  for (int k = 0; k < NK; ++k)
+    if (!g.plan[k].block_private) return XA_OK;
+  const int64_t nb = (g.n + B - 1) / B;
+  int64_t n_rec = 0, n_src = 0;
+  for (int k = 0; k < NK; ++k) {
+    n_rec += (int64_t)h[k].btgt.size();
+    n_src += (int64_t)h[k].bsrc.size();
+    for (int32_t t : h[k].btgt)
+      if ((uint32_t)t > kTargetMask) {
+        for (int kk = 0; kk < NK; ++kk) g.plan[kk].block_private = false;
+        return XA_OK;
+      }
+  }
+  if (n_rec + 1 > INT32_MAX || n_src > INT32_MAX) {
+    for (int kk = 0; kk < NK; ++kk) g.plan[kk].block_private = false;
+    return XA_OK;
+  }
+  std::vector<int32_t> fblk(nb + 1, 0), flblk(nb + 1, 0);
+  std::vector<FRec> frec;
+  std::vector<uint16_t> fsrc;
+  std::vector<FLong> flong;
+  frec.reserve(n_rec + 1);
+  fsrc.reserve(n_src);
+  for (int64_t b = 0; b < nb; ++b) {
+    fblk[b] = (int32_t)frec.size();
+    flblk[b] = (int32_t)flong.size();
+    for (int k = 0; k < NK; ++k) {
+      const BlockHost& hk = h[k];
+      if (hk.btgt.empty()) continue;
+      for (int32_t t = hk.bblk[b]; t < hk.bblk[b + 1]; ++t) {
+        const int32_t tk = (int32_t)((uint32_t)hk.btgt[t] | ((uint32_t)k << kKindShift));
+        const int32_t j0 = (int32_t)fsrc.size();
+        fsrc.insert(fsrc.end(), hk.bsrc.begin() + hk.bseg[t], hk.bsrc.begin() + hk.bseg[t + 1]);
+        const int32_t j1 = (int32_t)fsrc.size();
+        frec.push_back({tk, j0});
+        if (!kind_is_assign(k) && j1 - j0 > kWaveSegment) flong.push_back({tk, j0, j1, 0});
+      }
+    }
+  }
+  fblk[nb] = (int32_t)frec.size();
+  flblk[nb] = (int32_t)flong.size();
+  frec.push_back({0, (int32_t)fsrc.size()});
   int rc;
-  if ((rc = upload(&p.d_bblk, bblk)) != XA_OK) return rc;
-  if (!contig && (rc = upload(&p.d_btgt, btgt)) != XA_OK) return rc;
-  if ((rc = upload(&p.d_bseg, bseg)) != XA_OK) return rc;
-  return upload(&p.d_bsrc, bsrc);
+  if ((rc = upload(&g.d_fblk, fblk)) != XA_OK) return rc;
+  if ((rc = upload(&g.d_frec, frec)) != XA_OK) return rc;
+  if ((rc = upload(&g.d_fsrc, fsrc)) != XA_OK) return rc;
+  if ((rc = upload(&g.d_flblk, flblk)) != XA_OK) return rc;
+  return upload(&g.d_flong, flong);
 }
 
 // A pattern plan applies when each of the n instances feeds S targets at the
@@ -1004,7 +1063,7 @@ static int detect_pattern(const Group& g, Plan& p, const std::vector<int32_t>& t
 
 // Counting sort of (target, instance, entry) triples by target; stable, so
 // contributions of one target stay in instance order then stamp order.
-static int build_plan(Group& g, int k) {
+static int build_plan(Group& g, int k, BlockHost& bh) {
   Plan& p = g.plan[k];
   const int E = g.impl->max_entries[k];
   p.E = E;
@@ -1110,7 +1169,7 @@ static int build_plan(Group& g, int k) {
     p.d_chunk_j0 = c0;
     p.d_chunk_j1 = c1;
   }
-  if ((rc = build_block_plan(g, p, target, seg, src, g.impl->fused_block)) != XA_OK) return rc;
+  build_block_plan(g, p, target, seg, src, g.impl->fused_block, bh);
   XA_HIP(hipMalloc((void**)&p.d_vals, sizeof(double) * (size_t)E * g.n));
   XA_HIP(hipMemset(p.d_vals, 0, sizeof(double) * (size_t)E * g.n));
   return XA_OK;
@@ -1186,8 +1245,12 @@ int xa_group_finalize(xa_group gh) {
     g->plan[k].loff = rows;
     if (live) rows += g->impl->max_entries[k];
   }
-  for (int k = 0; k < NK; ++k)
-    if ((rc = build_plan(*g, k)) != XA_OK) return rc;
+  {
+    BlockHost bh[NK];
+    for (int k = 0; k < NK; ++k)
+      if ((rc = build_plan(*g, k, bh[k])) != XA_OK) return rc;
+    if ((rc = build_flat_plan(*g, g->impl->fused_block, bh)) != XA_OK) return rc;
+  }
   g->fused_rows = rows;
   set_fused_block(*g, g->impl->fused_block);
   g->pattern_ok = g->impl->launch_pattern != nullptr && g->n > 0;
@@ -1341,6 +1404,23 @@ int xa_load_matrices(xa_group gh, const xa_solver_state* ss, const xa_vectors* v
   return XA_OK;
 }
 
+// The fused write-out adds every kind's targets concurrently: it needs the
+// kinds on the call to write distinct vectors.
+static bool fused_outputs_distinct(const Group& g, const xa_solver_state& ss,
+                                   const xa_vectors& v) {
+  const double* seen[NK];
+  int n = 0;
+  for (int k = 0; k < NK; ++k) {
+    const double* o = kind_out(v, k);
+    if (!o || g.plan[k].n_targets == 0) continue;
+    if (g.impl->skip_kind && g.impl->skip_kind(g, ss, k)) continue;
+    for (int q = 0; q < n; ++q)
+      if (seen[q] == o) return false;
+    seen[n++] = o;
+  }
+  return true;
+}
+
 // The route xa_load_all takes (timed when the group's timing is on).
 static int load_all(Group* g, const xa_solver_state* ss, const xa_vectors* v) {
   int rc;
@@ -1359,7 +1439,8 @@ static int load_all(Group* g, const xa_solver_state* ss, const xa_vectors* v) {
     return rc;
   }
   if (g->fused_ok && !g->impl->launch_secondary &&
-      (g->route == 2 || (g->route == 0 && fused_policy))) {
+      (g->route == 2 || (g->route == 0 && fused_policy)) &&
+      fused_outputs_distinct(*g, *ss, *v)) {
     tic(g, 0);
     rc = g->impl->launch_update(*g, *ss, *v, true);
     toc(g, 0);
@@ -1481,13 +1562,15 @@ static int download_plan(const Plan& p, std::vector<int32_t>& target, std::vecto
 static int build_block_plans(Group& g, int B) {
   std::vector<int32_t> target;
   std::vector<int64_t> seg, src;
+  BlockHost bh[NK];
   int rc;
   for (int k = 0; k < NK; ++k) {
     Plan& p = g.plan[k];
     if (p.E == 0) continue;
     if ((rc = download_plan(p, target, seg, src)) != XA_OK) return rc;
-    if ((rc = build_block_plan(g, p, target, seg, src, B)) != XA_OK) return rc;
+    build_block_plan(g, p, target, seg, src, B, bh[k]);
   }
+  if ((rc = build_flat_plan(g, B, bh)) != XA_OK) return rc;
   set_fused_block(g, B);
   return XA_OK;
 }
