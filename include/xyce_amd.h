@@ -335,6 +335,25 @@ int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int k
                          const double* c_lin, const uint8_t* held, double h, int dc,
                          double* x, double* q_now, double* dx_max);
 
+/* The same step with the integrator named: XA_STEP_DCOP (dc = 1 above),
+ * XA_STEP_BE (dc = 0 above) or XA_STEP_TRAP, the trapezoid rule of the
+ * reference's one-step integrator at order 2 (N_TIA_OneStep.C:272-335
+ * obtainResidual, :463-500 obtainJacobian; f_prev = the previous step's F,
+ * the integrator's qHistory[2], N_TIA_OneStep.C:1399-1406):
+ *   rhs = -((q_now - q_prev)/h + f/2 + f_prev/2) + dFdxdVp/2 + dQdxdVp/h,
+ *   J = dFdx/2 + dQdx/h + diag(c_lin/h).
+ * f_prev may be NULL except for XA_STEP_TRAP. */
+#define XA_STEP_DCOP 0
+#define XA_STEP_BE 1
+#define XA_STEP_TRAP 2
+int xa_batch_newton_step_ex(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
+                            const int32_t* c_start, const int32_t* pos,
+                            const int32_t* row_ptr, const int32_t* col_idx, const double* dFdx,
+                            const double* dQdx, const double* f, const double* q,
+                            const double* dFdxdVp, const double* dQdxdVp, const double* q_prev,
+                            const double* f_prev, const double* c_lin, const uint8_t* held,
+                            double h, int method, double* x, double* q_now, double* dx_max);
+
 #ifdef __cplusplus
 }
 #endif

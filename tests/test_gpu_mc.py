@@ -60,3 +60,45 @@ def test_batch_periods_match_oracle():
         assert pb is not None and po is not None
         assert abs(pb - po) <= WAVE_RTOL * po
 
+
+def test_trapezoid_batch_matches_plain_and_oracle():
+    """The one-step integrator's trapezoid rule (two backward-Euler steps,
+    then order 2) in xa_batch_newton_step_ex against the same rings through
+    the plain drop-in and the oracle with the same integrator; replayed from
+    HIP graphs, and the periods of rings that oscillate twice."""
+    batch = RingBatch(stages=[3, 5, 11, 7, 15], seed=34)
+    print(f"trapezoid batch of {batch.n_rings} rings, stages {batch.stages.tolist()}", flush=True)
+    bt = BatchTransient(batch, use_graph=True)
+    t, V0 = bt.run(t_stop=1.2e-9, method="trap")
+    print(f"  batched: {bt.newton_iterations} Newton iterations, {len(bt.graphs)} graphs",
+          flush=True)
+    tb, Vbe = BatchTransient(batch).run(t_stop=1.2e-9)
+    for r in range(batch.n_rings):
+        ring = batch.ring(r)
+        tg, Vg, _ = transient(GpuLoader(ring, fused=True, graph=True), ring, t_stop=1.2e-9,
+                              method="trap")
+        to, Vo, _ = transient(OracleLoader(ring), ring, t_stop=1.2e-9, method="trap")
+        eg = np.max(np.abs(V0[:, r] - Vg[:, ring.ic_node])) / ring.vdd
+        eo = np.max(np.abs(V0[:, r] - Vo[:, ring.ic_node])) / ring.vdd
+        # the integrators differ by far more than the parity bar
+        d_be = np.max(np.abs(V0[:, r] - Vbe[:, r])) / ring.vdd
+        print(f"  ring {r} ({ring.N} stages): vs plain {eg:.2e}, vs oracle {eo:.2e}, "
+              f"trap vs BE {d_be:.2e}", flush=True)
+        assert eg <= WAVE_RTOL and eo <= WAVE_RTOL
+        assert d_be > 1e3 * WAVE_RTOL
+        pb = period(t, V0[:, r], ring.vdd / 2)
+        po = period(to, Vo[:, ring.ic_node], ring.vdd / 2)
+        if po is not None:
+            assert pb is not None and abs(pb - po) <= WAVE_RTOL * po
+
+
+def test_trapezoid_needs_previous_f():
+    """The entry point refuses a trapezoid step without F_prev (host check)."""
+    import ctypes as C
+    from xyce_amd import Context
+    from xyce_amd._abi import XA_ERR_ARG, XA_STEP_TRAP, lib
+    ctx = Context(0)
+    one = C.c_void_p(8)   # never dereferenced: the call fails its argument checks
+    args = [ctx.h, 1, 4, 1, 1] + [one] * 11 + [None] + [one, one, C.c_double(1e-12),
+                                                        XA_STEP_TRAP, one, one, one]
+    assert lib().xa_batch_newton_step_ex(*args) == XA_ERR_ARG

@@ -11,7 +11,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxyce_amd.so")
 
 XA_OK = 0
+XA_ERR_ARG, XA_ERR_STATE, XA_ERR_HIP, XA_ERR_UNSUPPORTED, XA_ERR_NOMEM, XA_ERR_DEVICE = range(-1, -7, -1)
 XA_DIODE, XA_MOSFET1, XA_BJT, XA_BSIM4, XA_RESISTOR, XA_CAPACITOR, XA_VSOURCE = range(1, 8)
+XA_STEP_DCOP, XA_STEP_BE, XA_STEP_TRAP = range(3)   # xa_batch_newton_step_ex methods
 MODEL_TYPES = {"diode": XA_DIODE, "mosfet1": XA_MOSFET1, "bjt": XA_BJT, "bsim4": XA_BSIM4,
                "resistor": XA_RESISTOR, "capacitor": XA_CAPACITOR, "vsource": XA_VSOURCE}
 
@@ -96,6 +98,8 @@ def _proto(lib):
         "xa_group_last_times": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "xa_batch_newton_step": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int] + [vp] * 13 +
                                  [C.c_double, C.c_int, vp, vp, vp]),
+        "xa_batch_newton_step_ex": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int] +
+                                    [vp] * 14 + [C.c_double, C.c_int, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

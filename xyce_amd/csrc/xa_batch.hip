@@ -4,12 +4,19 @@
 // one system: circuit c owns the unknowns [c_start[c], c_start[c+1]) and no
 // matrix entry couples two circuits.  After a device load of the whole batch,
 // this kernel forms each circuit's Newton system the way the reference's
-// backward-Euler integrator forms it (N_TIA_BackwardDifferentiation15.C:
-// 215-265, with one linear capacitor per node added to Q) and solves it:
+// integrators form it, with one linear capacitor per node added to Q, and
+// solves it.  Backward Euler (N_TIA_BackwardDifferentiation15.C:215-265, and
+// the one-step integrator at order 1):
 //
 //   q      = Q(x) + c x
 //   rhs    = -((q - q_prev) / h + F) + dFdxdVp + dQdxdVp / h
 //   J      = dF/dx + dQ/dx / h + diag(c / h)
+//
+// trapezoid, the one-step integrator at order 2 (N_TIA_OneStep.C:272-335 and
+// 463-500; F_prev is the previous step's F, its qHistory[2]):
+//
+//   rhs    = -((q - q_prev) / h + F / 2 + F_prev / 2) + dFdxdVp / 2 + dQdxdVp / h
+//   J      = dF/dx / 2 + dQ/dx / h + diag(c / h)
 //
 // or, at a DC operating point, rhs = -F + dFdxdVp and J = dF/dx.  Rows of
 // held nodes (a supply, an initial condition) are identity rows with zero
@@ -41,10 +48,11 @@ struct BatchArgs {
   const double* dFdxdVp;
   const double* dQdxdVp;
   const double* q_prev;
+  const double* f_prev;
   const double* c_lin;
   const uint8_t* held;
   double h;
-  int dc;
+  int method;   // XA_STEP_DCOP, XA_STEP_BE or XA_STEP_TRAP
   int max_n;
   int kl, ku;
   double* x;
@@ -102,10 +110,17 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
       if (j < 0 || j >= n || b.held[gj]) continue;
       const int jp = b.pos ? b.pos[gj] : j;
       if (jp < 0 || jp >= n || ip - jp > kl || jp - ip > ku) { s_bad = 1; continue; }
-      AB[(kv + ip - jp) * n + jp] += b.dc ? b.dFdx[p] : b.dFdx[p] + b.dQdx[p] / b.h;
+      AB[(kv + ip - jp) * n + jp] +=
+          b.method == XA_STEP_DCOP ? b.dFdx[p]
+          : b.method == XA_STEP_TRAP ? 0.5 * b.dFdx[p] + b.dQdx[p] / b.h
+                                     : b.dFdx[p] + b.dQdx[p] / b.h;
     }
-    if (b.dc) {
+    if (b.method == XA_STEP_DCOP) {
       rhs[ip] = -b.f[gi] + b.dFdxdVp[gi];
+    } else if (b.method == XA_STEP_TRAP) {
+      AB[kv * n + ip] += b.c_lin[gi] / b.h;
+      rhs[ip] = -((qn - b.q_prev[gi]) / b.h + 0.5 * b.f[gi] + 0.5 * b.f_prev[gi]) +
+                0.5 * b.dFdxdVp[gi] + b.dQdxdVp[gi] / b.h;
     } else {
       AB[kv * n + ip] += b.c_lin[gi] / b.h;
       rhs[ip] = -((qn - b.q_prev[gi]) / b.h + b.f[gi]) + b.dFdxdVp[gi] + b.dQdxdVp[gi] / b.h;
@@ -194,6 +209,39 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_newton(BatchArgs b) {
 
 using namespace xa;
 
+extern "C" int xa_batch_newton_step_ex(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
+                                       const int32_t* c_start, const int32_t* pos,
+                                       const int32_t* row_ptr, const int32_t* col_idx,
+                                       const double* dFdx, const double* dQdx, const double* f,
+                                       const double* q, const double* dFdxdVp,
+                                       const double* dQdxdVp, const double* q_prev,
+                                       const double* f_prev, const double* c_lin,
+                                       const uint8_t* held, double h, int method, double* x,
+                                       double* q_now, double* dx_max) {
+  Ctx* cx = reinterpret_cast<Ctx*>(ctx);
+  if (!cx) return fail(XA_ERR_ARG, "null context");
+  if (n_circ <= 0) return XA_OK;
+  if (max_unknowns <= 0 || kl < 0 || ku < 0)
+    return fail(XA_ERR_ARG, "batch: max_unknowns must be positive and kl, ku >= 0");
+  if (method != XA_STEP_DCOP && method != XA_STEP_BE && method != XA_STEP_TRAP)
+    return fail(XA_ERR_ARG, "batch: method must be XA_STEP_DCOP, XA_STEP_BE or XA_STEP_TRAP");
+  const size_t lds = ((size_t)(2 * kl + ku + 2) * (size_t)max_unknowns) * sizeof(double);
+  if (lds > kMaxLds - 1024)
+    return fail(XA_ERR_ARG, "batch: a circuit's band does not fit in LDS");
+  if (method != XA_STEP_DCOP && !(h > 0.0))
+    return fail(XA_ERR_ARG, "batch: transient step needs h > 0");
+  if (method == XA_STEP_TRAP && !f_prev)
+    return fail(XA_ERR_ARG, "batch: the trapezoid step needs the previous step's F");
+  BatchArgs a{n_circ, c_start, pos, row_ptr, col_idx, dFdx, dQdx, f, q, dFdxdVp, dQdxdVp,
+              q_prev, f_prev, c_lin, held, h, method, max_unknowns, kl, ku, x, q_now, dx_max};
+  if (lds > 65536)
+    XA_HIP(hipFuncSetAttribute((const void*)k_batch_newton,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_batch_newton, dim3(n_circ), dim3(kBatchThreads), lds, cx->stream, a);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
 extern "C" int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, int kl, int ku,
                                     const int32_t* c_start, const int32_t* pos,
                                     const int32_t* row_ptr, const int32_t* col_idx,
@@ -202,21 +250,8 @@ extern "C" int xa_batch_newton_step(xa_ctx ctx, int n_circ, int max_unknowns, in
                                     const double* dQdxdVp, const double* q_prev,
                                     const double* c_lin, const uint8_t* held, double h, int dc,
                                     double* x, double* q_now, double* dx_max) {
-  Ctx* cx = reinterpret_cast<Ctx*>(ctx);
-  if (!cx) return fail(XA_ERR_ARG, "null context");
-  if (n_circ <= 0) return XA_OK;
-  if (max_unknowns <= 0 || kl < 0 || ku < 0)
-    return fail(XA_ERR_ARG, "batch: max_unknowns must be positive and kl, ku >= 0");
-  const size_t lds = ((size_t)(2 * kl + ku + 2) * (size_t)max_unknowns) * sizeof(double);
-  if (lds > kMaxLds - 1024)
-    return fail(XA_ERR_ARG, "batch: a circuit's band does not fit in LDS");
-  if (!dc && !(h > 0.0)) return fail(XA_ERR_ARG, "batch: transient step needs h > 0");
-  BatchArgs a{n_circ, c_start, pos, row_ptr, col_idx, dFdx, dQdx, f, q, dFdxdVp, dQdxdVp,
-              q_prev, c_lin, held, h, dc, max_unknowns, kl, ku, x, q_now, dx_max};
-  if (lds > 65536)
-    XA_HIP(hipFuncSetAttribute((const void*)k_batch_newton,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_batch_newton, dim3(n_circ), dim3(kBatchThreads), lds, cx->stream, a);
-  XA_HIP(hipGetLastError());
-  return XA_OK;
+  return xa_batch_newton_step_ex(ctx, n_circ, max_unknowns, kl, ku, c_start, pos, row_ptr,
+                                 col_idx, dFdx, dQdx, f, q, dFdxdVp, dQdxdVp, q_prev, nullptr,
+                                 c_lin, held, h, dc ? XA_STEP_DCOP : XA_STEP_BE, x, q_now,
+                                 dx_max);
 }

@@ -23,7 +23,7 @@ import time
 import numpy as np
 
 from . import Context, Master, SolverState, make_vectors
-from ._abi import check, lib
+from ._abi import XA_STEP_BE, XA_STEP_DCOP, XA_STEP_TRAP, check, lib
 from .ring import Ring
 
 SEED = 20261203
@@ -168,6 +168,7 @@ class BatchTransient:
         self.col_idx = i32(t.col_idx)
         self.c_lin = torch.from_numpy(c_lin).to(dev)
         self.q_prev = z(t.n_nodes)
+        self.f_prev = z(t.n_nodes)
         self.q_now = z(t.n_nodes)
         self.dx_max = z(batch.n_rings)
         supply = t.base[1:] - 1
@@ -200,30 +201,27 @@ class BatchTransient:
         for k, v in flags.items():
             setattr(ss, k, v)
         self.outputs.zero_()
-        m = self.master
-        m.updateState(ss, self.vectors)
-        m.updateSecondaryState(ss, self.vectors)
-        m.loadDAEVectors(ss, self.vectors)
-        m.loadDAEMatrices(ss, self.vectors)
+        self.master.loadAll(ss, self.vectors)   # the four calls' results, fewer launches
         self.loads += 1
 
-    def _solve(self, h, dc):
+    def _solve(self, h, method):
         d, t = self.d, self.topo
         P = self._ptr
-        check(lib().xa_batch_newton_step(
+        dc = method == XA_STEP_DCOP
+        check(lib().xa_batch_newton_step_ex(
             self.ctx.h, self.batch.n_rings, t.max_unknowns, t.kl, t.ku, P(self.c_start),
-            P(self.pos), P(self.row_ptr), P(self.col_idx), P(d["dFdx"]), P(d["dQdx"]), P(d["f"]), P(d["q"]), P(d["dFdxdVp"]),
-            P(d["dQdxdVp"]), P(self.q_prev), P(self.c_lin),
-            P(self.held_dc if dc else self.held_tr), C.c_double(h), 1 if dc else 0, P(self.x),
-            P(self.q_now), P(self.dx_max)), "xa_batch_newton_step")
+            P(self.pos), P(self.row_ptr), P(self.col_idx), P(d["dFdx"]), P(d["dQdx"]), P(d["f"]),
+            P(d["q"]), P(d["dFdxdVp"]), P(d["dQdxdVp"]), P(self.q_prev), P(self.f_prev),
+            P(self.c_lin), P(self.held_dc if dc else self.held_tr), C.c_double(h), method,
+            P(self.x), P(self.q_now), P(self.dx_max)), "xa_batch_newton_step_ex")
 
-    def _iteration(self, flags, h, dc):
+    def _iteration(self, flags, h, method):
         """One Newton iteration's device work: zero, load, solve, reduce."""
         self._load(**flags)
-        self._solve(h, dc)
+        self._solve(h, method)
         return self.dx_max.max()
 
-    def _graph(self, key, flags, h, dc):
+    def _graph(self, key, flags, h, method):
         """The iteration captured as a HIP graph.  The solver-state flags are
         baked into the captured kernel arguments, so one graph serves every
         iteration with the same flags; the loads test newtonIter only for
@@ -236,23 +234,23 @@ class BatchTransient:
             with torch.cuda.graph(g):
                 self.ctx.set_stream(torch.cuda.current_stream())
                 try:
-                    out = self._iteration(flags, h, dc)
+                    out = self._iteration(flags, h, method)
                 finally:
                     self.ctx.set_stream(main)
             self.graphs[key] = g = (g, out)
         return g
 
-    def _newton(self, flags_of, h, dc, vtol, max_iter):
+    def _newton(self, flags_of, h, method, vtol, max_iter):
         for it in range(max_iter):
             flags = flags_of(it)
             if self.use_graph:
-                key = (dc, h, it == 0, tuple(sorted((k, v) for k, v in flags.items()
-                                                    if k != "newtonIter")))
-                g, worst_t = self._graph(key, dict(flags, newtonIter=min(it, 1)), h, dc)
+                key = (method, h, it == 0, tuple(sorted((k, v) for k, v in flags.items()
+                                                        if k != "newtonIter")))
+                g, worst_t = self._graph(key, dict(flags, newtonIter=min(it, 1)), h, method)
                 g.replay()
                 self.loads += 1
             else:
-                worst_t = self._iteration(flags, h, dc)
+                worst_t = self._iteration(flags, h, method)
             self.newton_iterations += 1
             worst = float(worst_t.item())
             if not np.isfinite(worst):
@@ -266,15 +264,21 @@ class BatchTransient:
         d["currSto"].copy_(d["nextSto"])
         d["currSta"].copy_(d["nextSta"])
         self.q_prev.copy_(self.q_now)
+        self.f_prev.copy_(d["f"])   # the last load's F: the trapezoid's history
 
-    def run(self, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50):
-        """DC operating point then backward Euler; returns (times, V0) with
-        V0[step, r] the voltage of ring r's node 0."""
+    def run(self, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, method="be"):
+        """DC operating point then fixed steps of backward Euler (method "be")
+        or of the one-step integrator's trapezoid rule ("trap": two backward
+        Euler steps, then order 2, as N_TIA_OneStep.C:2130,2211 raise the
+        order after the second step); returns (times, V0) with V0[step, r]
+        the voltage of ring r's node 0."""
+        if method not in ("be", "trap"):
+            raise ValueError("method must be 'be' or 'trap'")
         torch = self.torch
         t = self.topo
         first = torch.from_numpy(t.base[:-1].astype(np.int64)).to(self.dev)
         self._newton(lambda it: dict(newtonIter=it, dcopFlag=1, initJctFlag=1 if it == 0 else 0),
-                     1.0, True, vtol, max_iter)
+                     1.0, XA_STEP_DCOP, vtol, max_iter)
         self._accept()
         steps = int(round(t_stop / h))
         V0 = torch.zeros((steps + 1, self.batch.n_rings), dtype=torch.float64, device=self.dev)
@@ -286,7 +290,8 @@ class BatchTransient:
                 return dict(newtonIter=it, transientFlag=1, pdt=1.0 / h, currTimeStep=h,
                             lastTimeStep=h, initTranFlag=1 if f0 else 0,
                             beginIntegrationFlag=1 if f0 else 0)
-            self._newton(flags, h, False, vtol, max_iter)
+            order2 = method == "trap" and n >= 2
+            self._newton(flags, h, XA_STEP_TRAP if order2 else XA_STEP_BE, vtol, max_iter)
             self._accept()
             V0[n + 1] = self.x[first]
         torch.cuda.synchronize()

@@ -232,9 +232,16 @@ def _newton(loader, ring, x, fixed, flags_of, residual, vtol, max_iter):
     raise NewtonFailure(f"Newton did not converge in {max_iter} iterations")
 
 
-def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record=None):
-    """DC operating point then fixed-step backward Euler to t_stop.  Returns
+def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record=None,
+              method="be"):
+    """DC operating point then fixed steps to t_stop: backward Euler (method
+    "be") or the one-step integrator's trapezoid rule ("trap": two backward
+    Euler steps, then order 2 from the third step on, as N_TIA_OneStep.C:
+    2130,2211 raise the order; residual and Jacobian of N_TIA_OneStep.C:
+    272-335,463-500, with the previous step's F as qHistory[2]).  Returns
     (times, V[steps+1, n_nodes], stats)."""
+    if method not in ("be", "trap"):
+        raise ValueError("method must be 'be' or 'trap'")
     t = loader.topo
     N = ring.N
     C = np.zeros(t.n_nodes)
@@ -260,6 +267,7 @@ def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record
     # the charge history is the last load's: device Q and capacitor charge at
     # the voltages that load saw
     q_n = out["q"] + C * x_load
+    f_n = out["f"].copy()
     times = [0.0]
     V = [x.copy()]
     steps = int(round(t_stop / h))
@@ -278,11 +286,20 @@ def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record
             jac = t.dense(out["dFdx"]) + t.dense(out["dQdx"]) / h + np.diag(C / h)
             return rhs, jac
 
-        x, out, x_load, its = _newton(loader, ring, x_n, {ring.supply}, tr_flags, tr_residual,
+        def trap_residual(out, xv, q_n=q_n, f_n=f_n):
+            q = out["q"] + C * xv
+            rhs = (-((q - q_n) / h + 0.5 * out["f"] + 0.5 * f_n) + 0.5 * out["dFdxdVp"]
+                   + out["dQdxdVp"] / h)
+            jac = 0.5 * t.dense(out["dFdx"]) + t.dense(out["dQdx"]) / h + np.diag(C / h)
+            return rhs, jac
+
+        res = trap_residual if method == "trap" and n >= 2 else tr_residual
+        x, out, x_load, its = _newton(loader, ring, x_n, {ring.supply}, tr_flags, res,
                                       vtol, max_iter)
         newton_total += its
         loader.accept()
         q_n = out["q"] + C * x_load
+        f_n = out["f"].copy()
         times.append((n + 1) * h)
         V.append(x.copy())
     stats = {"steps": steps, "newton_iterations": newton_total, "loads": loader.loads,
