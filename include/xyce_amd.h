@@ -354,6 +354,81 @@ int xa_batch_newton_step_ex(xa_ctx ctx, int n_circ, int max_unknowns, int kl, in
                             const double* f_prev, const double* c_lin, const uint8_t* held,
                             double h, int method, double* x, double* q_now, double* dx_max);
 
+/* ---- batched dense LU in FP32 on the matrix cores ------------------------ */
+/* No reference counterpart as code: LAPACK's blocked dgetrf / dgetrs order
+ * (partial pivoting, first largest magnitude) on `batch` matrices at once,
+ * the trailing updates on v_mfma_f32_16x16x4f32; the FP32 half of the
+ * mixed-precision solves (FP64 refinement around it, as in
+ * xa_hb_precond_solve).  Each matrix is ld x ld, row-major, the batch
+ * contiguous (matrix b at a + b*ld*ld); ld a multiple of 64, at most 3072;
+ * rows and columns [m, ld) must hold the identity.  ipiv: batch x ld, ipiv[i]
+ * = the 0-based row interchanged with row i at step i (rows < m rounded up to
+ * 64).  info[b] = 0, or the first zero pivot's column + 1 (the factor is then
+ * singular and a solve divides by zero).  solve: b holds batch x nrhs
+ * right-hand sides of ld floats each (rhs r of matrix q at b + (q*nrhs+r)*ld),
+ * overwritten by the solutions.  Device pointers; stream-ordered. */
+int xa_lu_f32_factor(xa_ctx ctx, int batch, int m, int ld, float* a, int32_t* ipiv,
+                     int32_t* info);
+int xa_lu_f32_solve(xa_ctx ctx, int batch, int nrhs, int m, int ld, const float* lu,
+                    const int32_t* ipiv, float* b);
+
+/* ---- harmonic balance ----------------------------------------------------- */
+/* The HB loader's transforms and batched-sample load (N_LOA_HBLoader.C:
+ * 540-775) and its block-Jacobi preconditioner (N_LAS_HBBlockJacobiPrecond.C:
+ * 178-425), on the device.  N = n_samples = 2H + 1 (odd, N_ANP_HB.C:1470),
+ * n = unknowns of one sample, T = period.
+ *   time domain:  xt[s*n + j], sample s (time s*T/N) of unknown j;
+ *   frequency:    xf[j*2N + 2p], xf[j*2N + 2p + 1] = (Re, Im) of unknown j at
+ *                 position p: harmonic p for p <= H, p - N above (the
+ *                 permuted block layout of N_LAS_BlockSystemHelpers.C:512-609).
+ * xa_hb_ift:  xt[s][j] = Re_0 + 2 sum_{k=1..H} (Re_k cos(2 pi k s/N) - Im_k sin(..));
+ * xa_hb_dft:  (Re, Im)_k = (1/N) sum_s x_s (cos, -sin)(2 pi k s/N), harmonic -k
+ *             the conjugate; (computePermutedIFT / computePermutedDFT with
+ *             the FFTW packing and scalings, N_UTL_FFTW_Interface.C:66-130);
+ * xa_hb_add_jwq: ff += j w k qf for k = -H..H, w = 2 pi / T
+ *             (N_LOA_HBLoader.C:733-775).
+ * xa_hb_load: one HB load: xt = ift(xf) into vt->nextSol (N*n entries), the
+ *             time-domain outputs of vt zeroed (f, q, dFdxdVp, dQdxdVp: N*n;
+ *             dFdx, dQdx: nnz_t), xa_load_all(g) over every sample at once --
+ *             g is the group of all samples' instances, sample s's unknown j
+ *             at LID s*n + j -- then ff = dft(f) + j w k dft(q), qf = dft(q),
+ *             dfdxdvp_f = dft(dFdxdVp) + j w k dft(dQdxdVp), dqdxdvp_f =
+ *             dft(dQdxdVp) (NULL outputs are skipped).  The per-sample
+ *             Jacobians stay in vt->dFdx / vt->dQdx for the preconditioner.
+ * Preconditioner: on a CSR pattern (row_ptr[n+1], col_idx[nnz], host arrays)
+ * shared by every sample, whose sample s values are dFdx_t[s*nnz + e]:
+ *   factor: G = (sum_s dFdx_s) (1/N), C likewise; for k = 0..H the real-
+ *           equivalent block [G, -k w C; k w C, G] (2n rows, padded to a
+ *           multiple of 64, at most 3072) in FP32, factored by
+ *           xa_lu_f32_factor; info[k] per block (host array of H+1 or NULL).
+ *           Harmonic -k uses harmonic k's factor: K_-k = P K_k P,
+ *           P = diag(I, -I).
+ *   solve:  x = K_k^-1 b for every position of bf (frequency layout) into xf:
+ *           the FP32 solve, then up to max_refine FP64 iterative-refinement
+ *           steps r = b - K_k x (FP64, from G and C), x += K_k^-1 r (FP32),
+ *           until max|r| <= tol max|b| for every system.  Per position p
+ *           (host arrays of N, or NULL): rel_fp32[p] = max|r|/max|b| after the
+ *           FP32 solve, rel_final[p] after refinement, iterations[p] = the
+ *           refinement steps it needed. */
+typedef struct xa_hb_plan_s* xa_hb_plan;
+typedef struct xa_hb_precond_s* xa_hb_precond;
+int xa_hb_plan_create(xa_ctx ctx, int n_samples, int n_unknowns, double period,
+                      xa_hb_plan* out);
+int xa_hb_plan_destroy(xa_hb_plan p);
+int xa_hb_ift(xa_hb_plan p, const double* xf, double* xt);
+int xa_hb_dft(xa_hb_plan p, const double* xt, double* xf);
+int xa_hb_add_jwq(xa_hb_plan p, double* ff, const double* qf);
+int xa_hb_load(xa_hb_plan p, xa_group g, const xa_solver_state* ss, const double* xf,
+               const xa_vectors* vt, int64_t nnz_t, double* ff, double* qf, double* dfdxdvp_f,
+               double* dqdxdvp_f);
+int xa_hb_precond_create(xa_hb_plan p, int nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                         xa_hb_precond* out);
+int xa_hb_precond_destroy(xa_hb_precond pc);
+int xa_hb_precond_factor(xa_hb_precond pc, const double* dFdx_t, const double* dQdx_t,
+                         int32_t* info);
+int xa_hb_precond_solve(xa_hb_precond pc, const double* bf, double* xf, int max_refine,
+                        double tol, double* rel_fp32, double* rel_final, int32_t* iterations);
+
 #ifdef __cplusplus
 }
 #endif

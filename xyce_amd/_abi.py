@@ -100,6 +100,19 @@ def _proto(lib):
                                  [C.c_double, C.c_int, vp, vp, vp]),
         "xa_batch_newton_step_ex": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int] +
                                     [vp] * 14 + [C.c_double, C.c_int, vp, vp, vp]),
+        "xa_lu_f32_factor": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
+        "xa_lu_f32_solve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
+        "xa_hb_plan_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, P(vp)]),
+        "xa_hb_plan_destroy": (C.c_int, [vp]),
+        "xa_hb_ift": (C.c_int, [vp, vp, vp]),
+        "xa_hb_dft": (C.c_int, [vp, vp, vp]),
+        "xa_hb_add_jwq": (C.c_int, [vp, vp, vp]),
+        "xa_hb_load": (C.c_int, [vp, vp, P(SolverState), vp, P(Vectors), C.c_int64, vp, vp, vp,
+                                 vp]),
+        "xa_hb_precond_create": (C.c_int, [vp, C.c_int, vp, vp, P(vp)]),
+        "xa_hb_precond_destroy": (C.c_int, [vp]),
+        "xa_hb_precond_factor": (C.c_int, [vp, vp, vp, vp]),
+        "xa_hb_precond_solve": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
