@@ -406,7 +406,8 @@ int xa_lu_f32_solve(xa_ctx ctx, int batch, int nrhs, int m, int ld, const float*
  *   solve:  x = K_k^-1 b for every position of bf (frequency layout) into xf:
  *           the FP32 solve, then up to max_refine FP64 iterative-refinement
  *           steps r = b - K_k x (FP64, from G and C), x += K_k^-1 r (FP32),
- *           until max|r| <= tol max|b| for every system.  Per position p
+ *           until every system has max|r| <= tol max|b| or a residual that
+ *           no longer halves in a step.  Per position p
  *           (host arrays of N, or NULL): rel_fp32[p] = max|r|/max|b| after the
  *           FP32 solve, rel_final[p] after refinement, iterations[p] = the
  *           refinement steps it needed. */

@@ -25,6 +25,10 @@ int lu_check(int batch, int m, int ld);
 int lu_factor(Ctx* cx, int batch, int m, int ld, float* a, int32_t* ipiv, int32_t* info);
 int lu_solve(Ctx* cx, int batch, int nrhs, int m, int ld, const float* lu, const int32_t* ipiv,
              float* b);
+int lu_prepare(Ctx* cx, int batch, int m, int ld, const float* lu, const int32_t* ipiv,
+               int32_t* perm, float* linv, float* uinv);
+int lu_solve_prepared(Ctx* cx, int batch, int nrhs, int m, int ld, const float* lu,
+                      const int32_t* perm, const float* linv, const float* uinv, float* b);
 
 struct HBPlan {
   Ctx* ctx = nullptr;
@@ -44,6 +48,9 @@ struct HBPrecond {
   float* d_lu = nullptr;     // nh blocks ld x ld
   int32_t* d_ipiv = nullptr;
   int32_t* d_info = nullptr;
+  int32_t* d_perm = nullptr;  // the exchanges as one permutation, per block
+  float* d_linv = nullptr;    // inverses of the 64x64 diagonal blocks of L and U
+  float* d_uinv = nullptr;
   double* d_b = nullptr;     // N systems x m: right-hand sides, solutions, residuals
   double* d_x = nullptr;
   double* d_r = nullptr;
@@ -405,7 +412,8 @@ extern "C" int xa_hb_load(xa_hb_plan p, xa_group g, const xa_solver_state* ss, c
 extern "C" int xa_hb_precond_destroy(xa_hb_precond pc) {
   if (!pc) return XA_OK;
   void* bufs[] = {pc->d_row_ptr, pc->d_col_idx, pc->d_G, pc->d_C, pc->d_lu, pc->d_ipiv,
-                  pc->d_info, pc->d_b, pc->d_x, pc->d_r, pc->d_rhs, pc->d_norm};
+                  pc->d_info, pc->d_perm, pc->d_linv, pc->d_uinv, pc->d_b, pc->d_x, pc->d_r,
+                  pc->d_rhs, pc->d_norm};
   for (void* b : bufs) (void)hipFree(b);
   delete pc;
   return XA_OK;
@@ -443,6 +451,9 @@ extern "C" int xa_hb_precond_create(xa_hb_plan p, int nnz, const int32_t* row_pt
   alloc((void**)&pc->d_lu, sizeof(float) * pc->nh * (size_t)ld * ld);
   alloc((void**)&pc->d_ipiv, sizeof(int32_t) * pc->nh * (size_t)ld);
   alloc((void**)&pc->d_info, sizeof(int32_t) * pc->nh);
+  alloc((void**)&pc->d_perm, sizeof(int32_t) * pc->nh * (size_t)ld);
+  alloc((void**)&pc->d_linv, sizeof(float) * pc->nh * (size_t)ld * 64);
+  alloc((void**)&pc->d_uinv, sizeof(float) * pc->nh * (size_t)ld * 64);
   alloc((void**)&pc->d_b, sizeof(double) * N * m);
   alloc((void**)&pc->d_x, sizeof(double) * N * m);
   alloc((void**)&pc->d_r, sizeof(double) * N * m);
@@ -473,6 +484,9 @@ extern "C" int xa_hb_precond_factor(xa_hb_precond pc, const double* dFdx_t, cons
                      pc->d_row_ptr, pc->d_col_idx, pc->d_G, pc->d_C, h->n, pc->ld, w);
   XA_HIP(hipGetLastError());
   int rc = lu_factor(h->ctx, pc->nh, pc->m, pc->ld, pc->d_lu, pc->d_ipiv, pc->d_info);
+  if (rc != XA_OK) return rc;
+  rc = lu_prepare(h->ctx, pc->nh, pc->m, pc->ld, pc->d_lu, pc->d_ipiv, pc->d_perm, pc->d_linv,
+                  pc->d_uinv);
   if (rc != XA_OK) return rc;
   if (info) {
     XA_HIP(hipMemcpyAsync(info, pc->d_info, sizeof(int32_t) * pc->nh, hipMemcpyDeviceToHost, st));
@@ -514,7 +528,8 @@ extern "C" int xa_hb_precond_solve(xa_hb_precond pc, const double* bf, double* x
     hipLaunchKernelGGL(k_hb_to_rhs, hb_grid(ld, N), dim3(256), 0, st, pc->d_r, pc->d_rhs, N, H, n,
                        ld);
     XA_HIP(hipGetLastError());
-    int rc = lu_solve(h->ctx, pc->nh, 2, pc->m, ld, pc->d_lu, pc->d_ipiv, pc->d_rhs);
+    int rc = lu_solve_prepared(h->ctx, pc->nh, 2, pc->m, ld, pc->d_lu, pc->d_perm, pc->d_linv,
+                               pc->d_uinv, pc->d_rhs);
     if (rc != XA_OK) return rc;
     hipLaunchKernelGGL(k_hb_from_rhs, hb_grid(pc->m, N), dim3(256), 0, st, pc->d_rhs, pc->d_x, N,
                        H, n, ld);
@@ -530,16 +545,24 @@ extern "C" int xa_hb_precond_solve(xa_hb_precond pc, const double* bf, double* x
   if ((rc = correct()) != XA_OK) return rc;         // the FP32 solve
   if ((rc = residual(true)) != XA_OK) return rc;
   std::vector<int32_t> its(N, 0);
+  // a system is done when its residual meets tol or stops falling (halving
+  // no more: the FP64 residual's own rounding floor, or a factor whose
+  // condition number times the FP32 precision is near one)
+  std::vector<char> done(N, 0);
   if (rel_fp32) for (int q = 0; q < N; ++q) rel_fp32[q] = rel[q];
+  for (int q = 0; q < N; ++q) done[q] = rel[q] <= tol;
   for (int it = 1; it <= max_refine; ++it) {
-    bool done = true;
-    for (int q = 0; q < N; ++q) done = done && rel[q] <= tol;
-    if (done) break;
+    bool all = true;
+    for (int q = 0; q < N; ++q) all = all && done[q];
+    if (all) break;
     if ((rc = correct()) != XA_OK) return rc;
     std::vector<double> before = rel;
     if ((rc = residual(true)) != XA_OK) return rc;
-    for (int q = 0; q < N; ++q)
-      if (!(before[q] <= tol)) its[q] = it;   // refinements this system needed
+    for (int q = 0; q < N; ++q) {
+      if (done[q]) continue;
+      its[q] = it;   // refinements this system took
+      if (rel[q] <= tol || !(rel[q] < 0.5 * before[q])) done[q] = 1;
+    }
   }
   if (rel_final) for (int q = 0; q < N; ++q) rel_final[q] = rel[q];
   if (iterations) for (int q = 0; q < N; ++q) iterations[q] = its[q];

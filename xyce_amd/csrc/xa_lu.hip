@@ -31,6 +31,7 @@ constexpr int kPanelThreads = 1024;
 constexpr int kPanelQ = 3;             // panel rows per thread
 constexpr int kLuMaxLd = kPanelThreads * kPanelQ;
 constexpr int kLuTile = 64;            // trailing-update tile
+constexpr int kLuOuter = 64;           // columns per trailing update
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -188,24 +189,28 @@ k_lu_panel(float* __restrict__ a, int32_t* __restrict__ ipiv, int32_t* __restric
   if (t == 0 && first_zero && info[bt] == 0) info[bt] = k0 + first_zero;
 }
 
-// The panel's row exchanges on every column outside it, then, right of the
-// panel, U12 = L11^-1 A12 (L11 unit lower, staged in LDS).
+// The row exchanges ipiv[k0, k0 + NB) on the columns [jlo, jhi) outside the
+// block's own columns [k0, k0 + NB), then, on those right of it,
+// U12 = L11^-1 A12 (L11 the block's unit lower triangle, staged in LDS).
+// One thread a column.
+template <int NB>
 __global__ void __launch_bounds__(256)
-k_lu_swap_trsm(float* __restrict__ a, const int32_t* __restrict__ ipiv, int ld, int k0) {
+k_lu_swap_trsm(float* __restrict__ a, const int32_t* __restrict__ ipiv, int ld, int k0, int jlo,
+               int jhi) {
   const int bt = blockIdx.y;
   float* A = a + (size_t)bt * ld * ld;
-  __shared__ float sL[kLuNB][kLuNB + 1];
-  __shared__ int sp[kLuNB];
+  __shared__ float sL[NB][NB + 1];
+  __shared__ int sp[NB];
   const int t = threadIdx.x;
-  for (int x = t; x < kLuNB * kLuNB; x += 256) {
-    const int r = x / kLuNB, c = x % kLuNB;
+  for (int x = t; x < NB * NB; x += 256) {
+    const int r = x / NB, c = x % NB;
     sL[r][c] = A[(size_t)(k0 + r) * ld + k0 + c];
   }
-  if (t < kLuNB) sp[t] = ipiv[(size_t)bt * ld + k0 + t];
+  if (t < NB) sp[t] = ipiv[(size_t)bt * ld + k0 + t];
   __syncthreads();
-  const int j = blockIdx.x * 256 + t;
-  if (j >= ld || (j >= k0 && j < k0 + kLuNB)) return;
-  for (int c = 0; c < kLuNB; ++c) {
+  const int j = jlo + blockIdx.x * 256 + t;
+  if (j >= jhi || (j >= k0 && j < k0 + NB)) return;
+  for (int c = 0; c < NB; ++c) {
     const int r2 = sp[c];
     if (r2 != k0 + c) {
       float* x1 = &A[(size_t)(k0 + c) * ld + j];
@@ -215,49 +220,52 @@ k_lu_swap_trsm(float* __restrict__ a, const int32_t* __restrict__ ipiv, int ld, 
       *x2 = tmp;
     }
   }
-  if (j < k0) return;   // left of the panel: the exchanges only
-  float u[kLuNB];
+  if (j < k0) return;   // left of the block: the exchanges only
+  float u[NB];
 #pragma unroll
-  for (int c = 0; c < kLuNB; ++c) u[c] = A[(size_t)(k0 + c) * ld + j];
+  for (int c = 0; c < NB; ++c) u[c] = A[(size_t)(k0 + c) * ld + j];
 #pragma unroll
-  for (int c = 1; c < kLuNB; ++c) {
+  for (int c = 1; c < NB; ++c) {
 #pragma unroll
     for (int cc = 0; cc < c; ++cc) u[c] -= sL[c][cc] * u[cc];
   }
 #pragma unroll
-  for (int c = 0; c < kLuNB; ++c) A[(size_t)(k0 + c) * ld + j] = u[c];
+  for (int c = 0; c < NB; ++c) A[(size_t)(k0 + c) * ld + j] = u[c];
 }
 
-// A22 -= L21 U12 over the trailing block [k0 + 16, ld)^2.  A 64x64 tile per
-// workgroup, a 32x32 quarter per wavefront as 2x2 blocks of the 16x16x4 FP32
-// MFMA: lane l feeds A(i = l % 16, k = l / 16), B(k = l / 16, j = l % 16)
-// and holds D(i = 4 (l / 16) + v, j = l % 16), v = 0..3.  The trailing size
-// is a multiple of 16, so a 16x16 block lies wholly inside or outside.
+// A22 -= L21 U12 on rows [k0 + K, ld) and columns [k0 + K, jhi), L21 the
+// columns [k0, k0 + K) below the block and U12 its rows right of it.  A 64x64
+// tile per workgroup, a 32x32 quarter per wavefront as 2x2 blocks of the
+// 16x16x4 FP32 MFMA: lane l feeds A(i = l % 16, k = l / 16),
+// B(k = l / 16, j = l % 16) and holds D(i = 4 (l / 16) + v, j = l % 16),
+// v = 0..3.  The extents are multiples of 16, so a 16x16 block lies wholly
+// inside or outside.
+template <int K>
 __global__ void __launch_bounds__(256)
-k_lu_update(float* __restrict__ a, int ld, int k0, int nt) {
+k_lu_update(float* __restrict__ a, int ld, int k0, int jhi, int ntj) {
   const int bt = blockIdx.y;
-  const int s0 = k0 + kLuNB;
-  const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+  const int s0 = k0 + K;
+  const int ti = blockIdx.x / ntj, tj = blockIdx.x % ntj;
   const int i0 = s0 + ti * kLuTile, j0 = s0 + tj * kLuTile;
   float* A = a + (size_t)bt * ld * ld;
-  __shared__ float Ls[kLuTile][kLuNB + 1];   // -L21 rows of the tile
-  __shared__ float Us[kLuNB][kLuTile];       // U12 columns of the tile
+  __shared__ float Ls[kLuTile][K + 1];   // -L21 rows of the tile
+  __shared__ float Us[K][kLuTile];       // U12 columns of the tile
   const int t = threadIdx.x;
-  for (int x = t; x < kLuTile * kLuNB; x += 256) {
-    const int r = x / kLuNB, c = x % kLuNB;
+  for (int x = t; x < kLuTile * K; x += 256) {
+    const int r = x / K, c = x % K;
     Ls[r][c] = i0 + r < ld ? -A[(size_t)(i0 + r) * ld + k0 + c] : 0.0f;
   }
-  for (int x = t; x < kLuNB * kLuTile; x += 256) {
+  for (int x = t; x < K * kLuTile; x += 256) {
     const int r = x / kLuTile, c = x % kLuTile;
-    Us[r][c] = j0 + c < ld ? A[(size_t)(k0 + r) * ld + j0 + c] : 0.0f;
+    Us[r][c] = j0 + c < jhi ? A[(size_t)(k0 + r) * ld + j0 + c] : 0.0f;
   }
   __syncthreads();
   const int w = t >> 6, lane = t & 63;
   const int wr = 32 * (w >> 1), wc = 32 * (w & 1);
-  if (i0 + wr >= ld || j0 + wc >= ld) return;
+  if (i0 + wr >= ld || j0 + wc >= jhi) return;
   const int li = lane & 15, lk = lane >> 4;
-  // blocks inside the matrix (uniform over the wavefront)
-  const bool in_r[2] = {true, i0 + wr + 16 < ld}, in_c[2] = {true, j0 + wc + 16 < ld};
+  // blocks inside the updated range (uniform over the wavefront)
+  const bool in_r[2] = {true, i0 + wr + 16 < ld}, in_c[2] = {true, j0 + wc + 16 < jhi};
   floatx4 acc[2][2];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
@@ -268,7 +276,7 @@ k_lu_update(float* __restrict__ a, int ld, int k0, int nt) {
         acc[bi][bj][v] = in_r[bi] && in_c[bj]
             ? A[(size_t)(i0 + wr + 16 * bi + 4 * lk + v) * ld + j0 + wc + 16 * bj + li] : 0.0f;
 #pragma unroll
-  for (int kk = 0; kk < kLuNB; kk += 4) {
+  for (int kk = 0; kk < K; kk += 4) {
     float av[2], bv[2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) av[bi] = Ls[wr + 16 * bi + li][kk + lk];
@@ -292,76 +300,247 @@ k_lu_update(float* __restrict__ a, int ld, int k0, int nt) {
       }
 }
 
-// x = A^-1 b for one right-hand side per workgroup (matrix blockIdx.y, rhs
-// blockIdx.x): the row exchanges, then forward (unit L) and backward (U)
-// substitution in 64-row blocks -- the block's diagonal part solved by one
-// wavefront from LDS, the rows beyond it updated by all threads.
-__global__ void __launch_bounds__(256)
+// x = A^-1 b for up to kSolveRhs right-hand sides of one matrix per
+// workgroup (matrix blockIdx.y, right-hand sides kSolveRhs * blockIdx.x on):
+// the row exchanges, then forward (unit L) and backward (U) substitution in
+// 64-row blocks -- the block's diagonal part solved by one wavefront from
+// LDS, the rows beyond it updated by all 1024 threads, every row of the
+// factor read once for all the right-hand sides.
+constexpr int kSolveRhs = 2;
+constexpr int kSolveThreads = 1024;
+
+// The 64-column segment of a factor row (16-byte aligned: ld and the block
+// start are multiples of 64) against the block's entries of each
+// right-hand side, in 16-byte loads.
+__device__ __forceinline__ void row_dot(const float* __restrict__ row,
+                                        const float (*sx)[kLuMaxLd], int b0,
+                                        float (&s)[kSolveRhs]) {
+  const float4* r4 = reinterpret_cast<const float4*>(row);
+  float4 a[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = r4[k];
+#pragma unroll
+  for (int r = 0; r < kSolveRhs; ++r) {
+    const float4* x4 = reinterpret_cast<const float4*>(&sx[r][b0]);
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float4 x = x4[k];
+      acc += a[k].x * x.x;
+      acc += a[k].y * x.y;
+      acc += a[k].z * x.z;
+      acc += a[k].w * x.w;
+    }
+    s[r] = acc;
+  }
+}
+__global__ void __launch_bounds__(kSolveThreads)
 k_lu_solve(const float* __restrict__ lu, const int32_t* __restrict__ ipiv, float* __restrict__ b,
            int ld, int mr, int nrhs) {
-  const int bt = blockIdx.y, rhs = blockIdx.x;
+  const int bt = blockIdx.y, r0 = blockIdx.x * kSolveRhs;
+  const int nr = nrhs - r0 < kSolveRhs ? nrhs - r0 : kSolveRhs;
   const float* A = lu + (size_t)bt * ld * ld;
-  float* x = b + ((size_t)bt * nrhs + rhs) * ld;
-  __shared__ float sx[kLuMaxLd];
+  __shared__ __attribute__((aligned(16))) float sx[kSolveRhs][kLuMaxLd];
   __shared__ float sd[64][65];
   const int t = threadIdx.x;
-  for (int i = t; i < ld; i += 256) sx[i] = x[i];
+  for (int r = 0; r < nr; ++r) {
+    const float* x = b + ((size_t)bt * nrhs + r0 + r) * ld;
+    for (int i = t; i < ld; i += kSolveThreads) sx[r][i] = x[i];
+  }
   __syncthreads();
-  if (t == 0) {
+  if (t < nr) {
     for (int i = 0; i < mr; ++i) {
       const int p = ipiv[(size_t)bt * ld + i];
       if (p != i) {
-        const float tmp = sx[i];
-        sx[i] = sx[p];
-        sx[p] = tmp;
+        const float tmp = sx[t][i];
+        sx[t][i] = sx[t][p];
+        sx[t][p] = tmp;
       }
     }
   }
   __syncthreads();
   for (int b0 = 0; b0 < mr; b0 += 64) {
-    for (int xi = t; xi < 64 * 64; xi += 256) sd[xi / 64][xi % 64] = A[(size_t)(b0 + xi / 64) * ld + b0 + xi % 64];
+    for (int xi = t; xi < 64 * 64; xi += kSolveThreads)
+      sd[xi / 64][xi % 64] = A[(size_t)(b0 + xi / 64) * ld + b0 + xi % 64];
     __syncthreads();
     if (t < 64) {
-      float xv = sx[b0 + t];
+      float xv[kSolveRhs];
+      for (int r = 0; r < kSolveRhs; ++r) xv[r] = r < nr ? sx[r][b0 + t] : 0.0f;
       for (int i = 0; i < 64; ++i) {
-        const float xi = __shfl(xv, i);
-        if (t > i) xv -= sd[t][i] * xi;
+        const float l = sd[t][i];
+#pragma unroll
+        for (int r = 0; r < kSolveRhs; ++r) {
+          const float xi = __shfl(xv[r], i);
+          if (t > i) xv[r] -= l * xi;
+        }
       }
-      sx[b0 + t] = xv;
+      for (int r = 0; r < nr; ++r) sx[r][b0 + t] = xv[r];
     }
     __syncthreads();
-    for (int i = b0 + 64 + t; i < mr; i += 256) {
-      const float* row = A + (size_t)i * ld + b0;
-      float s = 0.0f;
-#pragma unroll 16
-      for (int k = 0; k < 64; ++k) s += row[k] * sx[b0 + k];
-      sx[i] -= s;
+    for (int i = b0 + 64 + t; i < mr; i += kSolveThreads) {
+      float s[kSolveRhs];
+      row_dot(A + (size_t)i * ld + b0, sx, b0, s);
+      for (int r = 0; r < nr; ++r) sx[r][i] -= s[r];
     }
     __syncthreads();
   }
   for (int b0 = mr - 64; b0 >= 0; b0 -= 64) {
-    for (int xi = t; xi < 64 * 64; xi += 256) sd[xi / 64][xi % 64] = A[(size_t)(b0 + xi / 64) * ld + b0 + xi % 64];
+    for (int xi = t; xi < 64 * 64; xi += kSolveThreads)
+      sd[xi / 64][xi % 64] = A[(size_t)(b0 + xi / 64) * ld + b0 + xi % 64];
     __syncthreads();
     if (t < 64) {
-      float xv = sx[b0 + t];
+      float xv[kSolveRhs];
+      for (int r = 0; r < kSolveRhs; ++r) xv[r] = r < nr ? sx[r][b0 + t] : 0.0f;
       for (int i = 63; i >= 0; --i) {
-        if (t == i) xv /= sd[i][i];
-        const float xi = __shfl(xv, i);
-        if (t < i) xv -= sd[t][i] * xi;
+        const float u = sd[t][i], d = sd[i][i];
+#pragma unroll
+        for (int r = 0; r < kSolveRhs; ++r) {
+          if (t == i) xv[r] /= d;
+          const float xi = __shfl(xv[r], i);
+          if (t < i) xv[r] -= u * xi;
+        }
       }
-      sx[b0 + t] = xv;
+      for (int r = 0; r < nr; ++r) sx[r][b0 + t] = xv[r];
     }
     __syncthreads();
-    for (int i = t; i < b0; i += 256) {
-      const float* row = A + (size_t)i * ld + b0;
-      float s = 0.0f;
-#pragma unroll 16
-      for (int k = 0; k < 64; ++k) s += row[k] * sx[b0 + k];
-      sx[i] -= s;
+    for (int i = t; i < b0; i += kSolveThreads) {
+      float s[kSolveRhs];
+      row_dot(A + (size_t)i * ld + b0, sx, b0, s);
+      for (int r = 0; r < nr; ++r) sx[r][i] -= s[r];
     }
     __syncthreads();
   }
-  for (int i = t; i < ld; i += 256) x[i] = sx[i];
+  for (int r = 0; r < nr; ++r) {
+    float* x = b + ((size_t)bt * nrhs + r0 + r) * ld;
+    for (int i = t; i < ld; i += kSolveThreads) x[i] = sx[r][i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solve support for repeated solves with one factor (the refinement steps):
+// the row exchanges composed into one permutation, and the inverses of the
+// 64x64 diagonal blocks of L (unit lower) and U, so that a solve's block
+// steps are small products instead of 64 dependent substitution steps.
+
+// perm[i] = the original row that ends at position i after the exchanges
+__global__ void __launch_bounds__(256)
+k_lu_perm(const int32_t* __restrict__ ipiv, int32_t* __restrict__ perm, int ld, int mr) {
+  const int bt = blockIdx.x;
+  __shared__ int32_t idx[kLuMaxLd];
+  for (int i = threadIdx.x; i < ld; i += 256) idx[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < mr; ++i) {
+      const int p = ipiv[(size_t)bt * ld + i];
+      const int tmp = idx[i];
+      idx[i] = idx[p];
+      idx[p] = tmp;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ld; i += 256) perm[(size_t)bt * ld + i] = idx[i];
+}
+
+// Inverse of diagonal block blockIdx.x of matrix blockIdx.y: of L (unit
+// lower) for blockIdx.z = 0, of U for 1; thread c forms column c by
+// substitution against the block staged in LDS.  inv: [batch][blocks][64][64].
+__global__ void __launch_bounds__(64)
+k_lu_diag_inverse(const float* __restrict__ lu, float* __restrict__ linv, float* __restrict__ uinv,
+                  int ld, int nblk) {
+  const int blk = blockIdx.x, bt = blockIdx.y, upper = blockIdx.z;
+  const float* A = lu + (size_t)bt * ld * ld;
+  __shared__ float sb[64][65];
+  const int c = threadIdx.x, b0 = blk * 64;
+  for (int x = c; x < 64 * 64; x += 64) sb[x / 64][x % 64] = A[(size_t)(b0 + x / 64) * ld + b0 + x % 64];
+  __syncthreads();
+  float col[64];
+  float* out = (upper ? uinv : linv) + ((size_t)bt * nblk + blk) * 64 * 64;
+  if (!upper) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      float v = i == c ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= sb[i][k] * col[k];
+      col[i] = i < c ? 0.0f : v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 63; i >= 0; --i) {
+      float v = i == c ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = i + 1; k < 64; ++k) v -= sb[i][k] * col[k];
+      col[i] = i > c ? 0.0f : v / sb[i][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i) out[i * 64 + c] = col[i];
+}
+
+// x_blk = inv x_blk for every right-hand side: 1024 threads, sixteen lanes a
+// row of the 64x64 inverse, four columns a lane, summed over the sixteen.
+__device__ __forceinline__ void block_apply(const float* __restrict__ inv,
+                                            float (*sx)[kLuMaxLd], int b0, int nr) {
+  const int t = threadIdx.x, row = t >> 4, part = t & 15;
+  float s[kSolveRhs];
+  const float4 a = reinterpret_cast<const float4*>(inv + row * 64)[part];
+#pragma unroll
+  for (int r = 0; r < kSolveRhs; ++r) {
+    const float4 x = reinterpret_cast<const float4*>(&sx[r][b0])[part];
+    float acc = a.x * x.x;
+    acc += a.y * x.y;
+    acc += a.z * x.z;
+    acc += a.w * x.w;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    s[r] = acc;
+  }
+  __syncthreads();   // every lane has read the block before it is overwritten
+  if (part == 0)
+    for (int r = 0; r < nr; ++r) sx[r][b0 + row] = s[r];
+  __syncthreads();
+}
+
+// The solve with the composed permutation and the inverted diagonal blocks.
+__global__ void __launch_bounds__(kSolveThreads)
+k_lu_solve_inv(const float* __restrict__ lu, const int32_t* __restrict__ perm,
+               const float* __restrict__ linv, const float* __restrict__ uinv,
+               float* __restrict__ b, int ld, int mr, int nrhs) {
+  const int bt = blockIdx.y, r0 = blockIdx.x * kSolveRhs;
+  const int nr = nrhs - r0 < kSolveRhs ? nrhs - r0 : kSolveRhs;
+  const int nblk = mr / 64;
+  const float* A = lu + (size_t)bt * ld * ld;
+  __shared__ __attribute__((aligned(16))) float sx[kSolveRhs][kLuMaxLd];
+  const int t = threadIdx.x;
+  for (int r = 0; r < kSolveRhs; ++r) {
+    const float* x = b + ((size_t)bt * nrhs + r0 + (r < nr ? r : 0)) * ld;
+    for (int i = t; i < ld; i += kSolveThreads)
+      sx[r][i] = r < nr ? x[perm[(size_t)bt * ld + i]] : 0.0f;
+  }
+  __syncthreads();
+  for (int blk = 0; blk < nblk; ++blk) {
+    const int b0 = blk * 64;
+    block_apply(linv + ((size_t)bt * nblk + blk) * 64 * 64, sx, b0, kSolveRhs);
+    for (int i = b0 + 64 + t; i < mr; i += kSolveThreads) {
+      float s[kSolveRhs];
+      row_dot(A + (size_t)i * ld + b0, sx, b0, s);
+      for (int r = 0; r < kSolveRhs; ++r) sx[r][i] -= s[r];
+    }
+    __syncthreads();
+  }
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int b0 = blk * 64;
+    block_apply(uinv + ((size_t)bt * nblk + blk) * 64 * 64, sx, b0, kSolveRhs);
+    for (int i = t; i < b0; i += kSolveThreads) {
+      float s[kSolveRhs];
+      row_dot(A + (size_t)i * ld + b0, sx, b0, s);
+      for (int r = 0; r < kSolveRhs; ++r) sx[r][i] -= s[r];
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < nr; ++r) {
+    float* x = b + ((size_t)bt * nrhs + r0 + r) * ld;
+    for (int i = t; i < ld; i += kSolveThreads) x[i] = sx[r][i];
+  }
 }
 
 int lu_check(int batch, int m, int ld) {
@@ -380,15 +559,27 @@ int lu_factor(Ctx* cx, int batch, int m, int ld, float* a, int32_t* ipiv, int32_
   const int mr = lu_rows(m);
   hipStream_t st = cx->stream;
   hipLaunchKernelGGL(k_lu_zero_i32, dim3(grid_for(batch)), dim3(kBlock), 0, st, info, (int64_t)batch);
-  for (int k0 = 0; k0 < mr; k0 += kLuNB) {
-    hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(kPanelThreads), 0, st, a, ipiv, info, ld, k0);
-    hipLaunchKernelGGL(k_lu_swap_trsm, dim3((ld + 255) / 256, batch), dim3(256), 0, st, a, ipiv,
-                       ld, k0);
-    const int rest = ld - k0 - kLuNB;
-    if (rest > 0) {
-      const int nt = (rest + kLuTile - 1) / kLuTile;
-      hipLaunchKernelGGL(k_lu_update, dim3(nt * nt, batch), dim3(256), 0, st, a, ld, k0, nt);
+  // Blocks of kLuOuter columns, each factored as four 16-column panels with
+  // their exchanges, U12 and updates kept inside the block; then the block's
+  // exchanges on the other columns, its U12 and one trailing update of depth
+  // kLuOuter (a quarter of the trailing-matrix traffic of 16-deep updates).
+  auto update = [&](auto kernel, int K, int k0, int jhi) {
+    const int rows = ld - k0 - K, cols = jhi - k0 - K;
+    if (rows <= 0 || cols <= 0) return;
+    const int nti = (rows + kLuTile - 1) / kLuTile, ntj = (cols + kLuTile - 1) / kLuTile;
+    hipLaunchKernelGGL(kernel, dim3(nti * ntj, batch), dim3(256), 0, st, a, ld, k0, jhi, ntj);
+  };
+  for (int K0 = 0; K0 < mr; K0 += kLuOuter) {
+    for (int c0 = K0; c0 < K0 + kLuOuter; c0 += kLuNB) {
+      hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(kPanelThreads), 0, st, a, ipiv, info, ld,
+                         c0);
+      hipLaunchKernelGGL(k_lu_swap_trsm<kLuNB>, dim3(1, batch), dim3(256), 0, st, a, ipiv, ld, c0,
+                         K0, K0 + kLuOuter);
+      update(k_lu_update<kLuNB>, kLuNB, c0, K0 + kLuOuter);
     }
+    hipLaunchKernelGGL(k_lu_swap_trsm<kLuOuter>, dim3((ld + 255) / 256, batch), dim3(256), 0, st,
+                       a, ipiv, ld, K0, 0, ld);
+    update(k_lu_update<kLuOuter>, kLuOuter, K0, ld);
   }
   XA_HIP(hipGetLastError());
   return XA_OK;
@@ -398,8 +589,32 @@ int lu_solve(Ctx* cx, int batch, int nrhs, int m, int ld, const float* lu, const
              float* b) {
   int rc = lu_check(batch, m, ld);
   if (rc != XA_OK || batch == 0 || m == 0 || nrhs <= 0) return rc;
-  hipLaunchKernelGGL(k_lu_solve, dim3(nrhs, batch), dim3(256), 0, cx->stream, lu, ipiv, b, ld,
-                     lu_rows(m), nrhs);
+  hipLaunchKernelGGL(k_lu_solve, dim3((nrhs + kSolveRhs - 1) / kSolveRhs, batch),
+                     dim3(kSolveThreads), 0, cx->stream, lu, ipiv, b, ld, lu_rows(m), nrhs);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
+// perm: batch x ld; linv, uinv: batch x (mr / 64) x 64 x 64 floats
+int lu_prepare(Ctx* cx, int batch, int m, int ld, const float* lu, const int32_t* ipiv,
+               int32_t* perm, float* linv, float* uinv) {
+  int rc = lu_check(batch, m, ld);
+  if (rc != XA_OK || batch == 0 || m == 0) return rc;
+  const int mr = lu_rows(m), nblk = mr / 64;
+  hipLaunchKernelGGL(k_lu_perm, dim3(batch), dim3(256), 0, cx->stream, ipiv, perm, ld, mr);
+  hipLaunchKernelGGL(k_lu_diag_inverse, dim3(nblk, batch, 2), dim3(64), 0, cx->stream, lu, linv,
+                     uinv, ld, nblk);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
+int lu_solve_prepared(Ctx* cx, int batch, int nrhs, int m, int ld, const float* lu,
+                      const int32_t* perm, const float* linv, const float* uinv, float* b) {
+  int rc = lu_check(batch, m, ld);
+  if (rc != XA_OK || batch == 0 || m == 0 || nrhs <= 0) return rc;
+  hipLaunchKernelGGL(k_lu_solve_inv, dim3((nrhs + kSolveRhs - 1) / kSolveRhs, batch),
+                     dim3(kSolveThreads), 0, cx->stream, lu, perm, linv, uinv, b, ld, lu_rows(m),
+                     nrhs);
   XA_HIP(hipGetLastError());
   return XA_OK;
 }
