@@ -70,6 +70,52 @@ struct b4size_s {
   double inv_od_ref, rho_ref;
 };
 
+/* ---- Jacobian pairs -------------------------------------------------------
+ * Every (row node, column node) pair loadDAEMatrices addresses
+ * (N_DEV_MOSFET_B4.C:16577-17268), named as the reference's matrix pointers:
+ * GE gate (external), GP gate prime, GM gate mid, D/S/B external,
+ * DP/SP/BP primes, DB/SB drain/source body, Q the NQS charge, IBS/IDS/IGS the
+ * initial-condition branch currents. */
+#define B4_PAIRS(X) \
+  X(GEge, GateExt, GateExt) X(GEgp, GateExt, GatePrime) X(GEdp, GateExt, DrainPrime) \
+  X(GEsp, GateExt, SourcePrime) X(GEbp, GateExt, BodyPrime) X(GEgm, GateExt, GateMid) \
+  X(GPge, GatePrime, GateExt) X(GPgp, GatePrime, GatePrime) X(GPdp, GatePrime, DrainPrime) \
+  X(GPsp, GatePrime, SourcePrime) X(GPbp, GatePrime, BodyPrime) X(GPgm, GatePrime, GateMid) \
+  X(GPq, GatePrime, Charge) \
+  X(GMge, GateMid, GateExt) X(GMgm, GateMid, GateMid) X(GMdp, GateMid, DrainPrime) \
+  X(GMgp, GateMid, GatePrime) X(GMsp, GateMid, SourcePrime) X(GMbp, GateMid, BodyPrime) \
+  X(Dgp, Drain, GatePrime) X(Dsp, Drain, SourcePrime) X(Dbp, Drain, BodyPrime) \
+  X(Ddp, Drain, DrainPrime) X(Dd, Drain, Drain) X(Dids, Drain, Ids) \
+  X(Sdp, Source, DrainPrime) X(Sgp, Source, GatePrime) X(Sbp, Source, BodyPrime) \
+  X(Ssp, Source, SourcePrime) X(Ss, Source, Source) X(Sibs, Source, Ibs) X(Sids, Source, Ids) \
+  X(Sigs, Source, Igs) \
+  X(DPdp, DrainPrime, DrainPrime) X(DPd, DrainPrime, Drain) X(DPgp, DrainPrime, GatePrime) \
+  X(DPsp, DrainPrime, SourcePrime) X(DPbp, DrainPrime, BodyPrime) X(DPdb, DrainPrime, DrainBody) \
+  X(DPq, DrainPrime, Charge) X(DPgm, DrainPrime, GateMid) \
+  X(SPdp, SourcePrime, DrainPrime) X(SPgp, SourcePrime, GatePrime) \
+  X(SPsp, SourcePrime, SourcePrime) X(SPs, SourcePrime, Source) X(SPbp, SourcePrime, BodyPrime) \
+  X(SPsb, SourcePrime, SourceBody) X(SPq, SourcePrime, Charge) X(SPgm, SourcePrime, GateMid) \
+  X(BPdp, BodyPrime, DrainPrime) X(BPgp, BodyPrime, GatePrime) X(BPsp, BodyPrime, SourcePrime) \
+  X(BPbp, BodyPrime, BodyPrime) X(BPdb, BodyPrime, DrainBody) X(BPb, BodyPrime, Body) \
+  X(BPsb, BodyPrime, SourceBody) X(BPgm, BodyPrime, GateMid) \
+  X(DBdp, DrainBody, DrainPrime) X(DBdb, DrainBody, DrainBody) X(DBbp, DrainBody, BodyPrime) \
+  X(DBb, DrainBody, Body) \
+  X(SBsp, SourceBody, SourcePrime) X(SBbp, SourceBody, BodyPrime) X(SBb, SourceBody, Body) \
+  X(SBsb, SourceBody, SourceBody) \
+  X(Bdb, Body, DrainBody) X(Bbp, Body, BodyPrime) X(Bsb, Body, SourceBody) X(Bb, Body, Body) \
+  X(Bibs, Body, Ibs) \
+  X(Qq, Charge, Charge) X(Qgp, Charge, GatePrime) X(Qdp, Charge, DrainPrime) \
+  X(Qsp, Charge, SourcePrime) X(Qbp, Charge, BodyPrime) \
+  X(IBSb, Ibs, Body) X(IBSs, Ibs, Source) X(IBSibs, Ibs, Ibs) \
+  X(IDSd, Ids, Drain) X(IDSs, Ids, Source) X(IDSids, Ids, Ids) \
+  X(GEigs, GateExt, Igs) X(IGSg, Igs, GateExt) X(IGSs, Igs, Source) X(IGSigs, Igs, Igs)
+enum {
+#define B4_PAIR_ENUM(name, r, c) B4_##name,
+  B4_PAIRS(B4_PAIR_ENUM)
+#undef B4_PAIR_ENUM
+  B4_NPAIR
+};
+
 /* ---- instance ----------------------------------------------------------- */
 typedef struct {
   b4model* model;
@@ -138,16 +184,17 @@ typedef struct {
   double Igtoteq_Jdxp, ceqgcrg_Jdxp, ceqgstot_Jdxp, ceqgdtot_Jdxp, ceqjs_Jdxp, ceqjd_Jdxp;
   double Qeqqg_Jdxp, Qeqqd_Jdxp, Qeqqb_Jdxp, Qeqqgmid_Jdxp, Qeqqjs_Jdxp, Qeqqjd_Jdxp;
   double gjbd, gjbs, gdpr, gspr, geltd;
+  double Qqcheq_Jdxp;
   /* LIDs and CSR slots */
   int li_Drain, li_GateExt, li_Source, li_Body, li_DrainPrime, li_GatePrime, li_GateMid;
-  int li_SourcePrime, li_BodyPrime, li_DrainBody, li_SourceBody;
+  int li_SourcePrime, li_BodyPrime, li_DrainBody, li_SourceBody, li_Charge;
+  int li_Ibs, li_Ids, li_Igs;
   int li_store_vbd, li_store_vbs, li_store_vgs, li_store_vds, li_store_vges, li_store_vgms;
   int li_store_vdes, li_store_vses, li_store_vdbs, li_store_vsbs, li_store_vdbd;
   int li_store_vged, li_store_vgmd, li_store_von;
-  int li_state_qb, li_state_qg, li_state_qd;
-  int64_t s_GPgp, s_GPdp, s_GPsp, s_GPbp, s_DPdp, s_DPd, s_DPgp, s_DPsp, s_DPbp;
-  int64_t s_Ddp, s_Dd, s_SPdp, s_SPgp, s_SPsp, s_SPs, s_SPbp, s_Ssp, s_Ss;
-  int64_t s_BPdp, s_BPgp, s_BPsp, s_BPbp;
+  int li_state_qb, li_state_qg, li_state_qd, li_state_qgmid, li_state_qbs, li_state_qbd;
+  int li_state_qcheq, li_state_qcdump;
+  int64_t sp[B4_NPAIR];   /* CSR slot of each (row, col) pair the loads address */
 } b4inst;
 
 #define M(x)  (m->p.x)

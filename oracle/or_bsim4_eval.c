@@ -1,9 +1,8 @@
 /* or_bsim4_eval.c -- oracle restatement of BSIM4 Instance::updateIntermediateVars
  * (src/DeviceModelPKG/OpenModels/src/N_DEV_MOSFET_B4.C:7744-11919), test
- * infrastructure only.  Implemented option set: capMod = 2, rgateMod =
- * rbodyMod = trnqsMod = acnqsMod = 0, igcMod = igbMod = 0, rdsMod = 0,
- * mtrlMod = 0 (others are rejected at setup); every other branch of the
- * reference is restated. */
+ * infrastructure only.  Every branch of the reference is restated except
+ * mtrlMod != 0, which setup rejects: capMod 0/1/2, igcMod 1/2, igbMod,
+ * rdsMod, rgateMod 1-3, rbodyMod, trnqsMod/acnqsMod. */
 #include "or_bsim4.h"
 #include "or_eval_libm.h"
 
@@ -79,8 +78,19 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   double vs, Fsevl, dvs_dVg, dvs_dVd, dvs_dVb, dFsevl_dVg, dFsevl_dVd, dFsevl_dVb;
   double vgdx, vgsx, epssub, toxe, epsrox, von_local, Vds, Vgs, Vbs, Vdb_, Gm_, Vdseff_;
   double vbd_old, vbs_old, vgs_old, vds_old, vges_old, vgms_old, vdes_old, vses_old;
-  double vdbs_old, vsbs_old, vdbd_old, vged_old, vgmd_old;
-  int Check = 0;
+  double vdbs_old, vsbs_old, vdbd_old, vged_old, vgmd_old, vsbd_old, vsbd;
+  /* gate tunnelling, bias-dependent Rs/Rd, capMod 0/1 */
+  double Vfb, Voxacc, dVoxacc_dVg, dVoxacc_dVb, Voxdepinv, dVoxdepinv_dVg, dVoxdepinv_dVd;
+  double dVoxdepinv_dVb, VxNVt, ExpVxNVt, Vaux, dVaux_dVg, dVaux_dVd, dVaux_dVb, dT2_dVg_;
+  double dT7_dVg, dT7_dVd, dT7_dVb, Igc, dIgc_dVg, dIgc_dVd, dIgc_dVb, Pigcd, dPigcd_dVg;
+  double dPigcd_dVd, dPigcd_dVb, dIgcs_dVg, dIgcs_dVd, dIgcs_dVb, dIgcd_dVg, dIgcd_dVd, dIgcd_dVb;
+  double dIgs_dVg, dIgs_dVs, dIgd_dVg, dIgd_dVd, Igbacc, dIgbacc_dVg, dIgbacc_dVb;
+  double Igbinv, dIgbinv_dVg, dIgbinv_dVd, dIgbinv_dVb;
+  double dT0_dvg, dT1_dvb, dT3_dvg, dT3_dvb, Rs, dRs_dvg, dRs_dvb, Rd, dRd_dvg, dRd_dvb;
+  double dgstot_dvd, dgstot_dvg, dgstot_dvb, dgstot_dvs, dgdtot_dvd, dgdtot_dvg, dgdtot_dvb;
+  double dgdtot_dvs, Arg1, One_Third_CoxWL, Two_Third_CoxWL, Alphaz, dAlphaz_dVg, dAlphaz_dVb;
+  double dVgst_dVb, dVgst_dVg, dVdseffCV_dVg_, dVdseffCV_dVd_;
+  int Check = 0, Check1 = 0, Check2 = 0;
 
   dT1_dVb = dT2_dVb = dT3_dVb = dT4_dVb = dT5_dVb = dT6_dVb = 0.0;
   dT0_dVb = 0.0;
@@ -100,7 +110,7 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   I->Vgm = (I->li_GateMid >= 0) ? sol(v->nextSol, I->li_GateMid) : 0.0;
   I->Vdb = sol(v->nextSol, I->li_DrainBody);
   I->Vsb = sol(v->nextSol, I->li_SourceBody);
-  I->Qtotal = 0.0;
+  I->Qtotal = I->itrnqsMod ? sol(v->nextSol, I->li_Charge) : 0.0;
   I->Vddp = I->Vd - I->Vdp;
   I->Vssp = I->Vs - I->Vsp;
   I->Vdbb = I->Vdb - I->Vb;
@@ -239,7 +249,7 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   (void)vged_old; (void)vgmd_old; (void)vges_old; (void)vgms_old; (void)vdbs_old;
   (void)vsbs_old; (void)vdbd_old; (void)vbd_old;
 
-  /* voltage limiting (:8165-8359), rgateMod = rbodyMod = rdsMod = 0 */
+  /* voltage limiting (:8165-8359) */
   if (opt->voltageLimiterFlag && !(ss->initFixFlag && I->OFF)) {
     if (ss->newtonIter >= 0 && !(ss->initJctFlag)) {
       if (vds_old >= 0.0) {
@@ -247,18 +257,58 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
         I->vds = I->vgs - I->vgd;
         I->vds = or_limvds(I->vds, vds_old);
         I->vgd = I->vgs - I->vds;
+        if (I->irgateMod == 3) {
+          I->vges = or_fetlim(I->vges, vges_old, von_local);
+          I->vgms = or_fetlim(I->vgms, vgms_old, von_local);
+        } else if ((I->irgateMod == 1) || (I->irgateMod == 2)) {
+          I->vges = or_fetlim(I->vges, vges_old, von_local);
+          I->vged = I->vges - I->vds;
+        }
+        if (M(rdsMod)) {
+          I->vdes = or_limvds(I->vdes, vdes_old);
+          I->vses = -or_limvds(-I->vses, -vses_old);
+        }
       } else {
         I->vgd = or_fetlim(I->vgd, vgd_old, von_local);
         I->vds = I->vgs - I->vgd;
         I->vds = -or_limvds(-I->vds, -vds_old);
         I->vgs = I->vgd + I->vds;
+        if (I->irgateMod == 3) {
+          I->vged = or_fetlim(I->vged, vged_old, von_local);
+          I->vges = I->vged + I->vds;
+          I->vgmd = or_fetlim(I->vgmd, vgmd_old, von_local);
+          I->vgms = I->vgmd + I->vds;
+        }
+        if ((I->irgateMod == 1) || (I->irgateMod == 2)) {
+          I->vged = or_fetlim(I->vged, vged_old, von_local);
+          I->vges = I->vged + I->vds;
+        }
+        if (M(rdsMod)) {
+          I->vdes = -or_limvds(-I->vdes, -vdes_old);
+          I->vses = or_limvds(I->vses, vses_old);
+        }
       }
       if (I->vds >= 0.0) {
         I->vbs = or_pnjlim(I->vbs, vbs_old, B4_vt0, m->vcrit, &Check);
         I->vbd = I->vbs - I->vds;
+        if (I->irbodyMod) {
+          I->vdbs = or_pnjlim(I->vdbs, vdbs_old, B4_vt0, m->vcrit, &Check1);
+          I->vdbd = I->vdbs - I->vds;
+          I->vsbs = or_pnjlim(I->vsbs, vsbs_old, B4_vt0, m->vcrit, &Check2);
+          if ((Check1 != 0) || (Check2 != 0)) Check = 1;
+        }
       } else {
         I->vbd = or_pnjlim(I->vbd, vbd_old, B4_vt0, m->vcrit, &Check);
         I->vbs = I->vbd + I->vds;
+        if (I->irbodyMod) {
+          I->vdbd = or_pnjlim(I->vdbd, vdbd_old, B4_vt0, m->vcrit, &Check1);
+          I->vdbs = I->vdbd + I->vds;
+          vsbd_old = vsbs_old - vds_old;
+          vsbd = I->vsbs - I->vds;
+          vsbd = or_pnjlim(vsbd, vsbd_old, B4_vt0, m->vcrit, &Check2);
+          I->vsbs = vsbd + I->vds;
+          if ((Check1 != 0) || (Check2 != 0)) Check = 1;
+        }
       }
     }
     if (Check == 1) I->limitedFlag = 1;
@@ -1368,13 +1418,102 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   I->gmbs = Gmb;
   I->IdovVds = Ids;
   if (I->IdovVds <= 1.0e-9) I->IdovVds = 1.0e-9;
-  /* rgateMod <= 1 and no NQS: no gate resistance current conductances */
-  I->gcrg = I->gcrgd = I->gcrgg = I->gcrgs = I->gcrgb = 0.0;
-  /* rdsMod = 0 (:10063-10069) */
-  I->gstot = I->gstotd = I->gstotg = 0.0;
-  I->gstots = I->gstotb = 0.0;
-  I->gdtot = I->gdtotd = I->gdtotg = 0.0;
-  I->gdtots = I->gdtotb = 0.0;
+  /* gate resistance conductances (:9953-9988); without rgateMod > 1 or NQS
+     they stay at the zeros setupFVectorVars left */
+  if ((I->irgateMod > 1) || (I->itrnqsMod != 0) || (I->iacnqsMod != 0)) {
+    T9 = p->xrcrg2 * m->vtm;
+    T0 = T9 * beta;
+    dT0_dVd = (dbeta_dVd + dbeta_dVg * dVgsteff_dVd) * T9;
+    dT0_dVb = (dbeta_dVb + dbeta_dVg * dVgsteff_dVb) * T9;
+    dT0_dVg = dbeta_dVg * T9;
+    I->gcrg = p->xrcrg1 * (T0 + Ids);
+    I->gcrgd = p->xrcrg1 * (dT0_dVd + tmp1);
+    I->gcrgb = p->xrcrg1 * (dT0_dVb + tmp2) * dVbseff_dVb;
+    I->gcrgg = p->xrcrg1 * (dT0_dVg + tmp3) * dVgsteff_dVg;
+    if (nf != 1.0) {
+      I->gcrg *= nf;
+      I->gcrgg *= nf;
+      I->gcrgd *= nf;
+      I->gcrgb *= nf;
+    }
+    if (I->irgateMod == 2) {
+      T10 = I->grgeltd * I->grgeltd;
+      T11 = I->grgeltd + I->gcrg;
+      I->gcrg = I->grgeltd * I->gcrg / T11;
+      T12 = T10 / T11 / T11;
+      I->gcrgg *= T12;
+      I->gcrgd *= T12;
+      I->gcrgb *= T12;
+    }
+    I->gcrgs = -(I->gcrgg + I->gcrgd + I->gcrgb);
+  } else {
+    I->gcrg = I->gcrgd = I->gcrgg = I->gcrgs = I->gcrgb = 0.0;
+  }
+  /* bias-dependent external source/drain resistance (:9991-10069) */
+  if (M(rdsMod)) {
+    T0 = I->vgs - p->vfbsd;
+    T1 = sqrt(T0 * T0 + 1.0e-4);
+    I->vgs_eff = 0.5 * (T0 + T1);
+    I->dvgs_eff_dvg = I->vgs_eff / T1;
+    T0 = 1.0 + p->prwg * I->vgs_eff;
+    dT0_dvg = -p->prwg / T0 / T0 * I->dvgs_eff_dvg;
+    T1 = -p->prwb * I->vbs;
+    dT1_dvb = -p->prwb;
+    T2 = 1.0 / T0 + T1;
+    T3 = T2 + sqrt(T2 * T2 + 0.01);
+    dT3_dvg = T3 / (T3 - T2);
+    dT3_dvb = dT3_dvg * dT1_dvb;
+    dT3_dvg *= dT0_dvg;
+    T4 = p->rs0 * 0.5;
+    Rs = p->rswmin + T3 * T4;
+    dRs_dvg = T4 * dT3_dvg;
+    dRs_dvb = T4 * dT3_dvb;
+    T0 = 1.0 + I->sourceConductance * Rs;
+    I->gstot = I->sourceConductance / T0;
+    T0 = -I->gstot * I->gstot;
+    dgstot_dvd = 0.0;
+    dgstot_dvg = T0 * dRs_dvg;
+    dgstot_dvb = T0 * dRs_dvb;
+    dgstot_dvs = -(dgstot_dvg + dgstot_dvb + dgstot_dvd);
+    T0 = I->vgd - p->vfbsd;
+    T1 = sqrt(T0 * T0 + 1.0e-4);
+    I->vgd_eff = 0.5 * (T0 + T1);
+    I->dvgd_eff_dvg = I->vgd_eff / T1;
+    T0 = 1.0 + p->prwg * I->vgd_eff;
+    dT0_dvg = -p->prwg / T0 / T0 * I->dvgd_eff_dvg;
+    T1 = -p->prwb * I->vbd;
+    dT1_dvb = -p->prwb;
+    T2 = 1.0 / T0 + T1;
+    T3 = T2 + sqrt(T2 * T2 + 0.01);
+    dT3_dvg = T3 / (T3 - T2);
+    dT3_dvb = dT3_dvg * dT1_dvb;
+    dT3_dvg *= dT0_dvg;
+    T4 = p->rd0 * 0.5;
+    Rd = p->rdwmin + T3 * T4;
+    dRd_dvg = T4 * dT3_dvg;
+    dRd_dvb = T4 * dT3_dvb;
+    T0 = 1.0 + I->drainConductance * Rd;
+    I->gdtot = I->drainConductance / T0;
+    T0 = -I->gdtot * I->gdtot;
+    dgdtot_dvs = 0.0;
+    dgdtot_dvg = T0 * dRd_dvg;
+    dgdtot_dvb = T0 * dRd_dvb;
+    dgdtot_dvd = -(dgdtot_dvg + dgdtot_dvb + dgdtot_dvs);
+    I->gstotd = I->vses * dgstot_dvd;
+    I->gstotg = I->vses * dgstot_dvg;
+    I->gstots = I->vses * dgstot_dvs;
+    I->gstotb = I->vses * dgstot_dvb;
+    T2 = I->vdes - I->vds;
+    I->gdtotd = T2 * dgdtot_dvd;
+    I->gdtotg = T2 * dgdtot_dvg;
+    I->gdtots = T2 * dgdtot_dvs;
+    I->gdtotb = T2 * dgdtot_dvb;
+  } else {
+    I->gstot = I->gstotd = I->gstotg = 0.0;
+    I->gstots = I->gstotb = 0.0;
+    I->gdtot = I->gdtotd = I->gdtotg = 0.0;
+    I->gdtots = I->gdtotb = 0.0;
+  }
 
   /* GIDL (:10074-10126) */
   T0 = 3.0 * toxe;
@@ -1441,12 +1580,319 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   I->ggisls = Ggisls;
   I->ggislg = Ggislg;
   I->ggislb = Ggislb;
-  /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
-  I->Igcs = I->gIgcsg = I->gIgcsd = I->gIgcsb = 0.0;
-  I->Igcd = I->gIgcdg = I->gIgcdd = I->gIgcdb = 0.0;
-  I->Igs = I->gIgsg = I->gIgss = 0.0;
-  I->Igd = I->gIgdg = I->gIgdd = 0.0;
-  I->Igb = I->gIgbg = I->gIgbd = I->gIgbs = I->gIgbb = 0.0;
+  /* gate tunnelling current (:10182-10621) */
+  Vfb = 0.0;
+  Voxacc = dVoxacc_dVg = dVoxacc_dVb = 0.0;
+  Voxdepinv = dVoxdepinv_dVg = dVoxdepinv_dVd = dVoxdepinv_dVb = 0.0;
+  if ((M(igcMod) != 0) || (M(igbMod) != 0)) {
+    Vfb = I->vfbzb;
+    V3 = Vfb - Vgs_eff + Vbseff - B4_DELTA_3;
+    if (Vfb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * B4_DELTA_3 * Vfb);
+    else T0 = sqrt(V3 * V3 + 4.0 * B4_DELTA_3 * Vfb);
+    T1 = 0.5 * (1.0 + V3 / T0);
+    Vfbeff = Vfb - 0.5 * (V3 + T0);
+    dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    dVfbeff_dVb = -T1;
+    Voxacc = Vfb - Vfbeff;
+    dVoxacc_dVg = -dVfbeff_dVg;
+    dVoxacc_dVb = -dVfbeff_dVb;
+    if (Voxacc < 0.0) Voxacc = dVoxacc_dVg = dVoxacc_dVb = 0.0;
+    T0 = 0.5 * p->k1ox;
+    T3 = Vgs_eff - Vfbeff - Vbseff - I->Vgsteff;
+    if (p->k1ox == 0.0) {
+      Voxdepinv = dVoxdepinv_dVg = dVoxdepinv_dVd = dVoxdepinv_dVb = 0.0;
+    } else if (T3 < 0.0) {
+      Voxdepinv = -T3;
+      dVoxdepinv_dVg = -dVgs_eff_dVg + dVfbeff_dVg + dVgsteff_dVg;
+      dVoxdepinv_dVd = dVgsteff_dVd;
+      dVoxdepinv_dVb = dVfbeff_dVb + 1.0 + dVgsteff_dVb;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = T0 / T1;
+      Voxdepinv = p->k1ox * (T1 - T0);
+      dVoxdepinv_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+      dVoxdepinv_dVd = -T2 * dVgsteff_dVd;
+      dVoxdepinv_dVb = -T2 * (dVfbeff_dVb + 1.0 + dVgsteff_dVb);
+    }
+    Voxdepinv += I->Vgsteff;
+    dVoxdepinv_dVg += dVgsteff_dVg;
+    dVoxdepinv_dVd += dVgsteff_dVd;
+    dVoxdepinv_dVb += dVgsteff_dVb;
+  }
+  if (M(tempMod) < 2) tmp = Vtm;
+  else tmp = Vtm0;
+  if (M(igcMod)) {
+    T0 = tmp * p->nigc;
+    VxNVt = 0.0;
+    Vaux = dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+    if (M(igcMod) == 1) {
+      VxNVt = (Vgs_eff - m->dtype * I->vth0) / T0;
+      if (VxNVt > OR_EXP_THRESHOLD) {
+        Vaux = Vgs_eff - m->dtype * I->vth0;
+        dVaux_dVg = dVgs_eff_dVg;
+        dVaux_dVd = 0.0;
+        dVaux_dVb = 0.0;
+      }
+    } else if (M(igcMod) == 2) {
+      VxNVt = (Vgs_eff - I->von) / T0;
+      if (VxNVt > OR_EXP_THRESHOLD) {
+        Vaux = Vgs_eff - I->von;
+        dVaux_dVg = dVgs_eff_dVg;
+        dVaux_dVd = -dVth_dVd;
+        dVaux_dVb = -dVth_dVb;
+      }
+    }
+    if (VxNVt < -OR_EXP_THRESHOLD) {
+      Vaux = T0 * log(1.0 + OR_MIN_EXP);
+      dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+    } else if ((VxNVt >= -OR_EXP_THRESHOLD) && (VxNVt <= OR_EXP_THRESHOLD)) {
+      ExpVxNVt = exp(VxNVt);
+      Vaux = T0 * log(1.0 + ExpVxNVt);
+      dVaux_dVg = ExpVxNVt / (1.0 + ExpVxNVt);
+      if (M(igcMod) == 1) {
+        dVaux_dVd = 0.0;
+        dVaux_dVb = 0.0;
+      } else if (M(igcMod) == 2) {
+        dVaux_dVd = -dVgs_eff_dVg * dVth_dVd;
+        dVaux_dVb = -dVgs_eff_dVg * dVth_dVb;
+      }
+      dVaux_dVg *= dVgs_eff_dVg;
+    }
+    T2 = Vgs_eff * Vaux;
+    dT2_dVg = dVgs_eff_dVg * Vaux + Vgs_eff * dVaux_dVg;
+    dT2_dVd = Vgs_eff * dVaux_dVd;
+    dT2_dVb = Vgs_eff * dVaux_dVb;
+    T11 = p->Aechvb;
+    T12 = p->Bechvb;
+    T3 = p->aigc * p->cigc - p->bigc;
+    T4 = p->bigc * p->cigc;
+    T5 = T12 * (p->aigc + T3 * Voxdepinv - T4 * Voxdepinv * Voxdepinv);
+    if (T5 > OR_EXP_THRESHOLD) {
+      T6 = OR_MAX_EXP;
+      dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+    } else if (T5 < -OR_EXP_THRESHOLD) {
+      T6 = OR_MIN_EXP;
+      dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+    } else {
+      T6 = exp(T5);
+      dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxdepinv);
+      dT6_dVd = dT6_dVg * dVoxdepinv_dVd;
+      dT6_dVb = dT6_dVg * dVoxdepinv_dVb;
+      dT6_dVg *= dVoxdepinv_dVg;
+    }
+    Igc = T11 * T2 * T6;
+    dIgc_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+    dIgc_dVd = T11 * (T2 * dT6_dVd + T6 * dT2_dVd);
+    dIgc_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+    if (MG(pigcd)) {
+      Pigcd = p->pigcd;
+      dPigcd_dVg = dPigcd_dVd = dPigcd_dVb = 0.0;
+    } else {
+      T11 = p->Bechvb * toxe;
+      T12 = I->Vgsteff + 1.0e-20;
+      T13 = T11 / T12 / T12;
+      T14 = -T13 / T12;
+      Pigcd = T13 * (1.0 - 0.5 * Vdseff_ / T12);
+      dPigcd_dVg = T14 * (2.0 + 0.5 * (dVdseff_dVg - 3.0 * Vdseff_ / T12));
+      dPigcd_dVd = 0.5 * T14 * dVdseff_dVd;
+      dPigcd_dVb = 0.5 * T14 * dVdseff_dVb;
+    }
+    T7 = -Pigcd * Vdseff_;
+    dT7_dVg = -Vdseff_ * dPigcd_dVg - Pigcd * dVdseff_dVg;
+    dT7_dVd = -Vdseff_ * dPigcd_dVd - Pigcd * dVdseff_dVd + dT7_dVg * dVgsteff_dVd;
+    dT7_dVb = -Vdseff_ * dPigcd_dVb - Pigcd * dVdseff_dVb + dT7_dVg * dVgsteff_dVb;
+    dT7_dVg *= dVgsteff_dVg;
+    dT7_dVb *= dVbseff_dVb;
+    T8 = T7 * T7 + 2.0e-4;
+    dT8_dVg = 2.0 * T7;
+    dT8_dVd = dT8_dVg * dT7_dVd;
+    dT8_dVb = dT8_dVg * dT7_dVb;
+    dT8_dVg *= dT7_dVg;
+    if (T7 > OR_EXP_THRESHOLD) {
+      T9 = OR_MAX_EXP;
+      dT9_dVg = dT9_dVd = dT9_dVb = 0.0;
+    } else if (T7 < -OR_EXP_THRESHOLD) {
+      T9 = OR_MIN_EXP;
+      dT9_dVg = dT9_dVd = dT9_dVb = 0.0;
+    } else {
+      T9 = exp(T7);
+      dT9_dVg = T9 * dT7_dVg;
+      dT9_dVd = T9 * dT7_dVd;
+      dT9_dVb = T9 * dT7_dVb;
+    }
+    T0 = T8 * T8;
+    T1 = T9 - 1.0 + 1.0e-4;
+    T10 = (T1 - T7) / T8;
+    dT10_dVg = (dT9_dVg - dT7_dVg - T10 * dT8_dVg) / T8;
+    dT10_dVd = (dT9_dVd - dT7_dVd - T10 * dT8_dVd) / T8;
+    dT10_dVb = (dT9_dVb - dT7_dVb - T10 * dT8_dVb) / T8;
+    I->Igcs = Igc * T10;
+    dIgcs_dVg = dIgc_dVg * T10 + Igc * dT10_dVg;
+    dIgcs_dVd = dIgc_dVd * T10 + Igc * dT10_dVd;
+    dIgcs_dVb = dIgc_dVb * T10 + Igc * dT10_dVb;
+    T1 = T9 - 1.0 - 1.0e-4;
+    T10 = (T7 * T9 - T1) / T8;
+    dT10_dVg = (dT7_dVg * T9 + (T7 - 1.0) * dT9_dVg - T10 * dT8_dVg) / T8;
+    dT10_dVd = (dT7_dVd * T9 + (T7 - 1.0) * dT9_dVd - T10 * dT8_dVd) / T8;
+    dT10_dVb = (dT7_dVb * T9 + (T7 - 1.0) * dT9_dVb - T10 * dT8_dVb) / T8;
+    I->Igcd = Igc * T10;
+    dIgcd_dVg = dIgc_dVg * T10 + Igc * dT10_dVg;
+    dIgcd_dVd = dIgc_dVd * T10 + Igc * dT10_dVd;
+    dIgcd_dVb = dIgc_dVb * T10 + Igc * dT10_dVb;
+    I->gIgcsg = dIgcs_dVg;
+    I->gIgcsd = dIgcs_dVd;
+    I->gIgcsb = dIgcs_dVb * dVbseff_dVb;
+    I->gIgcdg = dIgcd_dVg;
+    I->gIgcdd = dIgcd_dVd;
+    I->gIgcdb = dIgcd_dVb * dVbseff_dVb;
+    /* source/drain edge tunnelling: rewrites the vgs_eff / vgd_eff members */
+    T0 = I->vgs - (p->vfbsd + p->vfbsdoff);
+    I->vgs_eff = sqrt(T0 * T0 + 1.0e-4);
+    I->dvgs_eff_dvg = T0 / I->vgs_eff;
+    T2 = I->vgs * I->vgs_eff;
+    dT2_dVg = I->vgs * I->dvgs_eff_dvg + I->vgs_eff;
+    T11 = p->AechvbEdgeS;
+    T12 = p->BechvbEdge;
+    T3 = p->aigs * p->cigs - p->bigs;
+    T4 = p->bigs * p->cigs;
+    T5 = T12 * (p->aigs + T3 * I->vgs_eff - T4 * I->vgs_eff * I->vgs_eff);
+    if (T5 > OR_EXP_THRESHOLD) {
+      T6 = OR_MAX_EXP;
+      dT6_dVg = 0.0;
+    } else if (T5 < -OR_EXP_THRESHOLD) {
+      T6 = OR_MIN_EXP;
+      dT6_dVg = 0.0;
+    } else {
+      T6 = exp(T5);
+      dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * I->vgs_eff) * I->dvgs_eff_dvg;
+    }
+    I->Igs = T11 * T2 * T6;
+    dIgs_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+    dIgs_dVs = -dIgs_dVg;
+    T0 = I->vgd - (p->vfbsd + p->vfbsdoff);
+    I->vgd_eff = sqrt(T0 * T0 + 1.0e-4);
+    I->dvgd_eff_dvg = T0 / I->vgd_eff;
+    T2 = I->vgd * I->vgd_eff;
+    dT2_dVg = I->vgd * I->dvgd_eff_dvg + I->vgd_eff;
+    T11 = p->AechvbEdgeD;
+    T3 = p->aigd * p->cigd - p->bigd;
+    T4 = p->bigd * p->cigd;
+    T5 = T12 * (p->aigd + T3 * I->vgd_eff - T4 * I->vgd_eff * I->vgd_eff);
+    if (T5 > OR_EXP_THRESHOLD) {
+      T6 = OR_MAX_EXP;
+      dT6_dVg = 0.0;
+    } else if (T5 < -OR_EXP_THRESHOLD) {
+      T6 = OR_MIN_EXP;
+      dT6_dVg = 0.0;
+    } else {
+      T6 = exp(T5);
+      dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * I->vgd_eff) * I->dvgd_eff_dvg;
+    }
+    I->Igd = T11 * T2 * T6;
+    dIgd_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+    dIgd_dVd = -dIgd_dVg;
+    I->gIgsg = dIgs_dVg;
+    I->gIgss = dIgs_dVs;
+    I->gIgdg = dIgd_dVg;
+    I->gIgdd = dIgd_dVd;
+  } else {
+    I->Igcs = I->gIgcsg = I->gIgcsd = I->gIgcsb = 0.0;
+    I->Igcd = I->gIgcdg = I->gIgcdd = I->gIgcdb = 0.0;
+    I->Igs = I->gIgsg = I->gIgss = 0.0;
+    I->Igd = I->gIgdg = I->gIgdd = 0.0;
+  }
+  if (M(igbMod)) {
+    T0 = tmp * p->nigbacc;
+    T1 = -Vgs_eff + Vbseff + Vfb;
+    VxNVt = T1 / T0;
+    if (VxNVt > OR_EXP_THRESHOLD) {
+      Vaux = T1;
+      dVaux_dVg = -dVgs_eff_dVg;
+      dVaux_dVb = 1.0;
+    } else if (VxNVt < -OR_EXP_THRESHOLD) {
+      Vaux = T0 * log(1.0 + OR_MIN_EXP);
+      dVaux_dVg = dVaux_dVb = 0.0;
+    } else {
+      ExpVxNVt = exp(VxNVt);
+      Vaux = T0 * log(1.0 + ExpVxNVt);
+      dVaux_dVb = ExpVxNVt / (1.0 + ExpVxNVt);
+      dVaux_dVg = -dVaux_dVb * dVgs_eff_dVg;
+    }
+    T2 = (Vgs_eff - Vbseff) * Vaux;
+    dT2_dVg = dVgs_eff_dVg * Vaux + (Vgs_eff - Vbseff) * dVaux_dVg;
+    dT2_dVb = -Vaux + (Vgs_eff - Vbseff) * dVaux_dVb;
+    T11 = 4.97232e-7 * p->weff * p->leff * p->ToxRatio;
+    T12 = -7.45669e11 * toxe;
+    T3 = p->aigbacc * p->cigbacc - p->bigbacc;
+    T4 = p->bigbacc * p->cigbacc;
+    T5 = T12 * (p->aigbacc + T3 * Voxacc - T4 * Voxacc * Voxacc);
+    if (T5 > OR_EXP_THRESHOLD) {
+      T6 = OR_MAX_EXP;
+      dT6_dVg = dT6_dVb = 0.0;
+    } else if (T5 < -OR_EXP_THRESHOLD) {
+      T6 = OR_MIN_EXP;
+      dT6_dVg = dT6_dVb = 0.0;
+    } else {
+      T6 = exp(T5);
+      dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxacc);
+      dT6_dVb = dT6_dVg * dVoxacc_dVb;
+      dT6_dVg *= dVoxacc_dVg;
+    }
+    Igbacc = T11 * T2 * T6;
+    dIgbacc_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+    dIgbacc_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+    T0 = tmp * p->nigbinv;
+    T1 = Voxdepinv - p->eigbinv;
+    VxNVt = T1 / T0;
+    if (VxNVt > OR_EXP_THRESHOLD) {
+      Vaux = T1;
+      dVaux_dVg = dVoxdepinv_dVg;
+      dVaux_dVd = dVoxdepinv_dVd;
+      dVaux_dVb = dVoxdepinv_dVb;
+    } else if (VxNVt < -OR_EXP_THRESHOLD) {
+      Vaux = T0 * log(1.0 + OR_MIN_EXP);
+      dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+    } else {
+      ExpVxNVt = exp(VxNVt);
+      Vaux = T0 * log(1.0 + ExpVxNVt);
+      dVaux_dVg = ExpVxNVt / (1.0 + ExpVxNVt);
+      dVaux_dVd = dVaux_dVg * dVoxdepinv_dVd;
+      dVaux_dVb = dVaux_dVg * dVoxdepinv_dVb;
+      dVaux_dVg *= dVoxdepinv_dVg;
+    }
+    T2 = (Vgs_eff - Vbseff) * Vaux;
+    dT2_dVg = dVgs_eff_dVg * Vaux + (Vgs_eff - Vbseff) * dVaux_dVg;
+    dT2_dVd = (Vgs_eff - Vbseff) * dVaux_dVd;
+    dT2_dVb = -Vaux + (Vgs_eff - Vbseff) * dVaux_dVb;
+    T11 *= 0.75610;
+    T12 *= 1.31724;
+    T3 = p->aigbinv * p->cigbinv - p->bigbinv;
+    T4 = p->bigbinv * p->cigbinv;
+    T5 = T12 * (p->aigbinv + T3 * Voxdepinv - T4 * Voxdepinv * Voxdepinv);
+    if (T5 > OR_EXP_THRESHOLD) {
+      T6 = OR_MAX_EXP;
+      dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+    } else if (T5 < -OR_EXP_THRESHOLD) {
+      T6 = OR_MIN_EXP;
+      dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+    } else {
+      T6 = exp(T5);
+      dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxdepinv);
+      dT6_dVd = dT6_dVg * dVoxdepinv_dVd;
+      dT6_dVb = dT6_dVg * dVoxdepinv_dVb;
+      dT6_dVg *= dVoxdepinv_dVg;
+    }
+    Igbinv = T11 * T2 * T6;
+    dIgbinv_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+    dIgbinv_dVd = T11 * (T2 * dT6_dVd + T6 * dT2_dVd);
+    dIgbinv_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+    I->Igb = Igbinv + Igbacc;
+    I->gIgbg = dIgbinv_dVg + dIgbacc_dVg;
+    I->gIgbd = dIgbinv_dVd;
+    I->gIgbb = (dIgbinv_dVb + dIgbacc_dVb) * dVbseff_dVb;
+  } else {
+    I->Igb = I->gIgbg = I->gIgbd = I->gIgbs = I->gIgbb = 0.0;
+  }
   if (nf != 1.0) {
     I->cdrain *= nf;
     I->gds *= nf;
@@ -1523,6 +1969,228 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
     I->cgbb = I->csbb = I->cdbb = I->cbbb = 0.0;
     I->cqdb = I->cqsb = I->cqgb = I->cqbb = 0.0;
     I->gtau = 0.0;
+  } else if (M(capMod) == 0) {
+    /* capMod = 0 (:10718-11012) */
+    if (Vbseff < 0.0) {
+      Vbseff = Vbs;
+      dVbseff_dVb = 1.0;
+    } else {
+      Vbseff = p->phi - Phis;
+      dVbseff_dVb = -dPhis_dVb;
+    }
+    Vfb = p->vfbcv;
+    Vth = Vfb + p->phi + p->k1ox * sqrtPhis;
+    Vgst = Vgs_eff - Vth;
+    dVth_dVb = p->k1ox * dsqrtPhis_dVb;
+    dVgst_dVb = -dVth_dVb;
+    dVgst_dVg = dVgs_eff_dVg;
+    I->CoxWL = m->coxe * p->weffCV * p->leffCV * nf;
+    Arg1 = Vgs_eff - Vbseff - Vfb;
+    if (Arg1 <= 0.0) {
+      I->qgate = I->CoxWL * Arg1;
+      I->qbulk = -I->qgate;
+      I->qdrn = 0.0;
+      I->cggb = I->CoxWL * dVgs_eff_dVg;
+      I->cgdb = 0.0;
+      I->cgsb = I->CoxWL * (dVbseff_dVb - dVgs_eff_dVg);
+      I->cdgb = 0.0;
+      I->cddb = 0.0;
+      I->cdsb = 0.0;
+      I->cbgb = -I->CoxWL * dVgs_eff_dVg;
+      I->cbdb = 0.0;
+      I->cbsb = -I->cgsb;
+    } else if (Vgst <= 0.0) {
+      T1 = 0.5 * p->k1ox;
+      T2 = sqrt(T1 * T1 + Arg1);
+      I->qgate = I->CoxWL * p->k1ox * (T2 - T1);
+      I->qbulk = -I->qgate;
+      I->qdrn = 0.0;
+      T0 = I->CoxWL * T1 / T2;
+      I->cggb = T0 * dVgs_eff_dVg;
+      I->cgdb = 0.0;
+      I->cgsb = T0 * (dVbseff_dVb - dVgs_eff_dVg);
+      I->cdgb = 0.0;
+      I->cddb = 0.0;
+      I->cdsb = 0.0;
+      I->cbgb = -I->cggb;
+      I->cbdb = 0.0;
+      I->cbsb = -I->cgsb;
+    } else {
+      One_Third_CoxWL = I->CoxWL / 3.0;
+      Two_Third_CoxWL = 2.0 * One_Third_CoxWL;
+      AbulkCV = Abulk0 * p->abulkCVfactor;
+      dAbulkCV_dVb = p->abulkCVfactor * dAbulk0_dVb;
+      Vdsat = Vgst / AbulkCV;
+      dVdsat_dVg = dVgs_eff_dVg / AbulkCV;
+      dVdsat_dVb = -(Vdsat * dAbulkCV_dVb + dVth_dVb) / AbulkCV;
+      if (M(xpart) > 0.5) {
+        /* 0/100 partition */
+        if (Vdsat <= Vds) {
+          T1 = Vdsat / 3.0;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          I->qbulk = -(I->qgate + T2);
+          I->qdrn = 0.0;
+          I->cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          I->cgsb = -(I->cggb + T2);
+          I->cgdb = 0.0;
+          I->cdgb = 0.0;
+          I->cddb = 0.0;
+          I->cdsb = 0.0;
+          I->cbgb = -(I->cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          I->cbsb = -(I->cbgb + T3);
+          I->cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * I->CoxWL;
+          T4 = T9 * Alphaz;
+          T7 = 2.0 * Vds - T1 - 3.0 * T3;
+          T8 = T3 - T1 - 2.0 * Vds;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - 0.5 * (Vds - T3));
+          T10 = T4 * T8;
+          I->qdrn = T4 * T7;
+          I->qbulk = -(I->qgate + I->qdrn + T10);
+          T5 = T3 / T1;
+          I->cggb = I->CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = -I->CoxWL * T5 * dVdsat_dVb;
+          I->cgdb = I->CoxWL * (T2 - 0.5 + 0.5 * T5);
+          I->cgsb = -(I->cggb + T11 + I->cgdb);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T7 = T9 * T7;
+          T8 = T9 * T8;
+          T9 = 2.0 * T4 * (1.0 - 3.0 * T5);
+          I->cdgb = (T7 * dAlphaz_dVg - T9 * dVdsat_dVg) * dVgs_eff_dVg;
+          T12 = T7 * dAlphaz_dVb - T9 * dVdsat_dVb;
+          I->cddb = T4 * (3.0 - 6.0 * T2 - 3.0 * T5);
+          I->cdsb = -(I->cdgb + T12 + I->cddb);
+          T9 = 2.0 * T4 * (1.0 + T5);
+          T10 = (T8 * dAlphaz_dVg - T9 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = T8 * dAlphaz_dVb - T9 * dVdsat_dVb;
+          T12 = T4 * (2.0 * T2 + T5 - 1.0);
+          T0 = -(T10 + T11 + T12);
+          I->cbgb = -(I->cggb + I->cdgb + T10);
+          I->cbdb = -(I->cgdb + I->cddb + T12);
+          I->cbsb = -(I->cgsb + I->cdsb + T0);
+        }
+      } else if (M(xpart) < 0.5) {
+        /* 40/60 partition */
+        if (Vds >= Vdsat) {
+          T1 = Vdsat / 3.0;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          I->qbulk = -(I->qgate + T2);
+          I->qdrn = 0.4 * T2;
+          I->cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          I->cgsb = -(I->cggb + T2);
+          I->cgdb = 0.0;
+          T3 = 0.4 * Two_Third_CoxWL;
+          I->cdgb = -T3 * dVgs_eff_dVg;
+          I->cddb = 0.0;
+          T4 = T3 * dVth_dVb;
+          I->cdsb = -(T4 + I->cdgb);
+          I->cbgb = -(I->cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          I->cbsb = -(I->cbgb + T3);
+          I->cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * I->CoxWL;
+          T4 = T9 * Alphaz;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - 0.5 * (Vds - T3));
+          T5 = T3 / T1;
+          I->cggb = I->CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          tmp = -I->CoxWL * T5 * dVdsat_dVb;
+          I->cgdb = I->CoxWL * (T2 - 0.5 + 0.5 * T5);
+          I->cgsb = -(I->cggb + I->cgdb + tmp);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T6 = 8.0 * Vdsat * Vdsat - 6.0 * Vdsat * Vds + 1.2 * Vds * Vds;
+          T8 = T2 / T1;
+          T7 = Vds - T1 - T8 * T6;
+          I->qdrn = T4 * T7;
+          T7 *= T9;
+          tmp = T8 / T1;
+          tmp1 = T4 * (2.0 - 4.0 * tmp * T6 + T8 * (16.0 * Vdsat - 6.0 * Vds));
+          I->cdgb = (T7 * dAlphaz_dVg - tmp1 * dVdsat_dVg) * dVgs_eff_dVg;
+          T10 = T7 * dAlphaz_dVb - tmp1 * dVdsat_dVb;
+          I->cddb = T4 * (2.0 - (1.0 / (3.0 * T1 * T1) + 2.0 * tmp) * T6
+                          + T8 * (6.0 * Vdsat - 2.4 * Vds));
+          I->cdsb = -(I->cdgb + T10 + I->cddb);
+          T7 = 2.0 * (T1 + T3);
+          I->qbulk = -(I->qgate - T4 * T7);
+          T7 *= T9;
+          T0 = 4.0 * T4 * (1.0 - T5);
+          T12 = (-T7 * dAlphaz_dVg - I->cdgb - T0 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = -T7 * dAlphaz_dVb - T10 - T0 * dVdsat_dVb;
+          T10 = -4.0 * T4 * (T2 - 0.5 + 0.5 * T5) - I->cddb;
+          tmp = -(T10 + T11 + T12);
+          I->cbgb = -(I->cggb + I->cdgb + T12);
+          I->cbdb = -(I->cgdb + I->cddb + T10);
+          I->cbsb = -(I->cgsb + I->cdsb + tmp);
+        }
+      } else {
+        /* 50/50 partition */
+        if (Vds >= Vdsat) {
+          T1 = Vdsat / 3.0;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          I->qbulk = -(I->qgate + T2);
+          I->qdrn = 0.5 * T2;
+          I->cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          I->cgsb = -(I->cggb + T2);
+          I->cgdb = 0.0;
+          I->cdgb = -One_Third_CoxWL * dVgs_eff_dVg;
+          I->cddb = 0.0;
+          T4 = One_Third_CoxWL * dVth_dVb;
+          I->cdsb = -(T4 + I->cdgb);
+          I->cbgb = -(I->cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          I->cbsb = -(I->cbgb + T3);
+          I->cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * I->CoxWL;
+          T4 = T9 * Alphaz;
+          I->qgate = I->CoxWL * (Vgs_eff - Vfb - p->phi - 0.5 * (Vds - T3));
+          T5 = T3 / T1;
+          I->cggb = I->CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          tmp = -I->CoxWL * T5 * dVdsat_dVb;
+          I->cgdb = I->CoxWL * (T2 - 0.5 + 0.5 * T5);
+          I->cgsb = -(I->cggb + I->cgdb + tmp);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T7 = T1 + T3;
+          I->qdrn = -T4 * T7;
+          I->qbulk = -(I->qgate + I->qdrn + I->qdrn);
+          T7 *= T9;
+          T0 = T4 * (2.0 * T5 - 2.0);
+          I->cdgb = (T0 * dVdsat_dVg - T7 * dAlphaz_dVg) * dVgs_eff_dVg;
+          T12 = T0 * dVdsat_dVb - T7 * dAlphaz_dVb;
+          I->cddb = T4 * (1.0 - 2.0 * T2 - T5);
+          I->cdsb = -(I->cdgb + T12 + I->cddb);
+          I->cbgb = -(I->cggb + 2.0 * I->cdgb);
+          I->cbdb = -(I->cgdb + 2.0 * I->cddb);
+          I->cbsb = -(I->cgsb + 2.0 * I->cdsb);
+        }
+      }
+    }
   } else {
     if (Vbseff < 0.0) {
       VbseffCV = Vbseff;
@@ -1615,6 +2283,140 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
       dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
       dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
     }
+    if (M(capMod) == 1) {
+    /* capMod = 1 (:11139-11319) */
+    Vfb = I->vfbzb;
+    V3 = Vfb - Vgs_eff + VbseffCV - B4_DELTA_3;
+    if (Vfb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * B4_DELTA_3 * Vfb);
+    else T0 = sqrt(V3 * V3 + 4.0 * B4_DELTA_3 * Vfb);
+    T1 = 0.5 * (1.0 + V3 / T0);
+    Vfbeff = Vfb - 0.5 * (V3 + T0);
+    dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+    Qac0 = I->CoxWL * (Vfbeff - Vfb);
+    dQac0_dVg = I->CoxWL * dVfbeff_dVg;
+    dQac0_dVb = I->CoxWL * dVfbeff_dVb;
+    T0 = 0.5 * p->k1ox;
+    T3 = Vgs_eff - Vfbeff - VbseffCV - I->Vgsteff;
+    if (p->k1ox == 0.0) {
+      T1 = 0.0;
+      T2 = 0.0;
+    } else if (T3 < 0.0) {
+      T1 = T0 + T3 / p->k1ox;
+      T2 = I->CoxWL;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = I->CoxWL * T0 / T1;
+    }
+    Qsub0 = I->CoxWL * p->k1ox * (T1 - T0);
+    dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+    dQsub0_dVd = -T2 * dVgsteff_dVd;
+    dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb);
+    AbulkCV = Abulk0 * p->abulkCVfactor;
+    dAbulkCV_dVb = p->abulkCVfactor * dAbulk0_dVb;
+    VdsatCV = I->Vgsteff / AbulkCV;
+    T0 = VdsatCV - Vds - B4_DELTA_4;
+    dT0_dVg = 1.0 / AbulkCV;
+    dT0_dVb = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+    T1 = sqrt(T0 * T0 + 4.0 * B4_DELTA_4 * VdsatCV);
+    dT1_dVg = (T0 + B4_DELTA_4 + B4_DELTA_4) / T1;
+    dT1_dVd = -T0 / T1;
+    dT1_dVb = dT1_dVg * dT0_dVb;
+    dT1_dVg *= dT0_dVg;
+    if (T0 >= 0.0) {
+      VdseffCV = VdsatCV - 0.5 * (T0 + T1);
+      dVdseffCV_dVg = 0.5 * (dT0_dVg - dT1_dVg);
+      dVdseffCV_dVd = 0.5 * (1.0 - dT1_dVd);
+      dVdseffCV_dVb = 0.5 * (dT0_dVb - dT1_dVb);
+    } else {
+      T3 = (B4_DELTA_4 + B4_DELTA_4) / (T1 - T0);
+      T4 = 1.0 - T3;
+      T5 = VdsatCV * T3 / (T1 - T0);
+      VdseffCV = VdsatCV * T4;
+      dVdseffCV_dVg = dT0_dVg * T4 + T5 * (dT1_dVg - dT0_dVg);
+      dVdseffCV_dVd = T5 * (dT1_dVd + 1.0);
+      dVdseffCV_dVb = dT0_dVb * (T4 - T5) + T5 * dT1_dVb;
+    }
+    if (Vds == 0.0) {
+      VdseffCV = 0.0;
+      dVdseffCV_dVg = 0.0;
+      dVdseffCV_dVb = 0.0;
+    }
+    T0 = AbulkCV * VdseffCV;
+    T1 = 12.0 * (I->Vgsteff - 0.5 * T0 + 1.0e-20);
+    T2 = VdseffCV / T1;
+    T3 = T0 * T2;
+    T4 = (1.0 - 12.0 * T2 * T2 * AbulkCV);
+    T5 = (6.0 * T0 * (4.0 * I->Vgsteff - T0) / (T1 * T1) - 0.5);
+    T6 = 12.0 * T2 * T2 * I->Vgsteff;
+    I->qgate = I->CoxWL * (I->Vgsteff - 0.5 * VdseffCV + T3);
+    Cgg1 = I->CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cgd1 = I->CoxWL * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd;
+    Cgb1 = I->CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteff_dVb;
+    Cgg1 *= dVgsteff_dVg;
+    T7 = 1.0 - AbulkCV;
+    I->qbulk = I->CoxWL * T7 * (0.5 * VdseffCV - T3);
+    T4 = -T7 * (T4 - 1.0);
+    T5 = -T7 * T5;
+    T6 = -(T7 * T6 + (0.5 * VdseffCV - T3));
+    Cbg1 = I->CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cbd1 = I->CoxWL * T5 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd;
+    Cbb1 = I->CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cbg1 * dVgsteff_dVb;
+    Cbg1 *= dVgsteff_dVg;
+    if (M(xpart) > 0.5) {
+      T1 = T1 + T1;
+      I->qsrc = -I->CoxWL * (0.5 * I->Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+      T7 = (4.0 * I->Vgsteff - T0) / (T1 * T1);
+      T4 = -(0.5 + 24.0 * T0 * T0 / (T1 * T1));
+      T5 = -(0.25 * AbulkCV - 12.0 * AbulkCV * T0 * T7);
+      T6 = -(0.25 * VdseffCV - 12.0 * T0 * VdseffCV * T7);
+      Csg = I->CoxWL * (T4 + T5 * dVdseffCV_dVg);
+      Csd = I->CoxWL * T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = I->CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else if (M(xpart) < 0.5) {
+      T1 = T1 / 12.0;
+      T2 = 0.5 * I->CoxWL / (T1 * T1);
+      T3 = I->Vgsteff * (2.0 * T0 * T0 / 3.0 + I->Vgsteff * (I->Vgsteff - 4.0 * T0 / 3.0))
+           - 2.0 * T0 * T0 * T0 / 15.0;
+      I->qsrc = -T2 * T3;
+      T7 = 4.0 / 3.0 * I->Vgsteff * (I->Vgsteff - T0) + 0.4 * T0 * T0;
+      T4 = -2.0 * I->qsrc / T1 - T2 * (I->Vgsteff * (3.0 * I->Vgsteff - 8.0 * T0 / 3.0)
+                                       + 2.0 * T0 * T0 / 3.0);
+      T5 = (I->qsrc / T1 + T2 * T7) * AbulkCV;
+      T6 = (I->qsrc / T1 * VdseffCV + T2 * T7 * VdseffCV);
+      Csg = (T4 + T5 * dVdseffCV_dVg);
+      Csd = T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else {
+      I->qsrc = -0.5 * (I->qgate + I->qbulk);
+      Csg = -0.5 * (Cgg1 + Cbg1);
+      Csb = -0.5 * (Cgb1 + Cbb1);
+      Csd = -0.5 * (Cgd1 + Cbd1);
+    }
+    I->qgate += Qac0 + Qsub0;
+    I->qbulk -= (Qac0 + Qsub0);
+    I->qdrn = -(I->qgate + I->qbulk + I->qsrc);
+    Cgg = dQac0_dVg + dQsub0_dVg + Cgg1;
+    Cgd = dQsub0_dVd + Cgd1;
+    Cgb = dQac0_dVb + dQsub0_dVb + Cgb1;
+    Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+    Cbd = Cbd1 - dQsub0_dVd;
+    Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+    Cgb *= dVbseff_dVb;
+    Cbb *= dVbseff_dVb;
+    Csb *= dVbseff_dVb;
+    I->cggb = Cgg;
+    I->cgsb = -(Cgg + Cgd + Cgb);
+    I->cgdb = Cgd;
+    I->cdgb = -(Cgg + Cbg + Csg);
+    I->cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+    I->cddb = -(Cgd + Cbd + Csd);
+    I->cbgb = Cbg;
+    I->cbsb = -(Cbg + Cbd + Cbb);
+    I->cbdb = Cbd;
+    } else {
     /* capMod = 2 (:11321-11602) */
     V3 = I->vfbzb - Vgs_eff + VbseffCV - B4_DELTA_3;
     if (I->vfbzb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * B4_DELTA_3 * I->vfbzb);
@@ -1820,6 +2622,7 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
     I->cbgb = Cbg;
     I->cbsb = -(Cbg + Cbd + Cbb);
     I->cbdb = Cbd;
+    }
   }
   I->csgb = -I->cggb - I->cdgb - I->cbgb;
   I->csdb = -I->cgdb - I->cddb - I->cbdb;
@@ -1828,6 +2631,19 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   I->cdbb = -I->cddb - I->cdgb - I->cdsb;
   I->cbbb = -I->cbgb - I->cbdb - I->cbsb;
   I->csbb = -I->cgbb - I->cdbb - I->cbbb;
+
+  /* NQS (:11626-11660) */
+  if ((I->itrnqsMod) || (I->iacnqsMod)) {
+    I->qchqs = I->qcheq = -(I->qbulk + I->qgate);
+    I->cqgb = -(I->cggb + I->cbgb);
+    I->cqdb = -(I->cgdb + I->cbdb);
+    I->cqsb = -(I->cgsb + I->cbsb);
+    I->cqbb = -(I->cqgb + I->cqdb + I->cqsb);
+    I->CoxWL = m->coxe * p->weffCV * nf * p->leffCV;
+    T1 = I->gcrg / I->CoxWL;
+    I->gtau = T1 * I->ScalingFactor;
+    if (I->iacnqsMod) I->taunet = 1.0 / T1;
+  }
 
   /* junction charges (:11662-11810) */
   if (I->ChargeComputationNeeded) {
@@ -1913,11 +2729,30 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
     }
   }
 
-  /* overlap capacitances, capMod != 0 (:11812-11883) */
-  vgdx = I->vgd;
-  vgsx = I->vgs;
+  /* gate resistor currents and overlap capacitances (:11812-11883) */
+  if (I->irgateMod == 3) {
+    vgdx = I->vgmd;
+    vgsx = I->vgms;
+  } else {
+    vgdx = I->vgd;
+    vgsx = I->vgs;
+  }
   I->Igate = I->IgateMid = 0.0;
   I->Igate_Jdxp = I->IgateMid_Jdxp = 0.0;
+  if (I->irgateMod == 1) {
+    I->Igate = I->grgeltd * (I->Vgegp);
+  } else if (I->irgateMod == 2) {
+    I->Igate = (I->gcrg) * (I->Vgegp);
+  } else if (I->irgateMod == 3) {
+    I->Igate = I->grgeltd * (I->Vgegm);
+    I->IgateMid = I->gcrg * (I->Vgmgp);
+  }
+  if (M(capMod) == 0) {
+    I->cgdo = p->cgdo;
+    I->qgdo = p->cgdo * vgdx;
+    I->cgso = p->cgso;
+    I->qgso = p->cgso * vgsx;
+  } else {
   T0 = vgdx + B4_DELTA_1;
   T1 = sqrt(T0 * T0 + 4.0 * B4_DELTA_1);
   T2 = 0.5 * (T0 - T1);
@@ -1932,6 +2767,7 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   T4 = sqrt(1.0 - 4.0 * T2 / p->ckappas);
   I->cgso = p->cgso + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
   I->qgso = (p->cgso + T3) * vgsx - T3 * (T2 + 0.5 * p->ckappas * (T4 - 1.0));
+  }
   if (nf != 1.0) {
     I->cgdo *= nf;
     I->cgso *= nf;
@@ -1940,8 +2776,21 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   }
   b4_setup_capacitors_old(I, ss);
   b4_setup_capacitors_new(I);
-  I->Idrain = I->drainConductance * I->Vddp;
-  I->Isource = I->sourceConductance * I->Vssp;
+  if (M(rdsMod) == 1) {
+    I->Idrain = I->gdtot * I->Vddp;
+    I->Isource = I->gstot * I->Vssp;
+  } else {
+    I->Idrain = I->drainConductance * I->Vddp;
+    I->Isource = I->sourceConductance * I->Vssp;
+  }
+  /* body network currents: the reference tests the model's rbodyMod here */
+  if (M(rbodyMod) != 0) {
+    I->Idbb = I->grbdb * I->Vdbb;
+    I->Idbbp = I->grbpd * I->Vdbbp;
+    I->Isbb = I->grbsb * I->Vsbb;
+    I->Isbbp = I->grbps * I->Vsbbp;
+    I->Ibpb = I->grbpb * I->Vbpb;
+  }
   (void)Vgs; (void)dTcen_dVd;
   return 1;
 }

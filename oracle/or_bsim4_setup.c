@@ -335,12 +335,14 @@ int b4_model_init(b4model* m, const or_card* c, const xa_device_options* opt) {
   /* updateTemperature rejects other junction models (N_DEV_MOSFET_B4.C:7515) */
   if (M(dioMod) != 0 && M(dioMod) != 1 && M(dioMod) != 2)
     return or_fail(XA_ERR_ARG, "bsim4: specified dioMod not matched");
-  if (M(capMod) != 2) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only capMod = 2");
+  /* the evaluation has branches for these values only (any other would
+   * leave its charges or gate-current terms unset) */
+  if (M(capMod) != 0 && M(capMod) != 1 && M(capMod) != 2)
+    return or_fail(XA_ERR_UNSUPPORTED, "bsim4: capMod must be 0, 1 or 2");
+  if (M(igcMod) != 0 && M(igcMod) != 1 && M(igcMod) != 2)
+    return or_fail(XA_ERR_UNSUPPORTED, "bsim4: igcMod must be 0, 1 or 2");
+  if (M(igbMod) != 0 && M(igbMod) != 1) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: igbMod must be 0 or 1");
   if (M(mtrlMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
-  if (M(igcMod) != 0 || M(igbMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only igcMod = igbMod = 0");
-  if (M(rdsMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only rdsMod = 0");
-  if (M(rgateMod) != 0 || M(rbodyMod) != 0 || M(trnqsMod) != 0 || M(acnqsMod) != 0)
-    return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only rgateMod = rbodyMod = trnqsMod = acnqsMod = 0");
   return XA_OK;
 }
 
@@ -1012,10 +1014,6 @@ int b4_instance_init(struct or_group_s* g, b4inst* I, int64_t i) {
   else if (I->itrnqsMod != 0 && I->itrnqsMod != 1) I->itrnqsMod = (int)M(trnqsMod);
   if (!I->acnqsMod_given) I->iacnqsMod = (int)M(acnqsMod);
   else if (I->iacnqsMod != 0 && I->iacnqsMod != 1) I->iacnqsMod = (int)M(acnqsMod);
-  if (I->irbodyMod || I->irgateMod || I->itrnqsMod || I->iacnqsMod)
-    return or_fail(XA_ERR_UNSUPPORTED, "bsim4: instance rbodyMod/rgateMod/trnqsMod/acnqsMod must be 0");
-  if (I->icVDS_given || I->icVGS_given || I->icVBS_given)
-    return or_fail(XA_ERR_UNSUPPORTED, "bsim4: IC1/IC2/IC3 not supported");
   b4_update_temperature(I, I->temp, &g->opt);
   createNode = 0;
   if (M(rdsMod) != 0) createNode = 1;
@@ -1039,9 +1037,13 @@ int b4_instance_init(struct or_group_s* g, b4inst* I, int64_t i) {
     }
   }
   I->sourceMOSFET_B4Exists = createNode != 0;
+  /* numIntVars / numStateVars (:3754-3788) */
   g->n_ext[i] = 4;
-  g->n_int[i] = (I->drainMOSFET_B4Exists ? 1 : 0) + (I->sourceMOSFET_B4Exists ? 1 : 0);
-  g->n_sta[i] = 3;
+  g->n_int[i] = (I->drainMOSFET_B4Exists ? 1 : 0) + (I->sourceMOSFET_B4Exists ? 1 : 0)
+                + ((I->irgateMod == 1 || I->irgateMod == 2) ? 1 : (I->irgateMod == 3) ? 2 : 0)
+                + (I->itrnqsMod ? 1 : 0) + (I->irbodyMod ? 3 : 0)
+                + (I->icVBS_given ? 1 : 0) + (I->icVDS_given ? 1 : 0) + (I->icVGS_given ? 1 : 0);
+  g->n_sta[i] = 3 + (I->irgateMod == 3 ? 1 : 0) + (I->irbodyMod ? 2 : 0) + (I->itrnqsMod ? 2 : 0);
   g->n_sto[i] = 14;   /* the BSIM4 Master loads write no lead currents */
   I->ScalingFactor = 1.0e-9;
   return XA_OK;

@@ -66,6 +66,40 @@ BSIM4_CARDS = [
 # check does not.
 BSIM4_VTL_CARD = (54, {"TYPE": 1.0, "VTL": 2e5, "LAMBDA": 0.2, "MOBMOD": 2.0})
 
+# Option cards: gate tunnelling (igcMod 1/2, igbMod, PIGCD given or not; a
+# thin oxide so the currents matter), bias-dependent source/drain resistance
+# (rdsMod 1), the capMod 0 and 1 charge models under each charge partition,
+# the gate resistance networks (rgateMod 1-3), the body resistance networks
+# (rbodyMod 1/2) and the NQS charge node (trnqsMod; acnqsMod).
+BSIM4_OPT_CARDS = {
+    "igc1": (54, {"TYPE": 1.0, "IGCMOD": 1.0, "IGBMOD": 1.0, "TOXE": 1.2e-9, "TOXP": 1.0e-9}),
+    "igc2": (54, {"TYPE": -1.0, "IGCMOD": 2.0, "IGBMOD": 1.0, "PIGCD": 2.0, "TOXE": 1.5e-9,
+                  "TOXP": 1.3e-9, "TEMPMOD": 2.0, "NGATE": 1e20, "VFBSDOFF": 0.02}),
+    "igc2b": (54, {"TYPE": 1.0, "IGCMOD": 2.0, "TOXE": 1.1e-9, "TOXP": 0.9e-9, "AIGSD": 0.012,
+                   "POXEDGE": 1.1}),
+    "rds": (54, {"TYPE": 1.0, "RDSMOD": 1.0, "RSH": 20.0, "RDSW": 150.0, "RSW": 80.0, "RDW": 90.0,
+                 "RSWMIN": 10.0, "RDWMIN": 12.0, "PRWG": 0.5, "PRWB": 0.1, "IGCMOD": 1.0,
+                 "TOXE": 1.2e-9, "TOXP": 1.0e-9}),
+    "rdsp": (54, {"TYPE": -1.0, "RDSMOD": 1.0, "RSH": 15.0, "RSW": 60.0, "RDW": 70.0, "PRWG": 0.3,
+                  "PRWB": 0.05}),
+    "cap0x0": (54, {"TYPE": 1.0, "CAPMOD": 0.0, "XPART": 0.0}),
+    "cap0x5": (54, {"TYPE": -1.0, "CAPMOD": 0.0, "XPART": 0.5}),
+    "cap0x1": (54, {"TYPE": 1.0, "CAPMOD": 0.0, "XPART": 1.0}),
+    "cap1x0": (54, {"TYPE": 1.0, "CAPMOD": 1.0, "XPART": 0.0}),
+    "cap1x1": (54, {"TYPE": -1.0, "CAPMOD": 1.0, "XPART": 1.0, "CVCHARGEMOD": 1.0}),
+    "cap1x5": (54, {"TYPE": 1.0, "CAPMOD": 1.0, "XPART": 0.5}),
+    "rg1": (54, {"TYPE": 1.0, "RGATEMOD": 1.0, "RSHG": 5.0}),
+    "rg2": (54, {"TYPE": -1.0, "RGATEMOD": 2.0, "RSHG": 5.0, "XRCRG1": 12.0, "XRCRG2": 1.0}),
+    "rg3": (54, {"TYPE": 1.0, "RGATEMOD": 3.0, "RSHG": 5.0, "CAPMOD": 1.0}),
+    "rb1": (54, {"TYPE": 1.0, "RBODYMOD": 1.0, "RBDB": 40.0, "RBSB": 50.0, "RBPB": 30.0,
+                 "RBPS": 20.0, "RBPD": 25.0}),
+    "rb2": (54, {"TYPE": -1.0, "RBODYMOD": 2.0, "RBPS0": 40.0, "RBPD0": 45.0, "RBPBX0": 60.0,
+                 "RBPBY0": 70.0, "RBSBX0": 80.0, "RBSBY0": 90.0, "RBDBX0": 85.0, "RBDBY0": 95.0}),
+    "nqs": (54, {"TYPE": 1.0, "TRNQSMOD": 1.0, "XRCRG1": 12.0, "XRCRG2": 1.0}),
+    "nqsrg3": (54, {"TYPE": -1.0, "TRNQSMOD": 1.0, "RGATEMOD": 3.0, "RBODYMOD": 1.0, "RSHG": 4.0}),
+    "acnqs": (54, {"TYPE": 1.0, "ACNQSMOD": 1.0, "RGATEMOD": 2.0, "RSHG": 5.0}),
+}
+
 
 def bsim4_spec(n, seed=20261203, cards=None, geometry=True, temp_spread=False, nf_mix=False,
                sizes=None):

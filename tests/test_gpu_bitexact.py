@@ -40,7 +40,8 @@ def run_exact(p, rng, iters=ITERS, fused=False, flag_sol=None):
         for k in ro:
             a, b = rg[k], ro[k]
             total += a.size
-            diff = a.view(np.int64) != b.view(np.int64)
+            # NaN payloads differ between the host and the GPU: both NaN is equal
+            diff = (a.view(np.int64) != b.view(np.int64)) & ~(np.isnan(a) & np.isnan(b))
             if diff.any():
                 idx = np.nonzero(diff)[0]
                 rel = np.abs(a[idx] - b[idx]) / np.maximum(np.maximum(np.abs(a[idx]),

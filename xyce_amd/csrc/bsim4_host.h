@@ -29,6 +29,7 @@ constexpr double kChargeQ = 1.60219e-19;
 constexpr double kDelta1 = 0.02;
 constexpr double kDelta3 = 0.02;
 constexpr double kDelta4 = 0.02;
+constexpr double kScalingFactor = 1.0e-9;   // N_DEV_MOSFET_B4.C:7878 (NQS charge scale)
 
 // ---- model card: every table parameter as a named field ------------------
 struct Params {

@@ -9,9 +9,9 @@
 //   updateTemperature :5640-7734, DioIjthVjmEval :14292-14303,
 //   NumFingerDiff :14322-14351, PAeffGeo :14364-14462,
 //   RdseffGeo :14473-14588, RdsEndIso :14599-14657, RdsEndSha :14668-14726.
-// Option subset: capMod = 2, mtrlMod = 0, igcMod = igbMod = 0, rdsMod = 0,
-// rgateMod = rbodyMod = trnqsMod = acnqsMod = 0 and no IC1-IC3; other values
-// are rejected with XA_ERR_UNSUPPORTED.
+// Every option of the evaluation is taken except mtrlMod != 0, which is
+// rejected with XA_ERR_UNSUPPORTED (as are capMod, igcMod and igbMod values
+// the evaluation has no branch for).
 #include <cmath>
 #include <cstring>
 #include "bsim4_host.h"
@@ -351,12 +351,14 @@ int model_init(Model& mref, const Card& c, const xa_device_options& opt) {
   // updateTemperature rejects other junction models (N_DEV_MOSFET_B4.C:7515)
   if (m->p.dioMod != 0 && m->p.dioMod != 1 && m->p.dioMod != 2)
     return fail(XA_ERR_ARG, "bsim4: specified dioMod not matched");
-  if (m->p.capMod != 2) return fail(XA_ERR_UNSUPPORTED, "bsim4: only capMod = 2");
+  // the evaluation has branches for these values only (any other would
+  // leave its charges or gate-current terms unset)
+  if (m->p.capMod != 0 && m->p.capMod != 1 && m->p.capMod != 2)
+    return fail(XA_ERR_UNSUPPORTED, "bsim4: capMod must be 0, 1 or 2");
+  if (m->p.igcMod != 0 && m->p.igcMod != 1 && m->p.igcMod != 2)
+    return fail(XA_ERR_UNSUPPORTED, "bsim4: igcMod must be 0, 1 or 2");
+  if (m->p.igbMod != 0 && m->p.igbMod != 1) return fail(XA_ERR_UNSUPPORTED, "bsim4: igbMod must be 0 or 1");
   if (m->p.mtrlMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
-  if (m->p.igcMod != 0 || m->p.igbMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only igcMod = igbMod = 0");
-  if (m->p.rdsMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only rdsMod = 0");
-  if (m->p.rgateMod != 0 || m->p.rbodyMod != 0 || m->p.trnqsMod != 0 || m->p.acnqsMod != 0)
-    return fail(XA_ERR_UNSUPPORTED, "bsim4: only rgateMod = rbodyMod = trnqsMod = acnqsMod = 0");
   return XA_OK;
 }
 
@@ -1021,10 +1023,6 @@ int instance_init(Group& g, Inst& iref, int64_t i) {
   else if (I->itrnqsMod != 0 && I->itrnqsMod != 1) I->itrnqsMod = (int)m->p.trnqsMod;
   if (!I->acnqsMod_given) I->iacnqsMod = (int)m->p.acnqsMod;
   else if (I->iacnqsMod != 0 && I->iacnqsMod != 1) I->iacnqsMod = (int)m->p.acnqsMod;
-  if (I->irbodyMod || I->irgateMod || I->itrnqsMod || I->iacnqsMod)
-    return fail(XA_ERR_UNSUPPORTED, "bsim4: instance rbodyMod/rgateMod/trnqsMod/acnqsMod must be 0");
-  if (I->icVDS_given || I->icVGS_given || I->icVBS_given)
-    return fail(XA_ERR_UNSUPPORTED, "bsim4: IC1/IC2/IC3 not supported");
   update_temperature(I, I->temp, g.opt);
   createNode = 0;
   if (m->p.rdsMod != 0) createNode = 1;
@@ -1048,9 +1046,13 @@ int instance_init(Group& g, Inst& iref, int64_t i) {
     }
   }
   I->sourceMOSFET_B4Exists = createNode != 0;
+  /* numIntVars / numStateVars (:3754-3788) */
   g.n_ext[i] = 4;
-  g.n_int[i] = (I->drainMOSFET_B4Exists ? 1 : 0) + (I->sourceMOSFET_B4Exists ? 1 : 0);
-  g.n_sta[i] = 3;
+  g.n_int[i] = (I->drainMOSFET_B4Exists ? 1 : 0) + (I->sourceMOSFET_B4Exists ? 1 : 0)
+               + ((I->irgateMod == 1 || I->irgateMod == 2) ? 1 : (I->irgateMod == 3) ? 2 : 0)
+               + (I->itrnqsMod ? 1 : 0) + (I->irbodyMod ? 3 : 0)
+               + (I->icVBS_given ? 1 : 0) + (I->icVDS_given ? 1 : 0) + (I->icVGS_given ? 1 : 0);
+  g.n_sta[i] = 3 + (I->irgateMod == 3 ? 1 : 0) + (I->irbodyMod ? 2 : 0) + (I->itrnqsMod ? 2 : 0);
   g.n_sto[i] = 14;   /* the BSIM4 Master loads write no lead currents */
   return XA_OK;
 }

@@ -9,9 +9,12 @@
 //   registerStateLIDs :5200-5257, registerStoreLIDs :5267-5290,
 //   Master::updateState :16180-16293, Master::loadDAEVectors :16303-16567,
 //   Master::loadDAEMatrices :16577-17268.
-// Option subset: capMod = 2, mtrlMod = 0, igcMod = igbMod = 0, rdsMod = 0,
-// rgateMod = rbodyMod = trnqsMod = acnqsMod = 0 (setup rejects the rest);
-// every other branch of updateIntermediateVars is evaluated.
+// Every branch of updateIntermediateVars is evaluated except mtrlMod != 0
+// (setup rejects it).  Groups whose cards or instances use capMod 0/1,
+// igcMod/igbMod, rdsMod, rgateMod, rbodyMod, trnqsMod/acnqsMod or initial
+// conditions take the option kernel (k_update_x: eval with X) and the option
+// stamp lists; the others take the default kernels, which compile none of
+// those branches.
 #include <cmath>
 #include <cstring>
 #include "bsim4_host.h"
@@ -259,6 +262,29 @@ enum DField {
 #undef B4V
   ND
 };
+// Columns only the option kernel reads (gate tunnelling, bias-dependent
+// source/drain resistance, capMod 0, the gate and body resistance networks,
+// initial conditions): size-dependent values (B4X) and instance values
+// (B4Y), appended after the ND columns for groups that use an option and
+// always read from HBM.
+#define B4_XFIELDS \
+  B4X(nigc) B4X(aigc) B4X(bigc) B4X(cigc) B4X(aigs) B4X(bigs) B4X(cigs) B4X(aigd) \
+  B4X(bigd) B4X(cigd) B4X(pigcd) B4X(vfbsd) B4X(vfbsdoff) B4X(Aechvb) B4X(Bechvb) \
+  B4X(AechvbEdgeS) B4X(AechvbEdgeD) B4X(BechvbEdge) B4X(ToxRatio) B4X(nigbacc) \
+  B4X(aigbacc) B4X(bigbacc) B4X(cigbacc) B4X(nigbinv) B4X(eigbinv) B4X(aigbinv) \
+  B4X(bigbinv) B4X(cigbinv) B4X(rs0) B4X(rswmin) B4X(rd0) B4X(rdwmin) B4X(xrcrg1) \
+  B4X(xrcrg2) B4X(vfbcv) \
+  B4Y(grgeltd) B4Y(grbdb) B4Y(grbsb) B4Y(grbpb) B4Y(grbps) B4Y(grbpd) B4Y(icVDS) \
+  B4Y(icVGS) B4Y(icVBS)
+enum XField {
+  X_BEFORE_ = ND - 1,
+#define B4X(x) X_##x,
+#define B4Y(x) X_##x,
+  B4_XFIELDS
+#undef B4X
+#undef B4Y
+  NDX
+};
 // A model's entry in the card table: the DevCard, then one value per
 // per-instance column.  Columns whose value is the same for every instance
 // of every card (parameters with no geometry or instance dependence, which
@@ -304,22 +330,46 @@ __host__ __device__ constexpr bool card_col(int cm, int f) {
 }
 static_assert(P_a0 < kNumP && V_vth0 >= kNumP, "size-dependent columns precede instance values");
 
-enum IField { F_MODEL, F_DEXIST, F_SEXIST, NIF };
+enum IField { F_MODEL, F_DEXIST, F_SEXIST, NIF, F_XF = NIF };
+// Option bits of an option group's instance (ifield F_XF): the instance's
+// rgateMod, rbodyMod, trnqsMod and acnqsMod, its model's igcMod, igbMod,
+// capMod, rdsMod and rbodyMod (the reference tests the model's value for the
+// body currents), PIGCD given, and the initial conditions given.
+enum XBit {
+  XB_RGATE = 0,      // 2 bits
+  XB_RBODY = 2,      // 2 bits
+  XB_TRNQS = 4, XB_ACNQS = 5,
+  XB_IGC = 6,        // 2 bits
+  XB_IGB = 8,
+  XB_CAP = 9,        // 2 bits
+  XB_RDS = 11, XB_RBODYM = 12, XB_PIGCD = 13, XB_ICVBS = 14, XB_ICVDS = 15, XB_ICVGS = 16
+};
 // Values the reference's Instance keeps from one evaluation to the next and
 // reads before writing: vgb (as vgb_orig), and the junction charges and
 // capacitances, which are only recomputed when charges are needed (transient,
 // tranop, acop) and are otherwise the last computed ones.
 enum State { S_VGB, S_QBS, S_QBD, S_CAPBS, S_CAPBD, NS };
 static const char* const kStateNames[NS] = {"vgb", "qbs", "qbd", "capbs", "capbd"};
-// Local nodes.  GatePrime, GateMid and GateExt are one node, as are
-// BodyPrime, DrainBody, SourceBody and Body (rgateMod = rbodyMod = 0);
-// DP/SP are the drain/source nodes themselves when no internal node exists.
-enum Node { N_D, N_G, N_S, N_B, N_DP, N_SP, NNODE };
+// Local nodes: the four terminals (N_G is the external gate), the drain and
+// source primes (the terminals themselves when no internal node exists), the
+// gate prime and gate mid nodes (rgateMod > 0, rgateMod = 3; else the
+// external gate), the body prime, source-body and drain-body nodes (rbodyMod;
+// else the body), the NQS charge node (trnqsMod) and the branch currents of
+// the initial conditions IC3 (VBS), IC1 (VDS), IC2 (VGS) (absent: -1).  The
+// default kernel reads the first six only.
+enum Node { N_D, N_G, N_S, N_B, N_DP, N_SP, N_GP, N_GM, N_BP, N_SB, N_DB, N_Q, N_IBS, N_IDS,
+            N_IGS, NNODE };
 enum Sto {
   T_vbd, T_vbs, T_vgs, T_vds, T_vges, T_vgms, T_vdes, T_vses, T_vdbs, T_vsbs,
   T_vdbd, T_vged, T_vgmd, T_von, NSTO
 };
-enum Sta { A_qb, A_qg, A_qd, NSTA };
+// State slots: qb, qg, qd, then as the options register them qgmid
+// (rgateMod 3), qbs and qbd (rbodyMod), qcheq and qcdump (trnqsMod), so an
+// option instance's slot of each follows from its flags.
+enum Sta { A_qb, A_qg, A_qd, NSTA_BASE, NSTA = 8 };
+// Entries per instance of the option groups' stamp lists (F, Q, F limiter,
+// Q limiter, dF/dx, dQ/dx)
+constexpr int kXF = 27, kXQ = 8, kXFLim = 13, kXQLim = 8, kXDF = 99, kXDQ = 33;
 
 // ---- host setup ------------------------------------------------------------
 static DevCard make_card(const Model& m) {
@@ -403,10 +453,28 @@ static int setup(Group& g) {
     const int rc = model_init(models[k], g.models[k], g.opt);
     if (rc != XA_OK) return rc;
   }
-  g.n_dfield = ND;
-  g.n_ifield = NIF;
-  g.dfield.assign((size_t)ND * n, 0.0);
-  g.ifield.assign((size_t)NIF * n, 0);
+  // An option group (any card or instance with an option outside the
+  // default kernel's branches) takes the option kernel and its longer stamp
+  // lists, the extra columns and the option bits.
+  bool xopt = false;
+  for (const Model& m : models)
+    xopt = xopt || m.p.capMod != 2 || m.p.igcMod != 0 || m.p.igbMod != 0 || m.p.rdsMod != 0 ||
+           m.p.rbodyMod != 0 || m.p.rgateMod != 0 || m.p.trnqsMod != 0 || m.p.acnqsMod != 0;
+  for (int s : {IS_rgateMod, IS_rbodyMod, IS_trnqsMod, IS_acnqsMod})
+    if (g.igiven[s])
+      for (int64_t i = 0; i < n && !xopt; ++i) xopt = g.ival[s][i] != 0.0;
+  for (int s : {IS_icVDS, IS_icVGS, IS_icVBS}) xopt = xopt || g.igiven[s];
+  const int ncol = xopt ? NDX : ND;
+  g.variant = xopt ? 1 : 0;
+  g.split_only = xopt;
+  if (xopt) {
+    const int E[NK] = {kXF, kXQ, kXFLim, kXQLim, kXDF, kXDQ, 0, 0};
+    for (int k = 0; k < NK; ++k) g.ent[k] = E[k];
+  }
+  g.n_dfield = ncol;
+  g.n_ifield = xopt ? NIF + 1 : NIF;
+  g.dfield.assign((size_t)ncol * n, 0.0);
+  g.ifield.assign((size_t)g.n_ifield * n, 0);
   g.n_state = NS;
   g.state_init.assign((size_t)NS * n, 0.0);
   auto D = [&](int f, int64_t i) -> double& { return g.dfield[f * n + i]; };
@@ -431,6 +499,21 @@ static int setup(Group& g) {
     I(F_MODEL, i) = g.inst_model[i];
     I(F_DEXIST, i) = in.drainMOSFET_B4Exists;
     I(F_SEXIST, i) = in.sourceMOSFET_B4Exists;
+    if (xopt) {
+#define B4X(x) D(X_##x, i) = p.x;
+#define B4Y(x) D(X_##x, i) = in.x;
+      B4_XFIELDS
+#undef B4X
+#undef B4Y
+      const Model& m = *in.model;
+      I(F_XF, i) = (in.irgateMod << XB_RGATE) | (in.irbodyMod << XB_RBODY) |
+                   ((in.itrnqsMod ? 1 : 0) << XB_TRNQS) | ((in.iacnqsMod ? 1 : 0) << XB_ACNQS) |
+                   ((int)m.p.igcMod << XB_IGC) | ((int)m.p.igbMod << XB_IGB) |
+                   ((int)m.p.capMod << XB_CAP) | ((m.p.rdsMod != 0 ? 1 : 0) << XB_RDS) |
+                   ((m.p.rbodyMod != 0 ? 1 : 0) << XB_RBODYM) | ((m.g.pigcd ? 1 : 0) << XB_PIGCD) |
+                   ((in.icVBS_given ? 1 : 0) << XB_ICVBS) | ((in.icVDS_given ? 1 : 0) << XB_ICVDS) |
+                   ((in.icVGS_given ? 1 : 0) << XB_ICVGS);
+    }
   }
   // cards after every instance: the model's temperature values are those
   // the last instance's updateTemperature wrote
@@ -458,8 +541,17 @@ static int setup(Group& g) {
   return XA_OK;
 }
 
+// option bits of instance i of an option group (0 in a default group)
+static int xflags(const Group& g, int64_t i) {
+  return g.variant ? g.ifield[(size_t)F_XF * g.n + i] : 0;
+}
+
+// registerLIDs (N_DEV_MOSFET_B4.C:4996-5128): external nodes, then the
+// internal ones in the reference's order.
 static void resolve_nodes(const Group& g, int64_t i, const int32_t* reg, int32_t* full) {
   const int64_t n = g.n;
+  const int xf = xflags(g, i);
+  const int rgate = (xf >> XB_RGATE) & 3, rbody = (xf >> XB_RBODY) & 3;
   full[N_D] = reg[0];
   full[N_G] = reg[1];
   full[N_S] = reg[2];
@@ -467,6 +559,19 @@ static void resolve_nodes(const Group& g, int64_t i, const int32_t* reg, int32_t
   int loc = 4;
   full[N_DP] = g.ifield[F_DEXIST * n + i] ? reg[loc++] : reg[0];
   full[N_SP] = g.ifield[F_SEXIST * n + i] ? reg[loc++] : reg[2];
+  full[N_GP] = rgate > 0 ? reg[loc++] : reg[1];
+  full[N_GM] = rgate == 3 ? reg[loc++] : reg[1];
+  if (rbody) {
+    full[N_BP] = reg[loc++];
+    full[N_SB] = reg[loc++];
+    full[N_DB] = reg[loc++];
+  } else {
+    full[N_BP] = full[N_SB] = full[N_DB] = reg[3];
+  }
+  full[N_Q] = ((xf >> XB_TRNQS) & 1) ? reg[loc++] : -1;
+  full[N_IBS] = ((xf >> XB_ICVBS) & 1) ? reg[loc++] : -1;
+  full[N_IDS] = ((xf >> XB_ICVDS) & 1) ? reg[loc++] : -1;
+  full[N_IGS] = ((xf >> XB_ICVGS) & 1) ? reg[loc++] : -1;
 }
 
 // Stamp tables in the order of the reference's loads.  A row of -1 marks an
@@ -478,9 +583,123 @@ static const Stamp kStLim[] = {{N_DP, 0}, {N_G, 0}, {N_B, 0}, {N_SP, 0}};
 static const Stamp kStDF[] = {{N_G, N_G}, {N_G, N_DP}, {N_G, N_SP}, {N_G, N_B}, {N_DP, N_DP}, {N_DP, N_D}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_D, N_DP}, {N_D, N_D}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_S}, {N_SP, N_B}, {N_S, N_SP}, {N_S, N_S}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}, {N_DP, N_DP}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}};
 static const Stamp kStDQ[] = {{N_G, N_G}, {N_G, N_DP}, {N_G, N_SP}, {N_G, N_B}, {N_DP, N_DP}, {N_DP, N_G}, {N_DP, N_SP}, {N_DP, N_B}, {N_SP, N_DP}, {N_SP, N_G}, {N_SP, N_SP}, {N_SP, N_B}, {N_B, N_DP}, {N_B, N_G}, {N_B, N_SP}, {N_B, N_B}};
 
+// Option groups: one stamp list per kind holding every entry of every option
+// branch of the reference's loads in its order; an instance disables the
+// entries of branches its options do not take (row N_NONE), which keeps the
+// relative order of the entries it loads.
+constexpr int16_t GE = N_G, GP = N_GP, GM = N_GM, DP = N_DP, SP = N_SP, BP = N_BP, DB = N_DB,
+                  SB = N_SB, QN = N_Q, ND_ = N_D, NS_ = N_S, NB_ = N_B;
+static const Stamp kXStF[] = {
+    {DP, 0}, {GP, 0},                                             // 0-1
+    {GE, 0}, {GM, 0}, {GP, 0},                                    // 2-4 gate resistance
+    {DB, 0}, {BP, 0}, {NB_, 0}, {SB, 0}, {SP, 0},                 // 5-9 body
+    {ND_, 0}, {NS_, 0}, {DP, 0}, {SP, 0},                         // 10-13 rdsMod
+    {ND_, 0}, {DP, 0}, {NS_, 0}, {SP, 0},                         // 14-17 contact resistors
+    {NB_, 0}, {NS_, 0}, {N_IBS, 0}, {ND_, 0}, {NS_, 0}, {N_IDS, 0},
+    {GE, 0}, {NS_, 0}, {N_IGS, 0}};                               // 18-26 initial conditions
+static const Stamp kXStFLim[] = {{DP, 0}, {GP, 0}, {GE, 0}, {GM, 0}, {GP, 0}, {DB, 0}, {BP, 0},
+                                 {SB, 0}, {SP, 0}, {ND_, 0}, {NS_, 0}, {DP, 0}, {SP, 0}};
+static const Stamp kXStQ[] = {{DP, 0}, {GP, 0}, {GM, 0}, {DB, 0}, {BP, 0}, {SB, 0}, {SP, 0}, {QN, 0}};
+static const Stamp kXStDF[] = {
+    {GE, GE}, {GE, GP}, {GE, GM}, {GE, DP}, {GE, SP}, {GE, BP},            // 0-5
+    {GM, GE}, {GM, GM}, {GM, DP}, {GM, GP}, {GM, SP}, {GM, BP},            // 6-11
+    {GP, GE}, {GP, GM}, {GP, GP}, {GP, DP}, {GP, SP}, {GP, BP},            // 12-17
+    {ND_, GP}, {ND_, SP}, {ND_, BP}, {NS_, DP}, {NS_, GP}, {NS_, BP},      // 18-23 rdsMod
+    {DP, DP}, {DP, ND_}, {DP, GP}, {DP, SP}, {DP, BP}, {ND_, DP}, {ND_, ND_},
+    {SP, DP}, {SP, GP}, {SP, SP}, {SP, NS_}, {SP, BP}, {NS_, SP}, {NS_, NS_},
+    {BP, DP}, {BP, GP}, {BP, SP}, {BP, BP},                                // 24-41
+    {DP, DP}, {DP, GP}, {DP, SP}, {DP, BP}, {BP, DP}, {BP, GP}, {BP, SP}, {BP, BP},  // 42-49 GIDL
+    {SP, DP}, {SP, GP}, {SP, SP}, {SP, BP}, {BP, DP}, {BP, GP}, {BP, SP}, {BP, BP},  // 50-57 GISL
+    {DP, DB}, {SP, SB}, {DB, DP}, {DB, DB}, {DB, BP}, {DB, NB_}, {BP, DB}, {BP, NB_},
+    {BP, SB}, {BP, BP}, {SB, SP}, {SB, BP}, {SB, NB_}, {SB, SB}, {NB_, DB}, {NB_, BP},
+    {NB_, SB}, {NB_, NB_},                                                 // 58-75 body
+    {QN, QN}, {QN, GP}, {QN, DP}, {QN, SP}, {QN, BP}, {DP, QN}, {SP, QN}, {GP, QN},  // 76-83 NQS
+    {NB_, N_IBS}, {NS_, N_IBS}, {N_IBS, NB_}, {N_IBS, NS_}, {N_IBS, N_IBS},
+    {ND_, N_IDS}, {NS_, N_IDS}, {N_IDS, ND_}, {N_IDS, NS_}, {N_IDS, N_IDS},
+    {GE, N_IGS}, {NS_, N_IGS}, {N_IGS, GE}, {N_IGS, NS_}, {N_IGS, N_IGS}};   // 84-98 IC
+static const Stamp kXStDQ[] = {
+    {GM, GM}, {GM, DP}, {GM, SP}, {GM, BP}, {DP, GM}, {SP, GM}, {BP, GM},  // 0-6 rgateMod 3
+    {GP, GP}, {GP, DP}, {GP, SP}, {GP, BP}, {DP, DP}, {DP, GP}, {DP, SP}, {DP, BP},
+    {SP, DP}, {SP, GP}, {SP, SP}, {SP, BP}, {BP, DP}, {BP, GP}, {BP, SP}, {BP, BP},  // 7-22
+    {DP, DB}, {SP, SB}, {DB, DP}, {DB, DB}, {SB, SP}, {SB, SB},            // 23-28 body
+    {QN, GP}, {QN, DP}, {QN, SP}, {QN, BP}};                               // 29-32 NQS
+static_assert(sizeof(kXStF) / sizeof(Stamp) == kXF && sizeof(kXStFLim) / sizeof(Stamp) == kXFLim &&
+              sizeof(kXStQ) / sizeof(Stamp) == kXQ && sizeof(kXStDF) / sizeof(Stamp) == kXDF &&
+              sizeof(kXStDQ) / sizeof(Stamp) == kXDQ, "option stamp tables");
+
+// the entries of the option stamp lists an instance with option bits xf loads
+static bool xstamp_on(int kind, int e, int xf) {
+  const int rgate = (xf >> XB_RGATE) & 3;
+  const bool rbody = ((xf >> XB_RBODY) & 3) != 0, nqs = (xf >> XB_TRNQS) & 1;
+  const bool rds = (xf >> XB_RDS) & 1;
+  switch (kind) {
+    case K_F:
+      if (e == 2 || e == 4) return rgate >= 1;
+      if (e == 3) return rgate == 3;
+      if (e == 5 || e == 7 || e == 8) return rbody;
+      if (e >= 10 && e <= 13) return rds;
+      if (e >= 18 && e <= 20) return (xf >> XB_ICVBS) & 1;
+      if (e >= 21 && e <= 23) return (xf >> XB_ICVDS) & 1;
+      if (e >= 24) return (xf >> XB_ICVGS) & 1;
+      return true;   // 14-17 by the contact resistors' nodes, below
+    case K_FLIM:
+      if (e == 2 || e == 4) return rgate >= 1;
+      if (e == 3) return rgate == 3;
+      if (e == 5 || e == 7) return rbody;
+      if (e >= 9) return rds;
+      return true;
+    case K_Q: case K_QLIM:
+      if (e == 2) return rgate == 3;
+      if (e == 3 || e == 5) return rbody;
+      if (e == 7) return nqs;
+      return true;
+    case K_DFDX:
+      if (e == 0) return rgate >= 1;
+      if (e == 1) return rgate == 1 || rgate == 2;
+      if (e == 2) return rgate == 3;
+      if (e >= 3 && e <= 5) return rgate == 2;
+      if (e >= 6 && e <= 11) return rgate == 3;
+      if (e == 12) return rgate == 1 || rgate == 2;
+      if (e == 13) return rgate == 3;
+      if (e >= 18 && e <= 23) return rds;
+      if (e >= 58 && e <= 75) return rbody;
+      if (e >= 76 && e <= 83) return nqs;
+      if (e >= 84 && e <= 88) return (xf >> XB_ICVBS) & 1;
+      if (e >= 89 && e <= 93) return (xf >> XB_ICVDS) & 1;
+      if (e >= 94) return (xf >> XB_ICVGS) & 1;
+      return true;
+    case K_DQDX:
+      if (e <= 6) return rgate == 3;
+      if (e >= 23 && e <= 28) return rbody;
+      if (e >= 29) return nqs;
+      return true;
+    default: return false;
+  }
+}
+
 static void stamps(const Group& g, int64_t i, int kind, std::vector<Stamp>& out) {
   out.clear();
   auto add = [&](const Stamp* s, int m) { out.assign(s, s + m); };
+  if (g.variant) {
+    const int xf = xflags(g, i);
+    switch (kind) {
+      case K_F: add(kXStF, kXF); break;
+      case K_Q: add(kXStQ, kXQ); break;
+      case K_FLIM: if (g.opt.voltageLimiterFlag) add(kXStFLim, kXFLim); break;
+      case K_QLIM: if (g.opt.voltageLimiterFlag) add(kXStQ, kXQLim); break;
+      case K_DFDX: add(kXStDF, kXDF); break;
+      case K_DQDX: add(kXStDQ, kXDQ); break;
+      default: break;
+    }
+    for (size_t e = 0; e < out.size(); ++e)
+      if (!xstamp_on(kind, (int)e, xf)) out[e].row = N_NONE;
+    if (kind == K_F) {
+      const int64_t n = g.n;
+      if (!g.ifield[F_DEXIST * n + i]) out[14].row = out[15].row = N_NONE;
+      if (!g.ifield[F_SEXIST * n + i]) out[16].row = out[17].row = N_NONE;
+    }
+    return;
+  }
   switch (kind) {
     case K_F: {
       add(kStF, 8);
@@ -553,7 +772,11 @@ __device__ __forceinline__ void reload_params(const double*& dfb) {
 // LDSCARD: the block's model cards are staged in LDS by the kernel (s_card,
 // in the order of the block's card table; card_slot gives the instance's);
 // otherwise they are read from global memory.
-template <int MODE, bool LDSCARD, int CM, bool RELOAD = true>
+// X: the option kernel (option groups): every option branch of the
+// reference is evaluated and the stamps follow the option stamp lists.  The
+// default kernel (X false) compiles none of them.
+#define XP(x) (dfb[(int64_t)(X_##x) * a.n + i])
+template <int MODE, bool LDSCARD, int CM, bool RELOAD = true, bool X = false>
 __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_card) {
   if (i >= a.n) return;
   // the fused kernel stages every card of the group (slot = model index),
@@ -568,12 +791,20 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   const xa_device_options& opt = a.opt;
   const double gmin = opt.gmin;
   const double nf = IV(nf);
-  const int irbodyMod = 0;
+  const int xf = X ? ifld(a, F_XF, i) : 0;
+  const int irgateMod = (xf >> XB_RGATE) & 3, irbodyMod = (xf >> XB_RBODY) & 3;
+  const int itrnqsMod = (xf >> XB_TRNQS) & 1, iacnqsMod = (xf >> XB_ACNQS) & 1;
+  const int igcMod = (xf >> XB_IGC) & 3, igbMod = (xf >> XB_IGB) & 1;
+  const int capMod = X ? (xf >> XB_CAP) & 3 : 2, rdsMod = (xf >> XB_RDS) & 1;
   const int32_t li_Drain = node(a, N_D, i), li_GateExt = node(a, N_G, i);
   const int32_t li_Source = node(a, N_S, i), li_Body = node(a, N_B, i);
   const int32_t li_DrainPrime = node(a, N_DP, i), li_SourcePrime = node(a, N_SP, i);
-  const int32_t li_GatePrime = li_GateExt, li_GateMid = li_GateExt;
-  const int32_t li_BodyPrime = li_Body, li_DrainBody = li_Body, li_SourceBody = li_Body;
+  const int32_t li_GatePrime = X ? node(a, N_GP, i) : li_GateExt;
+  const int32_t li_GateMid = X ? node(a, N_GM, i) : li_GateExt;
+  const int32_t li_BodyPrime = X ? node(a, N_BP, i) : li_Body;
+  const int32_t li_DrainBody = X ? node(a, N_DB, i) : li_Body;
+  const int32_t li_SourceBody = X ? node(a, N_SB, i) : li_Body;
+  const int32_t li_Charge = X ? node(a, N_Q, i) : -1;
   const int32_t li_store_vbd = stolid(a, T_vbd, i), li_store_vbs = stolid(a, T_vbs, i);
   const int32_t li_store_vgs = stolid(a, T_vgs, i), li_store_vds = stolid(a, T_vds, i);
   const int32_t li_store_vges = stolid(a, T_vges, i), li_store_vgms = stolid(a, T_vgms, i);
@@ -602,7 +833,8 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   double qinv, qgate, qdrn, qsrc, qbulk, qgmid, qgdo, qgso, qgb, qgmb, qchqs, qcheq;
   double cggb, cgsb, cgdb, cdgb, cdsb, cddb, cbgb, cbsb, cbdb, csgb, cssb, csdb;
   double cgbb, csbb, cdbb, cbbb, cqdb, cqsb, cqgb, cqbb, CoxWL, taunet;
-  double gtau = 0.0;   // trnqsMod = 0: no NQS time constant
+  double gtau = 0.0;   // no NQS: no NQS time constant
+  double Qqcheq_Jdxp = 0.0;
   double qbs, qbd, capbs, capbd, Igate, IgateMid, Igate_Jdxp, IgateMid_Jdxp;
   double Idrain, Isource, Idbb, Idbbp, Isbb, Isbbp, Ibpb;
   double gcggb, gcgdb, gcgsb, gcgbb, gcdgb, gcsgb, gcbgb, gcddb, gcdsb, gcsdb, gcssb;
@@ -698,7 +930,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   Vgm = (li_GateMid >= 0) ? solv(a.v.nextSol, li_GateMid) : 0.0;
   Vdb = solv(a.v.nextSol, li_DrainBody);
   Vsb = solv(a.v.nextSol, li_SourceBody);
-  Qtotal = 0.0;
+  Qtotal = (X && itrnqsMod) ? solv(a.v.nextSol, li_Charge) : 0.0;
   Vddp = Vd - Vdp;
   Vssp = Vs - Vsp;
   Vdbb = Vdb - Vb;
@@ -840,7 +1072,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   (void)vsbs_old; (void)vdbd_old; (void)vbd_old;
 
   if (RELOAD) reload_params(dfb);
-  /* voltage limiting (:8165-8359), rgateMod = rbodyMod = rdsMod = 0 */
+  /* voltage limiting (:8165-8359) */
   if (opt.voltageLimiterFlag && !(ss.initFixFlag && IV(OFF))) {
     if (ss.newtonIter >= 0 && !(ss.initJctFlag)) {
       if (vds_old >= 0.0) {
@@ -848,18 +1080,64 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
         vds = vgs - vgd;
         vds = limvds(vds, vds_old);
         vgd = vgs - vds;
+        if (X) {
+          if (irgateMod == 3) {
+            vges = fetlim(vges, vges_old, von_local);
+            vgms = fetlim(vgms, vgms_old, von_local);
+          } else if ((irgateMod == 1) || (irgateMod == 2)) {
+            vges = fetlim(vges, vges_old, von_local);
+            vged = vges - vds;
+          }
+          if (rdsMod) {
+            vdes = limvds(vdes, vdes_old);
+            vses = -limvds(-vses, -vses_old);
+          }
+        }
       } else {
         vgd = fetlim(vgd, vgd_old, von_local);
         vds = vgs - vgd;
         vds = -limvds(-vds, -vds_old);
         vgs = vgd + vds;
+        if (X) {
+          if (irgateMod == 3) {
+            vged = fetlim(vged, vged_old, von_local);
+            vges = vged + vds;
+            vgmd = fetlim(vgmd, vgmd_old, von_local);
+            vgms = vgmd + vds;
+          }
+          if ((irgateMod == 1) || (irgateMod == 2)) {
+            vged = fetlim(vged, vged_old, von_local);
+            vges = vged + vds;
+          }
+          if (rdsMod) {
+            vdes = -limvds(-vdes, -vdes_old);
+            vses = limvds(vses, vses_old);
+          }
+        }
       }
       if (vds >= 0.0) {
         vbs = pnjlim(vbs, vbs_old, kVt0, C.vcrit, &Check);
         vbd = vbs - vds;
+        if (X && irbodyMod) {
+          int Check1, Check2;
+          vdbs = pnjlim(vdbs, vdbs_old, kVt0, C.vcrit, &Check1);
+          vdbd = vdbs - vds;
+          vsbs = pnjlim(vsbs, vsbs_old, kVt0, C.vcrit, &Check2);
+          if ((Check1 != 0) || (Check2 != 0)) Check = 1;
+        }
       } else {
         vbd = pnjlim(vbd, vbd_old, kVt0, C.vcrit, &Check);
         vbs = vbd + vds;
+        if (X && irbodyMod) {
+          int Check1, Check2;
+          vdbd = pnjlim(vdbd, vdbd_old, kVt0, C.vcrit, &Check1);
+          vdbs = vdbd + vds;
+          const double vsbd_old = vsbs_old - vds_old;
+          double vsbd = vsbs - vds;
+          vsbd = pnjlim(vsbd, vsbd_old, kVt0, C.vcrit, &Check2);
+          vsbs = vsbd + vds;
+          if ((Check1 != 0) || (Check2 != 0)) Check = 1;
+        }
       }
     }
     if (Check == 1) limitedFlag = 1;
@@ -1986,14 +2264,106 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   IdovVds = Ids;
   if (IdovVds <= 1.0e-9) IdovVds = 1.0e-9;
   if (RELOAD) reload_params(dfb);
-  /* rgateMod <= 1 and no NQS: no gate resistance current conductances */
-  gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+  /* gate resistance conductances (:9953-9988); without rgateMod > 1 or NQS
+     they stay at the zeros setupFVectorVars left */
+  if (X && ((irgateMod > 1) || (itrnqsMod != 0) || (iacnqsMod != 0))) {
+    T9 = XP(xrcrg2) * C.vtm;
+    T0 = T9 * beta;
+    dT0_dVd = (dbeta_dVd + dbeta_dVg * dVgsteff_dVd) * T9;
+    dT0_dVb = (dbeta_dVb + dbeta_dVg * dVgsteff_dVb) * T9;
+    dT0_dVg = dbeta_dVg * T9;
+    gcrg = XP(xrcrg1) * (T0 + Ids);
+    gcrgd = XP(xrcrg1) * (dT0_dVd + tmp1);
+    gcrgb = XP(xrcrg1) * (dT0_dVb + tmp2) * dVbseff_dVb;
+    gcrgg = XP(xrcrg1) * (dT0_dVg + tmp3) * dVgsteff_dVg;
+    if (nf != 1.0) {
+      gcrg *= nf;
+      gcrgg *= nf;
+      gcrgd *= nf;
+      gcrgb *= nf;
+    }
+    if (irgateMod == 2) {
+      T10 = XP(grgeltd) * XP(grgeltd);
+      T11 = XP(grgeltd) + gcrg;
+      gcrg = XP(grgeltd) * gcrg / T11;
+      T12 = T10 / T11 / T11;
+      gcrgg *= T12;
+      gcrgd *= T12;
+      gcrgb *= T12;
+    }
+    gcrgs = -(gcrgg + gcrgd + gcrgb);
+  } else {
+    gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+  }
   if (RELOAD) reload_params(dfb);
-  /* rdsMod = 0 (:10063-10069) */
-  gstot = gstotd = gstotg = 0.0;
-  gstots = gstotb = 0.0;
-  gdtot = gdtotd = gdtotg = 0.0;
-  gdtots = gdtotb = 0.0;
+  /* bias-dependent external source/drain resistance (:9991-10069) */
+  if (X && rdsMod) {
+    double dT0_dvg, dT1_dvb, dT3_dvg, dT3_dvb, Rs, dRs_dvg, dRs_dvb, Rd, dRd_dvg, dRd_dvb;
+    double dgstot_dvd, dgstot_dvg, dgstot_dvb, dgstot_dvs, dgdtot_dvd, dgdtot_dvg, dgdtot_dvb, dgdtot_dvs;
+    T0 = vgs - XP(vfbsd);
+    T1 = sqrt(T0 * T0 + 1.0e-4);
+    vgs_eff = 0.5 * (T0 + T1);
+    dvgs_eff_dvg = vgs_eff / T1;
+    T0 = 1.0 + PP(prwg) * vgs_eff;
+    dT0_dvg = -PP(prwg) / T0 / T0 * dvgs_eff_dvg;
+    T1 = -PP(prwb) * vbs;
+    dT1_dvb = -PP(prwb);
+    T2 = 1.0 / T0 + T1;
+    T3 = T2 + sqrt(T2 * T2 + 0.01);
+    dT3_dvg = T3 / (T3 - T2);
+    dT3_dvb = dT3_dvg * dT1_dvb;
+    dT3_dvg *= dT0_dvg;
+    T4 = XP(rs0) * 0.5;
+    Rs = XP(rswmin) + T3 * T4;
+    dRs_dvg = T4 * dT3_dvg;
+    dRs_dvb = T4 * dT3_dvb;
+    T0 = 1.0 + IV(sourceConductance) * Rs;
+    gstot = IV(sourceConductance) / T0;
+    T0 = -gstot * gstot;
+    dgstot_dvd = 0.0;
+    dgstot_dvg = T0 * dRs_dvg;
+    dgstot_dvb = T0 * dRs_dvb;
+    dgstot_dvs = -(dgstot_dvg + dgstot_dvb + dgstot_dvd);
+    T0 = vgd - XP(vfbsd);
+    T1 = sqrt(T0 * T0 + 1.0e-4);
+    vgd_eff = 0.5 * (T0 + T1);
+    dvgd_eff_dvg = vgd_eff / T1;
+    T0 = 1.0 + PP(prwg) * vgd_eff;
+    dT0_dvg = -PP(prwg) / T0 / T0 * dvgd_eff_dvg;
+    T1 = -PP(prwb) * vbd;
+    dT1_dvb = -PP(prwb);
+    T2 = 1.0 / T0 + T1;
+    T3 = T2 + sqrt(T2 * T2 + 0.01);
+    dT3_dvg = T3 / (T3 - T2);
+    dT3_dvb = dT3_dvg * dT1_dvb;
+    dT3_dvg *= dT0_dvg;
+    T4 = XP(rd0) * 0.5;
+    Rd = XP(rdwmin) + T3 * T4;
+    dRd_dvg = T4 * dT3_dvg;
+    dRd_dvb = T4 * dT3_dvb;
+    T0 = 1.0 + IV(drainConductance) * Rd;
+    gdtot = IV(drainConductance) / T0;
+    T0 = -gdtot * gdtot;
+    dgdtot_dvs = 0.0;
+    dgdtot_dvg = T0 * dRd_dvg;
+    dgdtot_dvb = T0 * dRd_dvb;
+    dgdtot_dvd = -(dgdtot_dvg + dgdtot_dvb + dgdtot_dvs);
+    gstotd = vses * dgstot_dvd;
+    gstotg = vses * dgstot_dvg;
+    gstots = vses * dgstot_dvs;
+    gstotb = vses * dgstot_dvb;
+    T2 = vdes - vds;
+    gdtotd = T2 * dgdtot_dvd;
+    gdtotg = T2 * dgdtot_dvg;
+    gdtots = T2 * dgdtot_dvs;
+    gdtotb = T2 * dgdtot_dvb;
+  } else {
+    gstot = gstotd = gstotg = 0.0;
+    gstots = gstotb = 0.0;
+    gdtot = gdtotd = gdtotg = 0.0;
+    gdtots = gdtotb = 0.0;
+  }
+
 
   if (RELOAD) reload_params(dfb);
   /* GIDL (:10074-10126) */
@@ -2063,12 +2433,334 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   ggislg = Ggislg;
   ggislb = Ggislb;
   if (RELOAD) reload_params(dfb);
-  /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
-  Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
-  Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
-  Igs = gIgsg = gIgss = 0.0;
-  Igd = gIgdg = gIgdd = 0.0;
-  Igb = gIgbg = gIgbd = gIgbs = gIgbb = 0.0;
+  if constexpr (X) {
+    double Vfb, Voxacc, dVoxacc_dVg, dVoxacc_dVb, Voxdepinv, dVoxdepinv_dVg, dVoxdepinv_dVd;
+    double dVoxdepinv_dVb, VxNVt, ExpVxNVt, Vaux, dVaux_dVg, dVaux_dVd, dVaux_dVb;
+    double dT7_dVg, dT7_dVd, dT7_dVb, Igc, dIgc_dVg, dIgc_dVd, dIgc_dVb, Pigcd, dPigcd_dVg;
+    double dPigcd_dVd, dPigcd_dVb, dIgcs_dVg, dIgcs_dVd, dIgcs_dVb, dIgcd_dVg, dIgcd_dVd, dIgcd_dVb;
+    double dIgs_dVg, dIgs_dVs, dIgd_dVg, dIgd_dVd, Igbacc, dIgbacc_dVg, dIgbacc_dVb;
+    double Igbinv, dIgbinv_dVg, dIgbinv_dVd, dIgbinv_dVb;
+    /* gate tunnelling current (:10182-10621) */
+    Vfb = 0.0;
+    Voxacc = dVoxacc_dVg = dVoxacc_dVb = 0.0;
+    Voxdepinv = dVoxdepinv_dVg = dVoxdepinv_dVd = dVoxdepinv_dVb = 0.0;
+    if ((igcMod != 0) || (igbMod != 0)) {
+      Vfb = IV(vfbzb);
+      V3 = Vfb - Vgs_eff + Vbseff - kDelta3;
+      if (Vfb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * Vfb);
+      else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * Vfb);
+      T1 = 0.5 * (1.0 + V3 / T0);
+      Vfbeff = Vfb - 0.5 * (V3 + T0);
+      dVfbeff_dVg = T1 * dVgs_eff_dVg;
+      dVfbeff_dVb = -T1;
+      Voxacc = Vfb - Vfbeff;
+      dVoxacc_dVg = -dVfbeff_dVg;
+      dVoxacc_dVb = -dVfbeff_dVb;
+      if (Voxacc < 0.0) Voxacc = dVoxacc_dVg = dVoxacc_dVb = 0.0;
+      T0 = 0.5 * PP(k1ox);
+      T3 = Vgs_eff - Vfbeff - Vbseff - Vgsteff;
+      if (PP(k1ox) == 0.0) {
+        Voxdepinv = dVoxdepinv_dVg = dVoxdepinv_dVd = dVoxdepinv_dVb = 0.0;
+      } else if (T3 < 0.0) {
+        Voxdepinv = -T3;
+        dVoxdepinv_dVg = -dVgs_eff_dVg + dVfbeff_dVg + dVgsteff_dVg;
+        dVoxdepinv_dVd = dVgsteff_dVd;
+        dVoxdepinv_dVb = dVfbeff_dVb + 1.0 + dVgsteff_dVb;
+      } else {
+        T1 = sqrt(T0 * T0 + T3);
+        T2 = T0 / T1;
+        Voxdepinv = PP(k1ox) * (T1 - T0);
+        dVoxdepinv_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+        dVoxdepinv_dVd = -T2 * dVgsteff_dVd;
+        dVoxdepinv_dVb = -T2 * (dVfbeff_dVb + 1.0 + dVgsteff_dVb);
+      }
+      Voxdepinv += Vgsteff;
+      dVoxdepinv_dVg += dVgsteff_dVg;
+      dVoxdepinv_dVd += dVgsteff_dVd;
+      dVoxdepinv_dVb += dVgsteff_dVb;
+    }
+    if (C.tempMod < 2) tmp = Vtm;
+    else tmp = Vtm0;
+    if (igcMod) {
+      T0 = tmp * XP(nigc);
+      VxNVt = 0.0;
+      Vaux = dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+      if (igcMod == 1) {
+        VxNVt = (Vgs_eff - C.dtype * IV(vth0)) / T0;
+        if (VxNVt > kExpThreshold) {
+          Vaux = Vgs_eff - C.dtype * IV(vth0);
+          dVaux_dVg = dVgs_eff_dVg;
+          dVaux_dVd = 0.0;
+          dVaux_dVb = 0.0;
+        }
+      } else if (igcMod == 2) {
+        VxNVt = (Vgs_eff - von) / T0;
+        if (VxNVt > kExpThreshold) {
+          Vaux = Vgs_eff - von;
+          dVaux_dVg = dVgs_eff_dVg;
+          dVaux_dVd = -dVth_dVd;
+          dVaux_dVb = -dVth_dVb;
+        }
+      }
+      if (VxNVt < -kExpThreshold) {
+        Vaux = T0 * log(1.0 + kMinExp);
+        dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+      } else if ((VxNVt >= -kExpThreshold) && (VxNVt <= kExpThreshold)) {
+        ExpVxNVt = exp(VxNVt);
+        Vaux = T0 * log(1.0 + ExpVxNVt);
+        dVaux_dVg = ExpVxNVt / (1.0 + ExpVxNVt);
+        if (igcMod == 1) {
+          dVaux_dVd = 0.0;
+          dVaux_dVb = 0.0;
+        } else if (igcMod == 2) {
+          dVaux_dVd = -dVgs_eff_dVg * dVth_dVd;
+          dVaux_dVb = -dVgs_eff_dVg * dVth_dVb;
+        }
+        dVaux_dVg *= dVgs_eff_dVg;
+      }
+      T2 = Vgs_eff * Vaux;
+      dT2_dVg = dVgs_eff_dVg * Vaux + Vgs_eff * dVaux_dVg;
+      dT2_dVd = Vgs_eff * dVaux_dVd;
+      dT2_dVb = Vgs_eff * dVaux_dVb;
+      T11 = XP(Aechvb);
+      T12 = XP(Bechvb);
+      T3 = XP(aigc) * XP(cigc) - XP(bigc);
+      T4 = XP(bigc) * XP(cigc);
+      T5 = T12 * (XP(aigc) + T3 * Voxdepinv - T4 * Voxdepinv * Voxdepinv);
+      if (T5 > kExpThreshold) {
+        T6 = kMaxExp;
+        dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+      } else if (T5 < -kExpThreshold) {
+        T6 = kMinExp;
+        dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+      } else {
+        T6 = exp(T5);
+        dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxdepinv);
+        dT6_dVd = dT6_dVg * dVoxdepinv_dVd;
+        dT6_dVb = dT6_dVg * dVoxdepinv_dVb;
+        dT6_dVg *= dVoxdepinv_dVg;
+      }
+      Igc = T11 * T2 * T6;
+      dIgc_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+      dIgc_dVd = T11 * (T2 * dT6_dVd + T6 * dT2_dVd);
+      dIgc_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+      if (((xf >> XB_PIGCD) & 1)) {
+        Pigcd = XP(pigcd);
+        dPigcd_dVg = dPigcd_dVd = dPigcd_dVb = 0.0;
+      } else {
+        T11 = XP(Bechvb) * toxe;
+        T12 = Vgsteff + 1.0e-20;
+        T13 = T11 / T12 / T12;
+        T14 = -T13 / T12;
+        Pigcd = T13 * (1.0 - 0.5 * Vdseff_ / T12);
+        dPigcd_dVg = T14 * (2.0 + 0.5 * (dVdseff_dVg - 3.0 * Vdseff_ / T12));
+        dPigcd_dVd = 0.5 * T14 * dVdseff_dVd;
+        dPigcd_dVb = 0.5 * T14 * dVdseff_dVb;
+      }
+      T7 = -Pigcd * Vdseff_;
+      dT7_dVg = -Vdseff_ * dPigcd_dVg - Pigcd * dVdseff_dVg;
+      dT7_dVd = -Vdseff_ * dPigcd_dVd - Pigcd * dVdseff_dVd + dT7_dVg * dVgsteff_dVd;
+      dT7_dVb = -Vdseff_ * dPigcd_dVb - Pigcd * dVdseff_dVb + dT7_dVg * dVgsteff_dVb;
+      dT7_dVg *= dVgsteff_dVg;
+      dT7_dVb *= dVbseff_dVb;
+      T8 = T7 * T7 + 2.0e-4;
+      dT8_dVg = 2.0 * T7;
+      dT8_dVd = dT8_dVg * dT7_dVd;
+      dT8_dVb = dT8_dVg * dT7_dVb;
+      dT8_dVg *= dT7_dVg;
+      if (T7 > kExpThreshold) {
+        T9 = kMaxExp;
+        dT9_dVg = dT9_dVd = dT9_dVb = 0.0;
+      } else if (T7 < -kExpThreshold) {
+        T9 = kMinExp;
+        dT9_dVg = dT9_dVd = dT9_dVb = 0.0;
+      } else {
+        T9 = exp(T7);
+        dT9_dVg = T9 * dT7_dVg;
+        dT9_dVd = T9 * dT7_dVd;
+        dT9_dVb = T9 * dT7_dVb;
+      }
+      T0 = T8 * T8;
+      T1 = T9 - 1.0 + 1.0e-4;
+      T10 = (T1 - T7) / T8;
+      dT10_dVg = (dT9_dVg - dT7_dVg - T10 * dT8_dVg) / T8;
+      dT10_dVd = (dT9_dVd - dT7_dVd - T10 * dT8_dVd) / T8;
+      dT10_dVb = (dT9_dVb - dT7_dVb - T10 * dT8_dVb) / T8;
+      Igcs = Igc * T10;
+      dIgcs_dVg = dIgc_dVg * T10 + Igc * dT10_dVg;
+      dIgcs_dVd = dIgc_dVd * T10 + Igc * dT10_dVd;
+      dIgcs_dVb = dIgc_dVb * T10 + Igc * dT10_dVb;
+      T1 = T9 - 1.0 - 1.0e-4;
+      T10 = (T7 * T9 - T1) / T8;
+      dT10_dVg = (dT7_dVg * T9 + (T7 - 1.0) * dT9_dVg - T10 * dT8_dVg) / T8;
+      dT10_dVd = (dT7_dVd * T9 + (T7 - 1.0) * dT9_dVd - T10 * dT8_dVd) / T8;
+      dT10_dVb = (dT7_dVb * T9 + (T7 - 1.0) * dT9_dVb - T10 * dT8_dVb) / T8;
+      Igcd = Igc * T10;
+      dIgcd_dVg = dIgc_dVg * T10 + Igc * dT10_dVg;
+      dIgcd_dVd = dIgc_dVd * T10 + Igc * dT10_dVd;
+      dIgcd_dVb = dIgc_dVb * T10 + Igc * dT10_dVb;
+      gIgcsg = dIgcs_dVg;
+      gIgcsd = dIgcs_dVd;
+      gIgcsb = dIgcs_dVb * dVbseff_dVb;
+      gIgcdg = dIgcd_dVg;
+      gIgcdd = dIgcd_dVd;
+      gIgcdb = dIgcd_dVb * dVbseff_dVb;
+      /* source/drain edge tunnelling: rewrites the vgs_eff / vgd_eff members */
+      T0 = vgs - (XP(vfbsd) + XP(vfbsdoff));
+      vgs_eff = sqrt(T0 * T0 + 1.0e-4);
+      dvgs_eff_dvg = T0 / vgs_eff;
+      T2 = vgs * vgs_eff;
+      dT2_dVg = vgs * dvgs_eff_dvg + vgs_eff;
+      T11 = XP(AechvbEdgeS);
+      T12 = XP(BechvbEdge);
+      T3 = XP(aigs) * XP(cigs) - XP(bigs);
+      T4 = XP(bigs) * XP(cigs);
+      T5 = T12 * (XP(aigs) + T3 * vgs_eff - T4 * vgs_eff * vgs_eff);
+      if (T5 > kExpThreshold) {
+        T6 = kMaxExp;
+        dT6_dVg = 0.0;
+      } else if (T5 < -kExpThreshold) {
+        T6 = kMinExp;
+        dT6_dVg = 0.0;
+      } else {
+        T6 = exp(T5);
+        dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * vgs_eff) * dvgs_eff_dvg;
+      }
+      Igs = T11 * T2 * T6;
+      dIgs_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+      dIgs_dVs = -dIgs_dVg;
+      T0 = vgd - (XP(vfbsd) + XP(vfbsdoff));
+      vgd_eff = sqrt(T0 * T0 + 1.0e-4);
+      dvgd_eff_dvg = T0 / vgd_eff;
+      T2 = vgd * vgd_eff;
+      dT2_dVg = vgd * dvgd_eff_dvg + vgd_eff;
+      T11 = XP(AechvbEdgeD);
+      T3 = XP(aigd) * XP(cigd) - XP(bigd);
+      T4 = XP(bigd) * XP(cigd);
+      T5 = T12 * (XP(aigd) + T3 * vgd_eff - T4 * vgd_eff * vgd_eff);
+      if (T5 > kExpThreshold) {
+        T6 = kMaxExp;
+        dT6_dVg = 0.0;
+      } else if (T5 < -kExpThreshold) {
+        T6 = kMinExp;
+        dT6_dVg = 0.0;
+      } else {
+        T6 = exp(T5);
+        dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * vgd_eff) * dvgd_eff_dvg;
+      }
+      Igd = T11 * T2 * T6;
+      dIgd_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+      dIgd_dVd = -dIgd_dVg;
+      gIgsg = dIgs_dVg;
+      gIgss = dIgs_dVs;
+      gIgdg = dIgd_dVg;
+      gIgdd = dIgd_dVd;
+    } else {
+      Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
+      Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
+      Igs = gIgsg = gIgss = 0.0;
+      Igd = gIgdg = gIgdd = 0.0;
+    }
+    if (igbMod) {
+      T0 = tmp * XP(nigbacc);
+      T1 = -Vgs_eff + Vbseff + Vfb;
+      VxNVt = T1 / T0;
+      if (VxNVt > kExpThreshold) {
+        Vaux = T1;
+        dVaux_dVg = -dVgs_eff_dVg;
+        dVaux_dVb = 1.0;
+      } else if (VxNVt < -kExpThreshold) {
+        Vaux = T0 * log(1.0 + kMinExp);
+        dVaux_dVg = dVaux_dVb = 0.0;
+      } else {
+        ExpVxNVt = exp(VxNVt);
+        Vaux = T0 * log(1.0 + ExpVxNVt);
+        dVaux_dVb = ExpVxNVt / (1.0 + ExpVxNVt);
+        dVaux_dVg = -dVaux_dVb * dVgs_eff_dVg;
+      }
+      T2 = (Vgs_eff - Vbseff) * Vaux;
+      dT2_dVg = dVgs_eff_dVg * Vaux + (Vgs_eff - Vbseff) * dVaux_dVg;
+      dT2_dVb = -Vaux + (Vgs_eff - Vbseff) * dVaux_dVb;
+      T11 = 4.97232e-7 * PP(weff) * PP(leff) * XP(ToxRatio);
+      T12 = -7.45669e11 * toxe;
+      T3 = XP(aigbacc) * XP(cigbacc) - XP(bigbacc);
+      T4 = XP(bigbacc) * XP(cigbacc);
+      T5 = T12 * (XP(aigbacc) + T3 * Voxacc - T4 * Voxacc * Voxacc);
+      if (T5 > kExpThreshold) {
+        T6 = kMaxExp;
+        dT6_dVg = dT6_dVb = 0.0;
+      } else if (T5 < -kExpThreshold) {
+        T6 = kMinExp;
+        dT6_dVg = dT6_dVb = 0.0;
+      } else {
+        T6 = exp(T5);
+        dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxacc);
+        dT6_dVb = dT6_dVg * dVoxacc_dVb;
+        dT6_dVg *= dVoxacc_dVg;
+      }
+      Igbacc = T11 * T2 * T6;
+      dIgbacc_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+      dIgbacc_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+      T0 = tmp * XP(nigbinv);
+      T1 = Voxdepinv - XP(eigbinv);
+      VxNVt = T1 / T0;
+      if (VxNVt > kExpThreshold) {
+        Vaux = T1;
+        dVaux_dVg = dVoxdepinv_dVg;
+        dVaux_dVd = dVoxdepinv_dVd;
+        dVaux_dVb = dVoxdepinv_dVb;
+      } else if (VxNVt < -kExpThreshold) {
+        Vaux = T0 * log(1.0 + kMinExp);
+        dVaux_dVg = dVaux_dVd = dVaux_dVb = 0.0;
+      } else {
+        ExpVxNVt = exp(VxNVt);
+        Vaux = T0 * log(1.0 + ExpVxNVt);
+        dVaux_dVg = ExpVxNVt / (1.0 + ExpVxNVt);
+        dVaux_dVd = dVaux_dVg * dVoxdepinv_dVd;
+        dVaux_dVb = dVaux_dVg * dVoxdepinv_dVb;
+        dVaux_dVg *= dVoxdepinv_dVg;
+      }
+      T2 = (Vgs_eff - Vbseff) * Vaux;
+      dT2_dVg = dVgs_eff_dVg * Vaux + (Vgs_eff - Vbseff) * dVaux_dVg;
+      dT2_dVd = (Vgs_eff - Vbseff) * dVaux_dVd;
+      dT2_dVb = -Vaux + (Vgs_eff - Vbseff) * dVaux_dVb;
+      T11 *= 0.75610;
+      T12 *= 1.31724;
+      T3 = XP(aigbinv) * XP(cigbinv) - XP(bigbinv);
+      T4 = XP(bigbinv) * XP(cigbinv);
+      T5 = T12 * (XP(aigbinv) + T3 * Voxdepinv - T4 * Voxdepinv * Voxdepinv);
+      if (T5 > kExpThreshold) {
+        T6 = kMaxExp;
+        dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+      } else if (T5 < -kExpThreshold) {
+        T6 = kMinExp;
+        dT6_dVg = dT6_dVd = dT6_dVb = 0.0;
+      } else {
+        T6 = exp(T5);
+        dT6_dVg = T6 * T12 * (T3 - 2.0 * T4 * Voxdepinv);
+        dT6_dVd = dT6_dVg * dVoxdepinv_dVd;
+        dT6_dVb = dT6_dVg * dVoxdepinv_dVb;
+        dT6_dVg *= dVoxdepinv_dVg;
+      }
+      Igbinv = T11 * T2 * T6;
+      dIgbinv_dVg = T11 * (T2 * dT6_dVg + T6 * dT2_dVg);
+      dIgbinv_dVd = T11 * (T2 * dT6_dVd + T6 * dT2_dVd);
+      dIgbinv_dVb = T11 * (T2 * dT6_dVb + T6 * dT2_dVb);
+      Igb = Igbinv + Igbacc;
+      gIgbg = dIgbinv_dVg + dIgbacc_dVg;
+      gIgbd = dIgbinv_dVd;
+      gIgbb = (dIgbinv_dVb + dIgbacc_dVb) * dVbseff_dVb;
+    } else {
+      Igb = gIgbg = gIgbd = gIgbs = gIgbb = 0.0;
+    }
+  } else {
+    /* igcMod = igbMod = 0 (:10482-10488, :10617-10620) */
+    Igcs = gIgcsg = gIgcsd = gIgcsb = 0.0;
+    Igcd = gIgcdg = gIgcdd = gIgcdb = 0.0;
+    Igs = gIgsg = gIgss = 0.0;
+    Igd = gIgdg = gIgdd = 0.0;
+    Igb = gIgbg = gIgbd = gIgbs = gIgbb = 0.0;
+  }
   if (nf != 1.0) {
     cdrain *= nf;
     gds *= nf;
@@ -2137,7 +2829,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
 
   if (RELOAD) reload_params(dfb);
-  /* intrinsic charges, capMod = 2 (:10707-11611) */
+  /* intrinsic charges (:10707-11611) */
   if ((C.xpart < 0) || (!ChargeComputationNeeded)) {
     qgate = qdrn = qsrc = qbulk = 0.0;
     cggb = cgsb = cgdb = 0.0;
@@ -2147,6 +2839,230 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     cgbb = csbb = cdbb = cbbb = 0.0;
     cqdb = cqsb = cqgb = cqbb = 0.0;
     gtau = 0.0;
+  } else if (X && capMod == 0) {
+    double Arg1, One_Third_CoxWL, Two_Third_CoxWL, Alphaz, dAlphaz_dVg, dAlphaz_dVb;
+    double Vfb, dVgst_dVb, dVgst_dVg;
+    /* capMod = 0 (:10718-11012) */
+    if (Vbseff < 0.0) {
+      Vbseff = Vbs;
+      dVbseff_dVb = 1.0;
+    } else {
+      Vbseff = PP(phi) - Phis;
+      dVbseff_dVb = -dPhis_dVb;
+    }
+    Vfb = XP(vfbcv);
+    Vth = Vfb + PP(phi) + PP(k1ox) * sqrtPhis;
+    Vgst = Vgs_eff - Vth;
+    dVth_dVb = PP(k1ox) * dsqrtPhis_dVb;
+    dVgst_dVb = -dVth_dVb;
+    dVgst_dVg = dVgs_eff_dVg;
+    CoxWL = C.coxe * PP(weffCV) * PP(leffCV) * nf;
+    Arg1 = Vgs_eff - Vbseff - Vfb;
+    if (Arg1 <= 0.0) {
+      qgate = CoxWL * Arg1;
+      qbulk = -qgate;
+      qdrn = 0.0;
+      cggb = CoxWL * dVgs_eff_dVg;
+      cgdb = 0.0;
+      cgsb = CoxWL * (dVbseff_dVb - dVgs_eff_dVg);
+      cdgb = 0.0;
+      cddb = 0.0;
+      cdsb = 0.0;
+      cbgb = -CoxWL * dVgs_eff_dVg;
+      cbdb = 0.0;
+      cbsb = -cgsb;
+    } else if (Vgst <= 0.0) {
+      T1 = 0.5 * PP(k1ox);
+      T2 = sqrt(T1 * T1 + Arg1);
+      qgate = CoxWL * PP(k1ox) * (T2 - T1);
+      qbulk = -qgate;
+      qdrn = 0.0;
+      T0 = CoxWL * T1 / T2;
+      cggb = T0 * dVgs_eff_dVg;
+      cgdb = 0.0;
+      cgsb = T0 * (dVbseff_dVb - dVgs_eff_dVg);
+      cdgb = 0.0;
+      cddb = 0.0;
+      cdsb = 0.0;
+      cbgb = -cggb;
+      cbdb = 0.0;
+      cbsb = -cgsb;
+    } else {
+      One_Third_CoxWL = CoxWL / 3.0;
+      Two_Third_CoxWL = 2.0 * One_Third_CoxWL;
+      AbulkCV = Abulk0 * PP(abulkCVfactor);
+      dAbulkCV_dVb = PP(abulkCVfactor) * dAbulk0_dVb;
+      Vdsat = Vgst / AbulkCV;
+      dVdsat_dVg = dVgs_eff_dVg / AbulkCV;
+      dVdsat_dVb = -(Vdsat * dAbulkCV_dVb + dVth_dVb) / AbulkCV;
+      if (C.xpart > 0.5) {
+        /* 0/100 partition */
+        if (Vdsat <= Vds) {
+          T1 = Vdsat / 3.0;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          qbulk = -(qgate + T2);
+          qdrn = 0.0;
+          cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          cgsb = -(cggb + T2);
+          cgdb = 0.0;
+          cdgb = 0.0;
+          cddb = 0.0;
+          cdsb = 0.0;
+          cbgb = -(cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          cbsb = -(cbgb + T3);
+          cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * CoxWL;
+          T4 = T9 * Alphaz;
+          T7 = 2.0 * Vds - T1 - 3.0 * T3;
+          T8 = T3 - T1 - 2.0 * Vds;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - 0.5 * (Vds - T3));
+          T10 = T4 * T8;
+          qdrn = T4 * T7;
+          qbulk = -(qgate + qdrn + T10);
+          T5 = T3 / T1;
+          cggb = CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = -CoxWL * T5 * dVdsat_dVb;
+          cgdb = CoxWL * (T2 - 0.5 + 0.5 * T5);
+          cgsb = -(cggb + T11 + cgdb);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T7 = T9 * T7;
+          T8 = T9 * T8;
+          T9 = 2.0 * T4 * (1.0 - 3.0 * T5);
+          cdgb = (T7 * dAlphaz_dVg - T9 * dVdsat_dVg) * dVgs_eff_dVg;
+          T12 = T7 * dAlphaz_dVb - T9 * dVdsat_dVb;
+          cddb = T4 * (3.0 - 6.0 * T2 - 3.0 * T5);
+          cdsb = -(cdgb + T12 + cddb);
+          T9 = 2.0 * T4 * (1.0 + T5);
+          T10 = (T8 * dAlphaz_dVg - T9 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = T8 * dAlphaz_dVb - T9 * dVdsat_dVb;
+          T12 = T4 * (2.0 * T2 + T5 - 1.0);
+          T0 = -(T10 + T11 + T12);
+          cbgb = -(cggb + cdgb + T10);
+          cbdb = -(cgdb + cddb + T12);
+          cbsb = -(cgsb + cdsb + T0);
+        }
+      } else if (C.xpart < 0.5) {
+        /* 40/60 partition */
+        if (Vds >= Vdsat) {
+          T1 = Vdsat / 3.0;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          qbulk = -(qgate + T2);
+          qdrn = 0.4 * T2;
+          cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          cgsb = -(cggb + T2);
+          cgdb = 0.0;
+          T3 = 0.4 * Two_Third_CoxWL;
+          cdgb = -T3 * dVgs_eff_dVg;
+          cddb = 0.0;
+          T4 = T3 * dVth_dVb;
+          cdsb = -(T4 + cdgb);
+          cbgb = -(cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          cbsb = -(cbgb + T3);
+          cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * CoxWL;
+          T4 = T9 * Alphaz;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - 0.5 * (Vds - T3));
+          T5 = T3 / T1;
+          cggb = CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          tmp = -CoxWL * T5 * dVdsat_dVb;
+          cgdb = CoxWL * (T2 - 0.5 + 0.5 * T5);
+          cgsb = -(cggb + cgdb + tmp);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T6 = 8.0 * Vdsat * Vdsat - 6.0 * Vdsat * Vds + 1.2 * Vds * Vds;
+          T8 = T2 / T1;
+          T7 = Vds - T1 - T8 * T6;
+          qdrn = T4 * T7;
+          T7 *= T9;
+          tmp = T8 / T1;
+          tmp1 = T4 * (2.0 - 4.0 * tmp * T6 + T8 * (16.0 * Vdsat - 6.0 * Vds));
+          cdgb = (T7 * dAlphaz_dVg - tmp1 * dVdsat_dVg) * dVgs_eff_dVg;
+          T10 = T7 * dAlphaz_dVb - tmp1 * dVdsat_dVb;
+          cddb = T4 * (2.0 - (1.0 / (3.0 * T1 * T1) + 2.0 * tmp) * T6
+                          + T8 * (6.0 * Vdsat - 2.4 * Vds));
+          cdsb = -(cdgb + T10 + cddb);
+          T7 = 2.0 * (T1 + T3);
+          qbulk = -(qgate - T4 * T7);
+          T7 *= T9;
+          T0 = 4.0 * T4 * (1.0 - T5);
+          T12 = (-T7 * dAlphaz_dVg - cdgb - T0 * dVdsat_dVg) * dVgs_eff_dVg;
+          T11 = -T7 * dAlphaz_dVb - T10 - T0 * dVdsat_dVb;
+          T10 = -4.0 * T4 * (T2 - 0.5 + 0.5 * T5) - cddb;
+          tmp = -(T10 + T11 + T12);
+          cbgb = -(cggb + cdgb + T12);
+          cbdb = -(cgdb + cddb + T10);
+          cbsb = -(cgsb + cdsb + tmp);
+        }
+      } else {
+        /* 50/50 partition */
+        if (Vds >= Vdsat) {
+          T1 = Vdsat / 3.0;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - T1);
+          T2 = -Two_Third_CoxWL * Vgst;
+          qbulk = -(qgate + T2);
+          qdrn = 0.5 * T2;
+          cggb = One_Third_CoxWL * (3.0 - dVdsat_dVg) * dVgs_eff_dVg;
+          T2 = -One_Third_CoxWL * dVdsat_dVb;
+          cgsb = -(cggb + T2);
+          cgdb = 0.0;
+          cdgb = -One_Third_CoxWL * dVgs_eff_dVg;
+          cddb = 0.0;
+          T4 = One_Third_CoxWL * dVth_dVb;
+          cdsb = -(T4 + cdgb);
+          cbgb = -(cggb - Two_Third_CoxWL * dVgs_eff_dVg);
+          T3 = -(T2 + Two_Third_CoxWL * dVth_dVb);
+          cbsb = -(cbgb + T3);
+          cbdb = 0.0;
+        } else {
+          Alphaz = Vgst / Vdsat;
+          T1 = 2.0 * Vdsat - Vds;
+          T2 = Vds / (3.0 * T1);
+          T3 = T2 * Vds;
+          T9 = 0.25 * CoxWL;
+          T4 = T9 * Alphaz;
+          qgate = CoxWL * (Vgs_eff - Vfb - PP(phi) - 0.5 * (Vds - T3));
+          T5 = T3 / T1;
+          cggb = CoxWL * (1.0 - T5 * dVdsat_dVg) * dVgs_eff_dVg;
+          tmp = -CoxWL * T5 * dVdsat_dVb;
+          cgdb = CoxWL * (T2 - 0.5 + 0.5 * T5);
+          cgsb = -(cggb + cgdb + tmp);
+          T6 = 1.0 / Vdsat;
+          dAlphaz_dVg = T6 * (1.0 - Alphaz * dVdsat_dVg);
+          dAlphaz_dVb = -T6 * (dVth_dVb + Alphaz * dVdsat_dVb);
+          T7 = T1 + T3;
+          qdrn = -T4 * T7;
+          qbulk = -(qgate + qdrn + qdrn);
+          T7 *= T9;
+          T0 = T4 * (2.0 * T5 - 2.0);
+          cdgb = (T0 * dVdsat_dVg - T7 * dAlphaz_dVg) * dVgs_eff_dVg;
+          T12 = T0 * dVdsat_dVb - T7 * dAlphaz_dVb;
+          cddb = T4 * (1.0 - 2.0 * T2 - T5);
+          cdsb = -(cdgb + T12 + cddb);
+          cbgb = -(cggb + 2.0 * cdgb);
+          cbdb = -(cgdb + 2.0 * cddb);
+          cbsb = -(cgsb + 2.0 * cdsb);
+        }
+      }
+    }
   } else {
     if (Vbseff < 0.0) {
       VbseffCV = Vbseff;
@@ -2239,6 +3155,141 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
       dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
       dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
     }
+    if (X && capMod == 1) {
+    double Vfb;
+    /* capMod = 1 (:11139-11319) */
+    Vfb = IV(vfbzb);
+    V3 = Vfb - Vgs_eff + VbseffCV - kDelta3;
+    if (Vfb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * Vfb);
+    else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * Vfb);
+    T1 = 0.5 * (1.0 + V3 / T0);
+    Vfbeff = Vfb - 0.5 * (V3 + T0);
+    dVfbeff_dVg = T1 * dVgs_eff_dVg;
+    dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+    Qac0 = CoxWL * (Vfbeff - Vfb);
+    dQac0_dVg = CoxWL * dVfbeff_dVg;
+    dQac0_dVb = CoxWL * dVfbeff_dVb;
+    T0 = 0.5 * PP(k1ox);
+    T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
+    if (PP(k1ox) == 0.0) {
+      T1 = 0.0;
+      T2 = 0.0;
+    } else if (T3 < 0.0) {
+      T1 = T0 + T3 / PP(k1ox);
+      T2 = CoxWL;
+    } else {
+      T1 = sqrt(T0 * T0 + T3);
+      T2 = CoxWL * T0 / T1;
+    }
+    Qsub0 = CoxWL * PP(k1ox) * (T1 - T0);
+    dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg);
+    dQsub0_dVd = -T2 * dVgsteff_dVd;
+    dQsub0_dVb = -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb);
+    AbulkCV = Abulk0 * PP(abulkCVfactor);
+    dAbulkCV_dVb = PP(abulkCVfactor) * dAbulk0_dVb;
+    VdsatCV = Vgsteff / AbulkCV;
+    T0 = VdsatCV - Vds - kDelta4;
+    dT0_dVg = 1.0 / AbulkCV;
+    dT0_dVb = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta4 * VdsatCV);
+    dT1_dVg = (T0 + kDelta4 + kDelta4) / T1;
+    dT1_dVd = -T0 / T1;
+    dT1_dVb = dT1_dVg * dT0_dVb;
+    dT1_dVg *= dT0_dVg;
+    if (T0 >= 0.0) {
+      VdseffCV = VdsatCV - 0.5 * (T0 + T1);
+      dVdseffCV_dVg = 0.5 * (dT0_dVg - dT1_dVg);
+      dVdseffCV_dVd = 0.5 * (1.0 - dT1_dVd);
+      dVdseffCV_dVb = 0.5 * (dT0_dVb - dT1_dVb);
+    } else {
+      T3 = (kDelta4 + kDelta4) / (T1 - T0);
+      T4 = 1.0 - T3;
+      T5 = VdsatCV * T3 / (T1 - T0);
+      VdseffCV = VdsatCV * T4;
+      dVdseffCV_dVg = dT0_dVg * T4 + T5 * (dT1_dVg - dT0_dVg);
+      dVdseffCV_dVd = T5 * (dT1_dVd + 1.0);
+      dVdseffCV_dVb = dT0_dVb * (T4 - T5) + T5 * dT1_dVb;
+    }
+    if (Vds == 0.0) {
+      VdseffCV = 0.0;
+      dVdseffCV_dVg = 0.0;
+      dVdseffCV_dVb = 0.0;
+    }
+    T0 = AbulkCV * VdseffCV;
+    T1 = 12.0 * (Vgsteff - 0.5 * T0 + 1.0e-20);
+    T2 = VdseffCV / T1;
+    T3 = T0 * T2;
+    T4 = (1.0 - 12.0 * T2 * T2 * AbulkCV);
+    T5 = (6.0 * T0 * (4.0 * Vgsteff - T0) / (T1 * T1) - 0.5);
+    T6 = 12.0 * T2 * T2 * Vgsteff;
+    qgate = CoxWL * (Vgsteff - 0.5 * VdseffCV + T3);
+    Cgg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cgd1 = CoxWL * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd;
+    Cgb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cgg1 * dVgsteff_dVb;
+    Cgg1 *= dVgsteff_dVg;
+    T7 = 1.0 - AbulkCV;
+    qbulk = CoxWL * T7 * (0.5 * VdseffCV - T3);
+    T4 = -T7 * (T4 - 1.0);
+    T5 = -T7 * T5;
+    T6 = -(T7 * T6 + (0.5 * VdseffCV - T3));
+    Cbg1 = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+    Cbd1 = CoxWL * T5 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd;
+    Cbb1 = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Cbg1 * dVgsteff_dVb;
+    Cbg1 *= dVgsteff_dVg;
+    if (C.xpart > 0.5) {
+      T1 = T1 + T1;
+      qsrc = -CoxWL * (0.5 * Vgsteff + 0.25 * T0 - T0 * T0 / T1);
+      T7 = (4.0 * Vgsteff - T0) / (T1 * T1);
+      T4 = -(0.5 + 24.0 * T0 * T0 / (T1 * T1));
+      T5 = -(0.25 * AbulkCV - 12.0 * AbulkCV * T0 * T7);
+      T6 = -(0.25 * VdseffCV - 12.0 * T0 * VdseffCV * T7);
+      Csg = CoxWL * (T4 + T5 * dVdseffCV_dVg);
+      Csd = CoxWL * T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = CoxWL * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else if (C.xpart < 0.5) {
+      T1 = T1 / 12.0;
+      T2 = 0.5 * CoxWL / (T1 * T1);
+      T3 = Vgsteff * (2.0 * T0 * T0 / 3.0 + Vgsteff * (Vgsteff - 4.0 * T0 / 3.0))
+           - 2.0 * T0 * T0 * T0 / 15.0;
+      qsrc = -T2 * T3;
+      T7 = 4.0 / 3.0 * Vgsteff * (Vgsteff - T0) + 0.4 * T0 * T0;
+      T4 = -2.0 * qsrc / T1 - T2 * (Vgsteff * (3.0 * Vgsteff - 8.0 * T0 / 3.0)
+                                       + 2.0 * T0 * T0 / 3.0);
+      T5 = (qsrc / T1 + T2 * T7) * AbulkCV;
+      T6 = (qsrc / T1 * VdseffCV + T2 * T7 * VdseffCV);
+      Csg = (T4 + T5 * dVdseffCV_dVg);
+      Csd = T5 * dVdseffCV_dVd + Csg * dVgsteff_dVd;
+      Csb = (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) + Csg * dVgsteff_dVb;
+      Csg *= dVgsteff_dVg;
+    } else {
+      qsrc = -0.5 * (qgate + qbulk);
+      Csg = -0.5 * (Cgg1 + Cbg1);
+      Csb = -0.5 * (Cgb1 + Cbb1);
+      Csd = -0.5 * (Cgd1 + Cbd1);
+    }
+    qgate += Qac0 + Qsub0;
+    qbulk -= (Qac0 + Qsub0);
+    qdrn = -(qgate + qbulk + qsrc);
+    Cgg = dQac0_dVg + dQsub0_dVg + Cgg1;
+    Cgd = dQsub0_dVd + Cgd1;
+    Cgb = dQac0_dVb + dQsub0_dVb + Cgb1;
+    Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+    Cbd = Cbd1 - dQsub0_dVd;
+    Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+    Cgb *= dVbseff_dVb;
+    Cbb *= dVbseff_dVb;
+    Csb *= dVbseff_dVb;
+    cggb = Cgg;
+    cgsb = -(Cgg + Cgd + Cgb);
+    cgdb = Cgd;
+    cdgb = -(Cgg + Cbg + Csg);
+    cdsb = (Cgg + Cgd + Cgb + Cbg + Cbd + Cbb + Csg + Csd + Csb);
+    cddb = -(Cgd + Cbd + Csd);
+    cbgb = Cbg;
+    cbsb = -(Cbg + Cbd + Cbb);
+    cbdb = Cbd;
+    } else {
     /* capMod = 2 (:11321-11602) */
     V3 = IV(vfbzb) - Vgs_eff + VbseffCV - kDelta3;
     if (IV(vfbzb) <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * IV(vfbzb));
@@ -2444,6 +3495,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     cbgb = Cbg;
     cbsb = -(Cbg + Cbd + Cbb);
     cbdb = Cbd;
+    }
   }
   csgb = -cggb - cdgb - cbgb;
   csdb = -cgdb - cddb - cbdb;
@@ -2454,6 +3506,19 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   csbb = -cgbb - cdbb - cbbb;
 
   if (RELOAD) reload_params(dfb);
+  /* NQS (:11626-11660) */
+  if (X && ((itrnqsMod) || (iacnqsMod))) {
+    qchqs = qcheq = -(qbulk + qgate);
+    cqgb = -(cggb + cbgb);
+    cqdb = -(cgdb + cbdb);
+    cqsb = -(cgsb + cbsb);
+    cqbb = -(cqgb + cqdb + cqsb);
+    CoxWL = C.coxe * PP(weffCV) * nf * PP(leffCV);
+    T1 = gcrg / CoxWL;
+    gtau = T1 * kScalingFactor;
+    if (iacnqsMod) taunet = 1.0 / T1;
+  }
+
   /* junction charges (:11662-11810) */
   if (ChargeComputationNeeded) {
     czbd = C.DunitAreaTempJctCap * IV(Adeff);
@@ -2539,143 +3604,687 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
 
   if (RELOAD) reload_params(dfb);
-  /* overlap capacitances, capMod != 0 (:11812-11883) */
-  vgdx = vgd;
-  vgsx = vgs;
-  Igate = IgateMid = 0.0;
-  Igate_Jdxp = IgateMid_Jdxp = 0.0;
-  T0 = vgdx + kDelta1;
-  T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
-  T2 = 0.5 * (T0 - T1);
-  T3 = PP(weffCV) * PP(cgdl);
-  T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappad));
-  cgdo = PP(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-  qgdo = (PP(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PP(ckappad) * (T4 - 1.0));
-  T0 = vgsx + kDelta1;
-  T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
-  T2 = 0.5 * (T0 - T1);
-  T3 = PP(weffCV) * PP(cgsl);
-  T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappas));
-  cgso = PP(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-  qgso = (PP(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PP(ckappas) * (T4 - 1.0));
-  if (nf != 1.0) {
-    cgdo *= nf;
-    cgso *= nf;
-    qgdo *= nf;
-    qgso *= nf;
+  if constexpr (X) {
+    /* gate resistor currents and overlap capacitances (:11812-11883) */
+    if (irgateMod == 3) {
+      vgdx = vgmd;
+      vgsx = vgms;
+    } else {
+      vgdx = vgd;
+      vgsx = vgs;
+    }
+    Igate = IgateMid = 0.0;
+    Igate_Jdxp = IgateMid_Jdxp = 0.0;
+    if (irgateMod == 1) {
+      Igate = XP(grgeltd) * (Vgegp);
+    } else if (irgateMod == 2) {
+      Igate = (gcrg) * (Vgegp);
+    } else if (irgateMod == 3) {
+      Igate = XP(grgeltd) * (Vgegm);
+      IgateMid = gcrg * (Vgmgp);
+    }
+    if (capMod == 0) {
+      cgdo = PP(cgdo);
+      qgdo = PP(cgdo) * vgdx;
+      cgso = PP(cgso);
+      qgso = PP(cgso) * vgsx;
+    } else {
+    T0 = vgdx + kDelta1;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+    T2 = 0.5 * (T0 - T1);
+    T3 = PP(weffCV) * PP(cgdl);
+    T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappad));
+    cgdo = PP(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+    qgdo = (PP(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PP(ckappad) * (T4 - 1.0));
+    T0 = vgsx + kDelta1;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+    T2 = 0.5 * (T0 - T1);
+    T3 = PP(weffCV) * PP(cgsl);
+    T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappas));
+    cgso = PP(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+    qgso = (PP(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PP(ckappas) * (T4 - 1.0));
+    }
+    if (nf != 1.0) {
+      cgdo *= nf;
+      cgso *= nf;
+      qgdo *= nf;
+      qgso *= nf;
+    }
+  } else {
+    /* overlap capacitances, capMod != 0 (:11812-11883) */
+    vgdx = vgd;
+    vgsx = vgs;
+    Igate = IgateMid = 0.0;
+    Igate_Jdxp = IgateMid_Jdxp = 0.0;
+    T0 = vgdx + kDelta1;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+    T2 = 0.5 * (T0 - T1);
+    T3 = PP(weffCV) * PP(cgdl);
+    T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappad));
+    cgdo = PP(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+    qgdo = (PP(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PP(ckappad) * (T4 - 1.0));
+    T0 = vgsx + kDelta1;
+    T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
+    T2 = 0.5 * (T0 - T1);
+    T3 = PP(weffCV) * PP(cgsl);
+    T4 = sqrt(1.0 - 4.0 * T2 / PP(ckappas));
+    cgso = PP(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+    qgso = (PP(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PP(ckappas) * (T4 - 1.0));
+    if (nf != 1.0) {
+      cgdo *= nf;
+      cgso *= nf;
+      qgdo *= nf;
+      qgso *= nf;
+    }
   }
 
   // ---- setupCapacitors_oldDAE / setupCapacitors_newDAE ----
-  {
-    double ag0 = ss.pdt;
+  if constexpr (X) {
+    {
+
+    double ag0 = ss.pdt, Cdd, Csd, Cdg, Csg, Cds, Css;
     if (!(ss.dcopFlag) && ss.initTranFlag && ss.newtonIter == 0) ag0 = 0.0;
     if (mode > 0) {
-      qdrn -= qgdo;
-      gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
-      gcgdb = (cgdb - cgdo) * ag0;
-      gcgsb = (cgsb - cgso) * ag0;
-      gcgbb = -(gcggb + gcgdb + gcgsb);
-      gcdgb = (cdgb - cgdo) * ag0;
-      gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
-      gcbgb = (cbgb - PP(cgbo)) * ag0;
-      gcdgmb = gcsgmb = gcbgmb = 0.0;
-      qgb = PP(cgbo) * vgb;
-      qgate += qgdo + qgso + qgb;
-      qbulk -= qgb;
-      qsrc = -(qgate + qbulk + qdrn);
-      gcddb = (cddb + capbd + cgdo) * ag0;
-      gcdsb = cdsb * ag0;
-      gcsdb = -(cgdb + cbdb + cddb) * ag0;
-      gcssb = (capbs + cgso - (cgsb + cbsb + cdsb)) * ag0;
-      gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
-      gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
-      gcbdb = (cbdb - capbd) * ag0;
-      gcbsb = (cbsb - capbs) * ag0;
-      gcdbdb = 0.0;
-      gcsbsb = 0.0;
-      gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
-      ggtg = ggtd = ggtb = ggts = 0.0;
-      sxpart = 0.6;
-      dxpart = 0.4;
-      ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
-      dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      if (itrnqsMod == 0) {
+        qdrn -= qgdo;
+        if (irgateMod == 3) {
+          gcgmgmb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgmdb = -cgdo * ag0;
+          gcgmsb = -cgso * ag0;
+          gcgmbb = -PP(cgbo) * ag0;
+          gcdgmb = gcgmdb;
+          gcsgmb = gcgmsb;
+          gcbgmb = gcgmbb;
+          gcggb = cggb * ag0;
+          gcgdb = cgdb * ag0;
+          gcgsb = cgsb * ag0;
+          gcgbb = -(gcggb + gcgdb + gcgsb);
+          gcdgb = cdgb * ag0;
+          gcsgb = -(cggb + cbgb + cdgb) * ag0;
+          gcbgb = cbgb * ag0;
+          qgmb = PP(cgbo) * vgmb;
+          qgmid = qgdo + qgso + qgmb;
+          qbulk -= qgmb;
+          qsrc = -(qgate + qgmid + qbulk + qdrn);
+        } else {
+          gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+          gcgdb = (cgdb - cgdo) * ag0;
+          gcgsb = (cgsb - cgso) * ag0;
+          gcgbb = -(gcggb + gcgdb + gcgsb);
+          gcdgb = (cdgb - cgdo) * ag0;
+          gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
+          gcbgb = (cbgb - PP(cgbo)) * ag0;
+          gcdgmb = gcsgmb = gcbgmb = 0.0;
+          qgb = PP(cgbo) * vgb;
+          qgate += qgdo + qgso + qgb;
+          qbulk -= qgb;
+          qsrc = -(qgate + qbulk + qdrn);
+        }
+        gcddb = (cddb + capbd + cgdo) * ag0;
+        gcdsb = cdsb * ag0;
+        gcsdb = -(cgdb + cbdb + cddb) * ag0;
+        gcssb = (capbs + cgso - (cgsb + cbsb + cdsb)) * ag0;
+        if (!irbodyMod) {
+          gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+          gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+          gcbdb = (cbdb - capbd) * ag0;
+          gcbsb = (cbsb - capbs) * ag0;
+          gcdbdb = 0.0;
+          gcsbsb = 0.0;
+        } else {
+          gcdbb = -(cddb + cdgb + cdsb) * ag0;
+          gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb) + capbs * ag0;
+          gcbdb = cbdb * ag0;
+          gcbsb = cbsb * ag0;
+          gcdbdb = -capbd * ag0;
+          gcsbsb = -capbs * ag0;
+        }
+        gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
+        ggtg = ggtd = ggtb = ggts = 0.0;
+        sxpart = 0.6;
+        dxpart = 0.4;
+        ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      } else {
+        qcheq = qchqs;
+        CoxWL = C.coxe * PP(weffCV) * IV(nf) * PP(leffCV);
+        T0 = qdef * kScalingFactor / CoxWL;
+        ggtg = gtg = T0 * gcrgg;
+        ggtd = gtd = T0 * gcrgd;
+        ggts = gts = T0 * gcrgs;
+        ggtb = gtb = T0 * gcrgb;
+        gqdef = kScalingFactor * ag0;
+        gcqgb = cqgb * ag0;
+        gcqdb = cqdb * ag0;
+        gcqsb = cqsb * ag0;
+        gcqbb = cqbb * ag0;
+        if (fabs(qcheq) <= 1.0e-5 * CoxWL) {
+          if (C.xpart < 0.5) dxpart = 0.4;
+          else if (C.xpart > 0.5) dxpart = 0.0;
+          else dxpart = 0.5;
+          ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        } else {
+          dxpart = qdrn / qcheq;
+          Cdd = cddb;
+          Csd = -(cgdb + cddb + cbdb);
+          ddxpart_dVd = (Cdd - dxpart * (Cdd + Csd)) / qcheq;
+          Cdg = cdgb;
+          Csg = -(cggb + cdgb + cbgb);
+          ddxpart_dVg = (Cdg - dxpart * (Cdg + Csg)) / qcheq;
+          Cds = cdsb;
+          Css = -(cgsb + cdsb + cbsb);
+          ddxpart_dVs = (Cds - dxpart * (Cds + Css)) / qcheq;
+          ddxpart_dVb = -(ddxpart_dVd + ddxpart_dVg + ddxpart_dVs);
+        }
+        sxpart = 1.0 - dxpart;
+        dsxpart_dVd = -ddxpart_dVd;
+        dsxpart_dVg = -ddxpart_dVg;
+        dsxpart_dVs = -ddxpart_dVs;
+        dsxpart_dVb = -(dsxpart_dVd + dsxpart_dVg + dsxpart_dVs);
+        if (irgateMod == 3) {
+          gcgmgmb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgmdb = -cgdo * ag0;
+          gcgmsb = -cgso * ag0;
+          gcgmbb = -PP(cgbo) * ag0;
+          gcdgmb = gcgmdb;
+          gcsgmb = gcgmsb;
+          gcbgmb = gcgmbb;
+          gcdgb = gcsgb = gcbgb = 0.0;
+          gcggb = gcgdb = gcgsb = gcgbb = 0.0;
+          qgmb = PP(cgbo) * vgmb;
+          qgmid = qgdo + qgso + qgmb;
+          qgate = 0.0;
+          qbulk = -qgmb;
+          qdrn = -qgdo;
+          qsrc = -(qgmid + qbulk + qdrn);
+        } else {
+          gcggb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgdb = -cgdo * ag0;
+          gcgsb = -cgso * ag0;
+          gcgbb = -PP(cgbo) * ag0;
+          gcdgb = gcgdb;
+          gcsgb = gcgsb;
+          gcbgb = gcgbb;
+          gcdgmb = gcsgmb = gcbgmb = 0.0;
+          qgb = PP(cgbo) * vgb;
+          qgate = qgdo + qgso + qgb;
+          qbulk = -qgb;
+          qdrn = -qgdo;
+          qsrc = -(qgate + qbulk + qdrn);
+        }
+        gcddb = (capbd + cgdo) * ag0;
+        gcdsb = gcsdb = 0.0;
+        gcssb = (capbs + cgso) * ag0;
+        if (!irbodyMod) {
+          gcdbb = -(gcdgb + gcddb + gcdgmb);
+          gcsbb = -(gcsgb + gcssb + gcsgmb);
+          gcbdb = -capbd * ag0;
+          gcbsb = -capbs * ag0;
+          gcdbdb = 0.0;
+          gcsbsb = 0.0;
+        } else {
+          gcdbb = gcsbb = gcbdb = gcbsb = 0.0;
+          gcdbdb = -capbd * ag0;
+          gcsbsb = -capbs * ag0;
+        }
+        gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
+      }
     } else {
-      qsrc = qdrn - qgso;
-      gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
-      gcgdb = (cgsb - cgdo) * ag0;
-      gcgsb = (cgdb - cgso) * ag0;
-      gcgbb = -(gcggb + gcgdb + gcgsb);
-      gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
-      gcsgb = (cdgb - cgso) * ag0;
-      gcbgb = (cbgb - PP(cgbo)) * ag0;
-      gcdgmb = gcsgmb = gcbgmb = 0.0;
-      qgb = PP(cgbo) * vgb;
-      qgate += qgdo + qgso + qgb;
-      qbulk -= qgb;
-      qdrn = -(qgate + qbulk + qsrc);
-      gcddb = (capbd + cgdo - (cgsb + cbsb + cdsb)) * ag0;
-      gcdsb = -(cgdb + cbdb + cddb) * ag0;
-      gcsdb = cdsb * ag0;
-      gcssb = (cddb + capbs + cgso) * ag0;
-      gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
-      gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
-      gcbdb = (cbsb - capbd) * ag0;
-      gcbsb = (cbdb - capbs) * ag0;
-      gcdbdb = 0.0;
-      gcsbsb = 0.0;
-      gcbbb = -(gcbgb + gcbdb + gcbsb + gcbgmb);
-      ggtg = ggtd = ggtb = ggts = 0.0;
-      sxpart = 0.4;
-      dxpart = 0.6;
-      ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
-      dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      if (itrnqsMod == 0) {
+        qsrc = qdrn - qgso;
+        if (irgateMod == 3) {
+          gcgmgmb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgmdb = -cgdo * ag0;
+          gcgmsb = -cgso * ag0;
+          gcgmbb = -PP(cgbo) * ag0;
+          gcdgmb = gcgmdb;
+          gcsgmb = gcgmsb;
+          gcbgmb = gcgmbb;
+          gcggb = cggb * ag0;
+          gcgdb = cgsb * ag0;
+          gcgsb = cgdb * ag0;
+          gcgbb = -(gcggb + gcgdb + gcgsb);
+          gcdgb = -(cggb + cbgb + cdgb) * ag0;
+          gcsgb = cdgb * ag0;
+          gcbgb = cbgb * ag0;
+          qgmb = PP(cgbo) * vgmb;
+          qgmid = qgdo + qgso + qgmb;
+          qbulk -= qgmb;
+          qdrn = -(qgate + qgmid + qbulk + qsrc);
+        } else {
+          gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+          gcgdb = (cgsb - cgdo) * ag0;
+          gcgsb = (cgdb - cgso) * ag0;
+          gcgbb = -(gcggb + gcgdb + gcgsb);
+          gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
+          gcsgb = (cdgb - cgso) * ag0;
+          gcbgb = (cbgb - PP(cgbo)) * ag0;
+          gcdgmb = gcsgmb = gcbgmb = 0.0;
+          qgb = PP(cgbo) * vgb;
+          qgate += qgdo + qgso + qgb;
+          qbulk -= qgb;
+          qdrn = -(qgate + qbulk + qsrc);
+        }
+        gcddb = (capbd + cgdo - (cgsb + cbsb + cdsb)) * ag0;
+        gcdsb = -(cgdb + cbdb + cddb) * ag0;
+        gcsdb = cdsb * ag0;
+        gcssb = (cddb + capbs + cgso) * ag0;
+        if (!irbodyMod) {
+          gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+          gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+          gcbdb = (cbsb - capbd) * ag0;
+          gcbsb = (cbdb - capbs) * ag0;
+          gcdbdb = 0.0;
+          gcsbsb = 0.0;
+        } else {
+          gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb) + capbd * ag0;
+          gcsbb = -(cddb + cdgb + cdsb) * ag0;
+          gcbdb = cbsb * ag0;
+          gcbsb = cbdb * ag0;
+          gcdbdb = -capbd * ag0;
+          gcsbsb = -capbs * ag0;
+        }
+        gcbbb = -(gcbgb + gcbdb + gcbsb + gcbgmb);
+        ggtg = ggtd = ggtb = ggts = 0.0;
+        sxpart = 0.4;
+        dxpart = 0.6;
+        ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      } else {
+        qcheq = qchqs;
+        CoxWL = C.coxe * PP(weffCV) * IV(nf) * PP(leffCV);
+        T0 = qdef * kScalingFactor / CoxWL;
+        ggtg = gtg = T0 * gcrgg;
+        ggts = gts = T0 * gcrgd;
+        ggtd = gtd = T0 * gcrgs;
+        ggtb = gtb = T0 * gcrgb;
+        gqdef = kScalingFactor * ag0;
+        gcqgb = cqgb * ag0;
+        gcqdb = cqsb * ag0;
+        gcqsb = cqdb * ag0;
+        gcqbb = cqbb * ag0;
+        if (fabs(qcheq) <= 1.0e-5 * CoxWL) {
+          if (C.xpart < 0.5) sxpart = 0.4;
+          else if (C.xpart > 0.5) sxpart = 0.0;
+          else sxpart = 0.5;
+          dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+        } else {
+          sxpart = qdrn / qcheq;
+          Css = cddb;
+          Cds = -(cgdb + cddb + cbdb);
+          dsxpart_dVs = (Css - sxpart * (Css + Cds)) / qcheq;
+          Csg = cdgb;
+          Cdg = -(cggb + cdgb + cbgb);
+          dsxpart_dVg = (Csg - sxpart * (Csg + Cdg)) / qcheq;
+          Csd = cdsb;
+          Cdd = -(cgsb + cdsb + cbsb);
+          dsxpart_dVd = (Csd - sxpart * (Csd + Cdd)) / qcheq;
+          dsxpart_dVb = -(dsxpart_dVd + dsxpart_dVg + dsxpart_dVs);
+        }
+        dxpart = 1.0 - sxpart;
+        ddxpart_dVd = -dsxpart_dVd;
+        ddxpart_dVg = -dsxpart_dVg;
+        ddxpart_dVs = -dsxpart_dVs;
+        ddxpart_dVb = -(ddxpart_dVd + ddxpart_dVg + ddxpart_dVs);
+        if (irgateMod == 3) {
+          gcgmgmb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgmdb = -cgdo * ag0;
+          gcgmsb = -cgso * ag0;
+          gcgmbb = -PP(cgbo) * ag0;
+          gcdgmb = gcgmdb;
+          gcsgmb = gcgmsb;
+          gcbgmb = gcgmbb;
+          gcdgb = gcsgb = gcbgb = 0.0;
+          gcggb = gcgdb = gcgsb = gcgbb = 0.0;
+          qgmb = PP(cgbo) * vgmb;
+          qgmid = qgdo + qgso + qgmb;
+          qgate = 0.0;
+          qbulk = -qgmb;
+          qdrn = -qgdo;
+          qsrc = -qgso;
+        } else {
+          gcggb = (cgdo + cgso + PP(cgbo)) * ag0;
+          gcgdb = -cgdo * ag0;
+          gcgsb = -cgso * ag0;
+          gcgbb = -PP(cgbo) * ag0;
+          gcdgb = gcgdb;
+          gcsgb = gcgsb;
+          gcbgb = gcgbb;
+          gcdgmb = gcsgmb = gcbgmb = 0.0;
+          qgb = PP(cgbo) * vgb;
+          qgate = qgdo + qgso + qgb;
+          qbulk = -qgb;
+          qdrn = -qgdo;
+          qsrc = -qgso;
+        }
+        gcddb = (capbd + cgdo) * ag0;
+        gcdsb = gcsdb = 0.0;
+        gcssb = (capbs + cgso) * ag0;
+        if (!irbodyMod) {
+          gcdbb = -(gcdgb + gcddb + gcdgmb);
+          gcsbb = -(gcsgb + gcssb + gcsgmb);
+          gcbdb = -capbd * ag0;
+          gcbsb = -capbs * ag0;
+          gcdbdb = 0.0;
+          gcsbsb = 0.0;
+        } else {
+          gcdbb = gcsbb = gcbdb = gcbsb = 0.0;
+          gcdbdb = -capbd * ag0;
+          gcsbsb = -capbs * ag0;
+        }
+        gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
+      }
     }
-  }
-  {
+    }
+    {
+
     if (mode > 0) {
-      CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
-      CAPcgdb = (cgdb - cgdo);
-      CAPcgsb = (cgsb - cgso);
-      CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
-      CAPcdgb = (cdgb - cgdo);
-      CAPcsgb = -(cggb + cbgb + cdgb + cgso);
-      CAPcbgb = (cbgb - PP(cgbo));
-      CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
-      CAPcddb = (cddb + capbd + cgdo);
-      CAPcdsb = cdsb;
-      CAPcsdb = -(cgdb + cbdb + cddb);
-      CAPcssb = (capbs + cgso - (cgsb + cbsb + cdsb));
-      CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
-      CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
-      CAPcbdb = (cbdb - capbd);
-      CAPcbsb = (cbsb - capbs);
-      CAPcdbdb = 0.0;
-      CAPcsbsb = 0.0;
-      CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+      if (itrnqsMod == 0) {
+        if (irgateMod == 3) {
+          CAPcgmgmb = (cgdo + cgso + PP(cgbo));
+          CAPcgmdb = -cgdo;
+          CAPcgmsb = -cgso;
+          CAPcgmbb = -PP(cgbo);
+          CAPcdgmb = CAPcgmdb;
+          CAPcsgmb = CAPcgmsb;
+          CAPcbgmb = CAPcgmbb;
+          CAPcggb = cggb;
+          CAPcgdb = cgdb;
+          CAPcgsb = cgsb;
+          CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+          CAPcdgb = cdgb;
+          CAPcsgb = -(cggb + cbgb + cdgb);
+          CAPcbgb = cbgb;
+        } else {
+          CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+          CAPcgdb = (cgdb - cgdo);
+          CAPcgsb = (cgsb - cgso);
+          CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+          CAPcdgb = (cdgb - cgdo);
+          CAPcsgb = -(cggb + cbgb + cdgb + cgso);
+          CAPcbgb = (cbgb - PP(cgbo));
+          CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        }
+        CAPcddb = (cddb + capbd + cgdo);
+        CAPcdsb = cdsb;
+        CAPcsdb = -(cgdb + cbdb + cddb);
+        CAPcssb = (capbs + cgso - (cgsb + cbsb + cdsb));
+        if (!irbodyMod) {
+          CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+          CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+          CAPcbdb = (cbdb - capbd);
+          CAPcbsb = (cbsb - capbs);
+          CAPcdbdb = 0.0;
+          CAPcsbsb = 0.0;
+        } else {
+          CAPcdbb = -(cddb + cdgb + cdsb);
+          CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb) + capbs;
+          CAPcbdb = cbdb;
+          CAPcbsb = cbsb;
+          CAPcdbdb = -capbd;
+          CAPcsbsb = -capbs;
+        }
+        CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+      } else {
+        CAPcqgb = cqgb;
+        CAPcqdb = cqdb;
+        CAPcqsb = cqsb;
+        CAPcqbb = cqbb;
+        if (irgateMod == 3) {
+          CAPcgmgmb = (cgdo + cgso + PP(cgbo));
+          CAPcgmdb = -cgdo;
+          CAPcgmsb = -cgso;
+          CAPcgmbb = -PP(cgbo);
+          CAPcdgmb = CAPcgmdb;
+          CAPcsgmb = CAPcgmsb;
+          CAPcbgmb = CAPcgmbb;
+          CAPcdgb = CAPcsgb = CAPcbgb = 0.0;
+          CAPcggb = CAPcgdb = CAPcgsb = CAPcgbb = 0.0;
+        } else {
+          CAPcggb = (cgdo + cgso + PP(cgbo));
+          CAPcgdb = -cgdo;
+          CAPcgsb = -cgso;
+          CAPcgbb = -PP(cgbo);
+          CAPcdgb = CAPcgdb;
+          CAPcsgb = CAPcgsb;
+          CAPcbgb = CAPcgbb;
+          CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        }
+        CAPcddb = (capbd + cgdo);
+        CAPcdsb = CAPcsdb = 0.0;
+        CAPcssb = (capbs + cgso);
+        if (!irbodyMod) {
+          CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdgmb);
+          CAPcsbb = -(CAPcsgb + CAPcssb + CAPcsgmb);
+          CAPcbdb = -capbd;
+          CAPcbsb = -capbs;
+          CAPcdbdb = 0.0;
+          CAPcsbsb = 0.0;
+        } else {
+          CAPcdbb = CAPcsbb = CAPcbdb = CAPcbsb = 0.0;
+          CAPcdbdb = -capbd;
+          CAPcsbsb = -capbs;
+        }
+        CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+      }
     } else {
-      CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
-      CAPcgdb = (cgsb - cgdo);
-      CAPcgsb = (cgdb - cgso);
-      CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
-      CAPcdgb = -(cggb + cbgb + cdgb + cgdo);
-      CAPcsgb = (cdgb - cgso);
-      CAPcbgb = (cbgb - PP(cgbo));
-      CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
-      CAPcddb = (capbd + cgdo - (cgsb + cbsb + cdsb));
-      CAPcdsb = -(cgdb + cbdb + cddb);
-      CAPcsdb = cdsb;
-      CAPcssb = (cddb + capbs + cgso);
-      CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
-      CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
-      CAPcbdb = (cbsb - capbd);
-      CAPcbsb = (cbdb - capbs);
-      CAPcdbdb = 0.0;
-      CAPcsbsb = 0.0;
-      CAPcbbb = -(CAPcbgb + CAPcbdb + CAPcbsb + CAPcbgmb);
+      if (itrnqsMod == 0) {
+        if (irgateMod == 3) {
+          CAPcgmgmb = (cgdo + cgso + PP(cgbo));
+          CAPcgmdb = -cgdo;
+          CAPcgmsb = -cgso;
+          CAPcgmbb = -PP(cgbo);
+          CAPcdgmb = CAPcgmdb;
+          CAPcsgmb = CAPcgmsb;
+          CAPcbgmb = CAPcgmbb;
+          CAPcggb = cggb;
+          CAPcgdb = cgsb;
+          CAPcgsb = cgdb;
+          CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+          CAPcdgb = -(cggb + cbgb + cdgb);
+          CAPcsgb = cdgb;
+          CAPcbgb = cbgb;
+        } else {
+          CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+          CAPcgdb = (cgsb - cgdo);
+          CAPcgsb = (cgdb - cgso);
+          CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+          CAPcdgb = -(cggb + cbgb + cdgb + cgdo);
+          CAPcsgb = (cdgb - cgso);
+          CAPcbgb = (cbgb - PP(cgbo));
+          CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        }
+        CAPcddb = (capbd + cgdo - (cgsb + cbsb + cdsb));
+        CAPcdsb = -(cgdb + cbdb + cddb);
+        CAPcsdb = cdsb;
+        CAPcssb = (cddb + capbs + cgso);
+        if (!irbodyMod) {
+          CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+          CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+          CAPcbdb = (cbsb - capbd);
+          CAPcbsb = (cbdb - capbs);
+          CAPcdbdb = 0.0;
+          CAPcsbsb = 0.0;
+        } else {
+          CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb) + capbd;
+          CAPcsbb = -(cddb + cdgb + cdsb);
+          CAPcbdb = cbsb;
+          CAPcbsb = cbdb;
+          CAPcdbdb = -capbd;
+          CAPcsbsb = -capbs;
+        }
+        CAPcbbb = -(CAPcbgb + CAPcbdb + CAPcbsb + CAPcbgmb);
+      } else {
+        CAPcqgb = cqgb;
+        CAPcqdb = cqsb;
+        CAPcqsb = cqdb;
+        CAPcqbb = cqbb;
+        if (irgateMod == 3) {
+          CAPcgmgmb = (cgdo + cgso + PP(cgbo));
+          CAPcgmdb = -cgdo;
+          CAPcgmsb = -cgso;
+          CAPcgmbb = -PP(cgbo);
+          CAPcdgmb = CAPcgmdb;
+          CAPcsgmb = CAPcgmsb;
+          CAPcbgmb = CAPcgmbb;
+          CAPcdgb = CAPcsgb = CAPcbgb = 0.0;
+          CAPcggb = CAPcgdb = CAPcgsb = CAPcgbb = 0.0;
+        } else {
+          CAPcggb = (cgdo + cgso + PP(cgbo));
+          CAPcgdb = -cgdo;
+          CAPcgsb = -cgso;
+          CAPcgbb = -PP(cgbo);
+          CAPcdgb = CAPcgdb;
+          CAPcsgb = CAPcgsb;
+          CAPcbgb = CAPcgbb;
+          CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        }
+        CAPcddb = (capbd + cgdo);
+        CAPcdsb = CAPcsdb = 0.0;
+        CAPcssb = (capbs + cgso);
+        if (!irbodyMod) {
+          CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdgmb);
+          CAPcsbb = -(CAPcsgb + CAPcssb + CAPcsgmb);
+          CAPcbdb = -capbd;
+          CAPcbsb = -capbs;
+          CAPcdbdb = 0.0;
+          CAPcsbsb = 0.0;
+        } else {
+          CAPcdbb = CAPcsbb = CAPcbdb = CAPcbsb = 0.0;
+          CAPcdbdb = -capbd;
+          CAPcsbsb = -capbs;
+        }
+        CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+      }
+    }
+    }
+  } else {
+    {
+      double ag0 = ss.pdt;
+      if (!(ss.dcopFlag) && ss.initTranFlag && ss.newtonIter == 0) ag0 = 0.0;
+      if (mode > 0) {
+        qdrn -= qgdo;
+        gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+        gcgdb = (cgdb - cgdo) * ag0;
+        gcgsb = (cgsb - cgso) * ag0;
+        gcgbb = -(gcggb + gcgdb + gcgsb);
+        gcdgb = (cdgb - cgdo) * ag0;
+        gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
+        gcbgb = (cbgb - PP(cgbo)) * ag0;
+        gcdgmb = gcsgmb = gcbgmb = 0.0;
+        qgb = PP(cgbo) * vgb;
+        qgate += qgdo + qgso + qgb;
+        qbulk -= qgb;
+        qsrc = -(qgate + qbulk + qdrn);
+        gcddb = (cddb + capbd + cgdo) * ag0;
+        gcdsb = cdsb * ag0;
+        gcsdb = -(cgdb + cbdb + cddb) * ag0;
+        gcssb = (capbs + cgso - (cgsb + cbsb + cdsb)) * ag0;
+        gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+        gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+        gcbdb = (cbdb - capbd) * ag0;
+        gcbsb = (cbsb - capbs) * ag0;
+        gcdbdb = 0.0;
+        gcsbsb = 0.0;
+        gcbbb = -(gcbdb + gcbgb + gcbsb + gcbgmb);
+        ggtg = ggtd = ggtb = ggts = 0.0;
+        sxpart = 0.6;
+        dxpart = 0.4;
+        ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      } else {
+        qsrc = qdrn - qgso;
+        gcggb = (cggb + cgdo + cgso + PP(cgbo)) * ag0;
+        gcgdb = (cgsb - cgdo) * ag0;
+        gcgsb = (cgdb - cgso) * ag0;
+        gcgbb = -(gcggb + gcgdb + gcgsb);
+        gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
+        gcsgb = (cdgb - cgso) * ag0;
+        gcbgb = (cbgb - PP(cgbo)) * ag0;
+        gcdgmb = gcsgmb = gcbgmb = 0.0;
+        qgb = PP(cgbo) * vgb;
+        qgate += qgdo + qgso + qgb;
+        qbulk -= qgb;
+        qdrn = -(qgate + qbulk + qsrc);
+        gcddb = (capbd + cgdo - (cgsb + cbsb + cdsb)) * ag0;
+        gcdsb = -(cgdb + cbdb + cddb) * ag0;
+        gcsdb = cdsb * ag0;
+        gcssb = (cddb + capbs + cgso) * ag0;
+        gcdbb = -(gcdgb + gcddb + gcdsb + gcdgmb);
+        gcsbb = -(gcsgb + gcsdb + gcssb + gcsgmb);
+        gcbdb = (cbsb - capbd) * ag0;
+        gcbsb = (cbdb - capbs) * ag0;
+        gcdbdb = 0.0;
+        gcsbsb = 0.0;
+        gcbbb = -(gcbgb + gcbdb + gcbsb + gcbgmb);
+        ggtg = ggtd = ggtb = ggts = 0.0;
+        sxpart = 0.4;
+        dxpart = 0.6;
+        ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+      }
+    }
+    {
+      if (mode > 0) {
+        CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+        CAPcgdb = (cgdb - cgdo);
+        CAPcgsb = (cgsb - cgso);
+        CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+        CAPcdgb = (cdgb - cgdo);
+        CAPcsgb = -(cggb + cbgb + cdgb + cgso);
+        CAPcbgb = (cbgb - PP(cgbo));
+        CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        CAPcddb = (cddb + capbd + cgdo);
+        CAPcdsb = cdsb;
+        CAPcsdb = -(cgdb + cbdb + cddb);
+        CAPcssb = (capbs + cgso - (cgsb + cbsb + cdsb));
+        CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+        CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+        CAPcbdb = (cbdb - capbd);
+        CAPcbsb = (cbsb - capbs);
+        CAPcdbdb = 0.0;
+        CAPcsbsb = 0.0;
+        CAPcbbb = -(CAPcbdb + CAPcbgb + CAPcbsb + CAPcbgmb);
+      } else {
+        CAPcggb = (cggb + cgdo + cgso + PP(cgbo));
+        CAPcgdb = (cgsb - cgdo);
+        CAPcgsb = (cgdb - cgso);
+        CAPcgbb = -(CAPcggb + CAPcgdb + CAPcgsb);
+        CAPcdgb = -(cggb + cbgb + cdgb + cgdo);
+        CAPcsgb = (cdgb - cgso);
+        CAPcbgb = (cbgb - PP(cgbo));
+        CAPcdgmb = CAPcsgmb = CAPcbgmb = 0.0;
+        CAPcddb = (capbd + cgdo - (cgsb + cbsb + cdsb));
+        CAPcdsb = -(cgdb + cbdb + cddb);
+        CAPcsdb = cdsb;
+        CAPcssb = (cddb + capbs + cgso);
+        CAPcdbb = -(CAPcdgb + CAPcddb + CAPcdsb + CAPcdgmb);
+        CAPcsbb = -(CAPcsgb + CAPcsdb + CAPcssb + CAPcsgmb);
+        CAPcbdb = (cbsb - capbd);
+        CAPcbsb = (cbdb - capbs);
+        CAPcdbdb = 0.0;
+        CAPcsbsb = 0.0;
+        CAPcbbb = -(CAPcbgb + CAPcbdb + CAPcbsb + CAPcbgmb);
+      }
     }
   }
-  Idrain = IV(drainConductance) * Vddp;
-  Isource = IV(sourceConductance) * Vssp;
+  if (X && rdsMod == 1) {
+    Idrain = gdtot * Vddp;
+    Isource = gstot * Vssp;
+  } else {
+    Idrain = IV(drainConductance) * Vddp;
+    Isource = IV(sourceConductance) * Vssp;
+  }
+  /* body network currents: the reference tests the model's rbodyMod here */
+  if (X && ((xf >> XB_RBODYM) & 1) != 0) {
+    Idbb = XP(grbdb) * Vdbb;
+    Idbbp = XP(grbpd) * Vdbbp;
+    Isbb = XP(grbsb) * Vsbb;
+    Isbbp = XP(grbps) * Vsbbp;
+    Ibpb = XP(grbpb) * Vbpb;
+  } else {
+    Idbb = Idbbp = Isbb = Isbbp = Ibpb = 0.0;
+  }
 
   // ---- Master::updateState: stored voltages and charges ----
   {
@@ -2689,7 +4298,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   }
   qg = qgate;
   qd = qdrn - qbd;
-  qb = qbulk + qbd + qbs;
+  qb = (X && irbodyMod) ? qbulk : qbulk + qbd + qbs;
   {
     const int32_t lqb = stalid(a, A_qb, i), lqg = stalid(a, A_qg, i), lqd = stalid(a, A_qd, i);
     if (double* sta = a.v.nextSta) {
@@ -2701,7 +4310,26 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
       if (double* cs = a.v.currSta) {
         if (lqg >= 0) cs[lqg] = qgate;
         if (lqd >= 0) cs[lqd] = qdrn - qbd;
-        if (lqb >= 0) cs[lqb] = qbulk + qbd + qbs;
+        if (lqb >= 0) cs[lqb] = (X && irbodyMod) ? qbulk : qbulk + qbd + qbs;
+      }
+    }
+    if (X) {
+      // the option state slots follow qb, qg, qd in registration order
+      int k = NSTA_BASE;
+      const int32_t lqgmid = irgateMod == 3 ? stalid(a, k++, i) : -1;
+      const int32_t lqbs = irbodyMod ? stalid(a, k++, i) : -1;
+      const int32_t lqbd = irbodyMod ? stalid(a, k++, i) : -1;
+      const int32_t lqcheq = itrnqsMod ? stalid(a, k++, i) : -1;
+      const int32_t lqcdump = itrnqsMod ? stalid(a, k++, i) : -1;
+      const bool first_tran = !ss.dcopFlag && ss.initTranFlag && ss.newtonIter == 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        double* v = pass == 0 ? a.v.nextSta : (first_tran ? a.v.currSta : nullptr);
+        if (!v) continue;
+        if (lqgmid >= 0) v[lqgmid] = qgmid;
+        if (lqbs >= 0) v[lqbs] = qbs;
+        if (lqbd >= 0) v[lqbd] = qbd;
+        if (lqcheq >= 0) v[lqcheq] = qcheq;
+        if (lqcdump >= 0) v[lqcdump] = qdef * kScalingFactor;
       }
     }
   }
@@ -2716,6 +4344,608 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   // ---- Master::loadDAEVectors ----
   Sink<MODE> s(a, i);
   const double Mp = IV(numberParallel);
+  if constexpr (X) {
+    // setupFVectorVars (:13525-13838)
+    {
+
+      const double dt = C.dtype;
+      double tmp_gcrgd;
+      T0 = 0.0;
+      ceqdrn_Jdxp = 0.0; ceqbd_Jdxp = 0.0; ceqbs_Jdxp = 0.0;
+      Istoteq_Jdxp = 0.0; Idtoteq_Jdxp = 0.0;
+      Ibtoteq_Jdxp = 0.0; Igtoteq_Jdxp = 0.0;
+      ceqgcrg_Jdxp = 0.0; ceqgstot_Jdxp = 0.0; ceqgdtot_Jdxp = 0.0;
+      ceqjs_Jdxp = 0.0; ceqjd_Jdxp = 0.0;
+      if (mode >= 0) {
+        Gm = gm;
+        Gmbs = gmbs;
+        FwdSum = Gm + Gmbs;
+        RevSum = 0.0;
+        ceqdrn = dt * cdrain;
+        ceqdrn_Jdxp = dt * (-gds * (vds - vds_orig) - Gm * (vgs - vgs_orig)
+                               - Gmbs * (vbs - vbs_orig));
+        ceqbd = dt * (csub + Igidl);
+        ceqbd_Jdxp = dt * (-(gbds + ggidld) * (vds - vds_orig)
+                              - (gbgs + ggidlg) * (vgs - vgs_orig)
+                              - (gbbs + ggidlb) * (vbs - vbs_orig));
+        ceqbs = dt * Igisl;
+        ceqbs_Jdxp = dt * (+ggisls * (vds - vds_orig) - ggislg * (vgd - vgd_orig)
+                              - ggislb * (vbd - vbd_orig));
+        gbbdp = -(gbds);
+        gbbsp = gbds + gbgs + gbbs;
+        gbdpg = gbgs;
+        gbdpdp = gbds;
+        gbdpb = gbbs;
+        gbdpsp = -(gbdpg + gbdpdp + gbdpb);
+        gbspg = 0.0;
+        gbspdp = 0.0;
+        gbspb = 0.0;
+        gbspsp = 0.0;
+        if (igcMod) {
+          gIstotg = gIgsg + gIgcsg;
+          gIstotd = gIgcsd;
+          gIstots = gIgss + gIgcss;
+          gIstotb = gIgcsb;
+          Istoteq = dt * (Igs + Igcs);
+          Istoteq_Jdxp = dt * (-gIstotg * (vgs - vgs_orig) - gIgcsd * (vds - vds_orig)
+                                  - gIgcsb * (vbs - vbs_orig));
+          gIdtotg = gIgdg + gIgcdg;
+          gIdtotd = gIgdd + gIgcdd;
+          gIdtots = gIgcds;
+          gIdtotb = gIgcdb;
+          Idtoteq = dt * (Igd + Igcd);
+          Idtoteq_Jdxp = dt * (-gIgdg * (vgd - vgd_orig) - gIgcdg * (vgs - vgs_orig)
+                                  - gIgcdd * (vds - vds_orig) - gIgcdb * (vbs - vbs_orig));
+        } else {
+          gIstotg = gIstotd = gIstots = gIstotb = Istoteq = 0.0;
+          gIdtotg = gIdtotd = gIdtots = gIdtotb = Idtoteq = 0.0;
+          Istoteq_Jdxp = 0.0;
+          Idtoteq_Jdxp = 0.0;
+        }
+        if (igbMod) {
+          gIbtotg = gIgbg;
+          gIbtotd = gIgbd;
+          gIbtots = gIgbs;
+          gIbtotb = gIgbb;
+          Ibtoteq = dt * Igb;
+          Ibtoteq_Jdxp = dt * (-gIgbg * (vgs - vgs_orig) - gIgbd * (vds - vds_orig)
+                                  - gIgbb * (vbs - vbs_orig));
+        } else {
+          gIbtotg = gIbtotd = gIbtots = gIbtotb = Ibtoteq = 0.0;
+          Ibtoteq_Jdxp = 0.0;
+        }
+        if ((igcMod != 0) || (igbMod != 0)) {
+          gIgtotg = gIstotg + gIdtotg + gIbtotg;
+          gIgtotd = gIstotd + gIdtotd + gIbtotd;
+          gIgtots = gIstots + gIdtots + gIbtots;
+          gIgtotb = gIstotb + gIdtotb + gIbtotb;
+          Igtoteq = Istoteq + Idtoteq + Ibtoteq;
+          Igtoteq_Jdxp = Istoteq_Jdxp + Idtoteq_Jdxp + Ibtoteq_Jdxp;
+        } else {
+          gIgtotg = gIgtotd = gIgtots = gIgtotb = Igtoteq = 0.0;
+          Igtoteq_Jdxp = 0.0;
+        }
+        if (irgateMod == 2) T0 = vges - vgs;
+        else if (irgateMod == 3) T0 = vgms - vgs;
+        if (irgateMod > 1) {
+          gcrgd = gcrgd * T0;
+          gcrgg = gcrgg * T0;
+          gcrgs = gcrgs * T0;
+          gcrgb = gcrgb * T0;
+          ceqgcrg = 0.0;
+          ceqgcrg_Jdxp = -(gcrgd * (vds - vds_orig) + gcrgg * (vgs - vgs_orig)
+                              + gcrgb * (vbs - vbs_orig));
+          gcrgg -= gcrg;
+        } else {
+          ceqgcrg = gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+          ceqgcrg_Jdxp = 0.0;
+        }
+      } else {
+        Gm = -gm;
+        Gmbs = -gmbs;
+        FwdSum = 0.0;
+        RevSum = -(Gm + Gmbs);
+        ceqdrn = -dt * cdrain;
+        ceqdrn_Jdxp = -dt * (+gds * (vds - vds_orig) + Gm * (vgd - vgd_orig)
+                                + Gmbs * (vbd - vbd_orig));
+        ceqbs = dt * (csub + Igisl);
+        ceqbs_Jdxp = dt * (+(gbds + ggisls) * (vds - vds_orig)
+                              - (gbgs + ggislg) * (vgd - vgd_orig)
+                              - (gbbs + ggislb) * (vbd - vbd_orig));
+        ceqbd = dt * Igidl;
+        ceqbd_Jdxp = dt * (-ggidld * (vds - vds_orig) - ggidlg * (vgs - vgs_orig)
+                              - ggidlb * (vbs - vbs_orig));
+        gbbsp = -(gbds);
+        gbbdp = gbds + gbgs + gbbs;
+        gbdpg = 0.0;
+        gbdpsp = 0.0;
+        gbdpb = 0.0;
+        gbdpdp = 0.0;
+        gbspg = gbgs;
+        gbspsp = gbds;
+        gbspb = gbbs;
+        gbspdp = -(gbspg + gbspsp + gbspb);
+        if (igcMod) {
+          gIstotg = gIgsg + gIgcdg;
+          gIstotd = gIgcds;
+          gIstots = gIgss + gIgcdd;
+          gIstotb = gIgcdb;
+          Istoteq = dt * (Igs + Igcd);
+          Istoteq_Jdxp = dt * (-gIgsg * (vgs - vgs_orig) - gIgcdg * (vgd - vgd_orig)
+                                  + gIgcdd * (vds - vds_orig) - gIgcdb * (vbd - vbd_orig));
+          gIdtotg = gIgdg + gIgcsg;
+          gIdtotd = gIgdd + gIgcss;
+          gIdtots = gIgcsd;
+          gIdtotb = gIgcsb;
+          Idtoteq = dt * (Igd + Igcs);
+          Idtoteq_Jdxp = dt * (-(gIgdg + gIgcsg) * (vgd - vgd_orig)
+                                  + gIgcsd * (vds - vds_orig) - gIgcsb * (vbd - vbd_orig));
+        } else {
+          gIstotg = gIstotd = gIstots = gIstotb = Istoteq = 0.0;
+          gIdtotg = gIdtotd = gIdtots = gIdtotb = Idtoteq = 0.0;
+          Istoteq_Jdxp = 0.0;
+          Idtoteq_Jdxp = 0.0;
+        }
+        if (igbMod) {
+          gIbtotg = gIgbg;
+          gIbtotd = gIgbs;
+          gIbtots = gIgbd;
+          gIbtotb = gIgbb;
+          Ibtoteq = dt * Igb;
+          Ibtoteq_Jdxp = dt * (-gIgbg * (vgd - vgd_orig) + gIgbd * (vds - vds_orig)
+                                  - gIgbb * (vbd - vbd_orig));
+        } else {
+          gIbtotg = gIbtotd = gIbtots = gIbtotb = Ibtoteq = 0.0;
+          Ibtoteq_Jdxp = 0.0;
+        }
+        if ((igcMod != 0) || (igbMod != 0)) {
+          gIgtotg = gIstotg + gIdtotg + gIbtotg;
+          gIgtotd = gIstotd + gIdtotd + gIbtotd;
+          gIgtots = gIstots + gIdtots + gIbtots;
+          gIgtotb = gIstotb + gIdtotb + gIbtotb;
+          Igtoteq = Istoteq + Idtoteq + Ibtoteq;
+          Igtoteq_Jdxp = Istoteq_Jdxp + Idtoteq_Jdxp + Ibtoteq_Jdxp;
+        } else {
+          gIgtotg = gIgtotd = gIgtots = gIgtotb = Igtoteq = 0.0;
+          Igtoteq_Jdxp = 0.0;
+        }
+        if (irgateMod == 2) T0 = vges - vgs;
+        else if (irgateMod == 3) T0 = vgms - vgs;
+        if (irgateMod > 1) {
+          tmp_gcrgd = gcrgd;
+          gcrgd = gcrgs * T0;
+          gcrgg = gcrgg * T0;
+          gcrgs = tmp_gcrgd * T0;
+          gcrgb = gcrgb * T0;
+          ceqgcrg = 0.0;
+          ceqgcrg_Jdxp = -(gcrgg * (vgd - vgd_orig) - gcrgs * (vds - vds_orig)
+                              + gcrgb * (vbd - vbd_orig));
+          gcrgg -= gcrg;
+        } else {
+          /* the reverse branch leaves ceqgcrg_Jdxp at its zero */
+          ceqgcrg = gcrg = gcrgd = gcrgg = gcrgs = gcrgb = 0.0;
+        }
+      }
+      if (rdsMod == 1) {
+        ceqgstot = 0.0;
+        ceqgstot_Jdxp = dt * (gstotd * (vds - vds_orig) + gstotg * (vgs - vgs_orig)
+                                 + gstotb * (vbs - vbs_orig));
+        gstots = gstots - gstot;
+        ceqgdtot = 0.0;
+        ceqgdtot_Jdxp = -dt * (gdtotd * (vds - vds_orig) + gdtotg * (vgs - vgs_orig)
+                                  + gdtotb * (vbs - vbs_orig));
+        gdtotd = gdtotd - gdtot;
+      } else {
+        gstot = gstotd = gstotg = gstots = gstotb = ceqgstot = 0.0;
+        gdtot = gdtotd = gdtotg = gdtots = gdtotb = ceqgdtot = 0.0;
+        ceqgstot_Jdxp = 0.0;
+        ceqgdtot_Jdxp = 0.0;
+      }
+      if (C.dtype > 0) {
+        ceqjs = (cbs);
+        ceqjs_Jdxp = (-gbs * (vbs_jct - vbs_jct_orig));
+        ceqjd = (cbd);
+        ceqjd_Jdxp = (-gbd * (vbd_jct - vbd_jct_orig));
+      } else {
+        ceqjs = -(cbs);
+        ceqjs_Jdxp = (gbs * (vbs_jct - vbs_jct_orig));
+        ceqjd = -(cbd);
+        ceqjd_Jdxp = (gbd * (vbd_jct - vbd_jct_orig));
+        ceqgcrg = -ceqgcrg;
+        ceqgcrg_Jdxp = -ceqgcrg_Jdxp;
+      }
+    }
+    const bool dc = ss.dcopFlag != 0;
+    s.put(K_F, 0, -(ceqjd - ceqbd - ceqdrn + Idtoteq) * Mp);
+    s.put(K_F, 1, -(-(-ceqgcrg + Igtoteq) * Mp));
+    if (irgateMod == 1) {
+      s.put(K_F, 2, -(-(Igate) * Mp));
+      s.put(K_F, 4, -(Igate) * Mp);
+    } else if (irgateMod == 2) {
+      s.put(K_F, 2, -(-(Igate + ceqgcrg) * Mp));
+      s.put(K_F, 4, -(Igate) * Mp);
+    } else if (irgateMod == 3) {
+      s.put(K_F, 2, -(-(Igate) * Mp));
+      s.put(K_F, 3, -(-(IgateMid - Igate + ceqgcrg) * Mp));
+      s.put(K_F, 4, -(IgateMid) * Mp);
+    }
+    if (!irbodyMod) {
+      s.put(K_F, 6, -(ceqbd + ceqbs - ceqjd - ceqjs + Ibtoteq) * Mp);
+      s.put(K_F, 9, -(ceqdrn - ceqbs + ceqjs + Istoteq) * Mp);
+    } else {
+      s.put(K_F, 5, -(-(ceqjd + Idbb + Idbbp) * Mp));
+      s.put(K_F, 6, -(ceqbd + ceqbs + Ibtoteq + Idbbp + Isbbp - Ibpb) * Mp);
+      s.put(K_F, 7, -(Isbb + Idbb + Ibpb) * Mp);
+      s.put(K_F, 8, -(-(ceqjs + Isbb + Isbbp) * Mp));
+      s.put(K_F, 9, -(ceqdrn - ceqbs + ceqjs + Istoteq) * Mp);
+    }
+    if (rdsMod) {
+      s.put(K_F, 10, -(-ceqgdtot) * Mp);
+      s.put(K_F, 11, -(ceqgstot) * Mp);
+      s.put(K_F, 12, -(ceqgdtot) * Mp);
+      s.put(K_F, 13, -(-ceqgstot) * Mp);
+    }
+    if (ifld(a, F_DEXIST, i)) {
+      s.put(K_F, 14, -(-Idrain) * Mp);
+      s.put(K_F, 15, -(Idrain) * Mp);
+    }
+    if (ifld(a, F_SEXIST, i)) {
+      s.put(K_F, 16, -(-Isource) * Mp);
+      s.put(K_F, 17, -(+Isource) * Mp);
+    }
+    // initial conditions: the branch equations in DC (:16382-16411); other
+    // calls load nothing there (-0.0 leaves every sum unchanged)
+    {
+      const int32_t lb[3] = {node(a, N_IBS, i), node(a, N_IDS, i), node(a, N_IGS, i)};
+      const int32_t lp[3] = {li_Body, li_Drain, li_GateExt};
+      const double icv[3] = {XP(icVBS), XP(icVDS), XP(icVGS)};
+      const int on[3] = {(xf >> XB_ICVBS) & 1, (xf >> XB_ICVDS) & 1, (xf >> XB_ICVGS) & 1};
+      for (int k = 0; k < 3; ++k) {
+        if (!on[k]) continue;
+        const double coef = solv(a.v.nextSol, lb[k]);
+        const double br = solv(a.v.nextSol, lp[k]) - solv(a.v.nextSol, li_Source) - icv[k];
+        s.put(K_F, 18 + 3 * k, dc ? coef : -0.0);
+        s.put(K_F, 19 + 3 * k, dc ? -coef : -0.0);
+        s.put(K_F, 20 + 3 * k, dc ? br : -0.0);
+      }
+    }
+    if (opt.voltageLimiterFlag) {
+      const bool lim = !origFlag;
+      const double z = -0.0;
+      s.put(K_FLIM, 0, lim ? (ceqjd_Jdxp - ceqbd_Jdxp - ceqdrn_Jdxp + Idtoteq_Jdxp) * Mp : z);
+      s.put(K_FLIM, 1, lim ? -((-ceqgcrg_Jdxp + Igtoteq_Jdxp) * Mp) : z);
+      if (irgateMod == 1) {
+        s.put(K_FLIM, 2, lim ? -((Igate_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 4, lim ? (Igate_Jdxp) * Mp : z);
+      } else if (irgateMod == 2) {
+        s.put(K_FLIM, 2, lim ? -((Igate_Jdxp + ceqgcrg_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 4, lim ? (Igate_Jdxp) * Mp : z);
+      } else if (irgateMod == 3) {
+        s.put(K_FLIM, 2, lim ? -((Igate_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 3, lim ? -((IgateMid_Jdxp - Igate_Jdxp + ceqgcrg_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 4, lim ? (IgateMid_Jdxp) * Mp : z);
+      }
+      if (!irbodyMod) {
+        s.put(K_FLIM, 6, lim ? (ceqbd_Jdxp + ceqbs_Jdxp - ceqjd_Jdxp - ceqjs_Jdxp + Ibtoteq_Jdxp) * Mp : z);
+        s.put(K_FLIM, 8, lim ? (ceqdrn_Jdxp - ceqbs_Jdxp + ceqjs_Jdxp + Istoteq_Jdxp) * Mp : z);
+      } else {
+        s.put(K_FLIM, 5, lim ? -((ceqjd_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 6, lim ? (ceqbd_Jdxp + ceqbs_Jdxp + Ibtoteq_Jdxp) * Mp : z);
+        s.put(K_FLIM, 7, lim ? -((ceqjs_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 8, lim ? (ceqdrn_Jdxp - ceqbs_Jdxp + ceqjs_Jdxp + Istoteq_Jdxp) * Mp : z);
+      }
+      if (rdsMod) {
+        s.put(K_FLIM, 9, lim ? -((ceqgdtot_Jdxp) * Mp) : z);
+        s.put(K_FLIM, 10, lim ? (ceqgstot_Jdxp) * Mp : z);
+        s.put(K_FLIM, 11, lim ? (ceqgdtot_Jdxp) * Mp : z);
+        s.put(K_FLIM, 12, lim ? -((ceqgstot_Jdxp) * Mp) : z);
+      }
+    }
+    // auxChargeCalculations (:12216-12347)
+    {
+
+      if (!ChargeComputationNeeded) {
+        sxpart = (1.0 - (dxpart = (mode > 0) ? 0.4 : 0.6));
+        ddxpart_dVd = ddxpart_dVg = ddxpart_dVb = ddxpart_dVs = 0.0;
+        dsxpart_dVd = dsxpart_dVg = dsxpart_dVb = dsxpart_dVs = 0.0;
+        Qeqqg_Jdxp = 0.0;
+        Qeqqd_Jdxp = 0.0;
+        Qeqqb_Jdxp = 0.0;
+        Qeqqgmid_Jdxp = 0.0;
+        Qeqqjs_Jdxp = 0.0;
+        Qeqqjd_Jdxp = 0.0;
+        if (itrnqsMod) {
+          CoxWL = C.coxe * PP(weffCV) * IV(nf) * PP(leffCV);
+          T1 = gcrg / CoxWL;
+          gtau = T1 * kScalingFactor;
+        } else {
+          gtau = 0.0;
+        }
+      } else {
+        Qeqqg_Jdxp = 0.0;
+        if (!origFlag)
+          Qeqqg_Jdxp = -CAPcggb * (vgb - vgb_orig) + CAPcgdb * (vbd - vbd_orig)
+                          + CAPcgsb * (vbs - vbs_orig);
+        Qeqqd_Jdxp = 0.0;
+        if (!origFlag)
+          Qeqqd_Jdxp = -CAPcdgb * (vgb - vgb_orig) - CAPcdgmb * (vgmb - vgmb_orig)
+                          + (CAPcddb + CAPcdbdb) * (vbd - vbd_orig)
+                          - CAPcdbdb * (vbd_jct - vbd_jct_orig) + CAPcdsb * (vbs - vbs_orig);
+        Qeqqb_Jdxp = 0.0;
+        if (!origFlag)
+          Qeqqb_Jdxp = -CAPcbgb * (vgb - vgb_orig) - CAPcbgmb * (vgmb - vgmb_orig)
+                          + CAPcbdb * (vbd - vbd_orig) + CAPcbsb * (vbs - vbs_orig);
+        if (irgateMod == 3) {
+          Qeqqgmid_Jdxp = 0.0;
+          if (!origFlag)
+            Qeqqgmid_Jdxp = +CAPcgmdb * (vbd - vbd_orig) + CAPcgmsb * (vbs - vbs_orig)
+                               - CAPcgmgmb * (vgmb - vgmb_orig);
+        } else {
+          Qeqqgmid_Jdxp = 0.0;
+        }
+        if (irbodyMod) {
+          Qeqqjs_Jdxp = 0.0;
+          Qeqqjd_Jdxp = 0.0;
+          if (!origFlag) {
+            Qeqqjs_Jdxp = CAPcsbsb * (vbs_jct - vbs_jct_orig);
+            Qeqqjd_Jdxp = CAPcdbdb * (vbd_jct - vbd_jct_orig);
+          }
+        }
+        if (itrnqsMod) {
+          T0 = ggtg * (vgb - vgb_orig) - ggtd * (vbd - vbd_orig)
+               - ggts * (vbs - vbs_orig);
+          Qeqqg_Jdxp += T0;
+          T1 = qdef * gtau;
+          Qeqqd_Jdxp -= dxpart * T0 + T1 * (ddxpart_dVg * (vgb - vgb_orig)
+                                                  - ddxpart_dVd * (vbd - vbd_orig)
+                                                  - ddxpart_dVs * (vbs - vbs_orig));
+          Qqcheq_Jdxp = -(CAPcqgb * (vgb - vgb_orig) - CAPcqdb * (vbd - vbd_orig)
+                             - CAPcqsb * (vbs - vbs_orig)) + T0;
+        }
+      }
+    }
+    double Qeqqg, Qeqqb, Qeqqd, Qeqqgmid = 0.0, Qeqqjs = 0.0, Qeqqjd = 0.0, Qqdef = 0.0, Qqcheq = 0.0;
+    if (C.dtype > 0) {
+      Qeqqg = qg;
+      Qeqqd = qd;
+      Qeqqb = qb;
+      if (itrnqsMod) {
+        Qqdef = qdef;
+        Qqcheq = qcheq;
+      }
+      if (irbodyMod) {
+        Qeqqjs = qbs;
+        Qeqqjd = qbd;
+      }
+      if (irgateMod == 3) Qeqqgmid = qgmid;
+    } else {
+      Qeqqg = -qg;
+      Qeqqd = -qd;
+      Qeqqb = -qb;
+      if (itrnqsMod) {
+        Qqdef = -qdef;
+        Qqcheq = -qcheq;
+      }
+      if (irbodyMod) {
+        Qeqqjs = -qbs;
+        Qeqqjd = -qbd;
+      }
+      if (irgateMod == 3) Qeqqgmid = -qgmid;
+    }
+    s.put(K_Q, 0, -(-Qeqqd) * Mp);
+    s.put(K_Q, 1, -(-(Qeqqg) * Mp));
+    if (irgateMod == 3) s.put(K_Q, 2, -(-(+Qeqqgmid) * Mp));
+    if (!irbodyMod) {
+      s.put(K_Q, 4, -(-Qeqqb) * Mp);
+      s.put(K_Q, 6, -(+Qeqqg + Qeqqb + Qeqqd + Qeqqgmid) * Mp);
+    } else {
+      s.put(K_Q, 3, -(-(Qeqqjd) * Mp));
+      s.put(K_Q, 4, -(-Qeqqb) * Mp);
+      s.put(K_Q, 5, -(-(Qeqqjs) * Mp));
+      s.put(K_Q, 6, -(Qeqqd + Qeqqg + Qeqqb + Qeqqjd + Qeqqjs + Qeqqgmid) * Mp);
+    }
+    if (itrnqsMod) s.put(K_Q, 7, -(Qqcheq - Qqdef) * Mp);
+    if (opt.voltageLimiterFlag) {
+      const bool lim = !origFlag;
+      const double z = -0.0;
+      s.put(K_QLIM, 0, lim ? (-Qeqqd_Jdxp) * Mp : z);
+      s.put(K_QLIM, 1, lim ? -((Qeqqg_Jdxp) * Mp) : z);
+      if (irgateMod == 3) s.put(K_QLIM, 2, lim ? -((+Qeqqgmid_Jdxp) * Mp) : z);
+      if (!irbodyMod) {
+        s.put(K_QLIM, 4, lim ? (-Qeqqb_Jdxp) * Mp : z);
+        s.put(K_QLIM, 6, lim ? (+Qeqqg_Jdxp + Qeqqb_Jdxp + Qeqqd_Jdxp + Qeqqgmid_Jdxp) * Mp : z);
+      } else {
+        s.put(K_QLIM, 3, lim ? -((Qeqqjd_Jdxp) * Mp) : z);
+        s.put(K_QLIM, 4, lim ? (-Qeqqb_Jdxp) * Mp : z);
+        s.put(K_QLIM, 5, lim ? -((+Qeqqjs_Jdxp) * Mp) : z);
+        s.put(K_QLIM, 6, lim ? (+Qeqqd_Jdxp + Qeqqg_Jdxp + Qeqqb_Jdxp + Qeqqjd_Jdxp + Qeqqjs_Jdxp
+                                + Qeqqgmid_Jdxp) * Mp : z);
+      }
+      if (itrnqsMod) s.put(K_QLIM, 7, lim ? (Qqcheq_Jdxp) * Mp : z);
+    }
+
+    // ---- Master::loadDAEMatrices (option stamp lists) ----
+    if (!irbodyMod) {
+      gjbd = gbd;
+      gjbs = gbs;
+    } else {
+      gjbd = gjbs = 0.0;
+    }
+    if (!rdsMod) {
+      gdpr = IV(drainConductance);
+      gspr = IV(sourceConductance);
+    } else {
+      gdpr = gspr = 0.0;
+    }
+    geltd = XP(grgeltd);
+    T1 = qdef * gtau;
+    if (irgateMod == 1) {
+      s.put(K_DFDX, 0, (geltd) * Mp);
+      s.put(K_DFDX, 1, -((geltd) * Mp));
+      s.put(K_DFDX, 12, -((geltd) * Mp));
+      s.put(K_DFDX, 14, (+geltd - ggtg + gIgtotg) * Mp);
+      s.put(K_DFDX, 15, (-ggtd + gIgtotd) * Mp);
+      s.put(K_DFDX, 16, (-ggts + gIgtots) * Mp);
+      s.put(K_DFDX, 17, (-ggtb + gIgtotb) * Mp);
+    } else if (irgateMod == 2) {
+      s.put(K_DFDX, 0, (gcrg) * Mp);
+      s.put(K_DFDX, 1, (gcrgg) * Mp);
+      s.put(K_DFDX, 3, (gcrgd) * Mp);
+      s.put(K_DFDX, 4, (gcrgs) * Mp);
+      s.put(K_DFDX, 5, (gcrgb) * Mp);
+      s.put(K_DFDX, 12, -((gcrg) * Mp));
+      s.put(K_DFDX, 14, (-gcrgg - ggtg + gIgtotg) * Mp);
+      s.put(K_DFDX, 15, (-gcrgd - ggtd + gIgtotd) * Mp);
+      s.put(K_DFDX, 16, (-gcrgs - ggts + gIgtots) * Mp);
+      s.put(K_DFDX, 17, (-gcrgb - ggtb + gIgtotb) * Mp);
+    } else if (irgateMod == 3) {
+      s.put(K_DFDX, 0, (geltd) * Mp);
+      s.put(K_DFDX, 2, -((geltd) * Mp));
+      s.put(K_DFDX, 6, -((geltd) * Mp));
+      s.put(K_DFDX, 7, (geltd + gcrg) * Mp);
+      s.put(K_DFDX, 8, (gcrgd) * Mp);
+      s.put(K_DFDX, 9, (gcrgg) * Mp);
+      s.put(K_DFDX, 10, (gcrgs) * Mp);
+      s.put(K_DFDX, 11, (gcrgb) * Mp);
+      s.put(K_DFDX, 13, -((gcrg) * Mp));
+      s.put(K_DFDX, 14, (-gcrgg - ggtg + gIgtotg) * Mp);
+      s.put(K_DFDX, 15, (-gcrgd - ggtd + gIgtotd) * Mp);
+      s.put(K_DFDX, 16, (-gcrgs - ggts + gIgtots) * Mp);
+      s.put(K_DFDX, 17, (-gcrgb - ggtb + gIgtotb) * Mp);
+    } else {
+      s.put(K_DFDX, 14, (-ggtg + gIgtotg) * Mp);
+      s.put(K_DFDX, 15, (-ggtd + gIgtotd) * Mp);
+      s.put(K_DFDX, 16, (-ggts + gIgtots) * Mp);
+      s.put(K_DFDX, 17, (-ggtb + gIgtotb) * Mp);
+    }
+    if (rdsMod) {
+      s.put(K_DFDX, 18, (gdtotg) * Mp);
+      s.put(K_DFDX, 19, (gdtots) * Mp);
+      s.put(K_DFDX, 20, (gdtotb) * Mp);
+      s.put(K_DFDX, 21, (gstotd) * Mp);
+      s.put(K_DFDX, 22, (gstotg) * Mp);
+      s.put(K_DFDX, 23, (gstotb) * Mp);
+    }
+    s.put(K_DFDX, 24, (gdpr + gds + gbd + T1 * ddxpart_dVd - gdtotd + RevSum + gbdpdp + dxpart * ggtd - gIdtotd) * Mp);
+    s.put(K_DFDX, 25, -((gdpr + gdtot) * Mp));
+    s.put(K_DFDX, 26, (Gm - gdtotg + gbdpg - gIdtotg + dxpart * ggtg + T1 * ddxpart_dVg) * Mp);
+    s.put(K_DFDX, 27, -((gds + gdtots - dxpart * ggts + gIdtots - T1 * ddxpart_dVs + FwdSum - gbdpsp) * Mp));
+    s.put(K_DFDX, 28, -((gjbd + gdtotb - Gmbs - gbdpb + gIdtotb - T1 * ddxpart_dVb - dxpart * ggtb) * Mp));
+    s.put(K_DFDX, 29, -((gdpr - gdtotd) * Mp));
+    s.put(K_DFDX, 30, (gdpr + gdtot) * Mp);
+    s.put(K_DFDX, 31, -((gds + gstotd + RevSum - gbspdp - T1 * dsxpart_dVd - sxpart * ggtd + gIstotd) * Mp));
+    s.put(K_DFDX, 32, (-Gm - gstotg + gbspg + sxpart * ggtg + T1 * dsxpart_dVg - gIstotg) * Mp);
+    s.put(K_DFDX, 33, (gspr + gds + gbs + T1 * dsxpart_dVs - gstots + FwdSum + gbspsp + sxpart * ggts - gIstots) * Mp);
+    s.put(K_DFDX, 34, -((gspr + gstot) * Mp));
+    s.put(K_DFDX, 35, -((gjbs + gstotb + Gmbs - gbspb - sxpart * ggtb - T1 * dsxpart_dVb + gIstotb) * Mp));
+    s.put(K_DFDX, 36, -((gspr - gstots) * Mp));
+    s.put(K_DFDX, 37, (gspr + gstot) * Mp);
+    s.put(K_DFDX, 38, (-gjbd + gbbdp - gIbtotd) * Mp);
+    s.put(K_DFDX, 39, (-gbgs - gIbtotg) * Mp);
+    s.put(K_DFDX, 40, (-gjbs + gbbsp - gIbtots) * Mp);
+    s.put(K_DFDX, 41, (gjbd + gjbs - gbbs - gIbtotb) * Mp);
+    s.put(K_DFDX, 42, (ggidld) * Mp);
+    s.put(K_DFDX, 43, (ggidlg) * Mp);
+    s.put(K_DFDX, 44, -(((ggidlg + ggidld + ggidlb)) * Mp));
+    s.put(K_DFDX, 45, (ggidlb) * Mp);
+    s.put(K_DFDX, 46, -((ggidld) * Mp));
+    s.put(K_DFDX, 47, -((ggidlg) * Mp));
+    s.put(K_DFDX, 48, ((ggidlg + ggidld + ggidlb)) * Mp);
+    s.put(K_DFDX, 49, -((ggidlb) * Mp));
+    s.put(K_DFDX, 50, -(((ggisls + ggislg + ggislb)) * Mp));
+    s.put(K_DFDX, 51, (ggislg) * Mp);
+    s.put(K_DFDX, 52, (ggisls) * Mp);
+    s.put(K_DFDX, 53, (ggislb) * Mp);
+    s.put(K_DFDX, 54, ((ggislg + ggisls + ggislb)) * Mp);
+    s.put(K_DFDX, 55, -((ggislg) * Mp));
+    s.put(K_DFDX, 56, -((ggisls) * Mp));
+    s.put(K_DFDX, 57, -((ggislb) * Mp));
+    if (irbodyMod) {
+      s.put(K_DFDX, 58, (-gbd) * Mp);
+      s.put(K_DFDX, 59, -((gbs) * Mp));
+      s.put(K_DFDX, 60, (-gbd) * Mp);
+      s.put(K_DFDX, 61, (gbd + XP(grbpd) + XP(grbdb)) * Mp);
+      s.put(K_DFDX, 62, -((XP(grbpd)) * Mp));
+      s.put(K_DFDX, 63, -((XP(grbdb)) * Mp));
+      s.put(K_DFDX, 64, -((XP(grbpd)) * Mp));
+      s.put(K_DFDX, 65, -((XP(grbpb)) * Mp));
+      s.put(K_DFDX, 66, -((XP(grbps)) * Mp));
+      s.put(K_DFDX, 67, (XP(grbpd) + XP(grbps) + XP(grbpb)) * Mp);
+      s.put(K_DFDX, 68, (-gbs) * Mp);
+      s.put(K_DFDX, 69, -((XP(grbps)) * Mp));
+      s.put(K_DFDX, 70, -((XP(grbsb)) * Mp));
+      s.put(K_DFDX, 71, (gbs + XP(grbps) + XP(grbsb)) * Mp);
+      s.put(K_DFDX, 72, -((XP(grbdb)) * Mp));
+      s.put(K_DFDX, 73, -((XP(grbpb)) * Mp));
+      s.put(K_DFDX, 74, -((XP(grbsb)) * Mp));
+      s.put(K_DFDX, 75, (XP(grbsb) + XP(grbdb) + XP(grbpb)) * Mp);
+    }
+    if (itrnqsMod) {
+      s.put(K_DFDX, 76, (gqdef + gtau) * Mp);
+      s.put(K_DFDX, 77, (ggtg) * Mp);
+      s.put(K_DFDX, 78, (ggtd) * Mp);
+      s.put(K_DFDX, 79, (ggts) * Mp);
+      s.put(K_DFDX, 80, (ggtb) * Mp);
+      s.put(K_DFDX, 81, (dxpart * gtau) * Mp);
+      s.put(K_DFDX, 82, (sxpart * gtau) * Mp);
+      s.put(K_DFDX, 83, -((gtau) * Mp));
+    }
+    // initial conditions (:16840-16887): the DC form adds the branch
+    // incidences, the transient form puts 1 on the branch diagonal (which
+    // only this instance addresses: assigning it equals adding to zero)
+    {
+      const int on[3] = {(xf >> XB_ICVBS) & 1, (xf >> XB_ICVDS) & 1, (xf >> XB_ICVGS) & 1};
+      for (int k = 0; k < 3; ++k) {
+        if (!on[k]) continue;
+        const int e = 84 + 5 * k;
+        s.put(K_DFDX, e, dc ? 1.0 : -0.0);
+        s.put(K_DFDX, e + 1, dc ? -1.0 : -0.0);
+        s.put(K_DFDX, e + 2, dc ? 1.0 : -0.0);
+        s.put(K_DFDX, e + 3, dc ? -1.0 : -0.0);
+        s.put(K_DFDX, e + 4, dc ? -0.0 : 1.0);
+      }
+    }
+    if (irgateMod == 3) {
+      s.put(K_DQDX, 0, (+CAPcgmgmb) * Mp);
+      s.put(K_DQDX, 1, (CAPcgmdb) * Mp);
+      s.put(K_DQDX, 2, (CAPcgmsb) * Mp);
+      s.put(K_DQDX, 3, (CAPcgmbb) * Mp);
+      s.put(K_DQDX, 4, (CAPcdgmb) * Mp);
+      s.put(K_DQDX, 5, (CAPcsgmb) * Mp);
+      s.put(K_DQDX, 6, (CAPcbgmb) * Mp);
+    }
+    s.put(K_DQDX, 7, (CAPcggb) * Mp);
+    s.put(K_DQDX, 8, (CAPcgdb) * Mp);
+    s.put(K_DQDX, 9, (CAPcgsb) * Mp);
+    s.put(K_DQDX, 10, (CAPcgbb) * Mp);
+    s.put(K_DQDX, 11, (CAPcddb) * Mp);
+    s.put(K_DQDX, 12, (+CAPcdgb) * Mp);
+    s.put(K_DQDX, 13, -((-CAPcdsb) * Mp));
+    s.put(K_DQDX, 14, -((-CAPcdbb) * Mp));
+    s.put(K_DQDX, 15, -((-CAPcsdb) * Mp));
+    s.put(K_DQDX, 16, (CAPcsgb) * Mp);
+    s.put(K_DQDX, 17, (CAPcssb) * Mp);
+    s.put(K_DQDX, 18, -((-CAPcsbb) * Mp));
+    s.put(K_DQDX, 19, (CAPcbdb) * Mp);
+    s.put(K_DQDX, 20, (CAPcbgb) * Mp);
+    s.put(K_DQDX, 21, (CAPcbsb) * Mp);
+    s.put(K_DQDX, 22, (CAPcbbb) * Mp);
+    if (irbodyMod) {
+      s.put(K_DQDX, 23, (CAPcdbdb) * Mp);
+      s.put(K_DQDX, 24, -((-CAPcsbsb) * Mp));
+      s.put(K_DQDX, 25, (CAPcdbdb) * Mp);
+      s.put(K_DQDX, 26, (-CAPcdbdb) * Mp);
+      s.put(K_DQDX, 27, (CAPcsbsb) * Mp);
+      s.put(K_DQDX, 28, (-CAPcsbsb) * Mp);
+    }
+    if (itrnqsMod) {
+      s.put(K_DQDX, 29, (-CAPcqgb) * Mp);
+      s.put(K_DQDX, 30, (-CAPcqdb) * Mp);
+      s.put(K_DQDX, 31, (-CAPcqsb) * Mp);
+      s.put(K_DQDX, 32, (-CAPcqbb) * Mp);
+    }
+    return;
+  }
   {
     const double dt = C.dtype;
     ceqdrn_Jdxp = 0.0; ceqbd_Jdxp = 0.0; ceqbs_Jdxp = 0.0;
@@ -2984,6 +5214,26 @@ __global__ void __launch_bounds__(256, 1) k_update_small(KArgs a) {
   eval<0, true, CM, false>(a, i, s_card);
 }
 
+// The option kernel: the stored-mode update of option groups (every option
+// branch compiled in), one wave per SIMD so that its larger register set
+// stays in registers; the block's cards in LDS when its card tables exist.
+template <bool LDSCARD>
+__global__ void __launch_bounds__(256, 1) k_update_x(KArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (LDSCARD) {
+    constexpr int kStride = kCardStride;
+    __shared__ double s_card[kBlockCards * kStride];
+    const int32_t* tab = a.blk_cards + (int64_t)blockIdx.x * (kBlockCards + 1);
+    const int used = tab[0];   // 1 <= used <= kBlockCards
+    for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+      s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+    __syncthreads();
+    eval<0, true, 0, false, true>(a, i, s_card);
+  } else {
+    eval<0, false, 0, false, true>(a, i, nullptr);
+  }
+}
+
 // Pattern-fused load (every kind a pattern plan): the block stages its output
 // spans in LDS, every instance adds its stamps into its own span in stamp
 // order as the eval produces them, and the block writes the spans back.  No
@@ -3057,6 +5307,11 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
                          g.fused_lds + sizeof(double) * kBlockCards * kCardStride <= kMaxLds;
   const void* small[3] = {(const void*)k_update_small<0>, (const void*)k_update_small<1>,
                          (const void*)k_update_small<2>};
+  if (g.variant) {   // option group: the option kernel, four calls only
+    const void* kx = lds_cards ? (const void*)k_update_x<true> : (const void*)k_update_x<false>;
+    if (fused) return fail(XA_ERR_STATE, "bsim4: option groups have no fused load");
+    return launch_model_kernel(g, a, false, kx, kx);
+  }
   const void* k_stored = lds_cards && g.n <= kSmallUpdate ? small[cm] : stored[lds_cards][cm];
   return launch_model_kernel(g, a, fused, k_stored, fused_k[fused_lds][cm]);
 }
@@ -3087,7 +5342,7 @@ static double traffic(const Group& g, int phase) {
   // per-instance columns read: those outside the card mode's set (the others
   // come with the card, one copy per model)
   const double cols = ND - card_columns(g);
-  const double lids = 6 + NSTO + NSTA + 1 /*model index*/;
+  const double lids = 6 + NSTO + NSTA_BASE + 1 /*model index*/;
   if (phase == 0)   // transient: the junction charge state is written, not read
     return 8.0 * (cols + 6 /*node values*/ + 12 /*sto*/ + 3 /*sta*/ + 2 /*vgb*/ + 4 /*qb,cap*/) + 4.0 * lids;
   const bool lim = g.opt.voltageLimiterFlag != 0;

@@ -164,6 +164,13 @@ struct Group {
   uint64_t ucol[3] = {0, 0, 0};
   int n_state = 0;
   std::vector<double> state_init;  // [state][inst]
+  // Value slots per instance of each kind (the stamp lists' length): the
+  // model's max_entries, unless setup chose a longer list for this group.
+  int ent[NK] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // The group loads only through the four calls (its kernels have no fused
+  // or pattern-fused form), and a model-specific variant number setup chose.
+  bool split_only = false;
+  int variant = 0;
 
   // filled by register_lids / csr
   std::vector<int32_t> nodes;    // [local node][inst]

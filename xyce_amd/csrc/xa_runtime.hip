@@ -654,6 +654,7 @@ int xa_group_create(xa_ctx ctx, int model_type, const xa_device_options* opt,
   if (opt) g->opt = *opt; else xa_default_options(&g->opt);
   g->ival.assign(impl->n_islots, {});
   g->igiven.assign(impl->n_islots, 0);
+  for (int k = 0; k < NK; ++k) g->ent[k] = impl->max_entries[k];
   *out = reinterpret_cast<xa_group>(g);
   return XA_OK;
 }
@@ -859,11 +860,11 @@ int xa_group_register_csr(xa_group gh, int n_rows, const int32_t* row_ptr,
   const int64_t n = g->n;
   std::vector<Stamp> st;
   for (int k = 0; k < NK; ++k) {
-    const int E = m->max_entries[k];
+    const int E = g->ent[k];
     g->tgt[k].assign((size_t)E * n, -1);
     for (int64_t i = 0; i < n; ++i) {
       g->impl->stamps(*g, i, k, st);
-      if ((int)st.size() > E) return fail(XA_ERR_STATE, "stamp list exceeds max_entries");
+      if ((int)st.size() > E) return fail(XA_ERR_STATE, "stamp list exceeds the group's entries");
       for (size_t e = 0; e < st.size(); ++e) {
         int32_t t = -1;
         if (st[e].row < 0 || (kind_is_matrix(k) && st[e].col < 0)) {
@@ -1065,7 +1066,7 @@ static int detect_pattern(const Group& g, Plan& p, const std::vector<int32_t>& t
 // contributions of one target stay in instance order then stamp order.
 static int build_plan(Group& g, int k, BlockHost& bh) {
   Plan& p = g.plan[k];
-  const int E = g.impl->max_entries[k];
+  const int E = g.ent[k];
   p.E = E;
   const std::vector<int32_t>& tg = g.tgt[k];
   if (E == 0 || g.n == 0) return XA_OK;
@@ -1218,7 +1219,7 @@ static bool fused_fits(const Group& g, int B) {
 static void set_fused_block(Group& g, int B) {
   g.fused_B = B;
   g.fused_lds = (size_t)g.fused_rows * (size_t)std::max(B, 0) * sizeof(double);
-  g.fused_ok = fused_fits(g, B);
+  g.fused_ok = fused_fits(g, B) && !g.split_only;
   for (int k = 0; k < NK; ++k) g.fused_ok = g.fused_ok && g.plan[k].block_private;
 }
 
@@ -1243,7 +1244,7 @@ int xa_group_finalize(xa_group gh) {
     bool live = false;
     for (int32_t t : g->tgt[k]) if (t >= 0) { live = true; break; }
     g->plan[k].loff = rows;
-    if (live) rows += g->impl->max_entries[k];
+    if (live) rows += g->ent[k];
   }
   {
     BlockHost bh[NK];
@@ -1253,7 +1254,7 @@ int xa_group_finalize(xa_group gh) {
   }
   g->fused_rows = rows;
   set_fused_block(*g, g->impl->fused_block);
-  g->pattern_ok = g->impl->launch_pattern != nullptr && g->n > 0;
+  g->pattern_ok = g->impl->launch_pattern != nullptr && g->n > 0 && !g->split_only;
   g->pattern_rows = 0;
   for (int k = 0; k < NK; ++k) {
     const Plan& p = g->plan[k];
