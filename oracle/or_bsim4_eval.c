@@ -1,8 +1,8 @@
 /* or_bsim4_eval.c -- oracle restatement of BSIM4 Instance::updateIntermediateVars
  * (src/DeviceModelPKG/OpenModels/src/N_DEV_MOSFET_B4.C:7744-11919), test
- * infrastructure only.  Every branch of the reference is restated except
- * mtrlMod != 0, which setup rejects: capMod 0/1/2, igcMod 1/2, igbMod,
- * rdsMod, rgateMod 1-3, rbodyMod, trnqsMod/acnqsMod. */
+ * infrastructure only.  Every branch of the reference is restated: mtrlMod,
+ * capMod 0/1/2, igcMod 1/2, igbMod, rdsMod, rgateMod 1-3, rbodyMod,
+ * trnqsMod/acnqsMod and the initial conditions. */
 #include "or_bsim4.h"
 #include "or_eval_libm.h"
 
@@ -584,9 +584,15 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
     Vdb_ = -I->vbs;
   }
   (void)Vdb_;
-  epsrox = M(epsrox);
-  toxe = M(toxe);
-  epssub = B4_EPSSI;
+  if (M(mtrlMod)) {
+    epsrox = 3.9;
+    toxe = M(eot);
+    epssub = B4_EPS0 * M(epsrsub);
+  } else {
+    epsrox = M(epsrox);
+    toxe = M(toxe);
+    epssub = B4_EPSSI;
+  }
   /* MOSFET homotopy (:8818-8838), one gain-scale block, no staggering */
   if (ss->artParameterFlag) {
     const double alpha = ss->gainScale;
@@ -742,7 +748,8 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
 
   /* poly gate depletion (:9055-9080) */
   T0 = I->vfb + p->phi;
-  T1 = B4_EPSSI;
+  if (M(mtrlMod) == 0) T1 = B4_EPSSI;
+  else T1 = M(epsrgate) * B4_EPS0;
   polyDepletion(T0, p->ngate, T1, m->coxe, I->vgs, &I->vgs_eff, &I->dvgs_eff_dvg);
   polyDepletion(T0, p->ngate, T1, m->coxe, I->vgd, &I->vgd_eff, &I->dvgd_eff_dvg);
   if (I->mode > 0) {
@@ -889,7 +896,8 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   Abulk0 *= T0;
 
   /* mobility (:9261-9341) */
-  T14 = 0.0;
+  if (M(mtrlMod)) T14 = 2.0 * m->dtype * (M(phig) - M(easub) - 0.5 * m->Eg0 + 0.45);
+  else T14 = 0.0;
   if (M(mobMod) == 0) {
     T0 = I->Vgsteff + Vth + Vth - T14;
     T2 = p->ua + p->uc * Vbseff;
@@ -1516,8 +1524,10 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   }
 
   /* GIDL (:10074-10126) */
-  T0 = 3.0 * toxe;
-  T1 = (I->vds - I->vgs_eff - p->egidl) / T0;
+  if (M(mtrlMod) == 0) T0 = 3.0 * toxe;
+  else T0 = M(epsrsub) * toxe / epsrox;
+  if (M(mtrlMod) == 0) T1 = (I->vds - I->vgs_eff - p->egidl) / T0;
+  else T1 = (I->vds - I->vgs_eff - p->egidl + p->vfbsd) / T0;
   if ((p->agidl <= 0.0) || (p->bgidl <= 0.0) || (T1 <= 0.0) || (p->cgidl <= 0.0) || (I->vbd > 0.0)) {
     I->Igidl = Ggidld = Ggidlg = Ggidlb = 0.0;
   } else {
@@ -1549,7 +1559,8 @@ int b4_update_intermediate(b4inst* I, const xa_solver_state* ss, const xa_device
   I->ggidlg = Ggidlg;
   I->ggidlb = Ggidlb;
   /* GISL (:10132-10179) */
-  T1 = (-I->vds - I->vgd_eff - p->egisl) / T0;
+  if (M(mtrlMod) == 0) T1 = (-I->vds - I->vgd_eff - p->egisl) / T0;
+  else T1 = (-I->vds - I->vgd_eff - p->egisl + p->vfbsd) / T0;
   if ((p->agisl <= 0.0) || (p->bgisl <= 0.0) || (T1 <= 0.0) || (p->cgisl <= 0.0) || (I->vbs > 0.0)) {
     I->Igisl = Ggisls = Ggislg = Ggislb = 0.0;
   } else {

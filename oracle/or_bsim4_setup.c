@@ -342,7 +342,6 @@ int b4_model_init(b4model* m, const or_card* c, const xa_device_options* opt) {
   if (M(igcMod) != 0 && M(igcMod) != 1 && M(igcMod) != 2)
     return or_fail(XA_ERR_UNSUPPORTED, "bsim4: igcMod must be 0, 1 or 2");
   if (M(igbMod) != 0 && M(igbMod) != 1) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: igbMod must be 0 or 1");
-  if (M(mtrlMod) != 0) return or_fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
   return XA_OK;
 }
 
@@ -355,7 +354,7 @@ static b4size* size_lookup(b4model* m, double l, double w, double nf) {
   return hit;
 }
 
-/* Instance::updateTemperature (N_DEV_MOSFET_B4.C:5640-7734), mtrlMod = 0 */
+/* Instance::updateTemperature (N_DEV_MOSFET_B4.C:5640-7734) */
 void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* opt) {
   b4model* m = I->model;
   b4size* p;
@@ -371,17 +370,25 @@ void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* 
   if (temp_tmp != -999.0) I->temp = temp_tmp;
   Tnom = M(tnom);
   TRatio = I->temp / Tnom;
-  if ((MG(toxe)) && (MG(toxp)) && (MG(dtox)) && (M(toxe) != (M(toxp) + M(dtox)))) {
-  } else if ((MG(toxe)) && (!MG(toxp))) {
-    M(toxp) = M(toxe) - M(dtox);
-  } else if ((!MG(toxe)) && (MG(toxp))) {
-    M(toxe) = M(toxp) + M(dtox);
+  if (M(mtrlMod) == 0) {
+    if ((MG(toxe)) && (MG(toxp)) && (MG(dtox)) && (M(toxe) != (M(toxp) + M(dtox)))) {
+    } else if ((MG(toxe)) && (!MG(toxp))) {
+      M(toxp) = M(toxe) - M(dtox);
+    } else if ((!MG(toxe)) && (MG(toxp))) {
+      M(toxe) = M(toxp) + M(dtox);
+    }
   }
-  epsrox = M(epsrox);
-  toxe = M(toxe);
-  epssub = B4_EPSSI;
+  if (M(mtrlMod)) {
+    epsrox = 3.9;
+    toxe = M(eot);
+    epssub = B4_EPS0 * M(epsrsub);
+  } else {
+    epsrox = M(epsrox);
+    toxe = M(toxe);
+    epssub = B4_EPSSI;
+  }
   m->coxe = epsrox * B4_EPS0 / toxe;
-  m->coxp = M(epsrox) * B4_EPS0 / M(toxp);
+  if (M(mtrlMod) == 0) m->coxp = M(epsrox) * B4_EPS0 / M(toxp);
   if (!MG(cgdo)) {
     if (MG(dlc) && (M(dlc) > 0.0)) M(cgdo) = M(dlc) * m->coxe - M(cgdl);
     else M(cgdo) = 0.6 * M(xj) * m->coxe;
@@ -394,11 +401,18 @@ void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* 
   m->vcrit = B4_vt0 * log(B4_vt0 / (sqrt(2.0) * 1.0e-14));
   m->factor1 = sqrt(epssub / (epsrox * B4_EPS0) * toxe);
   Vtm0 = m->vtm0 = B4_KboQ * Tnom;
-  Eg0 = 1.16 - 7.02e-4 * Tnom * Tnom / (Tnom + 1108.0);
-  ni = 1.45e10 * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp(21.5565981 - Eg0 / (2.0 * Vtm0));
+  if (M(mtrlMod) == 0) {
+    Eg0 = 1.16 - 7.02e-4 * Tnom * Tnom / (Tnom + 1108.0);
+    ni = 1.45e10 * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp(21.5565981 - Eg0 / (2.0 * Vtm0));
+  } else {
+    Eg0 = M(bg0sub) - M(tbgasub) * Tnom * Tnom / (Tnom + M(tbgbsub));
+    T0 = M(bg0sub) - M(tbgasub) * 90090.0225 / (300.15 + M(tbgbsub));
+    ni = M(ni0sub) * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp((T0 - Eg0) / (2.0 * Vtm0));
+  }
   m->Eg0 = Eg0;
   m->vtm = B4_KboQ * I->temp;
-  Eg = 1.16 - 7.02e-4 * I->temp * I->temp / (I->temp + 1108.0);
+  if (M(mtrlMod) == 0) Eg = 1.16 - 7.02e-4 * I->temp * I->temp / (I->temp + 1108.0);
+  else Eg = M(bg0sub) - M(tbgasub) * I->temp * I->temp / (I->temp + M(tbgbsub));
   if (I->temp != Tnom) {
     T0 = Eg0 / Vtm0 - Eg / m->vtm;
     T1 = log(I->temp / Tnom);
@@ -587,10 +601,19 @@ void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* 
     p->phis3 = p->sqrtPhi * p->phi;
     p->Xdep0 = sqrt(2.0 * epssub / (B4_Charge_q * p->ndep * 1.0e6)) * p->sqrtPhi;
     p->sqrtXdep0 = sqrt(p->Xdep0);
-    p->litl = sqrt(3.0 * p->xj * toxe);
+    if (M(mtrlMod) == 0) p->litl = sqrt(3.0 * p->xj * toxe);
+    else p->litl = sqrt(M(epsrsub) / epsrox * p->xj * toxe);
     p->vbi = Vtm0 * log(p->nsd * p->ndep / (ni * ni));
-    if (p->ngate > 0.0) p->vfbsd = Vtm0 * log(p->ngate / p->nsd);
-    else p->vfbsd = 0.0;
+    if (M(mtrlMod) == 0) {
+      if (p->ngate > 0.0) p->vfbsd = Vtm0 * log(p->ngate / p->nsd);
+      else p->vfbsd = 0.0;
+    } else {
+      T0 = Vtm0 * log(p->nsd / ni);
+      T1 = 0.5 * Eg0;
+      if (T0 > T1) T0 = T1;
+      T2 = M(easub) + T1 - m->dtype * T0;
+      p->vfbsd = M(phig) - T2;
+    }
     p->cdep0 = sqrt(B4_Charge_q * epssub * p->ndep * 1.0e6 / 2.0 / p->phi);
     p->ToxRatio = exp(p->ntox * log(M(toxref) / toxe)) / toxe / toxe;
     p->ToxRatioEdge = exp(p->ntox * log(M(toxref) / (toxe * p->poxedge))) / toxe / toxe
@@ -625,7 +648,15 @@ void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* 
     }
     if (!MG(vfb)) {
       if (MG(vth0)) p->vfb = m->dtype * p->vth0 - p->phi - p->k1 * p->sqrtPhi;
-      else p->vfb = -1.0;
+      else if ((M(mtrlMod)) && (MG(phig)) && (MG(nsub))) {
+        T0 = Vtm0 * log(p->nsub / ni);
+        T1 = 0.5 * Eg0;
+        if (T0 > T1) T0 = T1;
+        T2 = M(easub) + T1 + m->dtype * T0;
+        p->vfb = M(phig) - T2;
+      } else {
+        p->vfb = -1.0;
+      }
     }
     if (!MG(vth0)) p->vth0 = m->dtype * (p->vfb + p->phi + p->k1 * p->sqrtPhi);
     p->k1ox = p->k1 * toxe / M(toxm);
@@ -973,6 +1004,91 @@ void b4_update_temperature(b4inst* I, double temp_tmp, const xa_device_options* 
   I->DswTempRevSatCur = T4 * I->Pdeff * M(jtsswd);
   I->SswgTempRevSatCur = T5 * T10 * M(jtsswgs);
   I->DswgTempRevSatCur = T6 * T10 * M(jtsswgd);
+  if (M(mtrlMod)) {
+    /* TOXP from EOT (:7614-7716): Vgs_eff at VDDEOT with poly depletion,
+     * Vth at Vds = Vbs = 0, then the charge centroid iterated */
+    double vddeot, Vgs_eff, V0, lt1, ltw, Theta0, Delt_vth, TempRatio, Vth_NarrowW, Lpe_Vb, Vth;
+    double n, tmp3, Vgsteff, toxpf, toxpi, Tcen;
+    int niter;
+    tmp2 = I->vfb + p->phi;
+    vddeot = m->dtype * M(vddeot);
+    T0 = M(epsrgate) * B4_EPS0;
+    if ((p->ngate > 1.0e18) && (p->ngate < 1.0e25) && (vddeot > tmp2) && (T0 != 0)) {
+      T1 = 1.0e6 * OR_Q * T0 * p->ngate / (m->coxe * m->coxe);
+      T8 = vddeot - tmp2;
+      T4 = sqrt(1.0 + 2.0 * T8 / T1);
+      T2 = 2.0 * T8 / (T4 + 1.0);
+      T3 = 0.5 * T2 * T2 / T1;
+      T7 = 1.12 - T3 - 0.05;
+      T6 = sqrt(T7 * T7 + 0.224);
+      T5 = 1.12 - 0.5 * (T7 + T6);
+      Vgs_eff = vddeot - T5;
+    } else {
+      Vgs_eff = vddeot;
+    }
+    V0 = p->vbi - p->phi;
+    lt1 = m->factor1 * p->sqrtXdep0;
+    ltw = lt1;
+    T0 = p->dvt1 * p->leff / lt1;
+    if (T0 < OR_EXP_THRESHOLD) {
+      T1 = exp(T0);
+      T2 = T1 - 1.0;
+      T3 = T2 * T2;
+      T4 = T3 + 2.0 * T1 * OR_MIN_EXP;
+      Theta0 = T1 / T4;
+    } else {
+      Theta0 = 1.0 / (OR_MAX_EXP - 2.0);
+    }
+    Delt_vth = p->dvt0 * Theta0 * V0;
+    T0 = p->dvt1w * p->weff * p->leff / ltw;
+    if (T0 < OR_EXP_THRESHOLD) {
+      T1 = exp(T0);
+      T2 = T1 - 1.0;
+      T3 = T2 * T2;
+      T4 = T3 + 2.0 * T1 * OR_MIN_EXP;
+      T5 = T1 / T4;
+    } else {
+      T5 = 1.0 / (OR_MAX_EXP - 2.0);
+    }
+    T2 = p->dvt0w * T5 * V0;
+    TempRatio = I->temp / M(tnom) - 1.0;
+    T0 = sqrt(1.0 + p->lpe0 / p->leff);
+    T1 = p->k1ox * (T0 - 1.0) * p->sqrtPhi + (p->kt1 + p->kt1l / p->leff) * TempRatio;
+    Vth_NarrowW = toxe * p->phi / (p->weff + p->w0);
+    Lpe_Vb = sqrt(1.0 + p->lpeb / p->leff);
+    Vth = m->dtype * I->vth0 + (p->k1ox - p->k1) * p->sqrtPhi * Lpe_Vb - Delt_vth - T2
+          + p->k3 * Vth_NarrowW + T1;
+    tmp1 = epssub / p->Xdep0;
+    I->nstar = m->vtm / B4_Charge_q * (m->coxe + tmp1 + p->cit);
+    tmp2 = p->nfactor * tmp1;
+    tmp3 = (tmp2 + p->cdsc * Theta0 + p->cit) / m->coxe;
+    if (tmp3 >= -0.5) {
+      n = 1.0 + tmp3;
+    } else {
+      T0 = 1.0 / (3.0 + 8.0 * tmp3);
+      n = (1.0 + 3.0 * tmp3) * T0;
+    }
+    if (p->dvtp0 > 0.0) {
+      T3 = p->leff + p->dvtp0 * 2.0;
+      if (M(tempMod) < 2) T4 = m->vtm * log(p->leff / T3);
+      else T4 = m->vtm0 * log(p->leff / T3);
+      Vth -= n * T4;
+    }
+    Vgsteff = Vgs_eff - Vth;
+    niter = 0;
+    toxpf = toxe;
+    do {
+      toxpi = toxpf;
+      tmp2 = 2.0e8 * toxpf;
+      T0 = (Vgsteff + I->vtfbphi2) / tmp2;
+      T1 = 1.0 + exp(M(bdos) * 0.7 * log(T0));
+      Tcen = M(ados) * 1.9e-9 / T1;
+      toxpf = toxe - epsrox / M(epsrsub) * Tcen;
+      niter++;
+    } while ((niter <= 4) && (fabs(toxpf - toxpi) > 1e-12));
+    M(toxp) = toxpf;
+    m->coxp = epsrox * B4_EPS0 / M(toxp);
+  }
 }
 
 /* Instance ctor defaults (N_DEV_MOSFET_B4.C:4585-4618) and

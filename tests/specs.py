@@ -98,6 +98,11 @@ BSIM4_OPT_CARDS = {
     "nqs": (54, {"TYPE": 1.0, "TRNQSMOD": 1.0, "XRCRG1": 12.0, "XRCRG2": 1.0}),
     "nqsrg3": (54, {"TYPE": -1.0, "TRNQSMOD": 1.0, "RGATEMOD": 3.0, "RBODYMOD": 1.0, "RSHG": 4.0}),
     "acnqs": (54, {"TYPE": 1.0, "ACNQSMOD": 1.0, "RGATEMOD": 2.0, "RSHG": 5.0}),
+    "mtrl": (54, {"TYPE": 1.0, "MTRLMOD": 1.0, "EOT": 1.2e-9, "VDDEOT": 1.0, "PHIG": 4.2,
+                  "EPSRGATE": 11.7, "NGATE": 1e20, "NSUB": 6e16, "AGIDL": 1e-9, "BGIDL": 1e9,
+                  "CGIDL": 0.5, "EGIDL": 0.8, "AGISL": 1e-9, "BGISL": 1e9, "CGISL": 0.5}),
+    "mtrlp": (54, {"TYPE": -1.0, "MTRLMOD": 1.0, "EOT": 1.5e-9, "PHIG": 4.9, "EPSRSUB": 12.9,
+                   "BG0SUB": 1.1, "NI0SUB": 1.2e10, "MOBMOD": 0.0, "IGCMOD": 1.0}),
 }
 
 

@@ -39,7 +39,7 @@ from specs import BSIM4_OPT_CARDS, bsim4_spec
 
 TRAN = dict(newtonIter=1, transientFlag=1)
 DCOP = dict(newtonIter=1, dcopFlag=1)
-EXACT = ["rdsp", "cap1x0", "cap1x1", "cap1x5", "rg1", "rg2", "rg3", "rb1", "rb2", "acnqs"]
+EXACT = ["rdsp", "cap1x0", "cap1x1", "cap1x5", "rg1", "rg2", "rg3", "rb1", "rb2", "acnqs", "mtrl"]
 
 
 def _pair(spec, **opts):
@@ -74,7 +74,7 @@ def test_fd_catches_planted_error_in_option_rows():
     assert len(fd_check(p, x, TRAN, vecs=("f",))) > 20
 
 
-@pytest.mark.parametrize("name", ["igc1", "igc2", "igc2b", "rds"])
+@pytest.mark.parametrize("name", ["igc1", "igc2", "igc2b", "rds", "mtrlp"])
 def test_fd_tunnelling_cards_without_tunnelling(name):
     """The gate tunnelling cards with igcMod = igbMod = 0 (thin oxide,
     tempMod 2, edge flat-band offset, bias-dependent Rs/Rd) have exact
@@ -273,3 +273,28 @@ def test_nqs_time_constant_and_missing_charge_current():
     rp, ci = p.topo.row_ptr, p.topo.col_idx
     diag = np.array([ro["dFdx"][rp[q] + np.nonzero(ci[rp[q]:rp[q + 1]] == q)[0][0]] for q in qrow])
     assert np.all(diag > 0.0)
+
+
+@pytest.mark.parametrize("dtype", [1.0, -1.0])
+def test_material_model_setup_laws(dtype):
+    """mtrlMod 1 (:5703-5781, :6817-6846): Coxe = 3.9 eps0 / EOT; the
+    substrate band gap Eg0 = BG0SUB - TBGASUB Tnom^2 / (Tnom + TBGBSUB) and
+    intrinsic density ni = NI0SUB (Tnom/300.15)^1.5 exp((Eg(300.15) - Eg0) /
+    (2 Vtm0)); the source/drain flat band VFBSD = PHIG - (EASUB + Eg0/2 -
+    type min(Vtm0 ln(NSD/ni), Eg0/2)) with NSD = 1e20."""
+    card = (54, dict(BSIM4_OPT_CARDS["mtrlp"][1], TYPE=dtype, IGCMOD=0.0))
+    spec = bsim4_spec(8, seed=10, cards=[card], geometry=False)
+    p = _pair(spec)
+    p.set_inputs(nextSol=np.zeros(p.topo.n_nodes))
+    p.load(**DCOP)
+    r = p.orc.read
+    eot, phig, easub, bg0, tbga, tbgb, ni0 = 1.5e-9, 4.9, 4.05, 1.1, 7.02e-4, 1108.0, 1.2e10
+    np.testing.assert_array_equal(r("coxe"), 3.9 * 8.854214871e-12 / eot)
+    tnom = 27.0 + 273.15
+    vtm0 = 8.617087e-5 * tnom
+    eg0 = bg0 - tbga * tnom * tnom / (tnom + tbgb)
+    t0 = bg0 - tbga * 90090.0225 / (300.15 + tbgb)
+    ni = ni0 * (tnom / 300.15) * np.sqrt(tnom / 300.15) * np.exp((t0 - eg0) / (2.0 * vtm0))
+    t0 = min(vtm0 * np.log(1e20 / ni), 0.5 * eg0)
+    want = phig - (easub + 0.5 * eg0 - dtype * t0)
+    np.testing.assert_allclose(r("vfbsd"), want, rtol=1e-12)

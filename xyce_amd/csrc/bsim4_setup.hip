@@ -9,9 +9,8 @@
 //   updateTemperature :5640-7734, DioIjthVjmEval :14292-14303,
 //   NumFingerDiff :14322-14351, PAeffGeo :14364-14462,
 //   RdseffGeo :14473-14588, RdsEndIso :14599-14657, RdsEndSha :14668-14726.
-// Every option of the evaluation is taken except mtrlMod != 0, which is
-// rejected with XA_ERR_UNSUPPORTED (as are capMod, igcMod and igbMod values
-// the evaluation has no branch for).
+// Every option of the evaluation is taken; capMod, igcMod and igbMod values
+// the evaluation has no branch for are rejected with XA_ERR_UNSUPPORTED.
 #include <cmath>
 #include <cstring>
 #include "bsim4_host.h"
@@ -358,12 +357,11 @@ int model_init(Model& mref, const Card& c, const xa_device_options& opt) {
   if (m->p.igcMod != 0 && m->p.igcMod != 1 && m->p.igcMod != 2)
     return fail(XA_ERR_UNSUPPORTED, "bsim4: igcMod must be 0, 1 or 2");
   if (m->p.igbMod != 0 && m->p.igbMod != 1) return fail(XA_ERR_UNSUPPORTED, "bsim4: igbMod must be 0 or 1");
-  if (m->p.mtrlMod != 0) return fail(XA_ERR_UNSUPPORTED, "bsim4: only mtrlMod = 0");
   return XA_OK;
 }
 
 
-/* Instance::updateTemperature (N_DEV_MOSFET_B4.C:5640-7734), mtrlMod = 0 */
+/* Instance::updateTemperature (N_DEV_MOSFET_B4.C:5640-7734) */
 static void update_temperature(Inst* I, double temp_tmp, const xa_device_options& opt) {
   Model* m = I->model;
   Size* p;
@@ -379,17 +377,25 @@ static void update_temperature(Inst* I, double temp_tmp, const xa_device_options
   if (temp_tmp != -999.0) I->temp = temp_tmp;
   Tnom = m->p.tnom;
   TRatio = I->temp / Tnom;
-  if ((m->g.toxe) && (m->g.toxp) && (m->g.dtox) && (m->p.toxe != (m->p.toxp + m->p.dtox))) {
-  } else if ((m->g.toxe) && (!m->g.toxp)) {
-    m->p.toxp = m->p.toxe - m->p.dtox;
-  } else if ((!m->g.toxe) && (m->g.toxp)) {
-    m->p.toxe = m->p.toxp + m->p.dtox;
+  if (m->p.mtrlMod == 0) {
+    if ((m->g.toxe) && (m->g.toxp) && (m->g.dtox) && (m->p.toxe != (m->p.toxp + m->p.dtox))) {
+    } else if ((m->g.toxe) && (!m->g.toxp)) {
+      m->p.toxp = m->p.toxe - m->p.dtox;
+    } else if ((!m->g.toxe) && (m->g.toxp)) {
+      m->p.toxe = m->p.toxp + m->p.dtox;
+    }
   }
-  epsrox = m->p.epsrox;
-  toxe = m->p.toxe;
-  epssub = kEpsSi;
+  if (m->p.mtrlMod) {
+    epsrox = 3.9;
+    toxe = m->p.eot;
+    epssub = kEps0 * m->p.epsrsub;
+  } else {
+    epsrox = m->p.epsrox;
+    toxe = m->p.toxe;
+    epssub = kEpsSi;
+  }
   m->coxe = epsrox * kEps0 / toxe;
-  m->coxp = m->p.epsrox * kEps0 / m->p.toxp;
+  if (m->p.mtrlMod == 0) m->coxp = m->p.epsrox * kEps0 / m->p.toxp;
   if (!m->g.cgdo) {
     if (m->g.dlc && (m->p.dlc > 0.0)) m->p.cgdo = m->p.dlc * m->coxe - m->p.cgdl;
     else m->p.cgdo = 0.6 * m->p.xj * m->coxe;
@@ -402,11 +408,18 @@ static void update_temperature(Inst* I, double temp_tmp, const xa_device_options
   m->vcrit = kVt0 * log(kVt0 / (sqrt(2.0) * 1.0e-14));
   m->factor1 = sqrt(epssub / (epsrox * kEps0) * toxe);
   Vtm0 = m->vtm0 = kKboQ * Tnom;
-  Eg0 = 1.16 - 7.02e-4 * Tnom * Tnom / (Tnom + 1108.0);
-  ni = 1.45e10 * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp(21.5565981 - Eg0 / (2.0 * Vtm0));
+  if (m->p.mtrlMod == 0) {
+    Eg0 = 1.16 - 7.02e-4 * Tnom * Tnom / (Tnom + 1108.0);
+    ni = 1.45e10 * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp(21.5565981 - Eg0 / (2.0 * Vtm0));
+  } else {
+    Eg0 = m->p.bg0sub - m->p.tbgasub * Tnom * Tnom / (Tnom + m->p.tbgbsub);
+    T0 = m->p.bg0sub - m->p.tbgasub * 90090.0225 / (300.15 + m->p.tbgbsub);
+    ni = m->p.ni0sub * (Tnom / 300.15) * sqrt(Tnom / 300.15) * exp((T0 - Eg0) / (2.0 * Vtm0));
+  }
   m->Eg0 = Eg0;
   m->vtm = kKboQ * I->temp;
-  Eg = 1.16 - 7.02e-4 * I->temp * I->temp / (I->temp + 1108.0);
+  if (m->p.mtrlMod == 0) Eg = 1.16 - 7.02e-4 * I->temp * I->temp / (I->temp + 1108.0);
+  else Eg = m->p.bg0sub - m->p.tbgasub * I->temp * I->temp / (I->temp + m->p.tbgbsub);
   if (I->temp != Tnom) {
     T0 = Eg0 / Vtm0 - Eg / m->vtm;
     T1 = log(I->temp / Tnom);
@@ -595,10 +608,19 @@ static void update_temperature(Inst* I, double temp_tmp, const xa_device_options
     p->phis3 = p->sqrtPhi * p->phi;
     p->Xdep0 = sqrt(2.0 * epssub / (kChargeQ * p->ndep * 1.0e6)) * p->sqrtPhi;
     p->sqrtXdep0 = sqrt(p->Xdep0);
-    p->litl = sqrt(3.0 * p->xj * toxe);
+    if (m->p.mtrlMod == 0) p->litl = sqrt(3.0 * p->xj * toxe);
+    else p->litl = sqrt(m->p.epsrsub / epsrox * p->xj * toxe);
     p->vbi = Vtm0 * log(p->nsd * p->ndep / (ni * ni));
-    if (p->ngate > 0.0) p->vfbsd = Vtm0 * log(p->ngate / p->nsd);
-    else p->vfbsd = 0.0;
+    if (m->p.mtrlMod == 0) {
+      if (p->ngate > 0.0) p->vfbsd = Vtm0 * log(p->ngate / p->nsd);
+      else p->vfbsd = 0.0;
+    } else {
+      T0 = Vtm0 * log(p->nsd / ni);
+      T1 = 0.5 * Eg0;
+      if (T0 > T1) T0 = T1;
+      T2 = m->p.easub + T1 - m->dtype * T0;
+      p->vfbsd = m->p.phig - T2;
+    }
     p->cdep0 = sqrt(kChargeQ * epssub * p->ndep * 1.0e6 / 2.0 / p->phi);
     p->ToxRatio = exp(p->ntox * log(m->p.toxref / toxe)) / toxe / toxe;
     p->ToxRatioEdge = exp(p->ntox * log(m->p.toxref / (toxe * p->poxedge))) / toxe / toxe
@@ -633,7 +655,15 @@ static void update_temperature(Inst* I, double temp_tmp, const xa_device_options
     }
     if (!m->g.vfb) {
       if (m->g.vth0) p->vfb = m->dtype * p->vth0 - p->phi - p->k1 * p->sqrtPhi;
-      else p->vfb = -1.0;
+      else if ((m->p.mtrlMod) && (m->g.phig) && (m->g.nsub)) {
+        T0 = Vtm0 * log(p->nsub / ni);
+        T1 = 0.5 * Eg0;
+        if (T0 > T1) T0 = T1;
+        T2 = m->p.easub + T1 + m->dtype * T0;
+        p->vfb = m->p.phig - T2;
+      } else {
+        p->vfb = -1.0;
+      }
     }
     if (!m->g.vth0) p->vth0 = m->dtype * (p->vfb + p->phi + p->k1 * p->sqrtPhi);
     p->k1ox = p->k1 * toxe / m->p.toxm;
@@ -981,6 +1011,90 @@ static void update_temperature(Inst* I, double temp_tmp, const xa_device_options
   I->DswTempRevSatCur = T4 * I->Pdeff * m->p.jtsswd;
   I->SswgTempRevSatCur = T5 * T10 * m->p.jtsswgs;
   I->DswgTempRevSatCur = T6 * T10 * m->p.jtsswgd;
+  if (m->p.mtrlMod) {
+    /* TOXP from EOT (:7614-7716): Vgs_eff at VDDEOT with poly depletion,
+     * Vth at Vds = Vbs = 0, then the charge centroid iterated */
+    double vddeot, Vgs_eff, V0, lt1, ltw, Theta0, Delt_vth, TempRatio, Vth_NarrowW, Lpe_Vb, Vth;
+    double n, tmp3, Vgsteff, toxpf, toxpi, Tcen;
+    int niter;
+    tmp2 = I->vfb + p->phi;
+    vddeot = m->dtype * m->p.vddeot;
+    T0 = m->p.epsrgate * kEps0;
+    if ((p->ngate > 1.0e18) && (p->ngate < 1.0e25) && (vddeot > tmp2) && (T0 != 0)) {
+      T1 = 1.0e6 * kQ * T0 * p->ngate / (m->coxe * m->coxe);
+      T8 = vddeot - tmp2;
+      T4 = sqrt(1.0 + 2.0 * T8 / T1);
+      T2 = 2.0 * T8 / (T4 + 1.0);
+      T3 = 0.5 * T2 * T2 / T1;
+      T7 = 1.12 - T3 - 0.05;
+      T6 = sqrt(T7 * T7 + 0.224);
+      T5 = 1.12 - 0.5 * (T7 + T6);
+      Vgs_eff = vddeot - T5;
+    } else {
+      Vgs_eff = vddeot;
+    }
+    V0 = p->vbi - p->phi;
+    lt1 = m->factor1 * p->sqrtXdep0;
+    ltw = lt1;
+    T0 = p->dvt1 * p->leff / lt1;
+    if (T0 < kExpThreshold) {
+      T1 = exp(T0);
+      T2 = T1 - 1.0;
+      T3 = T2 * T2;
+      T4 = T3 + 2.0 * T1 * kMinExp;
+      Theta0 = T1 / T4;
+    } else {
+      Theta0 = 1.0 / (kMaxExp - 2.0);
+    }
+    Delt_vth = p->dvt0 * Theta0 * V0;
+    T0 = p->dvt1w * p->weff * p->leff / ltw;
+    if (T0 < kExpThreshold) {
+      T1 = exp(T0);
+      T2 = T1 - 1.0;
+      T3 = T2 * T2;
+      T4 = T3 + 2.0 * T1 * kMinExp;
+      T5 = T1 / T4;
+    } else {
+      T5 = 1.0 / (kMaxExp - 2.0);
+    }
+    T2 = p->dvt0w * T5 * V0;
+    TempRatio = I->temp / m->p.tnom - 1.0;
+    T0 = sqrt(1.0 + p->lpe0 / p->leff);
+    T1 = p->k1ox * (T0 - 1.0) * p->sqrtPhi + (p->kt1 + p->kt1l / p->leff) * TempRatio;
+    Vth_NarrowW = toxe * p->phi / (p->weff + p->w0);
+    Lpe_Vb = sqrt(1.0 + p->lpeb / p->leff);
+    Vth = m->dtype * I->vth0 + (p->k1ox - p->k1) * p->sqrtPhi * Lpe_Vb - Delt_vth - T2
+          + p->k3 * Vth_NarrowW + T1;
+    tmp1 = epssub / p->Xdep0;
+    tmp2 = p->nfactor * tmp1;
+    tmp3 = (tmp2 + p->cdsc * Theta0 + p->cit) / m->coxe;
+    if (tmp3 >= -0.5) {
+      n = 1.0 + tmp3;
+    } else {
+      T0 = 1.0 / (3.0 + 8.0 * tmp3);
+      n = (1.0 + 3.0 * tmp3) * T0;
+    }
+    if (p->dvtp0 > 0.0) {
+      T3 = p->leff + p->dvtp0 * 2.0;
+      if (m->p.tempMod < 2) T4 = m->vtm * log(p->leff / T3);
+      else T4 = m->vtm0 * log(p->leff / T3);
+      Vth -= n * T4;
+    }
+    Vgsteff = Vgs_eff - Vth;
+    niter = 0;
+    toxpf = toxe;
+    do {
+      toxpi = toxpf;
+      tmp2 = 2.0e8 * toxpf;
+      T0 = (Vgsteff + I->vtfbphi2) / tmp2;
+      T1 = 1.0 + exp(m->p.bdos * 0.7 * log(T0));
+      Tcen = m->p.ados * 1.9e-9 / T1;
+      toxpf = toxe - epsrox / m->p.epsrsub * Tcen;
+      niter++;
+    } while ((niter <= 4) && (fabs(toxpf - toxpi) > 1e-12));
+    m->p.toxp = toxpf;
+    m->coxp = epsrox * kEps0 / m->p.toxp;
+  }
 }
 
 /* Instance ctor defaults (N_DEV_MOSFET_B4.C:4585-4618) and

@@ -9,10 +9,9 @@
 //   registerStateLIDs :5200-5257, registerStoreLIDs :5267-5290,
 //   Master::updateState :16180-16293, Master::loadDAEVectors :16303-16567,
 //   Master::loadDAEMatrices :16577-17268.
-// Every branch of updateIntermediateVars is evaluated except mtrlMod != 0
-// (setup rejects it).  Groups whose cards or instances use capMod 0/1,
-// igcMod/igbMod, rdsMod, rgateMod, rbodyMod, trnqsMod/acnqsMod or initial
-// conditions take the option kernel (k_update_x: eval with X) and the option
+// Every branch of updateIntermediateVars is evaluated.  Groups whose cards
+// or instances use mtrlMod, capMod 0/1, igcMod/igbMod, rdsMod, rgateMod,
+// rbodyMod, trnqsMod/acnqsMod or initial conditions take the option kernel (k_update_x: eval with X) and the option
 // stamp lists; the others take the default kernels, which compile none of
 // those branches.
 #include <cmath>
@@ -264,9 +263,9 @@ enum DField {
 };
 // Columns only the option kernel reads (gate tunnelling, bias-dependent
 // source/drain resistance, capMod 0, the gate and body resistance networks,
-// initial conditions): size-dependent values (B4X) and instance values
-// (B4Y), appended after the ND columns for groups that use an option and
-// always read from HBM.
+// initial conditions, the mtrlMod material values): size-dependent values
+// (B4X), instance values (B4Y) and model values (B4Z), appended after the ND
+// columns for groups that use an option and always read from HBM.
 #define B4_XFIELDS \
   B4X(nigc) B4X(aigc) B4X(bigc) B4X(cigc) B4X(aigs) B4X(bigs) B4X(cigs) B4X(aigd) \
   B4X(bigd) B4X(cigd) B4X(pigcd) B4X(vfbsd) B4X(vfbsdoff) B4X(Aechvb) B4X(Bechvb) \
@@ -275,14 +274,17 @@ enum DField {
   B4X(bigbinv) B4X(cigbinv) B4X(rs0) B4X(rswmin) B4X(rd0) B4X(rdwmin) B4X(xrcrg1) \
   B4X(xrcrg2) B4X(vfbcv) \
   B4Y(grgeltd) B4Y(grbdb) B4Y(grbsb) B4Y(grbpb) B4Y(grbps) B4Y(grbpd) B4Y(icVDS) \
-  B4Y(icVGS) B4Y(icVBS)
+  B4Y(icVGS) B4Y(icVBS) \
+  B4Z(eot) B4Z(epsrsub) B4Z(epsrgate) B4Z(phig) B4Z(easub) B4Z(Eg0)
 enum XField {
   X_BEFORE_ = ND - 1,
 #define B4X(x) X_##x,
 #define B4Y(x) X_##x,
+#define B4Z(x) X_##x,
   B4_XFIELDS
 #undef B4X
 #undef B4Y
+#undef B4Z
   NDX
 };
 // A model's entry in the card table: the DevCard, then one value per
@@ -342,7 +344,8 @@ enum XBit {
   XB_IGC = 6,        // 2 bits
   XB_IGB = 8,
   XB_CAP = 9,        // 2 bits
-  XB_RDS = 11, XB_RBODYM = 12, XB_PIGCD = 13, XB_ICVBS = 14, XB_ICVDS = 15, XB_ICVGS = 16
+  XB_RDS = 11, XB_RBODYM = 12, XB_PIGCD = 13, XB_ICVBS = 14, XB_ICVDS = 15, XB_ICVGS = 16,
+  XB_MTRL = 17
 };
 // Values the reference's Instance keeps from one evaluation to the next and
 // reads before writing: vgb (as vgb_orig), and the junction charges and
@@ -444,6 +447,16 @@ static DevCard make_card(const Model& m) {
   return c;
 }
 
+// model values of the mtrlMod columns (the model's Eg0 is its temperature value)
+static double mtrl_value(const Model& m, const char* x) {
+  if (!std::strcmp(x, "Eg0")) return m.Eg0;
+  if (!std::strcmp(x, "eot")) return m.p.eot;
+  if (!std::strcmp(x, "epsrsub")) return m.p.epsrsub;
+  if (!std::strcmp(x, "epsrgate")) return m.p.epsrgate;
+  if (!std::strcmp(x, "phig")) return m.p.phig;
+  return m.p.easub;
+}
+
 static int setup(Group& g) {
   const int64_t n = g.n;
   std::vector<Model> models(g.models.size());
@@ -458,7 +471,7 @@ static int setup(Group& g) {
   // lists, the extra columns and the option bits.
   bool xopt = false;
   for (const Model& m : models)
-    xopt = xopt || m.p.capMod != 2 || m.p.igcMod != 0 || m.p.igbMod != 0 || m.p.rdsMod != 0 ||
+    xopt = xopt || m.p.mtrlMod != 0 || m.p.capMod != 2 || m.p.igcMod != 0 || m.p.igbMod != 0 || m.p.rdsMod != 0 ||
            m.p.rbodyMod != 0 || m.p.rgateMod != 0 || m.p.trnqsMod != 0 || m.p.acnqsMod != 0;
   for (int s : {IS_rgateMod, IS_rbodyMod, IS_trnqsMod, IS_acnqsMod})
     if (g.igiven[s])
@@ -502,9 +515,11 @@ static int setup(Group& g) {
     if (xopt) {
 #define B4X(x) D(X_##x, i) = p.x;
 #define B4Y(x) D(X_##x, i) = in.x;
+#define B4Z(x) D(X_##x, i) = mtrl_value(*in.model, #x);
       B4_XFIELDS
 #undef B4X
 #undef B4Y
+#undef B4Z
       const Model& m = *in.model;
       I(F_XF, i) = (in.irgateMod << XB_RGATE) | (in.irbodyMod << XB_RBODY) |
                    ((in.itrnqsMod ? 1 : 0) << XB_TRNQS) | ((in.iacnqsMod ? 1 : 0) << XB_ACNQS) |
@@ -512,7 +527,8 @@ static int setup(Group& g) {
                    ((int)m.p.capMod << XB_CAP) | ((m.p.rdsMod != 0 ? 1 : 0) << XB_RDS) |
                    ((m.p.rbodyMod != 0 ? 1 : 0) << XB_RBODYM) | ((m.g.pigcd ? 1 : 0) << XB_PIGCD) |
                    ((in.icVBS_given ? 1 : 0) << XB_ICVBS) | ((in.icVDS_given ? 1 : 0) << XB_ICVDS) |
-                   ((in.icVGS_given ? 1 : 0) << XB_ICVGS);
+                   ((in.icVGS_given ? 1 : 0) << XB_ICVGS) |
+                   ((m.p.mtrlMod != 0 ? 1 : 0) << XB_MTRL);
     }
   }
   // cards after every instance: the model's temperature values are those
@@ -796,6 +812,7 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   const int itrnqsMod = (xf >> XB_TRNQS) & 1, iacnqsMod = (xf >> XB_ACNQS) & 1;
   const int igcMod = (xf >> XB_IGC) & 3, igbMod = (xf >> XB_IGB) & 1;
   const int capMod = X ? (xf >> XB_CAP) & 3 : 2, rdsMod = (xf >> XB_RDS) & 1;
+  const int mtrlMod = (xf >> XB_MTRL) & 1;
   const int32_t li_Drain = node(a, N_D, i), li_GateExt = node(a, N_G, i);
   const int32_t li_Source = node(a, N_S, i), li_Body = node(a, N_B, i);
   const int32_t li_DrainPrime = node(a, N_DP, i), li_SourcePrime = node(a, N_SP, i);
@@ -1416,9 +1433,15 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
     Vdb_ = -vbs;
   }
   (void)Vdb_;
-  epsrox = C.epsrox;
-  toxe = C.toxe;
-  epssub = kEpsSi;
+  if (X && mtrlMod) {
+    epsrox = 3.9;
+    toxe = XP(eot);
+    epssub = kEps0 * XP(epsrsub);
+  } else {
+    epsrox = C.epsrox;
+    toxe = C.toxe;
+    epssub = kEpsSi;
+  }
   if (RELOAD) reload_params(dfb);
   /* MOSFET homotopy (:8818-8838), one gain-scale block, no staggering */
   if (ss.artParameterFlag) {
@@ -1578,7 +1601,8 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   if (RELOAD) reload_params(dfb);
   /* poly gate depletion (:9055-9080) */
   T0 = IV(vfb) + PP(phi);
-  T1 = kEpsSi;
+  if (X && mtrlMod) T1 = XP(epsrgate) * kEps0;
+  else T1 = kEpsSi;
   polyDepletion(T0, PP(ngate), T1, C.coxe, vgs, vgs_eff, dvgs_eff_dvg);
   polyDepletion(T0, PP(ngate), T1, C.coxe, vgd, vgd_eff, dvgd_eff_dvg);
   if (mode > 0) {
@@ -1729,7 +1753,8 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
 
   if (RELOAD) reload_params(dfb);
   /* mobility (:9261-9341) */
-  T14 = 0.0;
+  if (X && mtrlMod) T14 = 2.0 * C.dtype * (XP(phig) - XP(easub) - 0.5 * XP(Eg0) + 0.45);
+  else T14 = 0.0;
   if (C.mobMod == 0) {
     T0 = Vgsteff + Vth + Vth - T14;
     T2 = PP(ua) + PP(uc) * Vbseff;
@@ -2367,8 +2392,13 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
 
   if (RELOAD) reload_params(dfb);
   /* GIDL (:10074-10126) */
-  T0 = 3.0 * toxe;
-  T1 = (vds - vgs_eff - PP(egidl)) / T0;
+  if (X && mtrlMod) {
+    T0 = XP(epsrsub) * toxe / epsrox;
+    T1 = (vds - vgs_eff - PP(egidl) + XP(vfbsd)) / T0;
+  } else {
+    T0 = 3.0 * toxe;
+    T1 = (vds - vgs_eff - PP(egidl)) / T0;
+  }
   if ((PP(agidl) <= 0.0) || (PP(bgidl) <= 0.0) || (T1 <= 0.0) || (PP(cgidl) <= 0.0) || (vbd > 0.0)) {
     Igidl = Ggidld = Ggidlg = Ggidlb = 0.0;
   } else {
@@ -2401,7 +2431,8 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i, const double* s_
   ggidlb = Ggidlb;
   if (RELOAD) reload_params(dfb);
   /* GISL (:10132-10179) */
-  T1 = (-vds - vgd_eff - PP(egisl)) / T0;
+  if (X && mtrlMod) T1 = (-vds - vgd_eff - PP(egisl) + XP(vfbsd)) / T0;
+  else T1 = (-vds - vgd_eff - PP(egisl)) / T0;
   if ((PP(agisl) <= 0.0) || (PP(bgisl) <= 0.0) || (T1 <= 0.0) || (PP(cgisl) <= 0.0) || (vbs > 0.0)) {
     Igisl = Ggisls = Ggislg = Ggislb = 0.0;
   } else {
