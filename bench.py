@@ -220,6 +220,19 @@ def mc_measure(args):
     t0 = time.perf_counter()
     t, V0 = bt.run(t_stop=args.mc_tstop, h=h)
     batched = time.perf_counter() - t0
+    batched_persistent = None
+    if args.mc_persistent:
+        # the same transient with the load's update as one persistent launch
+        del bt
+        bp = BatchTransient(batch, use_graph=True, persistent=True)
+        print("[bench] batched transient, persistent update", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tp, Vp = bp.run(t_stop=args.mc_tstop, h=h)
+        batched_persistent = time.perf_counter() - t0
+        if not (np.array_equal(tp, t) and np.array_equal(Vp, V0)):
+            raise RuntimeError("persistent update changed the waveforms")
+        bt = bp
     rng = np.random.default_rng(np.random.SeedSequence(7))
     sample = sorted(rng.choice(batch.n_rings, size=min(args.mc_sample, batch.n_rings),
                                replace=False).tolist())
@@ -260,6 +273,7 @@ def mc_measure(args):
                     f"backward Euler 1ps to {args.mc_tstop:g}s",
         "rings": batch.n_rings, "instances": int(2 * batch.stages.sum()),
         "batched_seconds": batched,
+        "batched_seconds_persistent_update": batched_persistent,
         "hip_graphs": len(bt.graphs),
         "newton_iterations": bt.newton_iterations,
         "plain_seconds_per_ring": plain_times,
@@ -426,6 +440,8 @@ def main():
                          "129 samples: HB loads, the FP32 MFMA block LU and its refined solve")
     ap.add_argument("--mc-rings", type=int, default=4096)
     ap.add_argument("--mc-tstop", type=float, default=6e-9)
+    ap.add_argument("--mc-persistent", action="store_true",
+                    help="also run the batched transient with the persistent update kernel")
     ap.add_argument("--mc-sample", type=int, default=4,
                     help="rings run through the plain drop-in and the oracle")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -268,6 +268,17 @@ int xa_group_autotune(xa_group g, const xa_solver_state* ss, const xa_vectors* v
  * the same bits. */
 int xa_group_set_path(xa_group g, int path, int block);
 
+/* Persistent form of the stored-mode update (no reference counterpart; for
+ * batched runs, where one group holds every circuit of a model type): the
+ * update kernel is launched with `blocks` workgroups that stay resident and
+ * loop over the group's instance tiles (tile t by workgroup t mod blocks,
+ * which deals consecutive tiles round-robin over the XCDs), instead of one
+ * workgroup per tile.  blocks = 0 returns to one workgroup per tile;
+ * blocks < 0 takes the number that fits the GPU at the kernel's occupancy.
+ * Same bits either way.  XA_ERR_UNSUPPORTED for models without a persistent
+ * update (BSIM4 has one). */
+int xa_group_set_persistent(xa_group g, int blocks);
+
 /* Shape of the scatter plan of one output kind (XA_KIND_*), for diagnostics
  * and tests (no reference counterpart): targets, contributions, and the
  * targets per instance of a pattern plan (0 when the kind uses the general

@@ -92,3 +92,14 @@ def test_bsim4_card_modes(variant):
     if variant in CARD_LAYOUTS:
         assert p.gpu.param_layout() == CARD_LAYOUTS[variant]
     run_parity(p, np.random.default_rng(716))
+
+
+@pytest.mark.parametrize("blocks", [7, -1])
+def test_bsim4_persistent_update_equals_tiles(blocks):
+    """The persistent update (a resident grid looping over the tiles, here 7
+    workgroups or the GPU's capacity) gives the bits of one workgroup per
+    tile."""
+    spec = bsim4_spec(20000, seed=231, cards=ALL_CARDS, geometry=True, temp_spread=True)
+    print(f"bsim4 persistent update blocks={blocks} n=20000", flush=True)
+    twin_drive("bsim4", spec, 232, np.random.default_rng(233), fused_b=False,
+               prepare_b=lambda p: p.gpu.set_persistent(blocks))

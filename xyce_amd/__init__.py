@@ -232,6 +232,12 @@ class Master:
         size (xa_group_set_path); raises where the group has no such path."""
         check(self.L.xa_group_set_path(self.h, self.PATHS[path], int(block)), "set_path", self.L)
 
+    def set_persistent(self, blocks=-1):
+        """Run the stored-mode update as a persistent kernel of `blocks`
+        resident workgroups looping over the instance tiles (-1: as many as
+        fit the GPU; 0: one workgroup per tile) (xa_group_set_persistent)."""
+        check(self.L.xa_group_set_persistent(self.h, int(blocks)), "set_persistent", self.L)
+
     def autotune(self, ss, vectors, reps=5):
         """Time the block shapes on these vectors and keep the fastest
         (xa_group_autotune); returns ("split" | "fused", block).  The tuning

@@ -90,6 +90,7 @@ def _proto(lib):
         "xa_group_plan_info": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(C.c_int)]),
         "xa_group_param_layout": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
         "xa_group_set_path": (C.c_int, [vp, C.c_int, C.c_int]),
+        "xa_group_set_persistent": (C.c_int, [vp, C.c_int]),
         "xa_group_autotune": (C.c_int, [vp, P(SolverState), P(Vectors), C.c_int, P(C.c_int),
                                         P(C.c_int)]),
         "xa_group_read_intermediate": (C.c_int, [vp, C.c_char_p, P(C.c_double)]),

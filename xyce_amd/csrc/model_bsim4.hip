@@ -5225,6 +5225,31 @@ __global__ void __launch_bounds__(256, FUSED ? 1 : XA_B4_STORED_WAVES) k_update(
   if (FUSED) fused_writeout(a);
 }
 
+// Persistent form of the stored-mode update (xa_group_set_persistent): a
+// resident grid whose workgroups loop over the instance tiles, tile t on
+// workgroup t mod gridDim.x, staging each tile's card table in LDS.  The
+// eval per instance is the stored kernel's, so the bits are the same.
+template <bool LDSCARD, int CM>
+__global__ void __launch_bounds__(256, XA_B4_STORED_WAVES) k_update_persist(KArgs a) {
+  constexpr int kStride = kCardStride;
+  __shared__ double s_card[LDSCARD ? kBlockCards * kStride : 1];
+  const int64_t tiles = (a.n + blockDim.x - 1) / blockDim.x;
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int64_t i = t * blockDim.x + threadIdx.x;
+    if (LDSCARD) {
+      const int32_t* tab = a.blk_cards + t * (kBlockCards + 1);
+      const int used = tab[0];   // 1 <= used <= kBlockCards
+      for (int k = threadIdx.x; k < used * kStride; k += blockDim.x)
+        s_card[k] = a.mcard[(int64_t)tab[1 + k / kStride] * kStride + k % kStride];
+      __syncthreads();
+      eval<0, true, CM, true>(a, i, s_card);
+      __syncthreads();   // every lane done with this tile's cards
+    } else {
+      eval<0, false, CM, true>(a, i, nullptr);
+    }
+  }
+}
+
 // Small launches (at most about one wavefront per SIMD over the GPU): the
 // stored-mode update with the block's cards in LDS, compiled for one wave per
 // SIMD so that every value stays in registers -- no spills, and no section
@@ -5343,8 +5368,35 @@ static int launch_update(Group& g, const xa_solver_state& ss, const xa_vectors& 
     if (fused) return fail(XA_ERR_STATE, "bsim4: option groups have no fused load");
     return launch_model_kernel(g, a, false, kx, kx);
   }
+  if (g.persist_blocks > 0 && !fused) {
+    const void* kp[2][3] = {
+        {(const void*)k_update_persist<false, 0>, (const void*)k_update_persist<false, 1>,
+         (const void*)k_update_persist<false, 2>},
+        {(const void*)k_update_persist<true, 0>, (const void*)k_update_persist<true, 1>,
+         (const void*)k_update_persist<true, 2>}};
+    const int64_t tiles = grid_for(g.n, g.split_block);
+    const unsigned grid = (unsigned)std::min<int64_t>(tiles, g.persist_blocks);
+    void* args[] = {(void*)&a};
+    XA_HIP(hipLaunchKernel(kp[lds_cards][cm], dim3(grid), dim3(g.split_block), args, 0,
+                           g.ctx->stream));
+    return XA_OK;
+  }
   const void* k_stored = lds_cards && g.n <= kSmallUpdate ? small[cm] : stored[lds_cards][cm];
   return launch_model_kernel(g, a, fused, k_stored, fused_k[fused_lds][cm]);
+}
+
+// Workgroups of the persistent update resident on the whole GPU at its
+// occupancy (the default groups' form; option groups keep one per tile).
+static int persistent_capacity(const Group& g) {
+  if (g.variant) return 0;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const bool lds = g.card_tables && g.card_block == g.split_block;
+  const void* k = lds ? (const void*)k_update_persist<true, 0> : (const void*)k_update_persist<false, 0>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, g.split_block, 0) != hipSuccess)
+    return 0;
+  return cus * per_cu;
 }
 
 static int launch_pattern(Group& g, const xa_solver_state& ss, const xa_vectors& v) {
@@ -5411,6 +5463,7 @@ const ModelImpl* model_bsim4() {
     x.state_names = kStateNames;
     x.n_state = NS;
     x.card_columns = card_columns;
+    x.persistent_capacity = persistent_capacity;
     x.traffic = traffic;
     x.fused_block = 128;
     x.launch_pattern = launch_pattern;

@@ -124,6 +124,9 @@ struct ModelImpl {
   int (*launch_pattern)(Group&, const xa_solver_state&, const xa_vectors&);
   // per-instance columns the update reads from its card (null: none)
   int (*card_columns)(const Group&);
+  // workgroups of the persistent update that fit the GPU (null: no
+  // persistent update)
+  int (*persistent_capacity)(const Group&);
 };
 
 const ModelImpl* model_diode();
@@ -171,6 +174,8 @@ struct Group {
   // or pattern-fused form), and a model-specific variant number setup chose.
   bool split_only = false;
   int variant = 0;
+  // workgroups of the persistent stored-mode update (0: one per tile)
+  int persist_blocks = 0;
 
   // filled by register_lids / csr
   std::vector<int32_t> nodes;    // [local node][inst]

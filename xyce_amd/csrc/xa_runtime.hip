@@ -1689,6 +1689,22 @@ int xa_group_set_path(xa_group gh, int path, int block) {
   return XA_OK;
 }
 
+int xa_group_set_persistent(xa_group gh, int blocks) {
+  Group* g = reinterpret_cast<Group*>(gh);
+  int rc = ready(g);
+  if (rc != XA_OK) return rc;
+  if (blocks == 0) {
+    g->persist_blocks = 0;
+    return XA_OK;
+  }
+  if (!g->impl->persistent_capacity)
+    return fail(XA_ERR_UNSUPPORTED, "set_persistent: no persistent update for this model");
+  XA_HIP(hipSetDevice(g->ctx->device));
+  g->persist_blocks = blocks > 0 ? blocks : g->impl->persistent_capacity(*g);
+  if (g->persist_blocks <= 0) return fail(XA_ERR_DEVICE, "set_persistent: no resident capacity");
+  return XA_OK;
+}
+
 int xa_group_autotune(xa_group gh, const xa_solver_state* ss, const xa_vectors* v, int reps,
                       int* route, int* block) {
   Group* g = reinterpret_cast<Group*>(gh);

@@ -131,7 +131,7 @@ class BatchTopology:
 class BatchTransient:
     """The whole batch on one GPU."""
 
-    def __init__(self, batch, device=0, options=None, use_graph=False):
+    def __init__(self, batch, device=0, options=None, use_graph=False, persistent=False):
         import torch
         self.use_graph = use_graph
         self.graphs = {}
@@ -142,6 +142,8 @@ class BatchTransient:
         t0 = time.perf_counter()
         self.topo = t = BatchTopology(self.master, batch)
         self.setup_seconds = time.perf_counter() - t0
+        if persistent:   # one resident grid loops over every ring's instances
+            self.master.set_persistent(-1)
         dev = torch.device("cuda", device)
         self.dev = dev
         z = lambda m: torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
