@@ -441,6 +441,66 @@ int xa_hb_precond_factor(xa_hb_precond pc, const double* dFdx_t, const double* d
 int xa_hb_precond_solve(xa_hb_precond pc, const double* bf, double* xf, int max_refine,
                         double tol, double* rel_fp32, double* rel_final, int32_t* iterations);
 
+/* ---- netlist reader and circuit builder --------------------------------- */
+/* The SPICE-netlist subset a device-load circuit needs, read into the
+ * groups above.  Replaces, for this subset, the reference's parser and
+ * device-instantiation path: IO::CircuitBlock reading the netlist
+ * (N_IO_CircuitBlock.C:483-511, numbers by N_UTL_Misc.C:197-236),
+ * DeviceMgr::addDeviceInstance creating one device type per model type as
+ * it is met (N_DEV_DeviceMgr.C:1446-1470, 1206-1229), Topology's LID
+ * assignment and the Jacobian graph (N_TOP_Topology.C, N_LAS_Builder.C).
+ *
+ * Read: a title line; `*` comment lines, `;` end-of-line comments, `+`
+ * continuations; M d g s b model (NMOS/PMOS LEVEL=1 -> MOSFET1, 14/54 ->
+ * BSIM4), D a c model [area], Q c b e [s] model [area] (NPN/PNP), R and C
+ * n1 n2 [model] [value], V n+ n- [DC] v | SIN(...) | EXP(...) | PULSE(...),
+ * name=value instance parameters, .MODEL, .TRAN tstep tstop [tstart
+ * [tmax]] [UIC], .IC V(n)=v or n=v, .STEP [LIN] target:param start stop step |
+ * target:param LIST v..., .END; .OPTIONS/.PRINT/.MEASURE lines are
+ * skipped.  Anything else -> XA_ERR_ARG naming the line.
+ *
+ * A part is every element of one device type with one set of instance
+ * parameter names (one group when built), in order of first appearance.
+ * Node ids are 0.. in order of first appearance, ground -1.  `step` selects
+ * a .STEP point (-1: the netlist's own values). */
+typedef struct xa_netlist_s* xa_netlist;
+typedef struct xa_circuit_s* xa_circuit;
+int xa_netlist_parse(const char* text, xa_netlist* out);
+int xa_netlist_destroy(xa_netlist h);
+int xa_netlist_sizes(xa_netlist h, int* n_nodes, int* n_parts, int* n_steps, int* n_ic);
+int xa_netlist_node_name(xa_netlist h, int node, char* buf, int len);
+/* tstep, tstop, tstart, tmax; XA_ERR_STATE without .TRAN */
+int xa_netlist_tran(xa_netlist h, double* out4);
+int xa_netlist_ic(xa_netlist h, int32_t* nodes, double* values);
+int xa_netlist_part(xa_netlist h, int part, int* model_type, int* n_models, int* n_inst,
+                    int* n_ext, int* n_iparams);
+/* model k of a part (the part's models in .MODEL order): level and
+ * name=value pairs (names NULL: count only); the names stay valid until the
+ * next call on this part */
+int xa_netlist_part_model(xa_netlist h, int part, int k, int step, int* level, int* n_params,
+                          const char** names, double* values);
+/* the part's instances: model index in the part, external node ids
+ * (instance-major), parameter names and values[p * n_inst + i] */
+int xa_netlist_part_instances(xa_netlist h, int part, int step, int32_t* model_index,
+                              int32_t* ext_nodes, const char** names, double* values);
+
+/* Builds every part as a group on ctx, with LIDs, state/store slots and the
+ * circuit's CSR, finalized.  step >= 0 or -1 as above; step -2 builds every
+ * .STEP point's copy of the circuit into the same groups (the batched
+ * form): copy c's unknowns are c*U .. c*U+U-1 (U = netlist nodes, then each
+ * part's internal nodes instance by instance), its state and store slots
+ * likewise contiguous.  The groups belong to the circuit. */
+int xa_circuit_build(xa_netlist h, xa_ctx ctx, const xa_device_options* opt, int step,
+                     xa_circuit* out);
+int xa_circuit_destroy(xa_circuit h);
+int xa_circuit_sizes(xa_circuit h, int* n_unknowns, int* n_sta, int* n_sto, int64_t* nnz,
+                     int* n_groups, int* n_copies);
+int xa_circuit_csr(xa_circuit h, int32_t* row_ptr, int32_t* col_idx);
+int xa_circuit_group(xa_circuit h, int k, xa_group* g, int* model_type);
+/* DeviceMgr's load over every device type (N_DEV_DeviceMgr.C:2053-2502):
+ * the groups in creation order onto the same vectors. */
+int xa_circuit_load(xa_circuit h, const xa_solver_state* ss, const xa_vectors* v);
+
 #ifdef __cplusplus
 }
 #endif

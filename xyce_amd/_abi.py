@@ -114,6 +114,23 @@ def _proto(lib):
         "xa_hb_precond_destroy": (C.c_int, [vp]),
         "xa_hb_precond_factor": (C.c_int, [vp, vp, vp, vp]),
         "xa_hb_precond_solve": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
+        "xa_netlist_parse": (C.c_int, [C.c_char_p, P(vp)]),
+        "xa_netlist_destroy": (C.c_int, [vp]),
+        "xa_netlist_sizes": (C.c_int, [vp, vp, vp, vp, vp]),
+        "xa_netlist_node_name": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int]),
+        "xa_netlist_tran": (C.c_int, [vp, vp]),
+        "xa_netlist_ic": (C.c_int, [vp, vp, vp]),
+        "xa_netlist_part": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
+        "xa_netlist_part_model": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp,
+                                            P(C.c_char_p), vp]),
+        "xa_netlist_part_instances": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, P(C.c_char_p),
+                                                vp]),
+        "xa_circuit_build": (C.c_int, [vp, vp, P(DeviceOptions), C.c_int, P(vp)]),
+        "xa_circuit_destroy": (C.c_int, [vp]),
+        "xa_circuit_sizes": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+        "xa_circuit_csr": (C.c_int, [vp, vp, vp]),
+        "xa_circuit_group": (C.c_int, [vp, C.c_int, P(vp), vp]),
+        "xa_circuit_load": (C.c_int, [vp, P(SolverState), P(Vectors)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

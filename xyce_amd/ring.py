@@ -69,6 +69,32 @@ class Ring:
             return [d, g, self.supply, self.supply]
         return [d, g, -1, -1]
 
+    def netlist(self, t_step=1e-12, t_stop=2e-9):
+        """The ring as SPICE text for the netlist reader: stage outputs
+        S0..S{N-1}, the supply VDD, the load capacitors, the supply source,
+        the initial condition and the transient.  Values print exactly (a
+        bare number with no scale suffix reads back bit for bit)."""
+        cards = [".model nch nmos " + " ".join(f"{k}={v!r}" for k, v in
+                                                 [("level", 54)] + list(self.models[0][1].items())
+                                                 if k != "TYPE"),
+                 ".model pch pmos " + " ".join(f"{k}={v!r}" for k, v in
+                                                 [("level", 54)] + list(self.models[1][1].items())
+                                                 if k != "TYPE")]
+        name = lambda x: "0" if x < 0 else ("VDD" if x == self.supply else f"S{x}")
+        lines = [f"CMOS ring oscillator, {self.N} stages"] + cards
+        for i in range(self.n_inst):
+            nodes = " ".join(name(x) for x in self.terminals(i))
+            card = "pch" if self.model_index[i] else "nch"
+            extra = " ".join(f"{k}={float(v[i])!r}" for k, v in self.params.items())
+            lines.append(f"M{i} {nodes} {card} {extra}")
+        for k in range(self.N):
+            lines.append(f"C{k} S{k} 0 {float(self.c_load)!r}")
+        lines.append(f"VDD VDD 0 DC {float(self.vdd)!r}")
+        lines.append(f".ic V(S{self.ic_node})={float(self.ic_value)!r}")
+        lines.append(f".tran {float(t_step)!r} {float(t_stop)!r}")
+        lines.append(".end")
+        return "\n".join(lines) + "\n"
+
 
 class Topology:
     """LIDs and CSR structure of a ring built into one BSIM4 group."""
