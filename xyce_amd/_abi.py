@@ -114,6 +114,7 @@ def _proto(lib):
         "xa_hb_precond_destroy": (C.c_int, [vp]),
         "xa_hb_precond_factor": (C.c_int, [vp, vp, vp, vp]),
         "xa_hb_precond_solve": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
+        "xa_csr_spmv": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
         "xa_netlist_parse": (C.c_int, [C.c_char_p, P(vp)]),
         "xa_netlist_destroy": (C.c_int, [vp]),
         "xa_netlist_sizes": (C.c_int, [vp, vp, vp, vp, vp]),

@@ -376,6 +376,33 @@ extern "C" int xa_hb_add_jwq(xa_hb_plan p, double* ff, const double* qf) {
   return hb_jwq(p, ff, qf);
 }
 
+// y = A x for a CSR matrix (device arrays): the Jacobian-vector products of
+// the HB Newton-Krylov solve (the time-domain dF/dx and dQ/dx applied to a
+// direction, N_LOA_HBLoader.C:360-430).  One thread per row, the row's
+// products summed in column-slot order, so the result does not depend on the
+// launch.
+__global__ void k_csr_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                           const double* __restrict__ a, const double* __restrict__ x,
+                           double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = rp[r]; k < rp[r + 1]; ++k) s += a[k] * x[ci[k]];
+  y[r] = s;
+}
+
+extern "C" int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                           const double* values, const double* x, double* y) {
+  Ctx* cx = reinterpret_cast<Ctx*>(ctx);
+  if (!cx || !row_ptr || !col_idx || !values || !x || !y || n_rows < 0)
+    return fail(XA_ERR_ARG, "csr spmv: null argument or negative size");
+  if (n_rows == 0) return XA_OK;
+  hipLaunchKernelGGL(k_csr_spmv, dim3((n_rows + 255) / 256), dim3(256), 0, cx->stream, n_rows,
+                     row_ptr, col_idx, values, x, y);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
 extern "C" int xa_hb_load(xa_hb_plan p, xa_group g, const xa_solver_state* ss, const double* xf,
                           const xa_vectors* vt, int64_t nnz_t, double* ff, double* qf,
                           double* dfdxdvp_f, double* dqdxdvp_f) {

@@ -715,7 +715,13 @@ int xa_circuit_build(xa_netlist h, xa_ctx ctx, const xa_device_options* opt, int
     C.types.push_back(P.type);
     const int nm = (int)std::max<size_t>(P.models.size(), 1);
     const int ni = (int)P.devices.size();
-    for (int c = 0; c < copies; ++c) {
+    // a part whose models no .STEP touches shares one set of cards among
+    // the copies (fewer cards to stage; the same values)
+    bool stepped = false;
+    for (const nl::Step& st : N.steps)
+      for (int m : P.models) stepped |= N.models[m].name == st.target;
+    const int model_copies = stepped ? copies : 1;
+    for (int c = 0; c < model_copies; ++c) {
       const int s = step == -2 ? c : step;
       for (int k = 0; k < nm; ++k) {
         int level, np;
@@ -741,7 +747,7 @@ int xa_circuit_build(xa_netlist h, xa_ctx ctx, const xa_device_options* opt, int
       std::vector<double> vals((size_t)np * ni);
       if ((rc = xa_netlist_part_instances(h, (int)p, s, mi.data(), nullptr, nullptr, vals.data())) != XA_OK)
         return rc;
-      for (int i = 0; i < ni; ++i) mi_all.push_back(mi[i] + c * nm);
+      for (int i = 0; i < ni; ++i) mi_all.push_back(mi[i] + (stepped ? c * nm : 0));
       for (int q = 0; q < np; ++q) cols[q].insert(cols[q].end(), vals.begin() + (size_t)q * ni,
                                                    vals.begin() + (size_t)(q + 1) * ni);
     }

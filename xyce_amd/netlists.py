@@ -196,3 +196,32 @@ def hb_sources(samples=128, n_transistors=2000, seed=SEED, **kw):
     nodes = np.stack([np.repeat(base, 4) + np.tile(np.arange(4, dtype=np.int32), samples),
                       np.full(4 * samples, -1, np.int32)], axis=1)
     return nodes, {"DCV0": volts[nodes[:, 0]]}
+
+
+def diffamp_netlist(n_transistors=2000, seed=SEED, r=1000.0, c=50e-15, vdd=1.0, bias=0.45,
+                    vcm=0.5, l=45e-9):
+    """The differential amplifier of ``hb_diffamp`` (one copy) with its
+    output RC filter and its sources, as SPICE text for the netlist reader.
+    Nodes are named after the generator's ids (``n<id>``; the filter nodes
+    follow the amplifier's), so a reader node maps back to the id by its
+    name.  The sources are DC: VDD, VBIAS, and the inputs VINP and VINN at
+    the common-mode level, ready for per-sample values (`.STEP` lines).
+    Returns (text, initial voltages by generator id)."""
+    (cards, mi, params, lids, n0, volts), outs = _diffamp(1, n_transistors, seed, vdd=vdd, l=l)
+    lid = np.asarray(lids).reshape(-1, 4)
+    name = lambda k: "0" if k < 0 else f"n{int(k)}"
+    lines = [f"differential amplifier, {len(mi)} BSIM4 transistors, RC filter"]
+    for j, (lev, card) in enumerate(cards):
+        kind = "nmos" if card["TYPE"] > 0 else "pmos"
+        lines.append(f".model c{j} {kind} level={lev}")
+    for i in range(len(mi)):
+        extra = " ".join(f"{k}={float(v[i])!r}" for k, v in params.items())
+        lines.append(f"M{i} {' '.join(name(x) for x in lid[i])} c{mi[i]} {extra}")
+    a, b = (int(x) for x in outs[0])
+    lines += [f"RF1 {name(a)} {name(n0)} {float(r)!r}", f"RF2 {name(b)} {name(n0 + 1)} {float(r)!r}",
+              f"CF1 {name(n0)} 0 {float(c)!r}", f"CF2 {name(n0 + 1)} 0 {float(c)!r}",
+              f"VDD {name(0)} 0 DC {float(vdd)!r}", f"VBIAS {name(1)} 0 DC {float(bias)!r}",
+              f"VINP {name(2)} 0 DC {float(vcm)!r}", f"VINN {name(3)} 0 DC {float(vcm)!r}"]
+    v0 = np.concatenate([volts, volts[[a, b]]])
+    v0[2] = v0[3] = vcm
+    return "\n".join(lines) + "\n", v0
