@@ -405,6 +405,120 @@ def hb_transient(args):
     print(json.dumps(out))
 
 
+def hb_solve_measure(args, cells_transistors=2000, N=129, T=1e-9, amp=0.01):
+    """The harmonic-balance solve of the 2000-transistor differential
+    amplifier with its RC filter and sources, driven differentially by a
+    10 mV tone at 1 GHz, 64 harmonics (129 samples): the netlist is read,
+    every sample's copy of it built in one set of groups, and the HB
+    Newton-Krylov solve run to a 1e-9 V update from the generator's bias
+    point.  Reports the Newton and Krylov iterations, the time, the final
+    residual and the output tone."""
+    import numpy as np
+    import torch
+    import xyce_amd
+    from xyce_amd import netlists
+    from xyce_amd.hb import HBSolver, sample_steps
+    from xyce_amd.netlist_reader import Circuit, Netlist
+    t0 = time.perf_counter()
+    text, v0 = netlists.diffamp_netlist(cells_transistors)
+    drive = lambda t: amp * np.sin(2 * np.pi * t / T)
+    nl = Netlist(text + sample_steps({"VINP": lambda t: 0.5 + drive(t),
+                                      "VINN": lambda t: 0.5 - drive(t)}, N, T))
+    ctx = xyce_amd.Context(0)
+    ckt = Circuit(ctx, nl, step=-2, options=xyce_amd.default_options(voltageLimiterFlag=0))
+    hb = HBSolver(ckt, T)
+    t_build = time.perf_counter() - t0
+    print(f"[bench] HB solve: {N} samples x {hb.n} unknowns, {ckt.n_copies} copies built in "
+          f"{t_build:.1f} s", file=sys.stderr, flush=True)
+    x0 = np.zeros(hb.n)
+    for k, name in enumerate(nl.node_names):
+        x0[k] = v0[int(name[1:])]
+    xt0 = torch.from_numpy(np.tile(x0, N)).cuda()
+    xf0 = torch.zeros((hb.n, 2 * N), dtype=torch.float64, device="cuda")
+    hb.plan.dft(xt0, xf0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xf, st = hb.solve(xf0, vtol=1e-9, krylov_tol=1e-10, verbose=True)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    out = [nl.node(f"n{len(v0) - 2}"), nl.node(f"n{len(v0) - 1}")]
+    x = xf.cpu().numpy()
+    tone = [float(np.hypot(x[j, 2], x[j, 3]) * 2) for j in out]
+    return {"samples": N, "harmonics": (N - 1) // 2, "unknowns_per_sample": hb.n,
+            "unknowns": hb.n * N, "instances_bsim4": cells_transistors * N,
+            "build_s": t_build, "solve_s": t_solve, "newton": st["newton"],
+            "krylov_iterations": [h["krylov_iterations"] for h in st["history"]],
+            "loads": st["loads"], "matvecs": st["matvecs"],
+            "final_residual_max": st["final_residual_max"],
+            "output_tone_amplitude_v": tone}
+
+
+def logic_transient_line(args):
+    """The flat logic netlist's FP64 transient (200k BSIM4 transistors in a
+    random INV/NAND2/NOR2 DAG, wire RC, 20 PULSE inputs at 1 GHz, 1 ps
+    steps): read from its netlist text, DC operating point then fixed steps
+    (backward Euler twice, then trapezoid), every Newton iteration's loads
+    and its GMRES + ILU(0) solve on the GPU."""
+    import numpy as np
+    import torch
+    import xyce_amd
+    from xyce_amd import netlists
+    from xyce_amd.netlist_reader import Circuit, Netlist
+    from xyce_amd.sparse_transient import SparseTransient
+    t0 = time.perf_counter()
+    text, volts = netlists.logic_netlist(args.n_logic, t_stop=args.tran_stop)
+    nl = Netlist(text)
+    ctx = xyce_amd.Context(0)
+    tr = SparseTransient(Circuit(ctx, nl), order=args.order, verbose=args.verbose)
+    t_build = time.perf_counter() - t0
+    print(f"[bench] logic transient: {args.n_logic} BSIM4, {tr.n} unknowns, J nnz {tr.J.nnz}, "
+          f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    x0 = np.zeros(tr.n)
+    for k, name in enumerate(nl.node_names):
+        x0[k] = volts[int(name[1:])]
+    rec = np.arange(1, 21)            # the 20 inputs' nodes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, V, nw = tr.run(torch.from_numpy(x0).cuda(), args.tran_stop, 1e-12, record=rec,
+                      progress=50)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps = len(t) - 1
+    st = tr.stats
+    out = {
+        "metric": "flat logic netlist FP64 transient: time steps/s (200k BSIM4, 1 ps steps)",
+        "value": steps / wall, "unit": "time steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": wall / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"flat logic netlist, {args.n_logic} BSIM4 transistors, wire RC, "
+                               f"20 PULSE inputs at 1 GHz, .TRAN 1ps {args.tran_stop:g}; DC op "
+                               "then BE x2 + trapezoid, GMRES + ILU(0) on the device",
+                   "unknowns": tr.n, "jacobian_nnz": tr.J.nnz},
+        "roofline": None, "cpu_baseline": None,
+        "transient": {"wall_s": wall, "build_s": t_build, "newton_total": int(nw.sum()),
+                      "newton_dcop": int(nw[0]), "newton_per_step_mean": float(nw[1:].mean()),
+                      "gmres_total": st["krylov"], "loads": st["loads"], "load_s": st["load_s"],
+                      "factor_s": st["factor_s"], "krylov_s": st["krylov_s"]},
+    }
+    print(json.dumps(out))
+
+
+def hb_solve_line(args):
+    m = hb_solve_measure(args)
+    out = {
+        "metric": "HB solves/s (129 samples x 2000-transistor BSIM4 differential amplifier + RC "
+                  "filter, Newton-Krylov to 1e-9 V)",
+        "value": 1.0 / m["solve_s"], "unit": "HB solves/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": m["solve_s"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "harmonic balance of a 2000-transistor BSIM4 differential "
+                               "amplifier with RC filter and sources, 1 GHz 10 mV differential "
+                               "drive, 64 harmonics (129 samples): the full Newton-Krylov solve"},
+        "roofline": None, "hb_solve": m, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,6 +549,19 @@ def main():
                          "its speedup over the plain drop-in")
     ap.add_argument("--no-mc", action="store_true",
                     help="leave the Monte Carlo measurement off the default line")
+    ap.add_argument("--logic-transient", action="store_true",
+                    help="the flat logic netlist's FP64 transient (read from its netlist, sparse "
+                         "Newton with GMRES + ILU(0) on the device)")
+    ap.add_argument("--n-logic", type=int, default=200_000,
+                    help="--logic-transient: transistor count")
+    ap.add_argument("--tran-stop", type=float, default=1e-9,
+                    help="--logic-transient: end time (s)")
+    ap.add_argument("--order", default="rcm", choices=["rcm", "natural"],
+                    help="--logic-transient: unknown order of the sparse solve")
+    ap.add_argument("--verbose", action="store_true", help="print every Newton iteration")
+    ap.add_argument("--hb-solve", action="store_true",
+                    help="the full harmonic-balance Newton-Krylov solve of the 2000-transistor "
+                         "differential amplifier with its filter and sources, 129 samples")
     ap.add_argument("--hb", action="store_true",
                     help="harmonic balance of the 2000-transistor differential amplifier at "
                          "129 samples: HB loads, the FP32 MFMA block LU and its refined solve")
@@ -459,6 +586,12 @@ def main():
         return
     if args.mc:
         mc_transient(args)
+        return
+    if args.hb_solve:
+        hb_solve_line(args)
+        return
+    if args.logic_transient:
+        logic_transient_line(args)
         return
     if args.hb:
         hb_transient(args)

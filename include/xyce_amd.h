@@ -448,6 +448,22 @@ int xa_hb_precond_factor(xa_hb_precond pc, const double* dFdx_t, const double* d
 int xa_hb_precond_solve(xa_hb_precond pc, const double* bf, double* xf, int max_refine,
                         double tol, double* rel_fp32, double* rel_final, int32_t* iterations);
 
+/* ---- sparse Newton systems: ILU(0) preconditioner ------------------------ */
+/* Incomplete LU with the matrix's own pattern, factored and applied on the
+ * device (rocSPARSE csrilu0 and triangular solves): the preconditioner of
+ * the device GMRES that solves a large circuit's transient Newton systems,
+ * where the reference uses KLU or AztecOO GMRES with an Ifpack ILU
+ * (N_LAS_AztecOOSolver.C, N_LAS_IfpackPrecond.C).  create: the CSR pattern
+ * (host arrays; columns sorted, every diagonal present); factor: the
+ * values (device, nnz), *zero_pivot = the first zero pivot's row or -1;
+ * solve: x = U^-1 L^-1 b (device vectors). */
+typedef struct xa_ilu0_s* xa_ilu0;
+int xa_ilu0_create(xa_ctx ctx, int n, int nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                   xa_ilu0* out);
+int xa_ilu0_destroy(xa_ilu0 h);
+int xa_ilu0_factor(xa_ilu0 h, const double* values, int* zero_pivot);
+int xa_ilu0_solve(xa_ilu0 h, const double* b, double* x);
+
 /* ---- netlist reader and circuit builder --------------------------------- */
 /* The SPICE-netlist subset a device-load circuit needs, read into the
  * groups above.  Replaces, for this subset, the reference's parser and
@@ -504,6 +520,9 @@ int xa_circuit_sizes(xa_circuit h, int* n_unknowns, int* n_sta, int* n_sto, int6
                      int* n_groups, int* n_copies);
 int xa_circuit_csr(xa_circuit h, int32_t* row_ptr, int32_t* col_idx);
 int xa_circuit_group(xa_circuit h, int k, xa_group* g, int* model_type);
+/* group k's node LIDs, instance-major (each instance's external, then
+ * internal nodes; ground -1); node_lids NULL: *count only */
+int xa_circuit_group_lids(xa_circuit h, int k, int64_t* count, int32_t* node_lids);
 /* DeviceMgr's load over every device type (N_DEV_DeviceMgr.C:2053-2502):
  * the groups in creation order onto the same vectors. */
 int xa_circuit_load(xa_circuit h, const xa_solver_state* ss, const xa_vectors* v);

@@ -541,6 +541,7 @@ struct Circuit {
   int n_copies = 1;
   int n_unknowns = 0, n_sta = 0, n_sto = 0;
   std::vector<int32_t> row_ptr, col_idx;
+  std::vector<std::vector<int32_t>> node_lids;   // per group, instance-major
   ~Circuit() {
     for (xa_group g : groups) xa_group_destroy(g);
   }
@@ -814,6 +815,7 @@ int xa_circuit_build(xa_netlist h, xa_ctx ctx, const xa_device_options* opt, int
     if ((rc = xa_group_jac_pairs(g, &cnt, r.data(), q.data())) != XA_OK) return rc;
     for (int64_t j = 0; j < cnt; ++j) keys.push_back((int64_t)r[j] * C.n_unknowns + q[j]);
   }
+  C.node_lids = node_lids;
   std::sort(keys.begin(), keys.end());
   keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
   C.row_ptr.assign((size_t)C.n_unknowns + 1, 0);
@@ -861,6 +863,17 @@ int xa_circuit_group(xa_circuit h, int k, xa_group* g, int* model_type) {
   if (!h || k < 0 || k >= (int)h->c.groups.size()) return fail(XA_ERR_ARG, "bad group index");
   if (g) *g = h->c.groups[k];
   if (model_type) *model_type = h->c.types[k];
+  return XA_OK;
+}
+
+// node LIDs of group k (instance-major: each instance's external then
+// internal nodes; ground -1); with node_lids NULL, *count is the length
+int xa_circuit_group_lids(xa_circuit h, int k, int64_t* count, int32_t* node_lids) {
+  if (!h || k < 0 || k >= (int)h->c.groups.size() || !count)
+    return fail(XA_ERR_ARG, "bad group index or null count");
+  const std::vector<int32_t>& v = h->c.node_lids[k];
+  *count = (int64_t)v.size();
+  if (node_lids) std::copy(v.begin(), v.end(), node_lids);
   return XA_OK;
 }
 

@@ -307,66 +307,8 @@ class HBSolver:
         return self.pack(xf)
 
     def fgmres(self, rhs, tol, restart=40, max_iter=400):
-        """Flexible GMRES with right preconditioning; returns (x, relative
-        residual, iterations)."""
-        torch = self.torch
-        shape = rhs.shape
-        b = rhs.reshape(-1)
-        bnorm = float(torch.linalg.vector_norm(b))
-        x = torch.zeros_like(b)
-        if bnorm == 0.0:
-            return x.reshape(shape), 0.0, 0
-        total = 0
-        r = b.clone()
-        rel = 1.0
-        while total < max_iter:
-            beta = float(torch.linalg.vector_norm(r))
-            rel = beta / bnorm
-            if rel <= tol:
-                break
-            m = min(restart, max_iter - total)
-            V = torch.zeros((m + 1, b.numel()), dtype=b.dtype, device=b.device)
-            Z = torch.zeros((m, b.numel()), dtype=b.dtype, device=b.device)
-            Hm = np.zeros((m + 1, m))
-            g = np.zeros(m + 1)
-            g[0] = beta
-            cs, sn = np.zeros(m), np.zeros(m)
-            V[0] = r / beta
-            k = 0
-            for j in range(m):
-                Z[j] = self.precond(V[j].reshape(shape)).reshape(-1)
-                w = self.matvec(Z[j].reshape(shape)).reshape(-1)
-                # classical Gram-Schmidt, twice
-                h = V[:j + 1] @ w
-                w = w - V[:j + 1].T @ h
-                h2 = V[:j + 1] @ w
-                w = w - V[:j + 1].T @ h2
-                hcol = (h + h2).cpu().numpy()
-                hn = float(torch.linalg.vector_norm(w))
-                Hm[:j + 1, j] = hcol
-                Hm[j + 1, j] = hn
-                if hn > 0:
-                    V[j + 1] = w / hn
-                for i in range(j):
-                    t = cs[i] * Hm[i, j] + sn[i] * Hm[i + 1, j]
-                    Hm[i + 1, j] = -sn[i] * Hm[i, j] + cs[i] * Hm[i + 1, j]
-                    Hm[i, j] = t
-                d = np.hypot(Hm[j, j], Hm[j + 1, j])
-                cs[j], sn[j] = (1.0, 0.0) if d == 0 else (Hm[j, j] / d, Hm[j + 1, j] / d)
-                Hm[j, j] = d
-                Hm[j + 1, j] = 0.0
-                g[j + 1] = -sn[j] * g[j]
-                g[j] = cs[j] * g[j]
-                k = j + 1
-                total += 1
-                rel = abs(g[j + 1]) / bnorm
-                if rel <= tol or hn == 0:
-                    break
-            yk = np.linalg.solve(np.triu(Hm[:k, :k]), g[:k])
-            x = x + Z[:k].T @ torch.from_numpy(yk).to(b.device)
-            r = b - self.matvec(x.reshape(shape)).reshape(-1)
-        rel = float(torch.linalg.vector_norm(r)) / bnorm
-        return x.reshape(shape), rel, total
+        from .krylov import fgmres
+        return fgmres(self.matvec, self.precond, rhs, tol, restart, max_iter)
 
     def solve(self, xf0, vtol=1e-9, max_newton=40, krylov_tol=1e-9, restart=40,
               max_krylov=400, max_step=0.5, verbose=False):

@@ -279,7 +279,11 @@ class BatchTransient:
         torch = self.torch
         t = self.topo
         first = torch.from_numpy(t.base[:-1].astype(np.int64)).to(self.dev)
-        self._newton(lambda it: dict(newtonIter=it, dcopFlag=1, initJctFlag=1 if it == 0 else 0),
+        # the transient's operating point (tranopFlag: the loads compute the
+        # charges the first step differences against, N_ANP_AnalysisManager.C:
+        # 2749-2753)
+        self._newton(lambda it: dict(newtonIter=it, dcopFlag=1, tranopFlag=1,
+                                     initJctFlag=1 if it == 0 else 0),
                      1.0, XA_STEP_DCOP, vtol, max_iter)
         self._accept()
         steps = int(round(t_stop / h))

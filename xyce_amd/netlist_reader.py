@@ -146,6 +146,15 @@ class Circuit:
         check(self.L.xa_circuit_group(self.h, k, C.byref(g), C.byref(t)), "group", self.L)
         return g, TYPE_NAMES[t.value]
 
+    def group_lids(self, k):
+        """Node LIDs of group k, instance-major (ground -1)."""
+        cnt = C.c_int64()
+        check(self.L.xa_circuit_group_lids(self.h, k, C.byref(cnt), None), "group_lids", self.L)
+        out = np.zeros(max(cnt.value, 1), np.int32)
+        check(self.L.xa_circuit_group_lids(self.h, k, C.byref(cnt), out.ctypes.data), "group_lids",
+              self.L)
+        return out[:cnt.value]
+
     def load(self, ss, **tensors):
         """One load of every group onto the given device vectors."""
         from . import make_vectors

@@ -225,3 +225,40 @@ def diffamp_netlist(n_transistors=2000, seed=SEED, r=1000.0, c=50e-15, vdd=1.0, 
     v0 = np.concatenate([volts, volts[[a, b]]])
     v0[2] = v0[3] = vcm
     return "\n".join(lines) + "\n", v0
+
+
+def logic_netlist(n_transistors=200_000, n_inputs=20, seed=SEED, t_step=1e-12, t_stop=1e-9,
+                  **kw):
+    """The flat logic netlist (``logic_dag`` with ``logic_wires`` and
+    ``logic_sources``) as SPICE text for the netlist reader: nodes named
+    after the generator's ids (``n<id>``), the supply and the PULSE inputs
+    first, then the transistors, the wire resistors and capacitors, and the
+    transient.  Values print exactly.  Returns (text, voltages by generator
+    id)."""
+    cards, mi, params, lids, n_nodes, volts = logic_dag(n_transistors, n_inputs, seed, **kw)
+    r_nodes, R, c_nodes, Cv, _ = logic_wires(n_transistors, n_inputs, seed, **kw)
+    src_nodes, src = logic_sources(n_inputs)
+    name = lambda k: "0" if k < 0 else f"n{int(k)}"
+    lines = [f"flat logic netlist, {len(mi)} BSIM4 transistors"]
+    for j, (lev, card) in enumerate(cards):
+        kind = "nmos" if card["TYPE"] > 0 else "pmos"
+        lines.append(f".model c{j} {kind} level={lev}")
+    for k, (a, b) in enumerate(src_nodes):
+        if k == 0:
+            lines.append(f"VS{k} {name(a)} {name(b)} DC {float(src['DCV0'][0])!r}")
+        else:
+            args = " ".join(repr(float(src[p][k]))
+                            for p in ("V1", "V2", "TD", "TR", "TF", "PW", "PER"))
+            lines.append(f"VS{k} {name(a)} {name(b)} PULSE({args})")
+    lid = np.asarray(lids).reshape(-1, 4)
+    keys = list(params.keys())
+    for i in range(len(mi)):
+        extra = " ".join(f"{k}={float(params[k][i])!r}" for k in keys)
+        lines.append(f"M{i} {' '.join(name(x) for x in lid[i])} c{mi[i]} {extra}")
+    for k, ((a, b), r) in enumerate(zip(np.asarray(r_nodes).reshape(-1, 2), R)):
+        lines.append(f"R{k} {name(a)} {name(b)} {float(r)!r}")
+    for k, ((a, b), c) in enumerate(zip(np.asarray(c_nodes).reshape(-1, 2), Cv)):
+        lines.append(f"C{k} {name(a)} {name(b)} {float(c)!r}")
+    lines.append(f".tran {float(t_step)!r} {float(t_stop)!r}")
+    lines.append(".end")
+    return "\n".join(lines) + "\n", np.asarray(volts)

@@ -12,8 +12,9 @@ initial condition on the first stage output that starts the oscillation.
   :class:`GpuLoader`; the capacitors and the supply are linear and are
   assembled here.
 * DC operating point: Newton on F(x) = 0 with the supply and the initial-
-  condition node held (initJct on the first iteration, as the reference's
-  DCOP does); the solution starts the transient with the condition released.
+  condition node held (initJct on the first iteration and tranopFlag set, as
+  the reference's operating point of a transient); the solution starts the
+  transient with the condition released.
 * Transient: fixed-step backward Euler.  The Newton residual and right-hand
   side are formed the way the reference's BDF integrator forms them
   (N_TIA_BackwardDifferentiation15.C:215-265): RHS = -((Q(x) - q_n)/h + F(x))
@@ -283,7 +284,10 @@ def transient(loader, ring, t_stop=2e-9, h=1e-12, vtol=1e-9, max_iter=50, record
         return -out["f"] + out["dFdxdVp"], t.dense(out["dFdx"])
 
     def dc_flags(it):
-        return dict(newtonIter=it, dcopFlag=1, initJctFlag=1 if it == 0 else 0)
+        # the operating point of a transient: tranopFlag set, so the loads
+        # compute the charges the first step differences against
+        # (N_ANP_AnalysisManager.C:2749-2753)
+        return dict(newtonIter=it, dcopFlag=1, tranopFlag=1, initJctFlag=1 if it == 0 else 0)
 
     newton_total = 0
     x, out, x_load, its = _newton(loader, ring, x, {ring.supply, ring.ic_node}, dc_flags,
