@@ -471,8 +471,8 @@ def logic_transient_line(args):
     ctx = xyce_amd.Context(0)
     tr = SparseTransient(Circuit(ctx, nl), order=args.order, verbose=args.verbose)
     t_build = time.perf_counter() - t0
-    print(f"[bench] logic transient: {args.n_logic} BSIM4, {tr.n} unknowns, J nnz {tr.J.nnz}, "
-          f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    print(f"[bench] logic transient: {args.n_logic} BSIM4, {tr.n} unknowns, circuit nnz "
+          f"{tr.c.nnz}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
     x0 = np.zeros(tr.n)
     for k, name in enumerate(nl.node_names):
         x0[k] = volts[int(name[1:])]
@@ -493,7 +493,8 @@ def logic_transient_line(args):
         "config": {"workload": f"flat logic netlist, {args.n_logic} BSIM4 transistors, wire RC, "
                                f"20 PULSE inputs at 1 GHz, .TRAN 1ps {args.tran_stop:g}; DC op "
                                "then BE x2 + trapezoid, GMRES + ILU(0) on the device",
-                   "unknowns": tr.n, "jacobian_nnz": tr.J.nnz},
+                   "unknowns": tr.n, "jacobian_nnz": tr.J.nnz, "order": args.order,
+                   "order_info": getattr(tr, "order_info", None)},
         "roofline": None, "cpu_baseline": None,
         "transient": {"wall_s": wall, "build_s": t_build, "newton_total": int(nw.sum()),
                       "newton_dcop": int(nw[0]), "newton_per_step_mean": float(nw[1:].mean()),
@@ -556,7 +557,7 @@ def main():
                     help="--logic-transient: transistor count")
     ap.add_argument("--tran-stop", type=float, default=1e-9,
                     help="--logic-transient: end time (s)")
-    ap.add_argument("--order", default="rcm", choices=["rcm", "natural"],
+    ap.add_argument("--order", default="topo", choices=["topo", "rcm", "natural"],
                     help="--logic-transient: unknown order of the sparse solve")
     ap.add_argument("--verbose", action="store_true", help="print every Newton iteration")
     ap.add_argument("--hb-solve", action="store_true",

@@ -436,8 +436,9 @@ int xa_hb_load(xa_hb_plan p, xa_group g, const xa_solver_state* ss, const double
 /* y = A x for a CSR matrix of n_rows rows (device arrays, double values):
  * the Jacobian-vector products of a Newton-Krylov solve -- for HB, the
  * time-domain dF/dx and dQ/dx applied to a direction between the transforms
- * (the matrix-free product of N_LOA_HBLoader.C:360-430).  Each row sums its
- * products in column-slot order. */
+ * (the matrix-free product of N_LOA_HBLoader.C:350-456).  Each row's sum
+ * is formed in a fixed order (lane-strided partial sums, then a fixed tree),
+ * the same on every call. */
 int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
                 const double* values, const double* x, double* y);
 int xa_hb_precond_create(xa_hb_plan p, int nnz, const int32_t* row_ptr, const int32_t* col_idx,

@@ -29,7 +29,7 @@ def test_logic_transient_matches_oracle(n_transistors, t_stop):
     for k, name in enumerate(nl.node_names):
         x0[k] = volts[int(name[1:])]
     rec = np.arange(nl.n_nodes)
-    print(f"logic netlist {n_transistors}: {tr.n} unknowns, J nnz {tr.J.nnz}", flush=True)
+    print(f"logic netlist {n_transistors}: {tr.n} unknowns, circuit nnz {tr.c.nnz}", flush=True)
     t, V, nw = tr.run(torch.from_numpy(x0).cuda(), t_stop, 1e-12, record=rec, progress=20)
     print(f"  device: Newton {nw.sum()} (DC {nw[0]}), GMRES {tr.stats['krylov']}", flush=True)
     ot = OracleTransient(nl)

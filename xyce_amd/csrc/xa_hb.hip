@@ -377,18 +377,24 @@ extern "C" int xa_hb_add_jwq(xa_hb_plan p, double* ff, const double* qf) {
 }
 
 // y = A x for a CSR matrix (device arrays): the Jacobian-vector products of
-// the HB Newton-Krylov solve (the time-domain dF/dx and dQ/dx applied to a
-// direction, N_LOA_HBLoader.C:360-430).  One thread per row, the row's
-// products summed in column-slot order, so the result does not depend on the
-// launch.
-__global__ void k_csr_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                           const double* __restrict__ a, const double* __restrict__ x,
-                           double* __restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// the Newton-Krylov solves (for HB the time-domain dF/dx and dQ/dx applied to
+// a direction, N_LOA_HBLoader.C:350-456).  One wavefront per row: lane l sums
+// the row's products l, l+64, ... in slot order, then the 64 partial sums are
+// added in a fixed tree, so the result does not depend on the launch -- and a
+// supply's KCL row (one entry per transistor on the rail) is spread over the
+// lanes instead of one thread.
+__global__ void __launch_bounds__(256) k_csr_spmv(int n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci,
+                                                  const double* __restrict__ a,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (r >= n) return;
   double s = 0.0;
-  for (int k = rp[r]; k < rp[r + 1]; ++k) s += a[k] * x[ci[k]];
-  y[r] = s;
+  for (int k = rp[r] + lane; k < rp[r + 1]; k += 64) s += a[k] * x[ci[k]];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) y[r] = s;
 }
 
 extern "C" int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
@@ -397,7 +403,7 @@ extern "C" int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const
   if (!cx || !row_ptr || !col_idx || !values || !x || !y || n_rows < 0)
     return fail(XA_ERR_ARG, "csr spmv: null argument or negative size");
   if (n_rows == 0) return XA_OK;
-  hipLaunchKernelGGL(k_csr_spmv, dim3((n_rows + 255) / 256), dim3(256), 0, cx->stream, n_rows,
+  hipLaunchKernelGGL(k_csr_spmv, dim3((n_rows + 3) / 4), dim3(256), 0, cx->stream, n_rows,
                      row_ptr, col_idx, values, x, y);
   XA_HIP(hipGetLastError());
   return XA_OK;

@@ -90,7 +90,7 @@ class CSRMatrix:
 
     def matvec(self, x, y=None, values=None):
         import torch
-        y = torch.empty_like(x) if y is None else y
+        y = torch.empty(self.n, dtype=x.dtype, device=x.device) if y is None else y
         vals = self.values if values is None else values
         check(self.L.xa_csr_spmv(self.ctx.h, self.n, self.rp.data_ptr(), self.ci.data_ptr(),
                                  vals.data_ptr(), x.data_ptr(), y.data_ptr()),

@@ -39,12 +39,13 @@ class NewtonFailure(RuntimeError):
 
 class SparseTransient:
     def __init__(self, circuit, n_voltages=None, vtol=1e-9, krylov_tol=1e-10, restart=60,
-                 max_krylov=600, order="rcm", verbose=False):
+                 max_krylov=600, order="topo", verbose=False):
         """n_voltages: the unknowns [0, n_voltages) that are node voltages
         (the convergence test reads them; default: the netlist's nodes).
-        order: "rcm" solves the system with its unknowns (and rows alike)
-        in reverse Cuthill-McKee order of the Jacobian's graph, which keeps
-        the incomplete factorisation's fill local; "natural" keeps the LID
+        order: the unknowns' (and rows') order in the solved system --
+        "topo" (default) the block-triangular order of the first system
+        solved (the DC operating point's Jacobian, see _topo_order), "rcm"
+        reverse Cuthill-McKee of the Jacobian's graph, "natural" the LID
         order."""
         import torch
         from . import default_solver_state
@@ -73,34 +74,16 @@ class SparseTransient:
                     raise ValueError("two voltage sources on one node: rows cannot be paired")
                 src[node], src[bra] = bra, node
         rp, ci = circuit.csr()
-        rows = src[np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))]   # src is its own inverse
-        cols = ci.astype(np.int64)
-        # symmetric ordering q: position p of the solved system holds unknown
-        # q[p] (and the row paired with it)
-        if order == "rcm":
-            import scipy.sparse as sps
-            from scipy.sparse.csgraph import reverse_cuthill_mckee
-            g = sps.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
-            q = np.asarray(reverse_cuthill_mckee((g + g.T).tocsr(), symmetric_mode=True),
-                           np.int64)
-        elif order == "natural":
-            q = np.arange(n, dtype=np.int64)
-        else:
-            raise ValueError("order must be 'rcm' or 'natural'")
-        qinv = np.empty(n, np.int64)
-        qinv[q] = np.arange(n)
+        self._rows = src[np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))]   # src: own inverse
+        self._cols = ci.astype(np.int64)
+        self._src = src
+        if order not in ("topo", "rcm", "natural"):
+            raise ValueError("order must be 'topo', 'rcm' or 'natural'")
+        self.order = order
         self.verbose = verbose
-        self.row_of = self.torch.from_numpy(src[q]).cuda()     # system row p <- circuit row
-        self.q = self.torch.from_numpy(q).cuda()
-        rows, cols = qinv[rows], qinv[cols]
-        ckeys = rows * n + cols
-        keys = np.unique(np.concatenate([ckeys, np.arange(n, dtype=np.int64) * (n + 1)]))
-        jr = keys // n
-        jrp = np.zeros(n + 1, np.int64)
-        np.add.at(jrp, jr + 1, 1)
-        self.J = CSRMatrix(self.ctx, np.cumsum(jrp).astype(np.int32), (keys % n).astype(np.int32))
-        self.map = torch.from_numpy(np.searchsorted(keys, ckeys)).cuda()
-        self.ilu = ILU0(self.J)
+        self.J = None
+        if order != "topo":
+            self._build(self._static_order(order))
         f64 = dict(dtype=torch.float64, device="cuda")
         ns, nt, nz = circuit.n_sta, circuit.n_sto, circuit.nnz
         sizes = dict(nextSol=n, currSol=n, f=n, q=n, b=n, dFdxdVp=n, dQdxdVp=n,
@@ -111,6 +94,110 @@ class SparseTransient:
                      "storeLeadQ")
         self.stats = {"loads": 0, "newton": 0, "krylov": 0, "load_s": 0.0, "factor_s": 0.0,
                       "krylov_s": 0.0}
+
+    def _static_order(self, order):
+        n = self.n
+        if order == "natural":
+            return np.arange(n, dtype=np.int64)
+        import scipy.sparse as sps
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        g = sps.csr_matrix((np.ones(len(self._rows)), (self._rows, self._cols)), shape=(n, n))
+        return np.asarray(reverse_cuthill_mckee((g + g.T).tocsr(), symmetric_mode=True), np.int64)
+
+    def _topo_order(self, vals):
+        """Unknown order from the numerically nonzero entries of a Jacobian:
+        the strongly connected components of its graph in topological order
+        (every entry below the diagonal blocks), and inside a component the
+        unknowns that other components read last -- so that the incomplete
+        factorisation drops no fill outside the diagonal blocks.  For the DC
+        Jacobian of a feed-forward logic netlist the blocks are a cell's
+        output net (driver, far end, stack nodes) and ILU(0) in this order is
+        nearly the exact LU (the topological order sparse direct solvers
+        reach by their block-triangular form)."""
+        import scipy.sparse as sps
+        from scipy.sparse.csgraph import connected_components
+        n = self.n
+        nz = vals != 0
+        r, c = self._rows[nz], self._cols[nz]
+        g = sps.csr_matrix((np.ones(len(r)), (r, c)), shape=(n, n))
+        ncomp, lab = connected_components(g, directed=True, connection="strong")
+        cr, cc = lab[r], lab[c]
+        m = cr != cc
+        E = sps.csr_matrix((np.ones(int(m.sum())), (cc[m], cr[m])), shape=(ncomp, ncomp))
+        E.sum_duplicates()
+        E.data[:] = 1.0
+        indeg = np.asarray(E.sum(axis=0)).ravel().astype(np.int64)
+        level = np.full(ncomp, -1, np.int64)
+        frontier = np.nonzero(indeg == 0)[0]
+        depth = 0
+        while len(frontier):
+            level[frontier] = depth
+            succ = E[frontier].indices
+            np.add.at(indeg, succ, -1)
+            cand = np.unique(succ)
+            frontier = cand[indeg[cand] == 0]
+            depth += 1
+        if (level < 0).any():
+            raise RuntimeError("component graph is not acyclic")
+        ext = np.zeros(n, np.int64)
+        np.add.at(ext, c[m], 1)
+        self.order_info = {"components": int(ncomp), "levels": int(depth),
+                           "largest_component": int(np.bincount(lab).max())}
+        return np.lexsort((np.arange(n), ext > 0, lab, level[lab])).astype(np.int64)
+
+    def _build(self, q):
+        """The solved system in order q (position p holds unknown q[p] and
+        the row paired with it): its CSR with every diagonal, the map from
+        the circuit's CSR slots and the ILU(0)."""
+        torch = self.torch
+        n = self.n
+        qinv = np.empty(n, np.int64)
+        qinv[q] = np.arange(n)
+        self.row_of = torch.from_numpy(self._src[q]).cuda()
+        self.q = torch.from_numpy(q).cuda()
+        ckeys = qinv[self._rows] * n + qinv[self._cols]
+        keys = np.unique(np.concatenate([ckeys, np.arange(n, dtype=np.int64) * (n + 1)]))
+        jrp = np.zeros(n + 1, np.int64)
+        np.add.at(jrp, keys // n + 1, 1)
+        jrp = np.cumsum(jrp).astype(np.int64)
+        jr, jc = keys // n, keys % n
+        self.J = CSRMatrix(self.ctx, jrp.astype(np.int32), jc.astype(np.int32))
+        self.map = torch.from_numpy(np.searchsorted(keys, ckeys)).cuda()
+        # terminal rows: an unknown that no other row reads (a voltage
+        # source's branch current, whose row is its node's KCL -- for a supply
+        # one entry per transistor on the rail).  The incomplete factorisation
+        # keeps only their diagonal; the preconditioner then solves them
+        # exactly after the rest (block forward substitution), so the long
+        # rows never enter the factorisation.
+        readers = np.bincount(jc[jr != jc], minlength=n)
+        term = np.nonzero(readers == 0)[0]
+        has_off = np.zeros(n, bool)
+        has_off[jr[jr != jc]] = True
+        term = term[has_off[term]]
+        keep = ~(np.isin(jr, term) & (jr != jc))
+        krp = np.zeros(n + 1, np.int64)
+        np.add.at(krp, jr[keep] + 1, 1)
+        self.M = CSRMatrix(self.ctx, np.cumsum(krp).astype(np.int32), jc[keep].astype(np.int32))
+        self.m_src = torch.from_numpy(np.nonzero(keep)[0]).cuda()
+        self.term = torch.from_numpy(term).cuda()
+        if len(term):
+            sel = np.isin(jr, term) & (jr != jc)
+            pos = np.searchsorted(term, jr[sel])
+            trp = np.zeros(len(term) + 1, np.int64)
+            np.add.at(trp, pos + 1, 1)
+            self.T = CSRMatrix(self.ctx, np.cumsum(trp).astype(np.int32), jc[sel].astype(np.int32))
+            self.t_src = torch.from_numpy(np.nonzero(sel)[0]).cuda()
+            self.t_diag = torch.from_numpy(jrp[term] + np.array(
+                [np.searchsorted(jc[jrp[t]:jrp[t + 1]], t) for t in term], np.int64)).cuda()
+        self.ilu = ILU0(self.M)
+        self.n_terminal = len(term)
+
+    def _precond(self, b):
+        y = self.ilu.solve(b)
+        if self.n_terminal:
+            t = self.T.matvec(y)
+            y[self.term] = (b[self.term] - t) / self.J.values[self.t_diag]
+        return y
 
     def load(self, x, flags):
         torch = self.torch
@@ -136,17 +223,24 @@ class SparseTransient:
         for it in range(max_iter):
             self.load(x, flags_of(it))
             rhs, a, b = system()
+            if self.J is None:
+                k = len(self._rows)
+                self._build(self._topo_order(
+                    (a * v["dFdx"][:k] + b * v["dQdx"][:k]).cpu().numpy()))
             rhs = rhs[self.row_of]
             t0 = time.perf_counter()
             Jv = self.J.values
             Jv.zero_()
             Jv[self.map] = a * v["dFdx"][:len(self.map)] + b * v["dQdx"][:len(self.map)]
+            self.M.values[:self.M.nnz] = Jv[self.m_src]
+            if self.n_terminal:
+                self.T.values[:self.T.nnz] = Jv[self.t_src]
             zp = self.ilu.factor()
             if zp >= 0:
                 raise NewtonFailure(f"ILU(0): zero pivot in row {zp}")
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            dy, rel, its = fgmres(self.J.matvec, self.ilu.solve, rhs, self.krylov_tol,
+            dy, rel, its = fgmres(self.J.matvec, self._precond, rhs, self.krylov_tol,
                                   self.restart, self.max_krylov)
             dx = torch.empty_like(dy)
             dx[self.q] = dy
