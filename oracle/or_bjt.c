@@ -9,8 +9,11 @@
  *   Model ctor :3394-3798, Master::updateState :3891-3937,
  *   Master::updateSecondaryState :3947-3968, Master::loadDAEVectors :3978-4142,
  *   Master::loadDAEMatrices :4152-4260.
- * newExcessPhase = 1 (the extra excess-phase nodes) and external internal
- * nodes are not restated; setup rejects the former.
+ * newExcessPhase = 1 (the new-DAE excess phase: two internal unknowns, the
+ * phase-shifted current i_fx and its derivative, :908-931, :1032-1036,
+ * :1149-1153, auxDAECalculations :1928-1980, the Master's excess-phase rows
+ * :3997-4024, :4033-4046, :4069-4100, :4177-4260) is restated; external
+ * internal nodes are not.
  *
  * Aliases (BF/BFM, VAF/VA/VBF, ...) write the same model value but keep
  * their own given flag: the reference reconciles the alias flags before it
@@ -92,8 +95,9 @@ typedef struct {
   double iC, iCE, iB, iE, diCEdvEp, diCEdvCp, diCEdvBp, gEpr, gCpr, gX;
   double diBrdvB, diBrdvCp, diBrdvEp, diBrdvBp, gBEtot, gBCtot;
   double iBEdiff, iBEdep, iCS, iBCdiff, iBCdep, iBX;
+  double diBEdvEp, diBEdvCp, diBEdvBp;   /* new excess phase (auxDAECalculations) */
   /* LIDs and slots */
-  int li_Coll, li_Base, li_Emit, li_Subst, li_CollP, li_BaseP, li_EmitP;
+  int li_Coll, li_Base, li_Emit, li_Subst, li_CollP, li_BaseP, li_EmitP, li_Ifx, li_dIfx;
   int li_qstateBEdiff, li_qstateBEdep, li_qstateCS, li_qstateBCdiff, li_qstateBCdep;
   int li_qstateBX, li_istateCEXBC;
   int li_storevBE, li_storevBC, li_store_capeqCB;
@@ -102,6 +106,9 @@ typedef struct {
   int64_t f_BpB, f_BpCp, f_BpBp, f_BpEp, f_EpE, f_EpCp, f_EpBp, f_EpEp;
   int64_t q_BB, q_BCp, q_SS, q_SCp, q_CpB, q_CpS, q_CpCp, q_CpBp, q_BpCp, q_BpBp, q_BpEp;
   int64_t q_EpCp, q_EpBp, q_EpEp;
+  /* new excess phase slots (-1 when the option is off) */
+  int64_t f_CpIfx, f_EpIfx, f_IfxdIfx, f_dIfxCp, f_dIfxBp, f_dIfxEp, f_dIfxdIfx, f_dIfxIfx;
+  int64_t f_IfxCp, f_IfxBp, f_IfxEp, f_IfxIfx, q_IfxIfx, q_dIfxdIfx;
 } binst;
 
 /* one value from a group of alias names; more than one given is an error */
@@ -258,7 +265,6 @@ static int setup(struct or_group_s* g) {
   binst* in = (binst*)calloc(g->n > 0 ? g->n : 1, sizeof(binst));
   g->mdata = mods;
   g->inst = in;
-  if (g->opt.newExcessPhase) return or_fail(XA_ERR_UNSUPPORTED, "bjt: newExcessPhase");
   for (k = 0; k < g->nmodels; ++k) {
     if (g->models[k].level != 1) return or_fail(XA_ERR_UNSUPPORTED, "bjt: level must be 1");
     if ((rc = model_init(&mods[k], &g->models[k], &g->opt)) != XA_OK) return rc;
@@ -278,6 +284,8 @@ static int setup(struct or_group_s* g) {
     g->n_ext[i] = 4;
     g->n_int[i] = (m->collectorResist == 0.0 ? 0 : 1) + (m->emitterResist == 0.0 ? 0 : 1) +
                   (m->baseResist == 0.0 ? 0 : 1);
+    /* the excess-phase current and its derivative (:1032-1036) */
+    if (g->opt.newExcessPhase) g->n_int[i] += 2;
     g->n_sta[i] = 7;
     g->n_sto[i] = 3 + (I->loadLeadCurrent ? 4 : 0);
     update_temperature(I, I->TEMP);   /* processParams -> updateTemperature(TEMP) */
@@ -297,6 +305,9 @@ static void register_lids(struct or_group_s* g, int64_t i, const int32_t* nodes,
   I->li_CollP = (m->collectorResist == 0.0) ? I->li_Coll : nodes[loc++];
   I->li_BaseP = (m->baseResist == 0.0) ? I->li_Base : nodes[loc++];
   I->li_EmitP = (m->emitterResist == 0.0) ? I->li_Emit : nodes[loc++];
+  /* :1149-1153 */
+  I->li_Ifx = g->opt.newExcessPhase ? nodes[loc++] : -1;
+  I->li_dIfx = g->opt.newExcessPhase ? nodes[loc++] : -1;
   I->li_qstateBEdiff = sta ? sta[0] : -1;
   I->li_qstateBEdep = sta ? sta[1] : -1;
   I->li_qstateCS = sta ? sta[2] : -1;
@@ -314,17 +325,30 @@ static void register_lids(struct or_group_s* g, int64_t i, const int32_t* nodes,
 }
 
 /* jacStamp_RB_RC_RE_ (:867-906) with the primed nodes folded into their
- * terminals when the resistance is zero (jacStampMap) */
+ * terminals when the resistance is zero (jacStampMap); with the new excess
+ * phase (:908-931) the C' and E' rows gain an Ifx column and the Ifx and
+ * dIfx rows follow */
 static int64_t jac_pairs(struct or_group_s* g, int64_t i, int32_t* r, int32_t* c) {
   const binst* I = &((binst*)g->inst)[i];
   const int C = I->li_Coll, B = I->li_Base, E = I->li_Emit, S = I->li_Subst;
   const int Cp = I->li_CollP, Bp = I->li_BaseP, Ep = I->li_EmitP;
-  const int32_t pr[24] = {C, C, B, B, B, B, E, E, S, S, Cp, Cp, Cp, Cp, Cp, Cp,
-                          Bp, Bp, Bp, Bp, Ep, Ep, Ep, Ep};
-  const int32_t pc[24] = {C, Cp, B, Cp, Bp, Ep, E, Ep, S, Cp, C, B, S, Cp, Bp, Ep,
-                          B, Cp, Bp, Ep, E, Cp, Bp, Ep};
+  const int X = I->li_Ifx, D = I->li_dIfx;
+  const int nep = g->opt.newExcessPhase != 0;
+  const int32_t pr0[24] = {C, C, B, B, B, B, E, E, S, S, Cp, Cp, Cp, Cp, Cp, Cp,
+                           Bp, Bp, Bp, Bp, Ep, Ep, Ep, Ep};
+  const int32_t pc0[24] = {C, Cp, B, Cp, Bp, Ep, E, Ep, S, Cp, C, B, S, Cp, Bp, Ep,
+                           B, Cp, Bp, Ep, E, Cp, Bp, Ep};
+  const int32_t pr1[36] = {C, C, B, B, B, B, E, E, S, S, Cp, Cp, Cp, Cp, Cp, Cp, Cp,
+                           Bp, Bp, Bp, Bp, Ep, Ep, Ep, Ep, Ep,
+                           X, X, X, X, X, D, D, D, D, D};
+  const int32_t pc1[36] = {C, Cp, B, Cp, Bp, Ep, E, Ep, S, Cp, C, B, S, Cp, Bp, Ep, X,
+                           B, Cp, Bp, Ep, E, Cp, Bp, Ep, X,
+                           Cp, Bp, Ep, X, D, Cp, Bp, Ep, X, D};
+  const int32_t* pr = nep ? pr1 : pr0;
+  const int32_t* pc = nep ? pc1 : pc0;
+  const int np = nep ? 36 : 24;
   int64_t k, n = 0;
-  for (k = 0; k < 24; ++k) {
+  for (k = 0; k < np; ++k) {
     if (pr[k] < 0 || pc[k] < 0) continue;
     if (r) { r[n] = pr[k]; c[n] = pc[k]; }
     ++n;
@@ -348,10 +372,24 @@ static int register_jac(struct or_group_s* g, int64_t i) {
   I->q_CpCp = or_slot(g, Cp, Cp); I->q_CpBp = or_slot(g, Cp, Bp); I->q_BpCp = or_slot(g, Bp, Cp);
   I->q_BpBp = or_slot(g, Bp, Bp); I->q_BpEp = or_slot(g, Bp, Ep); I->q_EpCp = or_slot(g, Ep, Cp);
   I->q_EpBp = or_slot(g, Ep, Bp); I->q_EpEp = or_slot(g, Ep, Ep);
+  if (g->opt.newExcessPhase) {
+    const int X = I->li_Ifx, D = I->li_dIfx;
+    I->f_CpIfx = or_slot(g, Cp, X); I->f_EpIfx = or_slot(g, Ep, X);
+    I->f_IfxdIfx = or_slot(g, X, D); I->f_dIfxCp = or_slot(g, D, Cp);
+    I->f_dIfxBp = or_slot(g, D, Bp); I->f_dIfxEp = or_slot(g, D, Ep);
+    I->f_dIfxdIfx = or_slot(g, D, D); I->f_dIfxIfx = or_slot(g, D, X);
+    I->f_IfxCp = or_slot(g, X, Cp); I->f_IfxBp = or_slot(g, X, Bp);
+    I->f_IfxEp = or_slot(g, X, Ep); I->f_IfxIfx = or_slot(g, X, X);
+    I->q_IfxIfx = or_slot(g, X, X); I->q_dIfxdIfx = or_slot(g, D, D);
+  } else {
+    I->f_CpIfx = I->f_EpIfx = I->f_IfxdIfx = I->f_dIfxCp = I->f_dIfxBp = I->f_dIfxEp = -1;
+    I->f_dIfxdIfx = I->f_dIfxIfx = I->f_IfxCp = I->f_IfxBp = I->f_IfxEp = I->f_IfxIfx = -1;
+    I->q_IfxIfx = I->q_dIfxdIfx = -1;
+  }
   {
     const int64_t* s = &I->f_CC;
     int k;
-    for (k = 0; k < 34; ++k) if (s[k] < -1) return XA_ERR_ARG;
+    for (k = 0; k < 48; ++k) if (s[k] < -1) return XA_ERR_ARG;
   }
   return XA_OK;
 }
@@ -370,7 +408,7 @@ static void lambertw_current(double* Id, double* Gd, double Vd, double Vte, doub
   *Gd = w / ((1 + w) * RS);
 }
 
-/* Instance::updateIntermediateVars (:2476-3207), newExcessPhase = 0 */
+/* Instance::updateIntermediateVars (:2476-3207) */
 static int update_intermediate(struct or_group_s* g, binst* I, const xa_solver_state* ss,
                                const xa_vectors* v) {
   const bmodel* m = I->model;
@@ -675,6 +713,39 @@ static int update_intermediate(struct or_group_s* g, binst* I, const xa_solver_s
       v->nextSta[I->li_istateCEXBC] = I->nextCexbc;
     }
   }
+  if (g->opt.newExcessPhase) {
+    /* auxDAECalculations (:1928-1980): the currents from the excess-phase
+     * unknown i_fx, which the new-DAE form carries in the solution */
+    const double td = m->excessPhaseFac;
+    if (td != 0 && !(ss->dcopFlag)) {
+      const double i_fx = sol(solVec, I->li_Ifx);
+      I->iCE = i_fx - I->iBC / I->qB;
+    } else {
+      I->iCE = (I->iBE - I->iBC) / I->qB;
+    }
+    I->iC = I->iCE - I->iBC / I->tBetaR - I->iBCleak;
+    I->iB = I->iBE / I->tBetaF + I->iBEleak + I->iBC / I->tBetaR + I->iBCleak;
+    I->iE = -I->iC - I->iB;
+    if (td != 0) {
+      if (!(ss->dcopFlag)) {
+        I->diCEdvBp = I->invqB * (-I->invqB * I->iBC * I->dqBdvBp - I->gBC);
+        I->diCEdvCp = I->invqB * (-I->invqB * I->iBC * I->dqBdvCp + I->gBC);
+        I->diCEdvEp = -I->invqB * I->invqB * I->iBC * I->dqBdvEp;
+      } else {
+        I->diCEdvBp = I->invqB * (I->iCE * I->dqBdvBp + I->gBE - I->gBC);
+        I->diCEdvCp = I->invqB * (I->iCE * I->dqBdvCp + I->gBC);
+        I->diCEdvEp = I->invqB * (I->iCE * I->dqBdvEp - I->gBE);
+      }
+      I->diBEdvBp = I->invqB * (I->invqB * I->iBE * I->dqBdvBp + I->gBE);
+      I->diBEdvCp = I->invqB * I->invqB * I->iBE * I->dqBdvCp;
+      I->diBEdvEp = I->invqB * (I->invqB * I->iBE * I->dqBdvEp - I->gBE);
+    } else {
+      I->diCEdvBp = I->invqB * (I->iCE * I->dqBdvBp + I->gBE - I->gBC);
+      I->diCEdvCp = I->invqB * (I->iCE * I->dqBdvCp + I->gBC);
+      I->diCEdvEp = I->invqB * (I->iCE * I->dqBdvEp - I->gBE);
+      I->diBEdvBp = I->diBEdvCp = I->diBEdvEp = 0.0;
+    }
+  } else {
   if (!(ss->dcopFlag)) {
     double pdt = ss->pdt;
     if (ss->initTranFlag && ss->newtonIter == 0) pdt = 0.0;
@@ -699,6 +770,7 @@ static int update_intermediate(struct or_group_s* g, binst* I, const xa_solver_s
     I->diCEdvEp = I->invqB * (I->iCE * I->dqBdvEp - gEX);
     I->diCEdvCp = I->invqB * (I->iCE * I->dqBdvCp + I->gBC);
     I->diCEdvBp = I->invqB * (I->iCE * I->dqBdvBp + gEX - I->gBC);
+  }
   }
   I->gEpr = m->emitterConduct * I->AREA;
   I->gCpr = m->collectorConduct * I->AREA;
@@ -777,7 +849,7 @@ static int update_secondary(struct or_group_s* g, const xa_solver_state* ss, con
   return XA_OK;
 }
 
-/* Master::loadDAEVectors (:3978-4142), newExcessPhase = 0 */
+/* Master::loadDAEVectors (:3978-4142) */
 static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const xa_vectors* v) {
   int64_t i;
   binst* in = (binst*)g->inst;
@@ -795,6 +867,25 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
       OR_SUB(v->f, bi->li_CollP, bi->vCCp * bi->gCpr + T * (-bi->iC));
       OR_SUB(v->f, bi->li_BaseP, bi->vBBp * bi->gX - T * (bi->iB));
       OR_SUB(v->f, bi->li_EmitP, bi->vEEp * bi->gEpr + T * (-bi->iE));
+      if (g->opt.newExcessPhase) {
+        /* excess phase rows (:3997-4024); the dc operating point's
+         * assignment of the dIfx entry is an addition of zero onto the
+         * zeroed vector (no other device writes that row) */
+        const double td = bi->model->excessPhaseFac;
+        const double i_fx = sol(v->nextSol, bi->li_Ifx), di_fx = sol(v->nextSol, bi->li_dIfx);
+        if (td != 0) {
+          if (!(ss->dcopFlag)) {
+            OR_ADD(v->f, bi->li_Ifx, -di_fx);
+            OR_ADD(v->f, bi->li_dIfx, 3 * di_fx * td + 3 * i_fx - 3 * bi->iBE / bi->qB);
+          } else {
+            OR_ADD(v->f, bi->li_Ifx, i_fx - bi->iBE / bi->qB);
+            OR_ADD(v->f, bi->li_dIfx, 0.0);
+          }
+        } else {
+          OR_ADD(v->f, bi->li_Ifx, i_fx);
+          OR_ADD(v->f, bi->li_dIfx, di_fx);
+        }
+      }
     }
     if (v->q) {
       OR_SUB(v->q, bi->li_Base, -T * bi->qBX);
@@ -802,9 +893,30 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
       OR_SUB(v->q, bi->li_CollP, T * (bi->qCS + bi->qBX + bi->qBCdep + bi->qBCdiff));
       OR_SUB(v->q, bi->li_BaseP, -T * (bi->qBEdep + bi->qBEdiff + bi->qBCdep + bi->qBCdiff));
       OR_SUB(v->q, bi->li_EmitP, T * (bi->qBEdep + bi->qBEdiff));
+      if (g->opt.newExcessPhase && bi->model->excessPhaseFac != 0) {
+        /* :4033-4046 */
+        const double td = bi->model->excessPhaseFac;
+        OR_ADD(v->q, bi->li_Ifx, sol(v->nextSol, bi->li_Ifx));
+        OR_ADD(v->q, bi->li_dIfx, !(ss->dcopFlag) ? sol(v->nextSol, bi->li_dIfx) * td * td : 0.0);
+      }
     }
     if (g->opt.voltageLimiterFlag) {
       double Cp_f = 0.0, Bp_f = 0.0, Ep_f = 0.0, Cp_q = 0.0, Ep_q = 0.0, Bp_q = 0.0;
+      double dIfx_f = 0.0, Ifx_f = 0.0;
+      const int nep = g->opt.newExcessPhase && bi->model->excessPhaseFac != 0;
+      if (nep && (!bi->origFlag || bi->offFlag)) {
+        /* :4069-4081 */
+        if (or_mag) {
+          const double t = fabs(bi->diBEdvBp * vbe_diff) + fabs(bi->diBEdvCp * vce_diff);
+          if (!(ss->dcopFlag)) dIfx_f = 3 * t; else Ifx_f = t;
+        } else if (!(ss->dcopFlag)) {
+          dIfx_f = -3 * (bi->diBEdvBp * vbe_diff + bi->diBEdvCp * vce_diff);
+          dIfx_f *= T;
+        } else {
+          Ifx_f = (bi->diBEdvBp * vbe_diff + bi->diBEdvCp * vce_diff);
+          Ifx_f *= T;
+        }
+      }
       if (or_mag && (!bi->origFlag || bi->offFlag)) {
         /* magnitude mode: the rounding scale of each correction is the sum
          * of its terms' magnitudes (they cancel near convergence) */
@@ -824,6 +936,10 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
         OR_ADD(v->dFdxdVp, bi->li_CollP, Cp_f);
         OR_ADD(v->dFdxdVp, bi->li_BaseP, Bp_f);
         OR_ADD(v->dFdxdVp, bi->li_EmitP, Ep_f);
+        if (nep) {
+          if (!(ss->dcopFlag)) OR_ADD(v->dFdxdVp, bi->li_dIfx, dIfx_f);
+          else OR_ADD(v->dFdxdVp, bi->li_Ifx, Ifx_f);
+        }
       }
       if (or_mag && (!bi->origFlag || bi->offFlag)) {
         Cp_q = fabs((bi->capBCdep + bi->capBCdiff) * vbc_diff);
@@ -861,15 +977,17 @@ static int load_vectors(struct or_group_s* g, const xa_solver_state* ss, const x
   return XA_OK;
 }
 
-/* Master::loadDAEMatrices (:4152-4260), newExcessPhase = 0 */
+/* Master::loadDAEMatrices (:4152-4260) */
 static int load_matrices(struct or_group_s* g, const xa_solver_state* ss, const xa_vectors* v) {
   int64_t i;
   binst* in = (binst*)g->inst;
   double* F = v->dFdx;
   double* Q = v->dQdx;
-  (void)ss;
   for (i = 0; i < g->n; ++i) {
     binst* bi = &in[i];
+    const int T = bi->model->TYPE;
+    const double td = bi->model->excessPhaseFac;
+    const int nep = g->opt.newExcessPhase && td != 0;
     if (F) {
       OR_ADD(F, bi->f_CC, bi->gCpr);
       OR_SUB(F, bi->f_CCp, bi->gCpr);
@@ -883,6 +1001,7 @@ static int load_matrices(struct or_group_s* g, const xa_solver_state* ss, const 
       OR_ADD(F, bi->f_CpCp, bi->diCEdvCp + bi->gBCtot + bi->gCpr);
       OR_ADD(F, bi->f_CpBp, bi->diCEdvBp - bi->gBCtot);
       OR_ADD(F, bi->f_CpEp, bi->diCEdvEp);
+      if (nep && !(ss->dcopFlag)) OR_ADD(F, bi->f_CpIfx, 1.0 * T);
       OR_SUB(F, bi->f_BpB, bi->diBrdvB);
       OR_ADD(F, bi->f_BpCp, -bi->diBrdvCp - bi->gBCtot);
       OR_ADD(F, bi->f_BpBp, -bi->diBrdvBp + bi->gBEtot + bi->gBCtot);
@@ -891,6 +1010,29 @@ static int load_matrices(struct or_group_s* g, const xa_solver_state* ss, const 
       OR_ADD(F, bi->f_EpCp, -bi->diCEdvCp);
       OR_ADD(F, bi->f_EpBp, -bi->diCEdvBp - bi->gBEtot);
       OR_ADD(F, bi->f_EpEp, bi->gBEtot + bi->gEpr + bi->diCEdvBp + bi->diCEdvCp);
+      if (nep && !(ss->dcopFlag)) OR_ADD(F, bi->f_EpIfx, -1.0 * T);
+      if (g->opt.newExcessPhase) {
+        /* :4196-4233 */
+        if (td != 0) {
+          if (!(ss->dcopFlag)) {
+            OR_ADD(F, bi->f_IfxdIfx, -1);
+            OR_ADD(F, bi->f_dIfxCp, -3 * bi->diBEdvCp * T);
+            OR_ADD(F, bi->f_dIfxBp, -3 * bi->diBEdvBp * T);
+            OR_ADD(F, bi->f_dIfxEp, -3 * bi->diBEdvEp * T);
+            OR_ADD(F, bi->f_dIfxdIfx, 3 * td);
+            OR_ADD(F, bi->f_dIfxIfx, 3);
+          } else {
+            OR_ADD(F, bi->f_IfxCp, -bi->diBEdvCp * T);
+            OR_ADD(F, bi->f_IfxBp, -bi->diBEdvBp * T);
+            OR_ADD(F, bi->f_IfxEp, -bi->diBEdvEp * T);
+            OR_ADD(F, bi->f_IfxIfx, 1);
+            OR_ADD(F, bi->f_dIfxdIfx, 1);
+          }
+        } else {
+          OR_ADD(F, bi->f_IfxIfx, 1);
+          OR_ADD(F, bi->f_dIfxdIfx, 1);
+        }
+      }
     }
     if (Q) {
       OR_ADD(Q, bi->q_BB, bi->capBX);
@@ -907,6 +1049,11 @@ static int load_matrices(struct or_group_s* g, const xa_solver_state* ss, const 
       OR_ADD(Q, bi->q_EpCp, bi->capeqCB);
       OR_ADD(Q, bi->q_EpBp, -bi->capBEdiff - bi->capBEdep - bi->capeqCB);
       OR_ADD(Q, bi->q_EpEp, bi->capBEdiff + bi->capBEdep);
+      if (nep && !(ss->dcopFlag)) {
+        /* :4253-4260 */
+        OR_ADD(Q, bi->q_IfxIfx, 1);
+        OR_ADD(Q, bi->q_dIfxdIfx, 1 * td * td);
+      }
     }
   }
   return XA_OK;
@@ -921,6 +1068,8 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
     else if (!strcmp(name, "iB")) out[i] = I->iB;
     else if (!strcmp(name, "iE")) out[i] = I->iE;
     else if (!strcmp(name, "qB")) out[i] = I->qB;
+    else if (!strcmp(name, "iBE")) out[i] = I->iBE;
+    else if (!strcmp(name, "iBC")) out[i] = I->iBC;
     else if (!strcmp(name, "vBE")) out[i] = I->vBE;
     else if (!strcmp(name, "vBC")) out[i] = I->vBC;
     else if (!strcmp(name, "gX")) out[i] = I->gX;

@@ -46,13 +46,15 @@ def run_parity(p, rng, iters=ITERS, fused=False):
 
 
 def twin_drive(model, spec, seed, rng, fused_b, topology="private", lead=None, iters=ITERS,
-               prepare_b=None, ground_frac=0.05):
+               prepare_b=None, ground_frac=0.05, options=None):
     """Two groups built from the same circuit and driven with the same inputs:
     the first through the four calls, the second fused (fused_b) or also
     through the four calls; their GPU outputs must be equal bit for bit.
     prepare_b(pair) runs on the second group before the inputs are set."""
-    pa = Pair(model, spec, topology=topology, seed=seed, ground_frac=ground_frac, lead=lead)
-    pb = Pair(model, spec, topology=topology, seed=seed, ground_frac=ground_frac, lead=lead)
+    pa = Pair(model, spec, topology=topology, seed=seed, ground_frac=ground_frac, lead=lead,
+              options=options)
+    pb = Pair(model, spec, topology=topology, seed=seed, ground_frac=ground_frac, lead=lead,
+              options=options)
     if prepare_b is not None:
         prepare_b(pb)
     cs = rng.uniform(-0.5, 1.5, pa.topo.n_sto)

@@ -114,6 +114,14 @@ def test_bjt_bitexact_ic_off_inputop():
               flag_sol=(rng.random(q.topo.n_nodes) < 0.9).astype(np.float64))
 
 
+def test_bjt_bitexact_new_excess_phase():
+    import oracle
+    spec = bjt_spec(6000, seed=426, temp_spread=True)
+    p = _pair("bjt", spec, options=oracle.default_options(newExcessPhase=1), seed=427,
+              topology="shared", ground_frac=0.05)
+    run_exact(p, np.random.default_rng(428), iters=BJT_ITERS)
+
+
 def test_bjt_bitexact_limiter_off():
     import oracle
     spec = bjt_spec(4000, seed=410, lambertw_frac=0.25)

@@ -5,7 +5,9 @@ libm noise; at most 0.05% outliers, none beyond 1e-6).  Covered: the parity
 cards (internal collector/base/emitter nodes, high injection with roll-off,
 Early effect both ways, leakage currents, base-current crowding, substrate
 and transit-time charges with the VTF/ITF bias dependence, old-style excess
-phase), Lambert-W junction currents, lead currents, Newton iterations of DC,
+phase, and the new-DAE excess phase of DeviceOptions newExcessPhase with its
+two unknowns per instance), Lambert-W junction currents, lead currents,
+Newton iterations of DC,
 transient-operating-point and transient analysis (first step with
 beginIntegration, later steps with unequal step sizes), initial conditions,
 OFF instances under initJct/initFix, inputOP flags, limiter on and off,
@@ -91,3 +93,26 @@ def test_bjt_run_to_run_bit_identical():
     print("bjt determinism n=6000", flush=True)
     twin_drive("bjt", spec, 417, np.random.default_rng(418), fused_b=False, topology="shared",
                iters=BJT_ITERS)
+
+
+@pytest.mark.parametrize("topology", ["private", "shared"])
+def test_bjt_new_excess_phase_parity(topology):
+    """newExcessPhase = 1: every card (one with PTF, three without: both
+    stamp forms in one group), DC, operating point and transient
+    iterations."""
+    import oracle
+    n = 12000
+    spec = bjt_spec(n, seed=420, temp_spread=True)
+    lead = (np.arange(n) % 5 == 0).astype(np.int32)
+    print(f"bjt new excess phase {topology} n={n}", flush=True)
+    p = Pair("bjt", spec, topology=topology, seed=421, lead=lead, ground_frac=0.05,
+             options=oracle.default_options(newExcessPhase=1), noise_runs=NOISE_RUNS)
+    run_parity(p, np.random.default_rng(422), iters=BJT_ITERS)
+
+
+def test_bjt_new_excess_phase_fused_equals_split():
+    import oracle
+    spec = bjt_spec(6000, seed=423)
+    print("bjt new excess phase fused vs split n=6000", flush=True)
+    twin_drive("bjt", spec, 424, np.random.default_rng(425), fused_b=True, iters=BJT_ITERS,
+               options=oracle.default_options(newExcessPhase=1))

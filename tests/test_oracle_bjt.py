@@ -154,3 +154,112 @@ def test_alias_name_acts_as_primary():
     rb, _ = pb.load(**TRAN)
     for k in ("f", "q", "dFdx", "dQdx"):
         np.testing.assert_array_equal(ra[k], rb[k], err_msg=k)
+
+
+# ---- the new-DAE excess phase (DeviceOptions newExcessPhase = 1) ----------
+# Two internal unknowns per instance, the phase-shifted transfer current i_fx
+# and its time derivative (N_DEV_BJT.C:1032-1036, :1149-1153); the collector
+# current takes i_fx in transient (auxDAECalculations :1928-1980).
+EP_CARD = BJT_CARDS[3]          # PTF = 30 degrees, TF = 0.3 ns
+
+
+def _ep_rows(p):
+    """(Ifx, dIfx) LIDs of every instance: its last two internal nodes."""
+    nodes_per = p.counts[0] + p.counts[1]
+    ends = np.cumsum(nodes_per)
+    return p.topo.node_lids[ends - 2], p.topo.node_lids[ends - 1]
+
+
+def _ep_pair(n=32, seed=21, card=EP_CARD):
+    p = _pair(bjt_spec(n, seed=seed, cards=[card], temp_spread=True), newExcessPhase=1)
+    p.set_inputs(nextSto=np.zeros(p.topo.n_sto))
+    return p
+
+
+def test_new_excess_phase_fd_jacobian():
+    p = _ep_pair()
+    x = _small_bias(p, 31)
+    ifx, difx = _ep_rows(p)
+    assert not fd_check(p, x, TRAN), "transient Jacobian"
+    # at the operating point the reference keeps a unit dIfx diagonal while
+    # the dIfx row of F is zero, and Q's Ifx row (i_fx) has no dQ/dx stamp:
+    # those two rows are the reference's own inconsistency, the rest is exact
+    ex_f = np.zeros(p.topo.n_nodes, bool)
+    ex_f[difx] = True
+    ex_q = np.zeros(p.topo.n_nodes, bool)
+    ex_q[ifx] = True
+    assert not fd_check(p, x, DCOP, vecs=("f",), exclude=ex_f)
+    assert not fd_check(p, x, DCOP, vecs=("q",), exclude=ex_q)
+
+
+def test_new_excess_phase_filter_is_second_order_bessel():
+    """The Ifx/dIfx block of the DAE is the excess-phase filter
+    H(s) = 3 / (td^2 s^2 + 3 td s + 3), td = PTF (radians) * TF: the
+    determinant of s dQ/dx + dF/dx over the block has those coefficients."""
+    p = _ep_pair(n=4)
+    x = _small_bias(p, 32)
+    p.set_inputs(nextSol=x)
+    ro, _ = p.load(**TRAN)
+    ifx, difx = _ep_rows(p)
+    rp, ci = p.topo.row_ptr, p.topo.col_idx
+    td = EP_CARD[1]["PTF"] * np.pi / 180.0 * EP_CARD[1]["TF"]
+
+    def entry(vals, r, c):
+        k = rp[r] + np.nonzero(ci[rp[r]:rp[r + 1]] == c)[0]
+        return float(vals[k].sum()) if len(k) else 0.0
+    for a, b in zip(ifx, difx):
+        G = np.array([[entry(ro["dFdx"], a, a), entry(ro["dFdx"], a, b)],
+                      [entry(ro["dFdx"], b, a), entry(ro["dFdx"], b, b)]])
+        Cm = np.array([[entry(ro["dQdx"], a, a), entry(ro["dQdx"], a, b)],
+                       [entry(ro["dQdx"], b, a), entry(ro["dQdx"], b, b)]])
+        # det(s C + G) = c2 s^2 + c1 s + c0
+        c2 = np.linalg.det(Cm)
+        c0 = np.linalg.det(G)
+        c1 = Cm[0, 0] * G[1, 1] + G[0, 0] * Cm[1, 1] - Cm[0, 1] * G[1, 0] - G[0, 1] * Cm[1, 0]
+        np.testing.assert_allclose([c2, c1, c0], [td * td, 3 * td, 3.0], rtol=1e-12)
+
+
+def test_new_excess_phase_steady_state_is_the_dc_current():
+    """With i_fx at its steady state iBE/qB and di_fx = 0, the transient
+    loads carry the operating point's terminal currents, and both excess
+    rows vanish."""
+    p = _ep_pair(n=16)
+    x = _small_bias(p, 33)
+    ifx, difx = _ep_rows(p)
+    p.set_inputs(nextSol=x)
+    p.load(**DCOP)
+    iBE = p.orc.read("iBE")
+    qB = p.orc.read("qB")
+    x[ifx] = iBE / qB
+    x[difx] = 0.0
+    p.set_inputs(nextSol=x)
+    dc, _ = p.load(**DCOP)
+    tr, _ = p.load(**TRAN)
+    mask = np.ones(p.topo.n_nodes, bool)
+    mask[ifx] = mask[difx] = False
+    np.testing.assert_allclose(tr["f"][mask], dc["f"][mask], rtol=1e-12,
+                               atol=1e-12 * np.max(np.abs(dc["f"][mask])))
+    scale = np.abs(iBE / qB).max()
+    assert np.max(np.abs(tr["f"][ifx])) == 0.0
+    assert np.max(np.abs(tr["f"][difx])) <= 1e-12 * 3 * scale
+
+
+def test_new_excess_phase_off_without_ptf():
+    """PTF = 0 (td = 0): the excess rows are identities (F = i_fx, di_fx;
+    unit Jacobian diagonal) and the terminal currents are the card's
+    without excess phase."""
+    card = (1, {k: v for k, v in EP_CARD[1].items() if k != "PTF"})
+    p = _ep_pair(n=16, card=card)
+    q = _pair(bjt_spec(16, seed=21, cards=[card], temp_spread=True))
+    q.set_inputs(nextSto=np.zeros(q.topo.n_sto))
+    x = _small_bias(p, 34)
+    ifx, difx = _ep_rows(p)
+    keep = np.ones(p.topo.n_nodes, bool)
+    keep[ifx] = keep[difx] = False
+    p.set_inputs(nextSol=x)
+    q.set_inputs(nextSol=x[keep])
+    rp_, _ = p.load(**TRAN)
+    rq_, _ = q.load(**TRAN)
+    np.testing.assert_array_equal(rp_["f"][ifx], x[ifx])
+    np.testing.assert_array_equal(rp_["f"][difx], x[difx])
+    np.testing.assert_array_equal(rp_["f"][keep], rq_["f"])

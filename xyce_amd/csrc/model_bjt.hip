@@ -10,8 +10,10 @@
 //   updateIntermediateVars :2476-3207, Model::processParams :3290-3361,
 //   Model ctor :3394-3798, Master::updateState :3891-3937,
 //   Master::loadDAEVectors :3978-4142, Master::loadDAEMatrices :4152-4260.
-// newExcessPhase = 1 (two extra excess-phase nodes) is rejected at setup
-// (XA_ERR_UNSUPPORTED).  updateSecondaryState only fills instance currents
+// newExcessPhase = 1 takes the new-DAE excess phase: two internal unknowns
+// per instance (i_fx and its derivative, :1032-1036), the currents from
+// auxDAECalculations (:1928-1980) and the Master's excess-phase rows
+// (:3997-4260).  updateSecondaryState only fills instance currents
 // that no load reads, so it launches nothing.
 #include <cmath>
 #include <cstring>
@@ -77,7 +79,10 @@ enum DField {
 enum IField { F_MODEL, F_FLAGS, F_INTNODES, NIF };
 enum Flag { FL_OFF = 1, FL_ICGIVEN = 2, FL_LAMBERTW = 4 };
 enum IntNode { IN_C = 1, IN_B = 2, IN_E = 4 };
-enum Node { N_C, N_B, N_E, N_S, N_CP, N_BP, N_EP, NNODE };
+// N_IFX, N_DIFX: the new-DAE excess phase's unknowns (i_fx and its time
+// derivative), present only with DeviceOptions newExcessPhase
+enum Node { N_C, N_B, N_E, N_S, N_CP, N_BP, N_EP, N_IFX, N_DIFX, NNODE };
+constexpr int16_t N_NONE = -1;
 enum Sto { T_vBE, T_vBC, T_capeqCB, T_ib, T_ie, T_ic, T_is, NSTO };
 enum Sta { A_qBEdiff, A_qBEdep, A_qCS, A_qBCdiff, A_qBCdep, A_qBX, A_CEXBC, NSTA };
 
@@ -177,8 +182,12 @@ static int model_init(HostModel& m, const Card& c, const xa_device_options& opt)
 }
 
 static int setup(Group& g) {
-  if (g.opt.newExcessPhase) return fail(XA_ERR_UNSUPPORTED, "bjt: newExcessPhase");
   const int64_t n = g.n;
+  if (g.opt.newExcessPhase) {
+    // the excess-phase rows and columns follow each kind's base entries
+    const int E[NK] = {8, 7, 5, 3, 32, 16, 4, 4};
+    for (int k = 0; k < NK; ++k) g.ent[k] = E[k];
+  }
   std::vector<HostModel> hm(g.models.size());
   for (size_t k = 0; k < g.models.size(); ++k) {
     if (g.models[k].level != 1) return fail(XA_ERR_UNSUPPORTED, "bjt: level must be 1");
@@ -200,7 +209,7 @@ static int setup(Group& g) {
     const double TEMP = g.igiven[I_TEMP] ? iv(I_TEMP, i) : g.opt.temp;
     g.n_ext[i] = 4;
     g.n_int[i] = (m.collectorResist == 0.0 ? 0 : 1) + (m.emitterResist == 0.0 ? 0 : 1) +
-                 (m.baseResist == 0.0 ? 0 : 1);
+                 (m.baseResist == 0.0 ? 0 : 1) + (g.opt.newExcessPhase ? 2 : 0);   // :1032-1036
     g.n_sta[i] = NSTA;
     g.n_sto[i] = 3 + (g.lead[i] ? 4 : 0);
 
@@ -302,6 +311,9 @@ static void resolve_nodes(const Group& g, int64_t i, const int32_t* reg, int32_t
   full[N_CP] = (in & IN_C) ? reg[loc++] : reg[0];
   full[N_BP] = (in & IN_B) ? reg[loc++] : reg[1];
   full[N_EP] = (in & IN_E) ? reg[loc++] : reg[2];
+  // :1149-1153
+  full[N_IFX] = g.opt.newExcessPhase ? reg[loc++] : -1;
+  full[N_DIFX] = g.opt.newExcessPhase ? reg[loc++] : -1;
 }
 
 // Stamp tables in the order of the reference's loads.
@@ -319,10 +331,60 @@ static const Stamp kStDQ[] = {
   {N_EP, N_EP}};
 // lead currents: store slots ib, ie, ic, is
 static const Stamp kStLead[] = {{T_ib, 0}, {T_ie, 0}, {T_ic, 0}, {T_is, 0}};
+// newExcessPhase (:908-931): the base lists, then the excess-phase entries --
+// F and Q rows Ifx, dIfx; the limiter's Ifx (operating point) and dIfx
+// (transient) rows; dF/dx (C',Ifx) (E',Ifx), the transient rows
+// (Ifx,dIfx) (dIfx,C') (dIfx,B') (dIfx,E') (dIfx,dIfx) (dIfx,Ifx) and the
+// operating point's (Ifx,C') (Ifx,B') (Ifx,E') (Ifx,Ifx); dQ/dx (Ifx,Ifx)
+// (dIfx,dIfx).  An entry the current mode does not stamp is loaded as 0.
+static const Stamp kStFX[] = {{N_C, 0}, {N_B, 0}, {N_E, 0}, {N_CP, 0}, {N_BP, 0}, {N_EP, 0},
+                              {N_IFX, 0}, {N_DIFX, 0}};
+static const Stamp kStQX[] = {{N_B, 0}, {N_S, 0}, {N_CP, 0}, {N_BP, 0}, {N_EP, 0},
+                              {N_IFX, 0}, {N_DIFX, 0}};
+static const Stamp kStFLimX[] = {{N_CP, 0}, {N_BP, 0}, {N_EP, 0}, {N_IFX, 0}, {N_DIFX, 0}};
+static const Stamp kStDFX[] = {
+  {N_C, N_C}, {N_C, N_CP}, {N_B, N_B}, {N_B, N_CP}, {N_B, N_BP}, {N_B, N_EP}, {N_E, N_E},
+  {N_E, N_EP}, {N_CP, N_C}, {N_CP, N_CP}, {N_CP, N_BP}, {N_CP, N_EP}, {N_BP, N_B},
+  {N_BP, N_CP}, {N_BP, N_BP}, {N_BP, N_EP}, {N_EP, N_E}, {N_EP, N_CP}, {N_EP, N_BP},
+  {N_EP, N_EP},
+  {N_CP, N_IFX}, {N_EP, N_IFX}, {N_IFX, N_DIFX}, {N_DIFX, N_CP}, {N_DIFX, N_BP},
+  {N_DIFX, N_EP}, {N_DIFX, N_DIFX}, {N_DIFX, N_IFX}, {N_IFX, N_CP}, {N_IFX, N_BP},
+  {N_IFX, N_EP}, {N_IFX, N_IFX}};
+static const Stamp kStDQX[] = {
+  {N_B, N_B}, {N_B, N_CP}, {N_S, N_S}, {N_S, N_CP}, {N_CP, N_B}, {N_CP, N_S}, {N_CP, N_CP},
+  {N_CP, N_BP}, {N_BP, N_CP}, {N_BP, N_BP}, {N_BP, N_EP}, {N_EP, N_CP}, {N_EP, N_BP},
+  {N_EP, N_EP}, {N_IFX, N_IFX}, {N_DIFX, N_DIFX}};
+
+static double card_td(const Group& g, int64_t i) {
+  return reinterpret_cast<const DevCard*>(&g.mcard[(size_t)g.inst_model[i] * g.mcard_stride])
+      ->tdExcess;
+}
 
 static void stamps(const Group& g, int64_t i, int kind, std::vector<Stamp>& out) {
   out.clear();
   auto add = [&](const Stamp* s, int m) { out.assign(s, s + m); };
+  if (g.opt.newExcessPhase) {
+    // an instance without excess phase (PTF 0) loads its F rows and the
+    // unit dF/dx diagonals of Ifx and dIfx (:4226-4231), zeros in its other
+    // excess-phase dF/dx entries (they are in the jacStamp all the same)
+    const bool td = card_td(g, i) != 0;
+    switch (kind) {
+      case K_F: add(kStFX, 8); break;
+      case K_Q: add(kStQX, 7); if (!td) out[5].row = out[6].row = N_NONE; break;
+      case K_FLIM:
+        if (g.opt.voltageLimiterFlag) {
+          add(kStFLimX, 5);
+          if (!td) out[3].row = out[4].row = N_NONE;
+        }
+        break;
+      case K_QLIM: if (g.opt.voltageLimiterFlag) add(kStLim, 3); break;
+      case K_DFDX: add(kStDFX, 32); break;   // the pattern is the jacStamp's, td or not
+      case K_DQDX: add(kStDQX, 16); if (!td) out[14].row = out[15].row = N_NONE; break;
+      case K_LEADF:
+      case K_LEADQ: if (g.lead[i]) add(kStLead, 4); break;
+    }
+    return;
+  }
   switch (kind) {
     case K_F: add(kStF, 6); break;
     case K_Q: add(kStQ, 5); break;
@@ -585,14 +647,45 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     gEX = gBE * phaseScalar;
     a.v.nextSta[acx] = iC + iEX / qB;
   }
-  const double iCE = (iEX - iBC) / qB;
+  double iCE = (iEX - iBC) / qB;
   const double tBetaF = dfld(a, D_TBETAF, i), tBetaR = dfld(a, D_TBETAR, i);
   iC += iCE - iBC / tBetaR - iBCleak;
   const double iB = iBE / tBetaF + iBEleak + iBC / tBetaR + iBCleak;
-  const double iE = -iC - iB;
-  const double diCEdvEp = invqB * (iCE * dqBdvEp - gEX);
-  const double diCEdvCp = invqB * (iCE * dqBdvCp + gBC);
-  const double diCEdvBp = invqB * (iCE * dqBdvBp + gEX - gBC);
+  double iE = -iC - iB;
+  double diCEdvEp = invqB * (iCE * dqBdvEp - gEX);
+  double diCEdvCp = invqB * (iCE * dqBdvCp + gBC);
+  double diCEdvBp = invqB * (iCE * dqBdvBp + gEX - gBC);
+  const bool nep = a.opt.newExcessPhase != 0;
+  double i_fx = 0.0, di_fx = 0.0, diBEdvBp = 0.0, diBEdvCp = 0.0, diBEdvEp = 0.0;
+  if (nep) {
+    // auxDAECalculations (:1928-1980): the new-DAE currents take i_fx from
+    // the solution; the old calculation above still updates its state
+    const int32_t lX = node(a, N_IFX, i), lD = node(a, N_DIFX, i);
+    i_fx = solv(a.v.nextSol, lX);
+    di_fx = solv(a.v.nextSol, lD);
+    if (td != 0 && !ss.dcopFlag) iCE = i_fx - iBC / qB;
+    else iCE = (iBE - iBC) / qB;
+    iC = iCE - iBC / tBetaR - iBCleak;
+    iE = -iC - iB;
+    if (td != 0) {
+      if (!ss.dcopFlag) {
+        diCEdvBp = invqB * (-invqB * iBC * dqBdvBp - gBC);
+        diCEdvCp = invqB * (-invqB * iBC * dqBdvCp + gBC);
+        diCEdvEp = -invqB * invqB * iBC * dqBdvEp;
+      } else {
+        diCEdvBp = invqB * (iCE * dqBdvBp + gBE - gBC);
+        diCEdvCp = invqB * (iCE * dqBdvCp + gBC);
+        diCEdvEp = invqB * (iCE * dqBdvEp - gBE);
+      }
+      diBEdvBp = invqB * (invqB * iBE * dqBdvBp + gBE);
+      diBEdvCp = invqB * invqB * iBE * dqBdvCp;
+      diBEdvEp = invqB * (invqB * iBE * dqBdvEp - gBE);
+    } else {
+      diCEdvBp = invqB * (iCE * dqBdvBp + gBE - gBC);
+      diCEdvCp = invqB * (iCE * dqBdvCp + gBC);
+      diCEdvEp = invqB * (iCE * dqBdvEp - gBE);
+    }
+  }
   const double gEpr = mc.gEmit * AREA, gCpr = mc.gColl * AREA;
   double gX;
   {
@@ -639,6 +732,26 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   s.put(K_F, 3, -(vCCp * gCpr + T * (-iC)));
   s.put(K_F, 4, -(vBBp * gX - T * (iB)));
   s.put(K_F, 5, -(vEEp * gEpr + T * (-iE)));
+  if (nep) {
+    // :3997-4024 (the operating point's assignment of the dIfx entry adds
+    // zero: no other device writes that row)
+    if (td != 0) {
+      if (!ss.dcopFlag) {
+        s.put(K_F, 6, -di_fx);
+        s.put(K_F, 7, 3 * di_fx * td + 3 * i_fx - 3 * iBE / qB);
+      } else {
+        s.put(K_F, 6, i_fx - iBE / qB);
+        s.put(K_F, 7, 0.0);
+      }
+    } else {
+      s.put(K_F, 6, i_fx);
+      s.put(K_F, 7, di_fx);
+    }
+    if (td != 0) {   // :4033-4046
+      s.put(K_Q, 5, i_fx);
+      s.put(K_Q, 6, !ss.dcopFlag ? di_fx * td * td : 0.0);
+    }
+  }
   s.put(K_Q, 0, -(-T * qBX));
   s.put(K_Q, 1, -(-T * qCS));
   s.put(K_Q, 2, -(T * (qCS + qBX + qBCdep + qBCdiff)));
@@ -667,6 +780,20 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
     s.put(K_FLIM, 0, Cp_f);
     s.put(K_FLIM, 1, Bp_f);
     s.put(K_FLIM, 2, Ep_f);
+    if (nep && td != 0) {
+      // :4069-4100: the dIfx row in transient, the Ifx row at the operating
+      // point
+      double ex = 0.0;
+      if (!origFlag || offFlag) {
+        const double vbe_diff = vBE - vBE_orig, vbc_diff = vBC - vBC_orig;
+        const double vce_diff = vbe_diff - vbc_diff;
+        ex = !ss.dcopFlag ? -3 * (diBEdvBp * vbe_diff + diBEdvCp * vce_diff)
+                          : (diBEdvBp * vbe_diff + diBEdvCp * vce_diff);
+        ex *= T;
+      }
+      s.put(K_FLIM, 3, ss.dcopFlag ? ex : 0.0);
+      s.put(K_FLIM, 4, ss.dcopFlag ? 0.0 : ex);
+    }
     s.put(K_QLIM, 0, Cp_q);
     s.put(K_QLIM, 1, Bp_q);
     s.put(K_QLIM, 2, Ep_q);
@@ -702,6 +829,22 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   s.put(K_DFDX, 17, -diCEdvCp);
   s.put(K_DFDX, 18, -diCEdvBp - gBEtot);
   s.put(K_DFDX, 19, gBEtot + gEpr + diCEdvBp + diCEdvCp);
+  if (nep) {
+    // :4177-4233 (an entry the mode does not stamp takes 0)
+    const bool tr = td != 0 && !ss.dcopFlag, dc = td != 0 && ss.dcopFlag;
+    s.put(K_DFDX, 20, tr ? 1.0 * T : 0.0);
+    s.put(K_DFDX, 21, tr ? -1.0 * T : 0.0);
+    s.put(K_DFDX, 22, tr ? -1.0 : 0.0);
+    s.put(K_DFDX, 23, tr ? -3 * diBEdvCp * T : 0.0);
+    s.put(K_DFDX, 24, tr ? -3 * diBEdvBp * T : 0.0);
+    s.put(K_DFDX, 25, tr ? -3 * diBEdvEp * T : 0.0);
+    s.put(K_DFDX, 26, tr ? 3 * td : 1.0);
+    s.put(K_DFDX, 27, tr ? 3.0 : 0.0);
+    s.put(K_DFDX, 28, dc ? -diBEdvCp * T : 0.0);
+    s.put(K_DFDX, 29, dc ? -diBEdvBp * T : 0.0);
+    s.put(K_DFDX, 30, dc ? -diBEdvEp * T : 0.0);
+    s.put(K_DFDX, 31, tr ? 0.0 : 1.0);
+  }
   s.put(K_DQDX, 0, capBX);
   s.put(K_DQDX, 1, -capBX);
   s.put(K_DQDX, 2, capCS);
@@ -716,6 +859,10 @@ __device__ __forceinline__ void eval(const KArgs& a, int64_t i) {
   s.put(K_DQDX, 11, capeqCB);
   s.put(K_DQDX, 12, -capBEdiff - capBEdep - capeqCB);
   s.put(K_DQDX, 13, capBEdiff + capBEdep);
+  if (nep && td != 0) {   // :4253-4260
+    s.put(K_DQDX, 14, !ss.dcopFlag ? 1.0 : 0.0);
+    s.put(K_DQDX, 15, !ss.dcopFlag ? 1 * td * td : 0.0);
+  }
 }
 
 template <bool FUSED>

@@ -10,6 +10,7 @@ that holds every sample's instances (N_LOA_HBLoader.C:540-775).
 factored in FP32 on the matrix cores and solved with FP64 refinement.
 """
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -337,7 +338,7 @@ class HBSolver:
                          "krylov_iterations": its, "krylov_rel": rel, "damping": scale})
             if verbose:
                 print(f"  HB Newton {it}: |R| {rnorm:.3e} |dx| {step:.3e} gmres {its} "
-                      f"(rel {rel:.1e}) damping {scale:.3f}", flush=True)
+                      f"(rel {rel:.1e}) damping {scale:.3f}", file=sys.stderr, flush=True)
             if scale == 1.0 and step <= vtol:
                 R = self.residual(xf)
                 return xf, {"converged": True, "newton": it + 1, "history": hist,

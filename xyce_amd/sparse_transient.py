@@ -26,6 +26,7 @@ voltage is below vtol volts.  J lives in the circuit's CSR pattern plus
 every diagonal (a voltage source's branch row has none of its own); the
 reference solves these systems with KLU or preconditioned GMRES
 (N_LAS_AztecOOSolver.C)."""
+import sys
 import time
 
 import numpy as np
@@ -253,7 +254,7 @@ class SparseTransient:
             x = x + dx
             dmax = float(torch.max(torch.abs(dx[:self.nv])))
             if self.verbose:
-                print(f"    Newton {it}: |dx| {dmax:.3e}, GMRES {its} (rel {rel:.1e})", flush=True)
+                print(f"    Newton {it}: |dx| {dmax:.3e}, GMRES {its} (rel {rel:.1e})", file=sys.stderr, flush=True)
             if rel > 1e3 * self.krylov_tol:
                 raise NewtonFailure(f"GMRES stalled at relative residual {rel:.2e}")
             if not np.isfinite(dmax):
@@ -326,5 +327,5 @@ class SparseTransient:
             out_v.append(x[rec].cpu().numpy())
             if progress and (k + 1) % progress == 0:
                 print(f"  step {k + 1}/{steps}: t = {tk:.3e} s, Newton {its}, "
-                      f"GMRES total {self.stats['krylov']}", flush=True)
+                      f"GMRES total {self.stats['krylov']}", file=sys.stderr, flush=True)
         return np.asarray(out_t), np.asarray(out_v), np.asarray(newton)
