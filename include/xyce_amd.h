@@ -441,6 +441,10 @@ int xa_hb_load(xa_hb_plan p, xa_group g, const xa_solver_state* ss, const double
  * the same on every call. */
 int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
                 const double* values, const double* x, double* y);
+/* the same for a matrix of few long rows (one 1024-thread block per row,
+ * a fixed summation tree): a supply's KCL row split off a Jacobian */
+int xa_csr_spmv_long(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                     const double* values, const double* x, double* y);
 int xa_hb_precond_create(xa_hb_plan p, int nnz, const int32_t* row_ptr, const int32_t* col_idx,
                          xa_hb_precond* out);
 int xa_hb_precond_destroy(xa_hb_precond pc);
@@ -464,6 +468,12 @@ int xa_ilu0_create(xa_ctx ctx, int n, int nnz, const int32_t* row_ptr, const int
 int xa_ilu0_destroy(xa_ilu0 h);
 int xa_ilu0_factor(xa_ilu0 h, const double* values, int* zero_pivot);
 int xa_ilu0_solve(xa_ilu0 h, const double* b, double* x);
+/* mode 1 (default): the solve runs level-scheduled kernels (rows grouped by
+ * the dependency depth of the pattern, one kernel per level); mode 2: the
+ * same kernels captured once as a HIP graph and replayed; mode 0: rocSPARSE
+ * csrsv; -1 keeps the mode.  Returns the level counts of the L and U
+ * solves. */
+int xa_ilu0_config(xa_ilu0 h, int mode, int* l_levels, int* u_levels);
 
 /* ---- netlist reader and circuit builder --------------------------------- */
 /* The SPICE-netlist subset a device-load circuit needs, read into the

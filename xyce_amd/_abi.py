@@ -115,10 +115,12 @@ def _proto(lib):
         "xa_hb_precond_factor": (C.c_int, [vp, vp, vp, vp]),
         "xa_hb_precond_solve": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
         "xa_csr_spmv": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
+        "xa_csr_spmv_long": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
         "xa_ilu0_create": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, P(vp)]),
         "xa_ilu0_destroy": (C.c_int, [vp]),
         "xa_ilu0_factor": (C.c_int, [vp, vp, P(C.c_int)]),
         "xa_ilu0_solve": (C.c_int, [vp, vp, vp]),
+        "xa_ilu0_config": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int)]),
         "xa_netlist_parse": (C.c_int, [C.c_char_p, P(vp)]),
         "xa_netlist_destroy": (C.c_int, [vp]),
         "xa_netlist_sizes": (C.c_int, [vp, vp, vp, vp, vp]),

@@ -397,6 +397,40 @@ __global__ void __launch_bounds__(256) k_csr_spmv(int n, const int32_t* __restri
   if (lane == 0) y[r] = s;
 }
 
+// y = A x for a matrix of few, long rows (a supply's KCL row): one
+// 1024-thread block per row, thread-strided partial sums added in a fixed
+// tree in LDS.
+__global__ void __launch_bounds__(1024) k_csr_spmv_long(const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci,
+                                                        const double* __restrict__ a,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+  __shared__ double part[1024];
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int k = rp[r] + (int)threadIdx.x; k < rp[r + 1]; k += 1024) s += a[k] * x[ci[k]];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[r] = part[0];
+}
+
+extern "C" int xa_csr_spmv_long(xa_ctx ctx, int n_rows, const int32_t* row_ptr,
+                                const int32_t* col_idx, const double* values, const double* x,
+                                double* y) {
+  Ctx* cx = reinterpret_cast<Ctx*>(ctx);
+  if (!cx || !row_ptr || !col_idx || !values || !x || !y || n_rows < 0)
+    return fail(XA_ERR_ARG, "csr spmv (long rows): null argument or negative size");
+  if (n_rows == 0) return XA_OK;
+  hipLaunchKernelGGL(k_csr_spmv_long, dim3(n_rows), dim3(1024), 0, cx->stream, row_ptr, col_idx,
+                     values, x, y);
+  XA_HIP(hipGetLastError());
+  return XA_OK;
+}
+
 extern "C" int xa_csr_spmv(xa_ctx ctx, int n_rows, const int32_t* row_ptr, const int32_t* col_idx,
                            const double* values, const double* x, double* y) {
   Ctx* cx = reinterpret_cast<Ctx*>(ctx);

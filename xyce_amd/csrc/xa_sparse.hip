@@ -6,8 +6,10 @@
 // N_LAS_AztecOOSolver.C / N_LAS_IfpackPrecond.C).  Here the Krylov solve
 // runs on the device (flexible GMRES in xyce_amd.krylov, products by
 // xa_csr_spmv) and the preconditioner is an incomplete LU with the matrix's
-// own pattern, factored and applied on the device: rocSPARSE's csrilu0 and
-// level-scheduled triangular solves (L unit lower, U upper).  The pattern
+// own pattern, factored on the device by rocSPARSE's csrilu0 and applied by
+// level-scheduled triangular solves of our own (one launch per level, a row
+// per thread; the levels come from the pattern at set-up) or by rocSPARSE's
+// csrsv.  The pattern
 // must hold every diagonal; a voltage source's branch row, whose diagonal is
 // structurally zero in the Jacobian, gets its pivot from the elimination of
 // its node column (the node precedes the branch in the LID order).
@@ -32,6 +34,18 @@ struct Ilu0 {
   double* tmp = nullptr;   // L^-1 b
   void* buf = nullptr;
   bool analysed = false;
+  // level schedules of the triangular solves (rows grouped so that a row's
+  // L (U) entries refer to rows of earlier levels only), host offsets
+  int32_t* diag = nullptr;           // slot of each row's diagonal
+  int32_t* lrows = nullptr;          // rows by L level
+  int32_t* urows = nullptr;          // rows by U level
+  std::vector<int> lptr, uptr;
+  int mode = 1;                      // 1 level kernels launched (default; a graph
+                                     // replay, 2, measured no faster), 0 rocSPARSE csrsv
+  double* bin = nullptr;             // the graph's fixed input and output
+  double* xout = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
   ~Ilu0() {
     if (info) rocsparse_destroy_mat_info(info);
     if (dM) rocsparse_destroy_mat_descr(dM);
@@ -43,8 +57,80 @@ struct Ilu0 {
     if (lu) (void)hipFree(lu);
     if (tmp) (void)hipFree(tmp);
     if (buf) (void)hipFree(buf);
+    if (diag) (void)hipFree(diag);
+    if (lrows) (void)hipFree(lrows);
+    if (urows) (void)hipFree(urows);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (bin) (void)hipFree(bin);
+    if (xout) (void)hipFree(xout);
   }
 };
+
+// forward substitution with the unit lower factor over one level's rows
+__global__ void k_lsolve_level(const int32_t* __restrict__ rows, int cnt,
+                               const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                               const int32_t* __restrict__ diag, const double* __restrict__ lu,
+                               const double* __restrict__ b, double* __restrict__ y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int r = rows[t];
+  double s = b[r];
+  for (int k = rp[r]; k < diag[r]; ++k) s -= lu[k] * y[ci[k]];
+  y[r] = s;
+}
+
+// back substitution with the upper factor over one level's rows
+__global__ void k_usolve_level(const int32_t* __restrict__ rows, int cnt,
+                               const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                               const int32_t* __restrict__ diag, const double* __restrict__ lu,
+                               const double* __restrict__ y, double* __restrict__ x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int r = rows[t];
+  double s = y[r];
+  for (int k = diag[r] + 1; k < rp[r + 1]; ++k) s -= lu[k] * x[ci[k]];
+  x[r] = s / lu[diag[r]];
+}
+
+static void launch_levels(const Ilu0& p, hipStream_t st, const double* b, double* x) {
+  for (size_t l = 0; l + 1 < p.lptr.size(); ++l) {
+    const int cnt = p.lptr[l + 1] - p.lptr[l];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_lsolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                       p.lrows + p.lptr[l], cnt, p.rp, p.ci, p.diag, p.lu, b, p.tmp);
+  }
+  for (size_t l = 0; l + 1 < p.uptr.size(); ++l) {
+    const int cnt = p.uptr[l + 1] - p.uptr[l];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_usolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                       p.urows + p.uptr[l], cnt, p.rp, p.ci, p.diag, p.lu, p.tmp, x);
+  }
+}
+
+// rows bucketed by level: level[i] = 1 + the largest level among the rows
+// its strictly lower (upper) entries name
+static void levels(int n, const int32_t* rp, const int32_t* ci, const std::vector<int>& dg,
+                   bool lower, std::vector<int32_t>& rows, std::vector<int>& ptr) {
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  for (int t = 0; t < n; ++t) {
+    const int i = lower ? t : n - 1 - t;
+    int m = 0;
+    const int k0 = lower ? rp[i] : dg[i] + 1, k1 = lower ? dg[i] : rp[i + 1];
+    for (int k = k0; k < k1; ++k) m = std::max(m, lev[ci[k]] + 1);
+    lev[i] = m;
+    nl = std::max(nl, m + 1);
+  }
+  ptr.assign(nl + 1, 0);
+  for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+  for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+  rows.assign(n, 0);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int t = 0; t < n; ++t) {     // increasing row order inside a level
+    rows[fill[lev[t]]++] = t;
+  }
+}
 
 static int sp_check(rocsparse_status s, const char* what) {
   if (s == rocsparse_status_success) return XA_OK;
@@ -105,6 +191,19 @@ int xa_ilu0_create(xa_ctx ctx, int n, int nnz, const int32_t* row_ptr, const int
   XA_SP(rocsparse_dcsrsv_buffer_size(p.h, rocsparse_operation_none, n, nnz, p.dU, p.lu, p.rp,
                                      p.ci, p.info, &sU));
   XA_HIP(hipMalloc(&p.buf, std::max(sM, std::max(sL, sU)) + 256));
+  std::vector<int> dg(n);
+  for (int r = 0; r < n; ++r)
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      if (col_idx[k] == r) dg[r] = k;
+  std::vector<int32_t> lr, ur;
+  levels(n, row_ptr, col_idx, dg, true, lr, p.lptr);
+  levels(n, row_ptr, col_idx, dg, false, ur, p.uptr);
+  XA_HIP(hipMalloc(&p.diag, sizeof(int32_t) * n));
+  XA_HIP(hipMalloc(&p.lrows, sizeof(int32_t) * n));
+  XA_HIP(hipMalloc(&p.urows, sizeof(int32_t) * n));
+  XA_HIP(hipMemcpy(p.diag, dg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  XA_HIP(hipMemcpy(p.lrows, lr.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  XA_HIP(hipMemcpy(p.urows, ur.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
   *out = hh.release();
   return XA_OK;
 }
@@ -148,12 +247,52 @@ int xa_ilu0_solve(xa_ilu0 hh, const double* b, double* x) {
   Ilu0& p = hh->p;
   if (!p.analysed) return fail(XA_ERR_STATE, "ilu0 solve before a factor");
   XA_HIP(hipSetDevice(p.ctx->device));
+  if (p.mode == 1) {
+    launch_levels(p, p.ctx->stream, b, x);
+    XA_HIP(hipGetLastError());
+    return XA_OK;
+  }
+  if (p.mode == 2) {
+    // the level launches captured once (on a private stream, kernels only)
+    // on fixed buffers, replayed on the context's stream between two copies
+    if (!p.exec) {
+      XA_HIP(hipMalloc(&p.bin, sizeof(double) * p.m));
+      XA_HIP(hipMalloc(&p.xout, sizeof(double) * p.m));
+      hipStream_t cs;
+      XA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        launch_levels(p, cs, p.bin, p.xout);
+        e = hipStreamEndCapture(cs, &p.graph);
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0);
+      (void)hipStreamDestroy(cs);
+      if (e != hipSuccess) return hip_check(e, "ilu0 solve graph capture");
+    }
+    hipStream_t st = p.ctx->stream;
+    XA_HIP(hipMemcpyAsync(p.bin, b, sizeof(double) * p.m, hipMemcpyDeviceToDevice, st));
+    XA_HIP(hipGraphLaunch(p.exec, st));
+    XA_HIP(hipMemcpyAsync(x, p.xout, sizeof(double) * p.m, hipMemcpyDeviceToDevice, st));
+    return XA_OK;
+  }
   XA_SP(rocsparse_set_stream(p.h, p.ctx->stream));
   const double one = 1.0;
   XA_SP(rocsparse_dcsrsv_solve(p.h, rocsparse_operation_none, p.m, p.nnz, &one, p.dL, p.lu, p.rp,
                                p.ci, p.info, b, p.tmp, rocsparse_solve_policy_auto, p.buf));
   XA_SP(rocsparse_dcsrsv_solve(p.h, rocsparse_operation_none, p.m, p.nnz, &one, p.dU, p.lu, p.rp,
                                p.ci, p.info, p.tmp, x, rocsparse_solve_policy_auto, p.buf));
+  return XA_OK;
+}
+
+// solve method (2 level kernels as a replayed HIP graph, 1 level kernels
+// launched, 0 rocSPARSE csrsv) and the level
+// counts of the two triangular solves
+int xa_ilu0_config(xa_ilu0 hh, int mode, int* l_levels, int* u_levels) {
+  if (!hh) return fail(XA_ERR_ARG, "ilu0 config: null handle");
+  if (mode == 0 || mode == 1 || mode == 2) hh->p.mode = mode;
+  else if (mode != -1) return fail(XA_ERR_ARG, "ilu0 config: mode must be 0, 1, 2 or -1 (keep)");
+  if (l_levels) *l_levels = (int)hh->p.lptr.size() - 1;
+  if (u_levels) *u_levels = (int)hh->p.uptr.size() - 1;
   return XA_OK;
 }
 

@@ -74,9 +74,12 @@ def fgmres(matvec, precond, rhs, tol, restart=40, max_iter=400):
 
 class CSRMatrix:
     """A CSR matrix on the device for xa_csr_spmv (pattern fixed, values a
-    float64 tensor that the caller refills)."""
+    float64 tensor that the caller refills, then calls sync()).  Rows longer
+    than long_row (a supply's KCL row) are split off into their own CSR and
+    multiplied by xa_csr_spmv_long, one block per row, so that one wavefront
+    does not walk them alone."""
 
-    def __init__(self, ctx, row_ptr, col_idx):
+    def __init__(self, ctx, row_ptr, col_idx, long_row=4096):
         import torch
         self.ctx = ctx
         self.L = ctx.L
@@ -84,17 +87,54 @@ class CSRMatrix:
         self.nnz = int(row_ptr[-1])
         self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
         self.col_idx = np.ascontiguousarray(col_idx, np.int32)
-        self.rp = torch.from_numpy(self.row_ptr).cuda()
-        self.ci = torch.from_numpy(self.col_idx).cuda()
         self.values = torch.zeros(max(self.nnz, 1), dtype=torch.float64, device="cuda")
+        lens = np.diff(self.row_ptr.astype(np.int64))
+        longr = np.nonzero(lens > long_row)[0]
+        self.split = len(longr) > 0
+        if not self.split:
+            self.rp = torch.from_numpy(self.row_ptr).cuda()
+            self.ci = torch.from_numpy(self.col_idx).cuda()
+            return
+        rows = np.repeat(np.arange(self.n), lens)
+        in_long = np.isin(rows, longr)
+        srp = np.zeros(self.n + 1, np.int64)
+        np.add.at(srp, rows[~in_long] + 1, 1)
+        lrp = np.concatenate([[0], np.cumsum(lens[longr])])
+        self.rp = torch.from_numpy(np.cumsum(srp).astype(np.int32)).cuda()
+        self.ci = torch.from_numpy(self.col_idx[~in_long]).cuda()
+        self.s_src = torch.from_numpy(np.nonzero(~in_long)[0]).cuda()
+        self.l_rp = torch.from_numpy(lrp.astype(np.int32)).cuda()
+        self.l_ci = torch.from_numpy(self.col_idx[in_long]).cuda()
+        self.l_src = torch.from_numpy(np.nonzero(in_long)[0]).cuda()
+        self.l_rows = torch.from_numpy(longr).cuda()
+        self.s_vals = torch.zeros(max(len(self.s_src), 1), dtype=torch.float64, device="cuda")
+        self.l_vals = torch.zeros(max(len(self.l_src), 1), dtype=torch.float64, device="cuda")
+        self.l_y = torch.zeros(len(longr), dtype=torch.float64, device="cuda")
+
+    def sync(self):
+        """Take the current values into the split parts (after a refill)."""
+        if self.split:
+            self.s_vals[:len(self.s_src)] = self.values[self.s_src]
+            self.l_vals[:len(self.l_src)] = self.values[self.l_src]
 
     def matvec(self, x, y=None, values=None):
         import torch
         y = torch.empty(self.n, dtype=x.dtype, device=x.device) if y is None else y
-        vals = self.values if values is None else values
+        if not self.split:
+            vals = self.values if values is None else values
+            check(self.L.xa_csr_spmv(self.ctx.h, self.n, self.rp.data_ptr(), self.ci.data_ptr(),
+                                     vals.data_ptr(), x.data_ptr(), y.data_ptr()),
+                  "xa_csr_spmv", self.L)
+            return y
+        if values is not None:
+            raise ValueError("a split matrix multiplies its synced values")
         check(self.L.xa_csr_spmv(self.ctx.h, self.n, self.rp.data_ptr(), self.ci.data_ptr(),
-                                 vals.data_ptr(), x.data_ptr(), y.data_ptr()),
+                                 self.s_vals.data_ptr(), x.data_ptr(), y.data_ptr()),
               "xa_csr_spmv", self.L)
+        check(self.L.xa_csr_spmv_long(self.ctx.h, len(self.l_rows), self.l_rp.data_ptr(),
+                                      self.l_ci.data_ptr(), self.l_vals.data_ptr(), x.data_ptr(),
+                                      self.l_y.data_ptr()), "xa_csr_spmv_long", self.L)
+        y[self.l_rows] = self.l_y
         return y
 
 
@@ -118,6 +158,15 @@ class ILU0:
         check(self.L.xa_ilu0_factor(self.h, vals.data_ptr(), C.byref(zp)), "xa_ilu0_factor",
               self.L)
         return zp.value
+
+    def config(self, mode=-1):
+        """Set the triangular-solve method (1 level-scheduled kernels, the
+        default; 2 the same replayed as a HIP graph; 0 rocSPARSE csrsv; -1
+        keep); returns (L levels, U levels)."""
+        lv, uv = C.c_int(), C.c_int()
+        check(self.L.xa_ilu0_config(self.h, int(mode), C.byref(lv), C.byref(uv)),
+              "xa_ilu0_config", self.L)
+        return lv.value, uv.value
 
     def solve(self, b, x=None):
         import torch

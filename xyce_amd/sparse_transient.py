@@ -234,8 +234,10 @@ class SparseTransient:
             Jv.zero_()
             Jv[self.map] = a * v["dFdx"][:len(self.map)] + b * v["dQdx"][:len(self.map)]
             self.M.values[:self.M.nnz] = Jv[self.m_src]
+            self.J.sync()
             if self.n_terminal:
                 self.T.values[:self.T.nnz] = Jv[self.t_src]
+                self.T.sync()
             zp = self.ilu.factor()
             if zp >= 0:
                 raise NewtonFailure(f"ILU(0): zero pivot in row {zp}")
