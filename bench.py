@@ -499,7 +499,8 @@ def logic_transient_line(args):
         "transient": {"wall_s": wall, "build_s": t_build, "newton_total": int(nw.sum()),
                       "newton_dcop": int(nw[0]), "newton_per_step_mean": float(nw[1:].mean()),
                       "gmres_total": st["krylov"], "loads": st["loads"], "load_s": st["load_s"],
-                      "factor_s": st["factor_s"], "krylov_s": st["krylov_s"]},
+                      "factor_s": st["factor_s"], "krylov_s": st["krylov_s"],
+                      "ilu_factorisations": st.get("factors")},
     }
     print(json.dumps(out))
 

@@ -40,14 +40,16 @@ class NewtonFailure(RuntimeError):
 
 class SparseTransient:
     def __init__(self, circuit, n_voltages=None, vtol=1e-9, krylov_tol=1e-10, restart=60,
-                 max_krylov=600, order="topo", verbose=False):
+                 max_krylov=600, order="topo", verbose=False, refactor_each_step_iter=False):
         """n_voltages: the unknowns [0, n_voltages) that are node voltages
         (the convergence test reads them; default: the netlist's nodes).
         order: the unknowns' (and rows') order in the solved system --
         "topo" (default) the block-triangular order of the first system
         solved (the DC operating point's Jacobian, see _topo_order), "rcm"
         reverse Cuthill-McKee of the Jacobian's graph, "natural" the LID
-        order."""
+        order.  refactor_each_step_iter: factor the ILU(0) at every Newton
+        iteration of a time step (default: at the first, reused after; the
+        operating point refactors every iteration)."""
         import torch
         from . import default_solver_state
         self.torch = torch
@@ -58,6 +60,7 @@ class SparseTransient:
         self.n = n
         self.nv = circuit.netlist.n_nodes if n_voltages is None else int(n_voltages)
         self.vtol, self.krylov_tol = vtol, krylov_tol
+        self.refactor_each_step_iter = refactor_each_step_iter
         self.restart, self.max_krylov = restart, max_krylov
         # rows of the system: a voltage source's branch row and its node's
         # row trade places, so that neither diagonal is structurally zero
@@ -218,7 +221,10 @@ class SparseTransient:
         self.v["currSta"].copy_(self.v["nextSta"])
         self.v["currSto"].copy_(self.v["nextSto"])
 
-    def _newton(self, x, flags_of, system, max_iter):
+    def _newton(self, x, flags_of, system, max_iter, refactor_each=True):
+        """refactor_each False: the ILU(0) of the first iteration
+        preconditions every iteration of this solve (GMRES still solves the
+        current Jacobian exactly)."""
         torch = self.torch
         v = self.v
         for it in range(max_iter):
@@ -233,14 +239,16 @@ class SparseTransient:
             Jv = self.J.values
             Jv.zero_()
             Jv[self.map] = a * v["dFdx"][:len(self.map)] + b * v["dQdx"][:len(self.map)]
-            self.M.values[:self.M.nnz] = Jv[self.m_src]
             self.J.sync()
             if self.n_terminal:
                 self.T.values[:self.T.nnz] = Jv[self.t_src]
                 self.T.sync()
-            zp = self.ilu.factor()
-            if zp >= 0:
-                raise NewtonFailure(f"ILU(0): zero pivot in row {zp}")
+            if refactor_each or it == 0:
+                self.M.values[:self.M.nnz] = Jv[self.m_src]
+                zp = self.ilu.factor()
+                if zp >= 0:
+                    raise NewtonFailure(f"ILU(0): zero pivot in row {zp}")
+                self.stats["factors"] = self.stats.get("factors", 0) + 1
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             dy, rel, its = fgmres(self.J.matvec, self._precond, rhs, self.krylov_tol,
@@ -320,7 +328,8 @@ class SparseTransient:
                 rhs = -((q - q_n) / h + f) + v["dFdxdVp"][:n] + v["dQdxdVp"][:n] / h
                 return rhs, 1.0, 1.0 / h
 
-            x, _, its = self._newton(x, flags, system, max_iter)
+            x, _, its = self._newton(x, flags, system, max_iter,
+                                     refactor_each=self.refactor_each_step_iter)
             self.accept()
             newton.append(its)
             q_n = v["q"][:n].clone()
