@@ -1461,6 +1461,12 @@ static int read_var(struct or_group_s* g, const char* name, double* out) {
     else if (!strcmp(name, "ueff")) out[i] = I->ueff;
     else if (!strcmp(name, "EsatL")) out[i] = I->EsatL;
     else if (!strcmp(name, "vsattemp")) out[i] = I->vsattemp;
+    else if (!strcmp(name, "ua")) out[i] = p->ua;
+    else if (!strcmp(name, "ub")) out[i] = p->ub;
+    else if (!strcmp(name, "uc")) out[i] = p->uc;
+    else if (!strcmp(name, "ud")) out[i] = p->ud;
+    else if (!strcmp(name, "rds0")) out[i] = p->rds0;
+    else if (!strcmp(name, "vfbzbfactor")) out[i] = p->vfbzbfactor;
     else if (!strcmp(name, "vtm")) out[i] = I->model->vtm;
     else if (!strcmp(name, "coxe")) out[i] = I->model->coxe;
     else if (!strcmp(name, "qsrc")) out[i] = I->qsrc;

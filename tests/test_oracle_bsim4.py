@@ -204,3 +204,42 @@ def test_mobility_temperature_law(u0, up):
     u = u0 / 1e4 if u0 > 1.0 else u0
     want = u * (1.0 - up * np.exp(-l / 1e-8)) * ((85.0 + 273.15) / (27.0 + 273.15)) ** -1.3
     np.testing.assert_allclose(p.orc.read("u0temp"), want, rtol=1e-13)
+
+
+@pytest.mark.parametrize("temp_mod", [0, 1, 2])
+def test_temperature_mode_laws(temp_mod):
+    """tempMod 0 scales UA..UD, VSAT and RDSW additively in T/TNOM - 1;
+    tempMod 1 and 2 multiplicatively in T - TNOM; tempMod 2 takes the
+    threshold's temperature term as -KT1 (T/TNOM - 1) instead of
+    (KT1 + KT1L/Leff)(T/TNOM - 1) (N_DEV_MOSFET_B4.C:6700-6740, 7050-7055):
+    the flat-band factor of tempMod 1 exceeds tempMod 2's by exactly
+    2 KT1 (T/TNOM - 1)."""
+    n = 8
+    rng = splitmix64(40)
+    w = 1e-6 * np.exp(0.2 * rng.standard_normal(n))
+    l = 1e-7 * np.exp(0.2 * rng.standard_normal(n))
+    card = {"TYPE": 1.0, "TNOM": 27.0, "UA": 2e-9, "UB": 1e-18, "UC": 5e-11, "UD": 0.01,
+            "UA1": 3e-3, "UB1": -2e-3, "UC1": 1e-3, "UD1": 5e-3, "VSAT": 8e4, "AT": 2e-3,
+            "RDSW": 150.0, "PRT": 0.4, "KT1": -0.11, "TEMPMOD": float(temp_mod)}
+    T, Tn = 85.0 + 273.15, 27.0 + 273.15
+
+    def build(c):
+        spec = {"models": [(54, c)], "n": n, "model_index": np.zeros(n, np.int32),
+                "inst_params": {"W": w, "L": l, "TEMP": np.full(n, 85.0)}}
+        return _pair(spec)
+    p = build(card)
+    t0, dt = T / Tn - 1.0, T - Tn
+    for name in ("UA", "UB", "UC", "UD"):
+        base, k1 = card[name], card[name + "1"]
+        want = base + k1 * t0 if temp_mod == 0 else base * (1.0 + k1 * dt)
+        np.testing.assert_allclose(p.orc.read(name.lower()), want, rtol=1e-13, err_msg=name)
+    want = 8e4 - 2e-3 * t0 if temp_mod == 0 else 8e4 * (1.0 - 2e-3 * dt)
+    np.testing.assert_allclose(p.orc.read("vsattemp"), want, rtol=1e-14)
+    # rds0 = RDSW(T) / (Weff,CJ in um)^WR with WR = 1 and NF = 1
+    rdsw = 150.0 + 0.4 * t0 if temp_mod == 0 else 150.0 * (1.0 + 0.4 * dt)
+    np.testing.assert_allclose(p.orc.read("rds0"), rdsw / (p.orc.read("weffCJ") * 1e6),
+                               rtol=1e-13)
+    if temp_mod == 1:
+        q = build(dict(card, TEMPMOD=2.0))
+        np.testing.assert_allclose(p.orc.read("vfbzbfactor") - q.orc.read("vfbzbfactor"),
+                                   2 * -0.11 * t0, rtol=1e-9)
