@@ -9,7 +9,10 @@ newtonIter = 1 with voltage limiting.  Inputs are resident in HBM before
 timing starts.  The default line also carries, under "mc", the batched Monte
 Carlo measurement (4096 ring oscillators through one transient, its speedup
 over the plain drop-in and its period error against the oracle; --no-mc
-leaves it off).  Prints one JSON line.
+leaves it off), under "hb_solve" the harmonic-balance Newton-Krylov solve of
+the 2000-transistor amplifier at 129 samples, and under "logic_transient"
+the first 50 ps of the 200k-transistor logic netlist's transient (--no-extra
+leaves those two off).  Prints one JSON line.
 """
 import argparse
 import json
@@ -453,7 +456,7 @@ def hb_solve_measure(args, cells_transistors=2000, N=129, T=1e-9, amp=0.01):
             "output_tone_amplitude_v": tone}
 
 
-def logic_transient_line(args):
+def logic_transient_measure(args, t_stop):
     """The flat logic netlist's FP64 transient (200k BSIM4 transistors in a
     random INV/NAND2/NOR2 DAG, wire RC, 20 PULSE inputs at 1 GHz, 1 ps
     steps): read from its netlist text, DC operating point then fixed steps
@@ -466,7 +469,7 @@ def logic_transient_line(args):
     from xyce_amd.netlist_reader import Circuit, Netlist
     from xyce_amd.sparse_transient import SparseTransient
     t0 = time.perf_counter()
-    text, volts = netlists.logic_netlist(args.n_logic, t_stop=args.tran_stop)
+    text, volts = netlists.logic_netlist(args.n_logic, t_stop=t_stop)
     nl = Netlist(text)
     ctx = xyce_amd.Context(0)
     tr = SparseTransient(Circuit(ctx, nl), order=args.order, verbose=args.verbose)
@@ -479,28 +482,33 @@ def logic_transient_line(args):
     rec = np.arange(1, 21)            # the 20 inputs' nodes
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    t, V, nw = tr.run(torch.from_numpy(x0).cuda(), args.tran_stop, 1e-12, record=rec,
-                      progress=50)
+    t, V, nw = tr.run(torch.from_numpy(x0).cuda(), t_stop, 1e-12, record=rec, progress=50)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    steps = len(t) - 1
     st = tr.stats
+    return {"transistors": args.n_logic, "unknowns": tr.n, "jacobian_nnz": tr.J.nnz,
+            "t_stop": t_stop, "steps": len(t) - 1, "wall_s": wall, "build_s": t_build,
+            "steps_per_s": (len(t) - 1) / wall, "order": args.order,
+            "order_info": getattr(tr, "order_info", None),
+            "newton_total": int(nw.sum()), "newton_dcop": int(nw[0]),
+            "newton_per_step_mean": float(nw[1:].mean()), "gmres_total": st["krylov"],
+            "loads": st["loads"], "load_s": st["load_s"], "factor_s": st["factor_s"],
+            "krylov_s": st["krylov_s"], "ilu_factorisations": st.get("factors")}
+
+
+def logic_transient_line(args):
+    m = logic_transient_measure(args, args.tran_stop)
     out = {
         "metric": "flat logic netlist FP64 transient: time steps/s (200k BSIM4, 1 ps steps)",
-        "value": steps / wall, "unit": "time steps/s", "n_gpus": 1, "steps": steps,
-        "warmup": 0, "ms_per_step": wall / steps * 1e3, "higher_is_better": True,
+        "value": m["steps_per_s"], "unit": "time steps/s", "n_gpus": 1, "steps": m["steps"],
+        "warmup": 0, "ms_per_step": m["wall_s"] / m["steps"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"flat logic netlist, {args.n_logic} BSIM4 transistors, wire RC, "
                                f"20 PULSE inputs at 1 GHz, .TRAN 1ps {args.tran_stop:g}; DC op "
                                "then BE x2 + trapezoid, GMRES + ILU(0) on the device",
-                   "unknowns": tr.n, "jacobian_nnz": tr.J.nnz, "order": args.order,
-                   "order_info": getattr(tr, "order_info", None)},
-        "roofline": None, "cpu_baseline": None,
-        "transient": {"wall_s": wall, "build_s": t_build, "newton_total": int(nw.sum()),
-                      "newton_dcop": int(nw[0]), "newton_per_step_mean": float(nw[1:].mean()),
-                      "gmres_total": st["krylov"], "loads": st["loads"], "load_s": st["load_s"],
-                      "factor_s": st["factor_s"], "krylov_s": st["krylov_s"],
-                      "ilu_factorisations": st.get("factors")},
+                   "unknowns": m["unknowns"], "jacobian_nnz": m["jacobian_nnz"],
+                   "order": m["order"], "order_info": m["order_info"]},
+        "roofline": None, "cpu_baseline": None, "transient": m,
     }
     print(json.dumps(out))
 
@@ -549,6 +557,8 @@ def main():
     ap.add_argument("--mc", action="store_true",
                     help="run the batched Monte Carlo ring-oscillator transient (4096 rings) and "
                          "its speedup over the plain drop-in")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="leave the HB solve and the 50 ps logic transient off the default line")
     ap.add_argument("--no-mc", action="store_true",
                     help="leave the Monte Carlo measurement off the default line")
     ap.add_argument("--logic-transient", action="store_true",
@@ -723,6 +733,13 @@ def main():
         del wl, step
         torch.cuda.empty_cache()
         out["mc"] = mc_measure(args)
+    if not args.no_extra and not args.netlist and args.model == "bsim4" and args.n == 1 << 20:
+        # the other two workloads, short forms: the HB solve of the amplifier
+        # and the first 50 ps of the logic netlist's transient
+        torch.cuda.empty_cache()
+        out["hb_solve"] = hb_solve_measure(args)
+        torch.cuda.empty_cache()
+        out["logic_transient"] = logic_transient_measure(args, 5e-11)
     print(json.dumps(out))
 
 
