@@ -7,9 +7,9 @@
 // runs on the device (flexible GMRES in xyce_amd.krylov, products by
 // xa_csr_spmv) and the preconditioner is an incomplete LU with the matrix's
 // own pattern, factored on the device by rocSPARSE's csrilu0 and applied by
-// level-scheduled triangular solves of our own (one launch per level, a row
-// per thread; the levels come from the pattern at set-up) or by rocSPARSE's
-// csrsv.  The pattern
+// level-scheduled triangular solves of our own (the levels come from the
+// pattern at set-up; a large level is one launch, a run of small levels is
+// walked by one workgroup with a barrier per level) or by rocSPARSE's csrsv.  The pattern
 // must hold every diagonal; a voltage source's branch row, whose diagonal is
 // structurally zero in the Jacobian, gets its pivot from the elimination of
 // its node column (the node precedes the branch in the LID order).
@@ -40,6 +40,8 @@ struct Ilu0 {
   int32_t* lrows = nullptr;          // rows by L level
   int32_t* urows = nullptr;          // rows by U level
   std::vector<int> lptr, uptr;
+  int32_t* d_lptr = nullptr;         // the level offsets on the device
+  int32_t* d_uptr = nullptr;
   int mode = 1;                      // 1 level kernels launched (default; a graph
                                      // replay, 2, measured no faster), 0 rocSPARSE csrsv
   double* bin = nullptr;             // the graph's fixed input and output
@@ -58,6 +60,8 @@ struct Ilu0 {
     if (tmp) (void)hipFree(tmp);
     if (buf) (void)hipFree(buf);
     if (diag) (void)hipFree(diag);
+    if (d_lptr) (void)hipFree(d_lptr);
+    if (d_uptr) (void)hipFree(d_uptr);
     if (lrows) (void)hipFree(lrows);
     if (urows) (void)hipFree(urows);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -93,19 +97,75 @@ __global__ void k_usolve_level(const int32_t* __restrict__ rows, int cnt,
   x[r] = s / lu[diag[r]];
 }
 
+// A run of consecutive small levels (each at most kChain rows) in one
+// workgroup: the block walks the levels in order with a barrier between
+// them, which makes the previous level's results visible to the whole block
+// -- one launch for the run instead of one per level.  Every thread takes
+// every barrier (the loop bounds are uniform).
+constexpr int kChain = 1024;
+
+__global__ void __launch_bounds__(kChain) k_lsolve_chain(
+    const int32_t* __restrict__ rows, const int32_t* __restrict__ lp, int l0, int l1,
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const int32_t* __restrict__ diag, const double* __restrict__ lu,
+    const double* __restrict__ b, double* y) {
+  for (int l = l0; l < l1; ++l) {
+    for (int t = lp[l] + (int)threadIdx.x; t < lp[l + 1]; t += kChain) {
+      const int r = rows[t];
+      double s = b[r];
+      for (int k = rp[r]; k < diag[r]; ++k) s -= lu[k] * y[ci[k]];
+      y[r] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kChain) k_usolve_chain(
+    const int32_t* __restrict__ rows, const int32_t* __restrict__ up, int l0, int l1,
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const int32_t* __restrict__ diag, const double* __restrict__ lu,
+    const double* __restrict__ y, double* x) {
+  for (int l = l0; l < l1; ++l) {
+    for (int t = up[l] + (int)threadIdx.x; t < up[l + 1]; t += kChain) {
+      const int r = rows[t];
+      double s = y[r];
+      for (int k = diag[r] + 1; k < rp[r + 1]; ++k) s -= lu[k] * x[ci[k]];
+      x[r] = s / lu[diag[r]];
+    }
+    __syncthreads();
+  }
+}
+
 static void launch_levels(const Ilu0& p, hipStream_t st, const double* b, double* x) {
-  for (size_t l = 0; l + 1 < p.lptr.size(); ++l) {
-    const int cnt = p.lptr[l + 1] - p.lptr[l];
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_lsolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
-                       p.lrows + p.lptr[l], cnt, p.rp, p.ci, p.diag, p.lu, b, p.tmp);
-  }
-  for (size_t l = 0; l + 1 < p.uptr.size(); ++l) {
-    const int cnt = p.uptr[l + 1] - p.uptr[l];
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_usolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
-                       p.urows + p.uptr[l], cnt, p.rp, p.ci, p.diag, p.lu, p.tmp, x);
-  }
+  // large levels one launch each (many blocks); runs of small ones chained
+  auto sweep = [&](const std::vector<int>& ptr, bool lower) {
+    const int nl = (int)ptr.size() - 1;
+    int l = 0;
+    while (l < nl) {
+      const int cnt = ptr[l + 1] - ptr[l];
+      if (cnt > kChain) {
+        if (lower)
+          hipLaunchKernelGGL(k_lsolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                             p.lrows + ptr[l], cnt, p.rp, p.ci, p.diag, p.lu, b, p.tmp);
+        else
+          hipLaunchKernelGGL(k_usolve_level, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                             p.urows + ptr[l], cnt, p.rp, p.ci, p.diag, p.lu, p.tmp, x);
+        ++l;
+        continue;
+      }
+      int l1 = l;
+      while (l1 < nl && ptr[l1 + 1] - ptr[l1] <= kChain) ++l1;
+      if (lower)
+        hipLaunchKernelGGL(k_lsolve_chain, dim3(1), dim3(kChain), 0, st, p.lrows, p.d_lptr, l,
+                           l1, p.rp, p.ci, p.diag, p.lu, b, p.tmp);
+      else
+        hipLaunchKernelGGL(k_usolve_chain, dim3(1), dim3(kChain), 0, st, p.urows, p.d_uptr, l,
+                           l1, p.rp, p.ci, p.diag, p.lu, p.tmp, x);
+      l = l1;
+    }
+  };
+  sweep(p.lptr, true);
+  sweep(p.uptr, false);
 }
 
 // rows bucketed by level: level[i] = 1 + the largest level among the rows
@@ -204,6 +264,12 @@ int xa_ilu0_create(xa_ctx ctx, int n, int nnz, const int32_t* row_ptr, const int
   XA_HIP(hipMemcpy(p.diag, dg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
   XA_HIP(hipMemcpy(p.lrows, lr.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
   XA_HIP(hipMemcpy(p.urows, ur.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  XA_HIP(hipMalloc(&p.d_lptr, sizeof(int32_t) * p.lptr.size()));
+  XA_HIP(hipMalloc(&p.d_uptr, sizeof(int32_t) * p.uptr.size()));
+  XA_HIP(hipMemcpy(p.d_lptr, p.lptr.data(), sizeof(int32_t) * p.lptr.size(),
+                   hipMemcpyHostToDevice));
+  XA_HIP(hipMemcpy(p.d_uptr, p.uptr.data(), sizeof(int32_t) * p.uptr.size(),
+                   hipMemcpyHostToDevice));
   *out = hh.release();
   return XA_OK;
 }
