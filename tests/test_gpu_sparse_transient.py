@@ -75,3 +75,20 @@ def test_ilu_level_solve_equals_csrsv():
     np.testing.assert_array_equal(x2, x1)
     np.testing.assert_array_equal(x2b, x1)
     del ctx
+
+
+def test_rc_step_follows_the_integrator_recurrences():
+    """The device transient on a linear RC step: the closed-form backward
+    Euler and trapezoid recurrences (the same pin as the oracle's)."""
+    import torch
+    import xyce_amd
+    from test_sparse_transient_oracle import TEXT, _expected
+    from xyce_amd.netlist_reader import Circuit, Netlist
+    from xyce_amd.sparse_transient import SparseTransient
+    nl = Netlist(TEXT)
+    ctx = xyce_amd.Context(0)
+    tr = SparseTransient(Circuit(ctx, nl), vtol=1e-13, krylov_tol=1e-13)
+    out = nl.node("out")
+    t, V, nw = tr.run(torch.zeros(tr.n, dtype=torch.float64, device="cuda"), 40e-12, 1e-12,
+                      record=[out])
+    np.testing.assert_allclose(V[:, 0], _expected(40, 1e-12, 1e-9), rtol=0, atol=1e-13)
