@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -695,8 +697,10 @@ int xa_circuit_build(xa_netlist h, xa_ctx ctx, const xa_device_options* opt, int
   const int n_steps = N.steps.empty() ? 0 : (int)N.steps[0].values.size();
   // step >= 0: that step's circuit; -1: the netlist's values; -2: every
   // step's copy in one build (batched)
-  if (step < -2 || step >= std::max(n_steps, step < 0 ? 0 : 1) || (step == -2 && n_steps == 0))
-    return fail(XA_ERR_ARG, "xa_circuit_build: step out of range");
+  const bool ok = step == -1 || (step == -2 && n_steps > 0) || (step >= 0 && step < n_steps);
+  if (!ok)
+    return fail(XA_ERR_ARG, "xa_circuit_build: step " + std::to_string(step) + " out of range (" +
+                                std::to_string(n_steps) + " .STEP points)");
   std::unique_ptr<xa_circuit_s> hc(new xa_circuit_s());
   Circuit& C = hc->c;
   C.ctx = reinterpret_cast<Ctx*>(ctx);
