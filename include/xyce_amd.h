@@ -488,6 +488,9 @@ int xa_ilu0_config(xa_ilu0 h, int mode, int* l_levels, int* u_levels);
  * continuations; M d g s b model (NMOS/PMOS LEVEL=1 -> MOSFET1, 14/54 ->
  * BSIM4), D a c model [area], Q c b e [s] model [area] (NPN/PNP), R and C
  * n1 n2 [model] [value], V n+ n- [DC] v | SIN(...) | EXP(...) | PULSE(...),
+ * X name nodes... subckt with .SUBCKT name ports ... .ENDS (flattened: an
+ * element inside X1 is named X1:<name>, its own nodes X1:<node>; no
+ * subcircuit parameters),
  * name=value instance parameters, .MODEL, .TRAN tstep tstop [tstart
  * [tmax]] [UIC], .IC V(n)=v or n=v, .STEP [LIN] target:param start stop step |
  * target:param LIST v..., .END; .OPTIONS/.PRINT/.MEASURE lines are

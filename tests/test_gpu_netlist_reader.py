@@ -198,3 +198,14 @@ def test_step_batched_equals_each_step():
                                               err_msg=f"iter {it} {k} copy {c}")
         # the copies differ (the sweep reaches the loads)
         assert not np.array_equal(outs[0]["f"], outs[3]["f"])
+
+
+def test_subcircuit_ring_circuit():
+    """The 17-stage ring written with an inverter subcircuit (the shape of
+    the reference's ringosc_17stg.sp): flattened, built and loaded like the
+    flat netlists."""
+    from test_netlist_reader import RING17_SUBCKT
+    from xyce_amd.netlist_reader import Netlist
+    nl = Netlist(RING17_SUBCKT)
+    print(f"subcircuit ring: parts {[p['type'] for p in nl.parts()]}", flush=True)
+    _check_against_pair(nl, 85)

@@ -141,13 +141,20 @@ def test_step_adds_parameter_not_on_the_element():
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("t\nX1 a b sub\n", "not a device-load element"),
+    ("t\nL1 a b 1n\n", "not a device-load element"),
+    ("t\nX1 a b sub\n", "subcircuit SUB not defined"),
+    ("t\n.subckt s a b\nR1 a b 1\n.ends\nX1 n1 s\n", "X1 connects 1 nodes, S has 2 ports"),
+    ("t\n.ends\n", ".ENDS without .SUBCKT"),
+    ("t\n.subckt s a\n.subckt u b\n.ends\n.ends\n", "nested .SUBCKT"),
+    ("t\n.subckt s a b params: w=1\n.ends\n", "subcircuit parameters"),
+    ("t\n.subckt s a\nR1 a 0 1\n", ".SUBCKT without .ENDS"),
+    ("t\n.subckt s a\nX1 a s\n.ends\nX1 n s\n", "nested deeper than 32"),
     ("t\n.model m nmos level=8\nM1 a b c d m\n", "MOSFET level 8"),
     ("t\nM1 a b c d nomodel\n", "model NOMODEL not defined"),
     ("t\n.model dm d\nM1 a b c d dm\n", "does not fit"),
     ("t\nR1 a 0 1k\n.STEP R2:R LIST 1 2\n", "target R2 not found"),
     ("t\n.STEP R1:R 1 2 -1\nR1 a 0 1\n", "wrong sign"),
-    ("t\n.subckt inv a b\n", "control line .SUBCKT is not read"),
+    ("t\n.include models.lib\n", "control line .INCLUDE is not read"),
     ("t\nV1 a 0 SIN(0 1\n", "unclosed SIN("),
     ("t\n.ic V(0)=1\n", ".IC on ground"),
 ])
@@ -218,3 +225,60 @@ def test_logic_netlist_text_reads_back_exactly():
     np.testing.assert_array_equal(to_gen(parts[3]["ext_nodes"]), np.asarray(r_nodes).reshape(-1))
     np.testing.assert_array_equal(parts[4]["params"]["C"], Cv)
     np.testing.assert_array_equal(to_gen(parts[4]["ext_nodes"]), np.asarray(c_nodes).reshape(-1))
+
+
+RING17_SUBCKT = """17-stage ring oscillator through a subcircuit (the shape of the
+* reference's ringosc_17stg.sp, with BSIM4 level-54 inverters)
+.options abstol=1e-6 reltol=1e-6
+.model nch nmos level=54
+.model pch pmos level=54
+vdd supply 0 dc=1.0
+.subckt inv vin vout vdd gnd
+Mp1 vout vin vdd vdd pch W=180n L=45n
+Mn1 vout vin gnd gnd nch W=90n L=45n
+.ends
+""" + "".join(f"Xinv{k} {k} {k % 17 + 1} supply 0 inv\n" for k in range(1, 18)) + """Cl 1 0 1f
+.ic 1=1
+.tran 1p 1n
+.print tran v(1)
+.measure tran t1 when v(1)=0.5 cross=1
+.end
+"""
+
+
+def test_subcircuit_ring_flattens():
+    nl = Netlist(RING17_SUBCKT)
+    names = nl.node_names
+    assert names[0] == "SUPPLY" and set(names[1:]) == {str(k) for k in range(1, 18)}
+    m, c, v = nl.parts()[1], nl.parts()[2], nl.parts()[0]
+    assert v["type"] == "vsource" and m["type"] == "bsim4" and c["type"] == "capacitor"
+    assert len(m["model_index"]) == 34
+    # instance k's PMOS: drain = node k+1 (the output), gate = node k, source = bulk = supply
+    ext = m["ext_nodes"].reshape(-1, 4)
+    for k in range(17):
+        inp, out = names.index(str(k + 1)), names.index(str((k + 1) % 17 + 1))
+        assert list(ext[2 * k]) == [out, inp, 0, 0]            # Mp1
+        assert list(ext[2 * k + 1]) == [out, inp, -1, -1]      # Mn1
+    np.testing.assert_array_equal(m["model_index"], [1, 0] * 17)
+    nodes, vals = nl.ic
+    assert [names[i] for i in nodes] == ["1"] and list(vals) == [1.0]
+
+
+def test_nested_subcircuits_and_internal_nodes():
+    nl = Netlist("""nested
+.subckt rdiv a b
+R1 a mid 1k
+R2 mid b 1k
+.ends
+.subckt two x y
+Xa x m rdiv
+Xb m y rdiv
+.ends
+X1 in 0 two
+V1 in 0 1
+""")
+    # nodes in order of first appearance in the expanded lines
+    assert nl.node_names == ["IN", "X1:XA:MID", "X1:M", "X1:XB:MID"]
+    r = nl.part(0)
+    assert r["type"] == "resistor"
+    np.testing.assert_array_equal(r["ext_nodes"], [0, 1, 1, 2, 2, 3, 3, -1])

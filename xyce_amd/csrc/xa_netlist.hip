@@ -255,16 +255,33 @@ static int parse_vsource(Device& d, const std::vector<std::string>& t, size_t k,
   return XA_OK;
 }
 
-static int parse_element(Netlist& n, const std::vector<std::string>& t, int line) {
+// A subcircuit instance being expanded: its name prefix ("X1:", "X1:X2:")
+// and its ports (definition name -> the node name in the caller).
+struct Scope {
+  std::string prefix;
+  std::map<std::string, std::string> ports;
+};
+
+// a node token of a line inside `sc`: ground stays ground, a port is the
+// caller's node, any other node is the instance's own ("X1:" + name)
+static std::string scoped_node(const Scope* sc, const std::string& tok) {
+  const std::string u = upper(tok);
+  if (!sc || u == "0" || u == "GND") return u;
+  auto it = sc->ports.find(u);
+  return it != sc->ports.end() ? it->second : sc->prefix + u;
+}
+
+static int parse_element(Netlist& n, const std::vector<std::string>& t, int line,
+                         const Scope* sc = nullptr) {
   Device d;
   d.line = line;
-  d.name = upper(t[0]);
-  d.letter = d.name[0];
+  d.name = (sc ? sc->prefix : std::string()) + upper(t[0]);
+  d.letter = (char)std::toupper((unsigned char)t[0][0]);
   size_t k = 1;
   double v;
   auto is_model = [&](const std::string& s) { return n.model_names.count(upper(s)) > 0; };
   auto nodes = [&](int cnt) {
-    for (int j = 0; j < cnt; ++j) d.nodes.push_back(node_of(n, t[k++]));
+    for (int j = 0; j < cnt; ++j) d.nodes.push_back(node_of(n, scoped_node(sc, t[k++])));
   };
   auto value = [&](const std::string& s, const char* name) {
     if (!parse_number(s, &v)) return perr(line, std::string("bad ") + name + " " + s);
@@ -300,7 +317,7 @@ static int parse_element(Netlist& n, const std::vector<std::string>& t, int line
       if (p.empty()) return perr(line, "Q needs a model");
       if (!is_model(p[0])) {
         if (p.size() < 2) return perr(line, "Q: no .MODEL card names " + p[0]);
-        d.nodes.push_back(node_of(n, p[0]));
+        d.nodes.push_back(node_of(n, scoped_node(sc, p[0])));
         j = 1;
       }
       d.model_name = upper(p[j++]);
@@ -329,7 +346,7 @@ static int parse_element(Netlist& n, const std::vector<std::string>& t, int line
       return XA_OK;
     }
     default:
-      return perr(line, "element " + d.name + " is not a device-load element (M D Q R C V)");
+      return perr(line, "element " + d.name + " is not a device-load element (M D Q R C V, X)");
   }
   if (rc == XA_OK) rc = read_pairs(t, k, line, d.params);
   if (rc != XA_OK) return rc;
@@ -482,6 +499,45 @@ static int resolve(Netlist& n) {
   return XA_OK;
 }
 
+struct Subckt {
+  std::vector<std::string> ports;
+  std::vector<std::pair<std::string, int>> body;   // logical lines, line numbers
+};
+
+// X<name> n1 .. nk <subckt>: the subcircuit's lines with its nodes renamed
+// and its elements named "<X name>:<element>" (flattened, as the reference's
+// parser expands subcircuits before the devices are made); nested instances
+// recurse.
+static int expand(Netlist& n, const std::map<std::string, Subckt>& defs,
+                  const std::vector<std::string>& t, int line, const Scope* sc, int depth) {
+  if (depth > 32) return perr(line, "subcircuits nested deeper than 32 (recursive definition?)");
+  std::vector<std::string> p;
+  for (size_t k = 1; k < t.size() && t[k] != "=" && upper(t[k]) != "PARAMS:"; ++k) p.push_back(t[k]);
+  if (!p.empty() && t.size() > p.size() + 1 && t[p.size() + 1] == "=") p.pop_back();
+  if (p.empty()) return perr(line, upper(t[0]) + " names no subcircuit");
+  const std::string sub = upper(p.back());
+  auto it = defs.find(sub);
+  if (it == defs.end()) return perr(line, "subcircuit " + sub + " not defined");
+  const Subckt& S = it->second;
+  if (p.size() - 1 != S.ports.size())
+    return perr(line, upper(t[0]) + " connects " + std::to_string(p.size() - 1) + " nodes, " +
+                          sub + " has " + std::to_string(S.ports.size()) + " ports");
+  if (p.size() + 1 < t.size()) return perr(line, "subcircuit parameters are not read");
+  Scope inner;
+  inner.prefix = (sc ? sc->prefix : std::string()) + upper(t[0]) + ":";
+  for (size_t j = 0; j < S.ports.size(); ++j) inner.ports[S.ports[j]] = scoped_node(sc, p[j]);
+  for (const auto& l : S.body) {
+    const std::vector<std::string> b = tokenize(l.first);
+    if (b.empty()) continue;
+    int rc;
+    if (b[0][0] == '.') rc = perr(l.second, "control line " + upper(b[0]) + " inside a subcircuit");
+    else if (std::toupper((unsigned char)b[0][0]) == 'X') rc = expand(n, defs, b, l.second, &inner, depth + 1);
+    else rc = parse_element(n, b, l.second, &inner);
+    if (rc != XA_OK) return rc;
+  }
+  return XA_OK;
+}
+
 static int parse(const char* text, Netlist& n) {
   std::istringstream in(text ? text : "");
   std::string raw;
@@ -507,12 +563,43 @@ static int parse(const char* text, Netlist& n) {
     const std::vector<std::string> t = tokenize(l.first);
     if (t.size() >= 2 && upper(t[0]) == ".MODEL") n.model_names.insert(upper(t[1]));
   }
-  bool end = false;
+  // .SUBCKT name ports ... .ENDS blocks out of the main line list (a .MODEL
+  // inside one is read as a global card)
+  std::map<std::string, Subckt> defs;
+  std::vector<std::pair<std::string, int>> main_lines;
+  Subckt* open = nullptr;
   for (auto& l : lines) {
+    const std::vector<std::string> t = tokenize(l.first);
+    const std::string c = t.empty() ? "" : upper(t[0]);
+    if (c == ".SUBCKT") {
+      if (open) return perr(l.second, "nested .SUBCKT definitions are not read");
+      if (t.size() < 2) return perr(l.second, ".SUBCKT needs a name");
+      const std::string name = upper(t[1]);
+      if (defs.count(name)) return perr(l.second, "subcircuit " + name + " defined twice");
+      open = &defs[name];
+      for (size_t k = 2; k < t.size(); ++k) {
+        if (t[k] == "=" || upper(t[k]) == "PARAMS:") return perr(l.second, "subcircuit parameters are not read");
+        open->ports.push_back(upper(t[k]));
+      }
+    } else if (c == ".ENDS") {
+      if (!open) return perr(l.second, ".ENDS without .SUBCKT");
+      open = nullptr;
+    } else if (open && c != ".MODEL") {
+      open->body.push_back(l);
+    } else {
+      main_lines.push_back(l);
+    }
+  }
+  if (open) return perr(lineno, ".SUBCKT without .ENDS");
+  bool end = false;
+  for (auto& l : main_lines) {
     if (end) break;
     const std::vector<std::string> t = tokenize(l.first);
     if (t.empty()) continue;
-    const int rc = t[0][0] == '.' ? parse_control(n, t, l.second, &end) : parse_element(n, t, l.second);
+    int rc;
+    if (t[0][0] == '.') rc = parse_control(n, t, l.second, &end);
+    else if (std::toupper((unsigned char)t[0][0]) == 'X') rc = expand(n, defs, t, l.second, nullptr, 0);
+    else rc = parse_element(n, t, l.second);
     if (rc != XA_OK) return rc;
   }
   return resolve(n);
