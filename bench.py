@@ -441,7 +441,8 @@ def hb_solve_measure(args, cells_transistors=2000, N=129, T=1e-9, amp=0.01):
     hb.plan.dft(xt0, xf0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xf, st = hb.solve(xf0, vtol=1e-9, krylov_tol=1e-10, verbose=True)
+    xf, st = hb.solve(xf0, vtol=1e-9, krylov_tol=1e-10, verbose=True,
+                      refactor_every=getattr(args, "hb_refactor", 2))
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     out = [nl.node(f"n{len(v0) - 2}"), nl.node(f"n{len(v0) - 1}")]
@@ -571,6 +572,8 @@ def main():
     ap.add_argument("--order", default="topo", choices=["topo", "rcm", "natural"],
                     help="--logic-transient: unknown order of the sparse solve")
     ap.add_argument("--verbose", action="store_true", help="print every Newton iteration")
+    ap.add_argument("--hb-refactor", type=int, default=2,
+                    help="--hb-solve: refactor the preconditioner every N Newton iterations")
     ap.add_argument("--hb-solve", action="store_true",
                     help="the full harmonic-balance Newton-Krylov solve of the 2000-transistor "
                          "differential amplifier with its filter and sources, 129 samples")

@@ -312,10 +312,13 @@ class HBSolver:
         return fgmres(self.matvec, self.precond, rhs, tol, restart, max_iter)
 
     def solve(self, xf0, vtol=1e-9, max_newton=40, krylov_tol=1e-9, restart=40,
-              max_krylov=400, max_step=0.5, verbose=False):
+              max_krylov=400, max_step=0.5, verbose=False, refactor_every=2):
         """Newton from xf0 ([n, 2N] spectrum) until the largest time-domain
         update is below vtol volts; each update is scaled so that no sample
-        moves by more than max_step.  Returns (xf, stats)."""
+        moves by more than max_step.  refactor_every: the preconditioner is
+        refactored from the current Jacobians every that many Newton
+        iterations (GMRES always applies the current ones).  Returns (xf,
+        stats)."""
         torch = self.torch
         xf = xf0.clone()
         hist = []
@@ -323,9 +326,10 @@ class HBSolver:
         for it in range(max_newton):
             R = self.residual(xf)
             rnorm = float(torch.max(torch.abs(self.pack(R))))
-            info = self.pc.factor(self.v["dFdx"], self.v["dQdx"])
-            if np.any(info != 0):
-                raise RuntimeError(f"HB preconditioner: singular block (info {info.tolist()})")
+            if it % max(1, int(refactor_every)) == 0:
+                info = self.pc.factor(self.v["dFdx"], self.v["dQdx"])
+                if np.any(info != 0):
+                    raise RuntimeError(f"HB preconditioner: singular block (info {info.tolist()})")
             dy, rel, its = self.fgmres(-self.pack(R), krylov_tol, restart, max_krylov)
             dxf = self.unpack(dy)
             self.plan.ift(dxf, upd_t)
